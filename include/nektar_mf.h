@@ -1,0 +1,125 @@
+/*
+ * nektar_mf.h -- C ABI of the MI355X matrix-free spectral/hp operator
+ * library (libnektar_amd.so).
+ *
+ * The surface mirrors the two plugin interfaces the reference path sits
+ * behind (paths relative to the Nektar++ 5.10.0 tree):
+ *   - MatrixFree::Operator and its string-keyed factory
+ *     (library/MatrixFreeOps/Operator.hpp:59-366, Operator.cpp:40-90);
+ *   - Collections::Helmholtz_MatrixFree / *_MatrixFree wrappers
+ *     (library/Collections/Helmholtz.cpp:586-783, BwdTrans.cpp:160-244,
+ *      IProductWRTBase.cpp:182-269, PhysDeriv.cpp:272-402),
+ * plus the iterative C0 Helmholtz solve they serve
+ * (MultiRegions/GlobalLinSysIterativeFull.cpp:96-383 driving
+ *  LibUtilities/LinearAlgebra/NekLinSysIterCG.cpp:118-372 with
+ *  MultiRegions/PreconditionerDiagonal.cpp:69-189).
+ *
+ * Conventions
+ *   - Every function returns NMF_OK (0) or a non-zero NMF_ERR_* code;
+ *     nmf_last_error() returns a thread-local description of the last
+ *     failure (the C++ wrappers turn it into an exception, as NEKERROR does).
+ *   - Array arguments of *_apply are DEVICE pointers of the operator's
+ *     precision, element-major and unpadded exactly as the Collections
+ *     interface passes them: coefficients [e][ncoeff], physical values
+ *     [e][nqtot] with direction 0 fastest.  *_host variants take host FP64
+ *     arrays and synchronise.
+ *   - `stream` is a hipStream_t (NULL = default stream).  A handle is not
+ *     thread-safe; concurrent use needs external synchronisation (as the
+ *     reference operators, which hold mutable lambda state).
+ *   - Geometry is given per element, not interleaved:
+ *       regular  jac[e],           df[e*9 + d]
+ *       deformed jac[e*nqtot + q], df[(e*nqtot + q)*9 + d]
+ *     with df[j*3 + k] = d xi_k / d x_j (GeomFactors.cpp:401-477).
+ */
+#ifndef NEKTAR_MF_H
+#define NEKTAR_MF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NMF_OK 0
+#define NMF_ERR_INVALID 1     /* bad argument                        */
+#define NMF_ERR_UNSUPPORTED 2 /* shape / order / key not provided    */
+#define NMF_ERR_STATE 3       /* e.g. geometry not set before apply  */
+#define NMF_ERR_DEVICE 4      /* HIP runtime or kernel launch error  */
+#define NMF_ERR_MEMORY 5      /* device allocation failed / budget   */
+
+#define NMF_FP64 0
+#define NMF_FP32 1
+
+#define NMF_SHAPE_HEX 0
+#define NMF_SHAPE_TET 1
+
+typedef struct nmf_operator_s *nmf_operator;
+typedef struct nmf_solver_s *nmf_solver;
+
+/* Last error message of the calling thread ("" if none). */
+const char *nmf_last_error(void);
+/* Library version as major*10000 + minor*100 + patch. */
+int nmf_version(void);
+
+/* ---------------------------------------------------------------------
+ * Device memory helpers (for callers without their own allocator).
+ * ------------------------------------------------------------------- */
+int nmf_device_malloc(void **ptr, size_t bytes);
+int nmf_device_free(void *ptr);
+int nmf_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int nmf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int nmf_device_synchronize(void);
+
+/* ---------------------------------------------------------------------
+ * Quadrature / basis tables of the MODIFIED expansion (host, FP64).
+ * Replaces LibUtilities::Basis::GetBdata/GetDbdata/GetZ/GetW and
+ * Points::GetD (Foundations/Basis.cpp:276-645, GaussPoints.cpp:117-281).
+ * shape NMF_SHAPE_*, dir 0..2, numpoints0 = points in direction 0.
+ * bdata/dbdata sizes nmodes_dir*nq_dir, D nq_dir^2 (D[k*nq+i]=l_k'(z_i)),
+ * z, w nq_dir (w without the Radau pre-scaling).  Any pointer may be NULL.
+ * ------------------------------------------------------------------- */
+int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0,
+                     int *nq_dir, int *nmodes_dir, double *bdata,
+                     double *dbdata, double *z, double *w, double *D);
+
+/* ---------------------------------------------------------------------
+ * MatrixFree::Operator
+ * ------------------------------------------------------------------- */
+/* MatrixFree::GetOperatorFactory().CreateInstance(key, basis, nElmt)
+ * (Operator.cpp:40-90).  key = "<Op>_<Shape>_<Regular|Deformed>" as built
+ * by GetOpstring, Op in {BwdTrans, IProduct, PhysDeriv, Helmholtz},
+ * Shape in {Hex, Tet}.  nummodes = P+1 per direction, numpoints0 = Q in
+ * direction 0 (tets use Q-1 Gauss-Radau points in directions 1, 2). */
+int nmf_operator_create(const char *key, int nummodes, int numpoints0,
+                        int nelmt, int precision, nmf_operator *out);
+int nmf_operator_destroy(nmf_operator op);
+/* GetInputSize/GetOutputSize per element (Collections/Operator.h:178-230) */
+int nmf_operator_info(nmf_operator op, int *ncoeff, int *nqtot, int *nelmt,
+                      int *deformed, int *precision);
+/* Operator::SetJac (Operator.hpp:104-105) -- host FP64, see layout above */
+int nmf_operator_set_jac(nmf_operator op, const double *jac);
+/* Operator::SetDF (Operator.hpp:99-101) */
+int nmf_operator_set_df(nmf_operator op, const double *df);
+/* Helmholtz::SetLambda (Operator.hpp:298-301) */
+int nmf_operator_set_lambda(nmf_operator op, double lambda);
+/* Helmholtz::SetConstVarDiffusion (Operator.hpp:303-313): d6 = {d00, d01,
+ * d11, d02, d12, d22}, symmetric positive definite; NULL resets to I. */
+int nmf_operator_set_const_var_diffusion(nmf_operator op, const double *d6);
+/* operator()(input, output) for BwdTrans / IProduct / Helmholtz
+ * (Operator.hpp:119-121, 181-183, 294-296) */
+int nmf_operator_apply(nmf_operator op, const void *in, void *out,
+                       void *stream);
+/* PhysDeriv operator()(input, output[3]) (Operator.hpp:238-240) */
+int nmf_operator_apply_physderiv(nmf_operator op, const void *in, void *out0,
+                                 void *out1, void *out2, void *stream);
+/* Host FP64 convenience forms (copy in, apply, copy out, synchronise). */
+int nmf_operator_apply_host(nmf_operator op, const double *in, double *out);
+int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in,
+                                      double *out0, double *out1,
+                                      double *out2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,243 @@
+"""MI355X-native matrix-free spectral/hp element operators (gfx950 HIP).
+
+Python mirror of the operator interface of Nektar++'s matrix-free path:
+
+* ``GetOperatorFactory().CreateInstance(key, nummodes, numpoints, nelmt)``
+  mirrors ``MatrixFree::GetOperatorFactory()``
+  (library/MatrixFreeOps/Operator.cpp:40-90) with keys built like
+  ``GetOpstring`` ("Helmholtz_Hex_Regular", "PhysDeriv_Tet_Deformed", ...);
+* :class:`Operator` mirrors ``MatrixFree::Operator`` and its subclasses
+  (Operator.hpp:70-366): ``SetJac``, ``SetDF``, ``SetLambda``,
+  ``SetConstVarDiffusion`` and ``operator()``.
+
+Everything runs through the C ABI of ``libnektar_amd.so`` (declared in
+``include/nektar_mf.h``).  There is no CPU fallback: importing this package
+raises if the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+try:  # torch first, so that one HIP runtime serves both torch and this library
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is optional for host-only use
+    torch = None
+
+import numpy as np
+
+__all__ = [
+    "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring",
+    "basis_tables", "FP64", "FP32",
+]
+
+FP64, FP32 = 0, 1
+SHAPE_HEX, SHAPE_TET = 0, 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnektar_amd.so")
+
+
+class NektarError(RuntimeError):
+    """Raised for a non-zero status from the C ABI (mirrors NekError)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[nmf error {code}] {msg}")
+        self.code = code
+
+
+_LIB = None
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+_SIGNATURES = {
+    "nmf_last_error": (ctypes.c_char_p, []),
+    "nmf_version": (ctypes.c_int, []),
+    "nmf_device_malloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
+    "nmf_device_free": (ctypes.c_int, [_P]),
+    "nmf_memcpy_h2d": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "nmf_memcpy_d2h": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "nmf_device_synchronize": (ctypes.c_int, []),
+    "nmf_basis_tables": (ctypes.c_int, [ctypes.c_int] * 4 + [_I, _I] + [_D] * 5),
+    "nmf_operator_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.POINTER(_P)]),
+    "nmf_operator_destroy": (ctypes.c_int, [_P]),
+    "nmf_operator_info": (ctypes.c_int, [_P, _I, _I, _I, _I, _I]),
+    "nmf_operator_set_jac": (ctypes.c_int, [_P, _D]),
+    "nmf_operator_set_df": (ctypes.c_int, [_P, _D]),
+    "nmf_operator_set_lambda": (ctypes.c_int, [_P, ctypes.c_double]),
+    "nmf_operator_set_const_var_diffusion": (ctypes.c_int, [_P, _D]),
+    "nmf_operator_apply": (ctypes.c_int, [_P, _P, _P, _P]),
+    "nmf_operator_apply_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "nmf_operator_apply_host": (ctypes.c_int, [_P, _D, _D]),
+    "nmf_operator_apply_physderiv_host": (ctypes.c_int, [_P, _D, _D, _D, _D]),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load libnektar_amd.so (built in-tree by ``__graft_entry__.build()``)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise NektarError(rc, lib().nmf_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def GetOpstring(shape: str, deformed: bool = False) -> str:
+    """``MatrixFree::GetOpstring`` (Operator.cpp:46-90) for Hex/Tet."""
+    return "_" + shape + ("_Deformed" if deformed else "_Regular")
+
+
+def basis_tables(shape: int, direction: int, nummodes: int, numpoints: int) -> dict:
+    """Host FP64 1-D tables of the MODIFIED expansion in one direction."""
+    nq, nmodes = ctypes.c_int(), ctypes.c_int()
+    check(lib().nmf_basis_tables(shape, direction, nummodes, numpoints, ctypes.byref(nq),
+                                 ctypes.byref(nmodes), None, None, None, None, None))
+    n, m = nq.value, nmodes.value
+    out = {k: np.zeros(v) for k, v in
+           dict(bdata=m * n, dbdata=m * n, z=n, w=n, D=n * n).items()}
+    check(lib().nmf_basis_tables(shape, direction, nummodes, numpoints, None, None,
+                                 _dptr(out["bdata"]), _dptr(out["dbdata"]), _dptr(out["z"]),
+                                 _dptr(out["w"]), _dptr(out["D"])))
+    out["nq"], out["nmodes"] = n, m
+    return out
+
+
+def _device_ptr(x, dtype_expected, count) -> int:
+    """Device pointer of a torch tensor (checked) or a raw int address."""
+    if isinstance(x, int):
+        return x
+    if torch is not None and isinstance(x, torch.Tensor):
+        if not x.is_cuda:
+            raise NektarError(1, "tensor is not on the GPU")
+        if x.dtype != dtype_expected:
+            raise NektarError(1, f"tensor dtype {x.dtype} != operator precision {dtype_expected}")
+        if not x.is_contiguous():
+            raise NektarError(1, "tensor must be contiguous")
+        if x.numel() < count:
+            raise NektarError(1, f"tensor has {x.numel()} entries, operator needs {count}")
+        return x.data_ptr()
+    raise NektarError(1, f"unsupported array type {type(x)}")
+
+
+class Operator:
+    """``MatrixFree::Operator`` instance for one element collection."""
+
+    def __init__(self, key: str, nummodes: int, numpoints: int, nelmt: int,
+                 precision: int = FP64):
+        self._h = None
+        h = ctypes.c_void_p()
+        check(lib().nmf_operator_create(key.encode(), nummodes, numpoints, nelmt, precision,
+                                        ctypes.byref(h)))
+        self._h = h
+        self.key = key
+        self.op = key.split("_")[0]
+        nc, nq, ne, de, pr = (ctypes.c_int() for _ in range(5))
+        check(lib().nmf_operator_info(h, ctypes.byref(nc), ctypes.byref(nq), ctypes.byref(ne),
+                                      ctypes.byref(de), ctypes.byref(pr)))
+        self.ncoeff, self.nqtot, self.nelmt = nc.value, nq.value, ne.value
+        self.deformed, self.precision = bool(de.value), pr.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nmf_operator_destroy(self._h)
+            self._h = None
+
+    # --- MatrixFree::Operator set-up -------------------------------------
+    def _geom_points(self):
+        return self.nelmt * (self.nqtot if self.deformed else 1)
+
+    def SetJac(self, jac) -> None:
+        jac = np.ascontiguousarray(jac, dtype=np.float64)
+        if jac.size != self._geom_points():
+            raise NektarError(1, f"SetJac: expected {self._geom_points()} values, got {jac.size}")
+        check(lib().nmf_operator_set_jac(self._h, _dptr(jac)))
+
+    def SetDF(self, df) -> None:
+        df = np.ascontiguousarray(df, dtype=np.float64)
+        if df.size != 9 * self._geom_points():
+            raise NektarError(1, f"SetDF: expected {9 * self._geom_points()} values, got {df.size}")
+        check(lib().nmf_operator_set_df(self._h, _dptr(df)))
+
+    def SetLambda(self, lam: float) -> None:
+        check(lib().nmf_operator_set_lambda(self._h, float(lam)))
+
+    def SetConstVarDiffusion(self, d6) -> None:
+        if d6 is None:
+            check(lib().nmf_operator_set_const_var_diffusion(self._h, None))
+            return
+        d = np.ascontiguousarray(d6, dtype=np.float64)
+        check(lib().nmf_operator_set_const_var_diffusion(self._h, _dptr(d)))
+
+    # --- sizes (Collections::Operator::GetInputSize/GetOutputSize) --------
+    def input_size(self) -> int:
+        return self.nelmt * (self.ncoeff if self.op in ("BwdTrans", "Helmholtz") else self.nqtot)
+
+    def output_size(self) -> int:
+        return self.nelmt * (self.ncoeff if self.op in ("IProduct", "Helmholtz") else self.nqtot)
+
+    def torch_dtype(self):
+        return torch.float32 if self.precision == FP32 else torch.float64
+
+    # --- operator() -------------------------------------------------------
+    def __call__(self, inp, out, out1=None, out2=None, stream=None):
+        """Apply on device arrays (torch tensors or raw device addresses)."""
+        dt = self.torch_dtype() if torch is not None else None
+        s = stream
+        if s is None and torch is not None:
+            s = torch.cuda.current_stream().cuda_stream
+        pin = _device_ptr(inp, dt, self.input_size())
+        if self.op == "PhysDeriv":
+            ptrs = [_device_ptr(o, dt, self.output_size()) for o in (out, out1, out2)]
+            check(lib().nmf_operator_apply_physderiv(self._h, pin, *ptrs, s))
+        else:
+            check(lib().nmf_operator_apply(self._h, pin, _device_ptr(out, dt, self.output_size()), s))
+
+    def apply_host(self, inp: np.ndarray):
+        """Host FP64 in / out (synchronous); returns ndarray(s)."""
+        x = np.ascontiguousarray(inp, dtype=np.float64)
+        if x.size != self.input_size():
+            raise NektarError(1, f"input has {x.size} values, expected {self.input_size()}")
+        if self.op == "PhysDeriv":
+            outs = [np.zeros(self.output_size()) for _ in range(3)]
+            check(lib().nmf_operator_apply_physderiv_host(self._h, _dptr(x), *map(_dptr, outs)))
+            return outs
+        y = np.zeros(self.output_size())
+        check(lib().nmf_operator_apply_host(self._h, _dptr(x), _dptr(y)))
+        return y
+
+
+class _Factory:
+    """``MatrixFree::GetOperatorFactory()`` (Operator.cpp:40-44)."""
+
+    def CreateInstance(self, key: str, nummodes: int, numpoints: int, nelmt: int,
+                       precision: int = FP64) -> Operator:
+        return Operator(key, nummodes, numpoints, nelmt, precision)
+
+
+_FACTORY = _Factory()
+
+
+def GetOperatorFactory() -> _Factory:
+    return _FACTORY

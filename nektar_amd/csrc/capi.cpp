@@ -1,0 +1,231 @@
+// C ABI entry points (include/nektar_mf.h).
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "basis.hpp"
+#include "common.hpp"
+#include "mf_operator.hpp"
+
+struct nmf_operator_s {
+    std::unique_ptr<nektar_amd::MFOperator> impl;
+};
+
+namespace nektar_amd {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+void clear_last_error() { g_last_error.clear(); }
+}  // namespace nektar_amd
+
+using namespace nektar_amd;
+
+extern "C" {
+
+const char *nmf_last_error(void) { return g_last_error.c_str(); }
+
+int nmf_version(void) { return 100; }
+
+int nmf_device_malloc(void **ptr, size_t bytes)
+{
+    return guard([&] {
+        NMF_REQUIRE(ptr, NMF_ERR_INVALID, "null output pointer");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        hipError_t e = hipMalloc(ptr, bytes);
+        NMF_REQUIRE(e == hipSuccess, NMF_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    });
+}
+
+int nmf_device_free(void *ptr)
+{
+    return guard([&] {
+        if (ptr) NMF_HIP(hipFree(ptr));
+    });
+}
+
+int nmf_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    return guard([&] {
+        if (bytes) NMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int nmf_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    return guard([&] {
+        if (bytes) NMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int nmf_device_synchronize(void)
+{
+    return guard([&] { NMF_HIP(hipDeviceSynchronize()); });
+}
+
+int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0, int *nq_dir, int *nmodes_dir,
+                     double *bdata, double *dbdata, double *z, double *w, double *D)
+{
+    return guard([&] {
+        NMF_REQUIRE(shape == NMF_SHAPE_HEX || shape == NMF_SHAPE_TET, NMF_ERR_UNSUPPORTED, "shape must be hex or tet");
+        NMF_REQUIRE(dir >= 0 && dir < 3, NMF_ERR_INVALID, "dir must be 0..2");
+        const Shape s = shape == NMF_SHAPE_HEX ? Shape::Hex : Shape::Tet;
+        int nq[3];
+        default_points(s, numpoints0, nq);
+        Basis1D b = make_basis(default_basis(s, dir), nummodes, nq[dir]);
+        if (nq_dir) *nq_dir = nq[dir];
+        if (nmodes_dir) *nmodes_dir = b.nmodes;
+        if (bdata) std::memcpy(bdata, b.bdata.data(), b.bdata.size() * sizeof(double));
+        if (dbdata) std::memcpy(dbdata, b.dbdata.data(), b.dbdata.size() * sizeof(double));
+        if (z) std::memcpy(z, b.pts.z.data(), b.pts.z.size() * sizeof(double));
+        if (w) std::memcpy(w, b.pts.w.data(), b.pts.w.size() * sizeof(double));
+        if (D) std::memcpy(D, b.pts.D.data(), b.pts.D.size() * sizeof(double));
+    });
+}
+
+int nmf_operator_create(const char *key, int nummodes, int numpoints0, int nelmt, int precision, nmf_operator *out)
+{
+    return guard([&] {
+        NMF_REQUIRE(key && out, NMF_ERR_INVALID, "null key or output handle");
+        *out = nullptr;
+        auto h = std::make_unique<nmf_operator_s>();
+        h->impl = std::make_unique<MFOperator>(key, nummodes, numpoints0, nelmt, precision);
+        *out = h.release();
+    });
+}
+
+int nmf_operator_destroy(nmf_operator op)
+{
+    return guard([&] { delete op; });
+}
+
+#define NMF_OP_CHECK() NMF_REQUIRE(op && op->impl, NMF_ERR_INVALID, "null operator handle")
+
+int nmf_operator_info(nmf_operator op, int *ncoeff, int *nqtot, int *nelmt, int *deformed, int *precision)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        const MFOperator &o = *op->impl;
+        if (ncoeff) *ncoeff = o.ncoeff;
+        if (nqtot) *nqtot = o.nqtot;
+        if (nelmt) *nelmt = o.nelmt;
+        if (deformed) *deformed = o.deformed;
+        if (precision) *precision = o.fp32 ? NMF_FP32 : NMF_FP64;
+    });
+}
+
+int nmf_operator_set_jac(nmf_operator op, const double *jac)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        op->impl->set_jac(jac);
+    });
+}
+
+int nmf_operator_set_df(nmf_operator op, const double *df)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        op->impl->set_df(df);
+    });
+}
+
+int nmf_operator_set_lambda(nmf_operator op, double lambda)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(op->impl->op == OpType::Helmholtz, NMF_ERR_INVALID, "SetLambda: not a Helmholtz operator");
+        op->impl->set_lambda(lambda);
+    });
+}
+
+int nmf_operator_set_const_var_diffusion(nmf_operator op, const double *d6)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(op->impl->op == OpType::Helmholtz, NMF_ERR_INVALID, "SetConstVarDiffusion: not a Helmholtz operator");
+        op->impl->set_diffusion(d6);
+    });
+}
+
+int nmf_operator_apply(nmf_operator op, const void *in, void *out, void *stream)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(op->impl->op != OpType::PhysDeriv, NMF_ERR_INVALID,
+                    "PhysDeriv has three outputs: use nmf_operator_apply_physderiv");
+        op->impl->apply(in, out, nullptr, nullptr, static_cast<hipStream_t>(stream));
+    });
+}
+
+int nmf_operator_apply_physderiv(nmf_operator op, const void *in, void *out0, void *out1, void *out2, void *stream)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(op->impl->op == OpType::PhysDeriv, NMF_ERR_INVALID, "not a PhysDeriv operator");
+        op->impl->apply(in, out0, out1, out2, static_cast<hipStream_t>(stream));
+    });
+}
+
+namespace {
+size_t in_count(const MFOperator &o)
+{
+    return (size_t)o.nelmt * (o.op == OpType::BwdTrans || o.op == OpType::Helmholtz ? o.ncoeff : o.nqtot);
+}
+size_t out_count(const MFOperator &o)
+{
+    return (size_t)o.nelmt * (o.op == OpType::IProduct || o.op == OpType::Helmholtz ? o.ncoeff : o.nqtot);
+}
+void to_device(DeviceBuffer &b, const double *h, size_t n, bool fp32)
+{
+    std::vector<double> v(h, h + n);
+    b.upload(v, fp32);
+}
+void to_host(const DeviceBuffer &b, double *h, size_t n, bool fp32)
+{
+    if (n == 0) return;
+    if (fp32) {
+        std::vector<float> f(n);
+        NMF_HIP(hipMemcpy(f.data(), b.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) h[i] = f[i];
+    } else {
+        NMF_HIP(hipMemcpy(h, b.get(), n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+}
+}  // namespace
+
+int nmf_operator_apply_host(nmf_operator op, const double *in, double *out)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        MFOperator &o = *op->impl;
+        NMF_REQUIRE(o.op != OpType::PhysDeriv, NMF_ERR_INVALID, "use nmf_operator_apply_physderiv_host");
+        const size_t ni = in_count(o), no = out_count(o);
+        DeviceBuffer di, dout;
+        to_device(di, in, ni, o.fp32);
+        dout.allocate(no * (o.fp32 ? sizeof(float) : sizeof(double)));
+        o.apply(di.get(), dout.get(), nullptr, nullptr, nullptr);
+        NMF_HIP(hipDeviceSynchronize());
+        to_host(dout, out, no, o.fp32);
+    });
+}
+
+int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in, double *out0, double *out1, double *out2)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        MFOperator &o = *op->impl;
+        NMF_REQUIRE(o.op == OpType::PhysDeriv, NMF_ERR_INVALID, "not a PhysDeriv operator");
+        const size_t n = (size_t)o.nelmt * o.nqtot, sz = o.fp32 ? sizeof(float) : sizeof(double);
+        DeviceBuffer di, d0(n * sz), d1(n * sz), d2(n * sz);
+        to_device(di, in, n, o.fp32);
+        o.apply(di.get(), d0.get(), d1.get(), d2.get(), nullptr);
+        NMF_HIP(hipDeviceSynchronize());
+        to_host(d0, out0, n, o.fp32);
+        to_host(d1, out1, n, o.fp32);
+        to_host(d2, out2, n, o.fp32);
+    });
+}
+
+}  // extern "C"

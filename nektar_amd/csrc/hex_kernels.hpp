@@ -1,0 +1,408 @@
+// Sum-factorised hex kernels for gfx950 (CDNA4), templated on the scalar
+// type T (double or float), modes NM and points NQ per direction.
+//
+// Work decomposition: a workgroup holds EPB elements; each element owns
+// NQ*NQ work-items.  A contraction along one direction is done by one
+// work-item per 1-D "line" of the tensor: it reads the line's NM (or NQ)
+// inputs once, keeps them in registers and produces all outputs of the line,
+// so every LDS read feeds NQ fused multiply-adds.  The 1-D tables (basis,
+// derivative matrix, weights) are wave-uniform and live in scalar registers
+// (loaded with scalar loads from the operator's table buffer).  Three LDS
+// slabs of NQ^3 values per element carry the tensor between directions.
+//
+// Layouts (element-major, the Collections contract):
+//   coefficients  in[e*NM^3 + (r*NM + q)*NM + p]     (p fastest)
+//   quadrature    u[e*NQ^3 + (k*NQ + j)*NQ + i]      (i fastest)
+//   table         B[p*NQ + i] | D[l*NQ + i] | W[i]    (D[l*NQ+i] = l_l'(z_i))
+//   Helmholtz geometry, regular : g[e*7 + c]
+//       c = 0..5: J*m_{00,01,02,11,12,22}, c = 6: J
+//   Helmholtz geometry, deformed: g[(e*7 + c)*NQ^3 + q]  (weights folded in)
+//   IProduct  jac: regular jac[e], deformed jac[e*NQ^3 + q]
+//   PhysDeriv df : regular df[e*9 + d], deformed df[(e*9 + d)*NQ^3 + q]
+//                  df[j*3 + k] = d xi_k / d x_j
+//
+// Reference: library/MatrixFreeOps/BwdTransKernels.hpp:160-227,
+// IProductKernels.hpp:270-350, PhysDerivKernels.hpp:184-345 and 436-507,
+// HelmholtzKernels.hpp:389-696, Helmholtz.h:549-599.  The fused Helmholtz
+// kernel uses IProduct(dbdata_d, v) == IProduct(bdata, D_d^T (W J v)) (exact
+// because dbdata = D * bdata, Basis.cpp:493-495) so that the four inner
+// products of the reference collapse into one transpose transform.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nektar_amd {
+namespace hex {
+
+template <int NQ>
+constexpr int elements_per_block()
+{
+    return (256 / (NQ * NQ)) > 0 ? (256 / (NQ * NQ)) : 1;
+}
+
+template <typename T, int NM, int NQ>
+struct Tab {
+    const T *__restrict__ base;
+    __device__ __forceinline__ T B(int p, int i) const { return base[p * NQ + i]; }
+    __device__ __forceinline__ T D(int l, int i) const { return base[NM * NQ + l * NQ + i]; }
+    __device__ __forceinline__ T W(int i) const { return base[NM * NQ + NQ * NQ + i]; }
+};
+
+// ---- forward transform: coefficients (global) -> u column in registers,
+//      also written to slab A[(k*NQ+j)*NQ+i] when `keep` is set.
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const T *__restrict__ xin,
+                                              bool valid, int t, T *A, T *Bs, T (&u)[NQ])
+{
+    // F1: contract p -> i along lines (q, r)
+    if (t < NM * NM) {
+        const int q = t % NM, r = t / NM;
+        T x[NM];
+#pragma unroll
+        for (int p = 0; p < NM; ++p) x[p] = valid ? xin[(r * NM + q) * NM + p] : T(0);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            T s = x[0] * tab.B(0, i);
+#pragma unroll
+            for (int p = 1; p < NM; ++p) s = fma(x[p], tab.B(p, i), s);
+            A[(r * NM + q) * NQ + i] = s;
+        }
+    }
+    __syncthreads();
+    // F2: contract q -> j along lines (i, r)
+    if (t < NQ * NM) {
+        const int i = t % NQ, r = t / NQ;
+        T a[NM];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) a[q] = A[(r * NM + q) * NQ + i];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            T s = a[0] * tab.B(0, j);
+#pragma unroll
+            for (int q = 1; q < NM; ++q) s = fma(a[q], tab.B(q, j), s);
+            Bs[(r * NQ + j) * NQ + i] = s;
+        }
+    }
+    __syncthreads();
+    // F3: contract r -> k along lines (i, j), result stays in registers
+    {
+        const int i = t % NQ, j = t / NQ;
+        T b[NM];
+#pragma unroll
+        for (int r = 0; r < NM; ++r) b[r] = Bs[(r * NQ + j) * NQ + i];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            T s = b[0] * tab.B(0, k);
+#pragma unroll
+            for (int r = 1; r < NM; ++r) s = fma(b[r], tab.B(r, k), s);
+            u[k] = s;
+        }
+    }
+}
+
+// ---- transpose transform: y column (registers, at quadrature points,
+//      already weighted) -> coefficients (global)
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const T (&y)[NQ], bool valid,
+                                                  int t, T *Bs, T *C, T *__restrict__ xout)
+{
+    {
+        const int i = t % NQ, j = t / NQ;
+#pragma unroll
+        for (int r = 0; r < NM; ++r) {
+            T s = y[0] * tab.B(r, 0);
+#pragma unroll
+            for (int k = 1; k < NQ; ++k) s = fma(y[k], tab.B(r, k), s);
+            Bs[(r * NQ + j) * NQ + i] = s;
+        }
+    }
+    __syncthreads();
+    if (t < NQ * NM) {
+        const int i = t % NQ, r = t / NQ;
+        T z[NQ];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) z[j] = Bs[(r * NQ + j) * NQ + i];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            T s = z[0] * tab.B(q, 0);
+#pragma unroll
+            for (int j = 1; j < NQ; ++j) s = fma(z[j], tab.B(q, j), s);
+            C[(r * NM + q) * NQ + i] = s;
+        }
+    }
+    __syncthreads();
+    if (t < NM * NM) {
+        const int q = t % NM, r = t / NM;
+        T z[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) z[i] = C[(r * NM + q) * NQ + i];
+        if (valid) {
+#pragma unroll
+            for (int p = 0; p < NM; ++p) {
+                T s = z[0] * tab.B(p, 0);
+#pragma unroll
+                for (int i = 1; i < NQ; ++i) s = fma(z[i], tab.B(p, i), s);
+                xout[(r * NM + q) * NM + p] = s;
+            }
+        }
+    }
+}
+
+// ---- collocation derivatives in directions 0 and 1 from slab A (u),
+//      written to slabs Bs (d0) and C (d1).  Direction 2 is in registers.
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const T *A, T *Bs, T *C)
+{
+    const int l0 = t % NQ, k = t / NQ;
+    {   // line (j = l0, k) along i
+        T v[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l0) * NQ + l];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            T s = v[0] * tab.D(0, i);
+#pragma unroll
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(l, i), s);
+            Bs[(k * NQ + l0) * NQ + i] = s;
+        }
+    }
+    {   // line (i = l0, k) along j
+        T v[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l) * NQ + l0];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            T s = v[0] * tab.D(0, j);
+#pragma unroll
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(l, j), s);
+            C[(k * NQ + j) * NQ + l0] = s;
+        }
+    }
+}
+
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void deriv2_column(const Tab<T, NM, NQ> &tab, const T (&u)[NQ], T (&d2)[NQ])
+{
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        T s = u[0] * tab.D(0, k);
+#pragma unroll
+        for (int l = 1; l < NQ; ++l) s = fma(u[l], tab.D(l, k), s);
+        d2[k] = s;
+    }
+}
+
+// ===================================================================
+// BwdTrans (BwdTransKernels.hpp:160-227)
+template <typename T, int NM, int NQ>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void bwdtrans_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ in, T *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ T lds[EPB][2][NQ3];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    T u[NQ];
+    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * NM * NM * NM, valid, t, lds[le][0], lds[le][1], u);
+    if (valid) {
+        const int i = t % NQ, j = t / NQ;
+        T *o = out + (size_t)e * NQ3;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) o[(k * NQ + j) * NQ + i] = u[k];
+    }
+}
+
+// IProductWRTBase (IProductKernels.hpp:270-350, SCALE=APPEND=false)
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void iproduct_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ jac, const T *__restrict__ in,
+    T *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ T lds[EPB][2][NQ3];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % NQ, j = t / NQ;
+    T y[NQ];
+    const T wij = tab.W(i) * tab.W(j);
+    const T *f = in + (size_t)e * NQ3;
+    T jv = T(0);
+    if (!DEF && valid) jv = jac[e];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const int c = (k * NQ + j) * NQ + i;
+        T fv = valid ? f[c] : T(0);
+        T jq = DEF ? (valid ? jac[(size_t)e * NQ3 + c] : T(0)) : jv;
+        y[k] = fv * jq * (wij * tab.W(k));
+    }
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, lds[le][0], lds[le][1], out + (size_t)e * NM * NM * NM);
+}
+
+// PhysDeriv (PhysDerivKernels.hpp:436-507 then :326-340)
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void physderiv_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ df, const T *__restrict__ in,
+    T *__restrict__ out0, T *__restrict__ out1, T *__restrict__ out2)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ T lds[EPB][3][NQ3];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % NQ, j = t / NQ;
+    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
+    T u[NQ], d2[NQ];
+    const T *f = in + (size_t)e * NQ3;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const int c = (k * NQ + j) * NQ + i;
+        u[k] = valid ? f[c] : T(0);
+        A[c] = u[k];
+    }
+    deriv2_column<T, NM, NQ>(tab, u, d2);
+    __syncthreads();
+    deriv01<T, NM, NQ>(tab, t, A, Bs, C);
+    __syncthreads();
+    if (!valid) return;
+    T g[9];
+    if (!DEF) {
+#pragma unroll
+        for (int d = 0; d < 9; ++d) g[d] = df[(size_t)e * 9 + d];
+    }
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        const int c = (k * NQ + j) * NQ + i;
+        if (DEF) {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) g[d] = df[((size_t)e * 9 + d) * NQ3 + c];
+        }
+        const T a0 = Bs[c], a1 = C[c], a2 = d2[k];
+        T o0 = a0 * g[0];
+        o0 = fma(a1, g[1], o0);
+        o0 = fma(a2, g[2], o0);
+        T o1 = a0 * g[3];
+        o1 = fma(a1, g[4], o1);
+        o1 = fma(a2, g[5], o1);
+        T o2 = a0 * g[6];
+        o2 = fma(a1, g[7], o2);
+        o2 = fma(a2, g[8], o2);
+        out0[(size_t)e * NQ3 + c] = o0;
+        out1[(size_t)e * NQ3 + c] = o1;
+        out2[(size_t)e * NQ3 + c] = o2;
+    }
+}
+
+// Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
+    T *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ T lds[EPB][3][NQ3];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % NQ, j = t / NQ;
+    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
+
+    T u[NQ];
+    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * NM * NM * NM, valid, t, A, Bs, u);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) A[(k * NQ + j) * NQ + i] = u[k];
+    T d2[NQ];
+    deriv2_column<T, NM, NQ>(tab, u, d2);
+    __syncthreads();
+    deriv01<T, NM, NQ>(tab, t, A, Bs, C);
+    __syncthreads();
+
+    // pointwise: metric, Jacobian, weights, mass term
+    T y[NQ], v2[NQ];
+    {
+        T g[7];
+        if (!DEF) {
+#pragma unroll
+            for (int c = 0; c < 7; ++c) g[c] = valid ? geo[(size_t)e * 7 + c] : T(0);
+        }
+        const T wij = tab.W(i) * tab.W(j);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int c = (k * NQ + j) * NQ + i;
+            T w = T(1);
+            if (DEF) {
+#pragma unroll
+                for (int cc = 0; cc < 7; ++cc) g[cc] = valid ? geo[((size_t)e * 7 + cc) * NQ3 + c] : T(0);
+            } else {
+                w = wij * tab.W(k);
+            }
+            const T a0 = Bs[c], a1 = C[c], a2 = d2[k];
+            T t0 = g[0] * a0;
+            t0 = fma(g[1], a1, t0);
+            t0 = fma(g[2], a2, t0);
+            T t1 = g[1] * a0;
+            t1 = fma(g[3], a1, t1);
+            t1 = fma(g[4], a2, t1);
+            T t2 = g[2] * a0;
+            t2 = fma(g[4], a1, t2);
+            t2 = fma(g[5], a2, t2);
+            if (!DEF) {
+                t0 *= w;
+                t1 *= w;
+                t2 *= w;
+            }
+            Bs[c] = t0;
+            C[c] = t1;
+            v2[k] = t2;
+            y[k] = (lambda * g[6]) * (DEF ? u[k] : w * u[k]);
+        }
+    }
+    // D2^T v2 (in registers): (D^T v)(k) = sum_l D(l,k) v(l), D(out,in)=tab.D(in,out)
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        T s = y[k];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) s = fma(tab.D(k, l), v2[l], s);
+        y[k] = s;
+    }
+    __syncthreads();
+    {   // D0^T along i on line (j = l0, k): A <- D0^T Bs
+        const int l0 = t % NQ, k = t / NQ;
+        T v[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = Bs[(k * NQ + l0) * NQ + l];
+#pragma unroll
+        for (int ii = 0; ii < NQ; ++ii) {
+            T s = v[0] * tab.D(ii, 0);
+#pragma unroll
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(ii, l), s);
+            A[(k * NQ + l0) * NQ + ii] = s;
+        }
+    }
+    __syncthreads();
+    {   // D1^T along j on line (i = l0, k): A += D1^T C
+        const int l0 = t % NQ, k = t / NQ;
+        T v[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = C[(k * NQ + l) * NQ + l0];
+#pragma unroll
+        for (int jj = 0; jj < NQ; ++jj) {
+            T s = A[(k * NQ + jj) * NQ + l0];
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) s = fma(v[l], tab.D(jj, l), s);
+            A[(k * NQ + jj) * NQ + l0] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) y[k] += A[(k * NQ + j) * NQ + i];
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, out + (size_t)e * NM * NM * NM);
+}
+
+}  // namespace hex
+}  // namespace nektar_amd

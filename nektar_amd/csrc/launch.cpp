@@ -1,0 +1,36 @@
+// Runtime (shape, NM, NQ) -> template instantiation dispatch.
+#include "launch.hpp"
+
+namespace nektar_amd {
+
+#define NMF_DECL(n)                                              \
+    hipError_t launch_hex_nm##n(int nq, const KernelArgs &a);    \
+    hipError_t launch_tet_nm##n(int nq, const KernelArgs &a);
+NMF_DECL(2) NMF_DECL(3) NMF_DECL(4) NMF_DECL(5) NMF_DECL(6) NMF_DECL(7) NMF_DECL(8) NMF_DECL(9) NMF_DECL(10)
+#undef NMF_DECL
+
+bool hex_supported(int nm, int nq) { return nm >= kMinModes && nm <= kMaxModes && (nq == nm + 1 || nq == nm + 2); }
+bool tet_supported(int nm, int nq0) { return nm >= kMinModes && nm <= kMaxModes && (nq0 == nm + 1 || nq0 == nm + 2); }
+
+#define NMF_CASE(shape, n) \
+    case n: return launch_##shape##_nm##n(nq, a);
+
+hipError_t launch_hex(int nm, int nq, const KernelArgs &a)
+{
+    switch (nm) {
+        NMF_CASE(hex, 2) NMF_CASE(hex, 3) NMF_CASE(hex, 4) NMF_CASE(hex, 5) NMF_CASE(hex, 6)
+        NMF_CASE(hex, 7) NMF_CASE(hex, 8) NMF_CASE(hex, 9) NMF_CASE(hex, 10)
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tet(int nm, int nq, const KernelArgs &a)
+{
+    switch (nm) {
+        NMF_CASE(tet, 2) NMF_CASE(tet, 3) NMF_CASE(tet, 4) NMF_CASE(tet, 5) NMF_CASE(tet, 6)
+        NMF_CASE(tet, 7) NMF_CASE(tet, 8) NMF_CASE(tet, 9) NMF_CASE(tet, 10)
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace nektar_amd

@@ -1,0 +1,32 @@
+// Kernel launch dispatch for the matrix-free operators.  Each (shape, NM)
+// pair is instantiated in its own translation unit (ops_<shape>_nm<NM>.hip)
+// for NQ = NM+1 (the MODIFIED default Q = P+2) and NQ = NM+2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nektar_amd {
+
+enum class OpType { BwdTrans = 0, IProduct = 1, PhysDeriv = 2, Helmholtz = 3 };
+
+struct KernelArgs {
+    OpType op;
+    bool fp32;
+    bool deformed;
+    int nelmt;
+    const void *tab;
+    const void *geo;
+    double lambda;
+    const void *in;
+    void *out0, *out1, *out2;
+    hipStream_t stream;
+};
+
+constexpr int kMinModes = 2, kMaxModes = 10;
+
+bool hex_supported(int nm, int nq);
+bool tet_supported(int nm, int nq0);
+hipError_t launch_hex(int nm, int nq, const KernelArgs &a);
+hipError_t launch_tet(int nm, int nq0, const KernelArgs &a);
+
+}  // namespace nektar_amd

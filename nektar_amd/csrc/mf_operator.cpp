@@ -1,0 +1,241 @@
+// MatrixFree operator object: key parsing, 1-D tables, geometry packing and
+// launch.  Reference behaviour: MatrixFree::Helper set-up
+// (Operator.hpp:529-641), Collections::Helmholtz_MatrixFree::UpdateFactors
+// (Collections/Helmholtz.cpp:615-691), CoalescedGeomData layouts
+// (Collections/CoalescedGeomData.cpp:90-342).
+#include "mf_operator.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace nektar_amd {
+
+void parse_operator_key(const std::string &key, OpType &op, Shape &shape, bool &deformed)
+{
+    auto p1 = key.find('_');
+    auto p2 = key.find('_', p1 == std::string::npos ? 0 : p1 + 1);
+    NMF_REQUIRE(p1 != std::string::npos && p2 != std::string::npos, NMF_ERR_INVALID,
+                "operator key '" + key + "' is not <Op>_<Shape>_<Regular|Deformed>");
+    const std::string o = key.substr(0, p1), s = key.substr(p1 + 1, p2 - p1 - 1), g = key.substr(p2 + 1);
+    if (o == "BwdTrans") op = OpType::BwdTrans;
+    else if (o == "IProduct") op = OpType::IProduct;
+    else if (o == "PhysDeriv") op = OpType::PhysDeriv;
+    else if (o == "Helmholtz") op = OpType::Helmholtz;
+    else throw Error(NMF_ERR_UNSUPPORTED, "operator '" + o + "' is not provided (BwdTrans, IProduct, PhysDeriv, Helmholtz)");
+    if (s == "Hex") shape = Shape::Hex;
+    else if (s == "Tet") shape = Shape::Tet;
+    else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet)");
+    if (g == "Regular") deformed = false;
+    else if (g == "Deformed") deformed = true;
+    else throw Error(NMF_ERR_INVALID, "geometry tag '" + g + "' must be Regular or Deformed");
+}
+
+std::vector<double> MFOperator::build_tables(Shape shape, int nm, int nq0)
+{
+    std::vector<double> t;
+    if (shape == Shape::Hex) {
+        Basis1D b = make_basis(BasisType::ModifiedA, nm, nq0);
+        t.insert(t.end(), b.bdata.begin(), b.bdata.end());
+        t.insert(t.end(), b.pts.D.begin(), b.pts.D.end());
+        t.insert(t.end(), b.pts.w.begin(), b.pts.w.end());
+        return t;
+    }
+    Basis1D b0 = make_basis(BasisType::ModifiedA, nm, nq0);
+    Basis1D b1 = make_basis(BasisType::ModifiedB, nm, nq0 - 1);
+    Basis1D b2 = make_basis(BasisType::ModifiedC, nm, nq0 - 1);
+    for (auto *b : {&b0, &b1, &b2}) t.insert(t.end(), b->bdata.begin(), b->bdata.end());
+    for (auto *b : {&b0, &b1, &b2}) t.insert(t.end(), b->pts.D.begin(), b->pts.D.end());
+    for (auto *b : {&b0, &b1, &b2}) {
+        const double f = weight_prescale(b->pts.type);
+        for (double w : b->pts.w) t.push_back(f * w);
+    }
+    for (double z : b0.pts.z) t.push_back(0.5 * (1.0 + z));
+    for (double z : b1.pts.z) t.push_back(0.5 * (1.0 + z));
+    for (double z : b1.pts.z) t.push_back(2.0 / (1.0 - z));
+    for (double z : b2.pts.z) t.push_back(2.0 / (1.0 - z));
+    return t;
+}
+
+MFOperator::MFOperator(const std::string &key, int nummodes, int numpoints0, int n, int precision)
+{
+    parse_operator_key(key, op, shape, deformed);
+    NMF_REQUIRE(precision == NMF_FP64 || precision == NMF_FP32, NMF_ERR_INVALID, "precision must be NMF_FP64 or NMF_FP32");
+    NMF_REQUIRE(n >= 0, NMF_ERR_INVALID, "negative element count");
+    fp32 = precision == NMF_FP32;
+    nm = nummodes;
+    nelmt = n;
+    const bool ok = shape == Shape::Hex ? hex_supported(nummodes, numpoints0) : tet_supported(nummodes, numpoints0);
+    NMF_REQUIRE(ok, NMF_ERR_UNSUPPORTED,
+                "(nummodes=" + std::to_string(nummodes) + ", numpoints=" + std::to_string(numpoints0) +
+                    ") not instantiated: need 2 <= nummodes <= 10 and numpoints0 in {nummodes+1, nummodes+2}");
+    default_points(shape, numpoints0, nq);
+    nqtot = nq[0] * nq[1] * nq[2];
+    ncoeff = num_coeffs(shape, nummodes);
+    tab.upload(build_tables(shape, nm, numpoints0), fp32);
+    if (op == OpType::BwdTrans) deformed = false;  // BwdTrans needs no geometry
+}
+
+void MFOperator::set_jac(const double *j)
+{
+    NMF_REQUIRE(j || nelmt == 0, NMF_ERR_INVALID, "SetJac: null pointer");
+    jac.assign(j, j + geom_points());
+    have_jac = true;
+    geo_dirty = true;
+}
+
+void MFOperator::set_df(const double *d)
+{
+    NMF_REQUIRE(d || nelmt == 0, NMF_ERR_INVALID, "SetDF: null pointer");
+    df.assign(d, d + 9 * geom_points());
+    have_df = true;
+    geo_dirty = true;
+}
+
+void MFOperator::set_diffusion(const double *d6)
+{
+    if (!d6) {
+        has_diff = false;
+        const double id[6] = {1, 0, 1, 0, 0, 1};
+        std::memcpy(diff, id, sizeof(diff));
+    } else {
+        has_diff = true;
+        std::memcpy(diff, d6, sizeof(diff));
+    }
+    geo_dirty = true;
+}
+
+namespace {
+
+// Lower Cholesky factor of the symmetric tensor {d00,d01,d11,d02,d12,d22}.
+void cholesky3(const double *d6, double L[3][3])
+{
+    const double K[3][3] = {{d6[0], d6[1], d6[3]}, {d6[1], d6[2], d6[4]}, {d6[3], d6[4], d6[5]}};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) L[i][j] = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        double s = K[j][j];
+        for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+        NMF_REQUIRE(s > 0.0, NMF_ERR_INVALID, "SetConstVarDiffusion: tensor is not positive definite");
+        L[j][j] = std::sqrt(s);
+        for (int i = j + 1; i < 3; ++i) {
+            double t = K[i][j];
+            for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+            L[i][j] = t / L[j][j];
+        }
+    }
+}
+
+// df'[a*3+d] = sum_j L[j][a] df[j*3+d], so that df'^T df' = df^T K df.
+void apply_diffusion(const double L[3][3], const double *f, double *out)
+{
+    for (int a = 0; a < 3; ++a)
+        for (int d = 0; d < 3; ++d) {
+            double s = 0.0;
+            for (int j = 0; j < 3; ++j) s += L[j][a] * f[j * 3 + d];
+            out[a * 3 + d] = s;
+        }
+}
+
+// metric m = {00, 01, 02, 11, 12, 22}, m_de = sum_j f[j*3+d] f[j*3+e]
+// (HelmholtzKernels.hpp:440-462)
+void metric6(const double *f, double *m)
+{
+    const int idx[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int c = 0; c < 6; ++c) {
+        const int d = idx[c][0], e = idx[c][1];
+        m[c] = f[d] * f[e] + f[3 + d] * f[3 + e] + f[6 + d] * f[6 + e];
+    }
+}
+
+}  // namespace
+
+std::vector<double> MFOperator::pack_geometry() const
+{
+    const size_t gp = geom_points();
+    std::vector<double> g;
+    if (op == OpType::IProduct) return jac;
+    if (op == OpType::PhysDeriv) {
+        if (!deformed) return df;
+        g.resize(df.size());
+        for (int e = 0; e < nelmt; ++e)
+            for (int q = 0; q < nqtot; ++q)
+                for (int d = 0; d < 9; ++d)
+                    g[((size_t)e * 9 + d) * nqtot + q] = df[((size_t)e * nqtot + q) * 9 + d];
+        return g;
+    }
+    // Helmholtz
+    double L[3][3];
+    if (has_diff) cholesky3(diff, L);
+    auto fetch_df = [&](size_t pt, double *f) {
+        if (has_diff) apply_diffusion(L, &df[pt * 9], f);
+        else std::memcpy(f, &df[pt * 9], 9 * sizeof(double));
+    };
+    if (shape == Shape::Hex) {
+        if (!deformed) {
+            g.resize(gp * 7);
+            for (size_t e = 0; e < gp; ++e) {
+                double f[9], m[6];
+                fetch_df(e, f);
+                metric6(f, m);
+                for (int c = 0; c < 6; ++c) g[e * 7 + c] = jac[e] * m[c];
+                g[e * 7 + 6] = jac[e];
+            }
+        } else {
+            Points1D P = make_points(PointsType::GLL, nq[0]);
+            const int Q = nq[0];
+            g.resize(gp * 7);
+            for (int e = 0; e < nelmt; ++e)
+                for (int k = 0, q = 0; k < Q; ++k)
+                    for (int j = 0; j < Q; ++j)
+                        for (int i = 0; i < Q; ++i, ++q) {
+                            const size_t pt = (size_t)e * nqtot + q;
+                            double f[9], m[6];
+                            fetch_df(pt, f);
+                            metric6(f, m);
+                            const double wj = P.w[i] * P.w[j] * P.w[k] * jac[pt];
+                            for (int c = 0; c < 6; ++c) g[((size_t)e * 7 + c) * nqtot + q] = wj * m[c];
+                            g[((size_t)e * 7 + 6) * nqtot + q] = wj;
+                        }
+        }
+        return g;
+    }
+    // tet: df (possibly diffusion-transformed) and J, per element or point
+    g.resize(gp * 10);
+    if (!deformed) {
+        for (size_t e = 0; e < gp; ++e) {
+            fetch_df(e, &g[e * 10]);
+            g[e * 10 + 9] = jac[e];
+        }
+    } else {
+        for (int e = 0; e < nelmt; ++e)
+            for (int q = 0; q < nqtot; ++q) {
+                const size_t pt = (size_t)e * nqtot + q;
+                double f[9];
+                fetch_df(pt, f);
+                for (int c = 0; c < 9; ++c) g[((size_t)e * 10 + c) * nqtot + q] = f[c];
+                g[((size_t)e * 10 + 9) * nqtot + q] = jac[pt];
+            }
+    }
+    return g;
+}
+
+void MFOperator::ensure_geometry()
+{
+    if (needs_jac()) NMF_REQUIRE(have_jac, NMF_ERR_STATE, "Jacobian not set (SetJac) before apply");
+    if (needs_df()) NMF_REQUIRE(have_df, NMF_ERR_STATE, "derivative factors not set (SetDF) before apply");
+    if (!geo_dirty || op == OpType::BwdTrans) return;
+    geo.upload(pack_geometry(), fp32);
+    geo_dirty = false;
+}
+
+void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s)
+{
+    NMF_REQUIRE(nelmt == 0 || (in && out0), NMF_ERR_INVALID, "null input/output pointer");
+    if (op == OpType::PhysDeriv) NMF_REQUIRE(nelmt == 0 || (out1 && out2), NMF_ERR_INVALID, "PhysDeriv needs three outputs");
+    ensure_geometry();
+    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in, out0, out1, out2, s};
+    hipError_t e = shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);
+    NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+}
+
+}  // namespace nektar_amd

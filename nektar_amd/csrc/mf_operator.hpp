@@ -1,0 +1,59 @@
+// MatrixFree operator object behind the C ABI: one operator type for one
+// collection of same-shape, same-order elements, exactly as one
+// MatrixFree::*Template instance is (library/MatrixFreeOps/Operator.hpp,
+// Helmholtz.h:62-110).  Host keeps the FP64 geometry as given; the device
+// holds the 1-D tables and the geometry packed for the kernels.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace nektar_amd {
+
+struct MFOperator {
+    OpType op = OpType::Helmholtz;
+    Shape shape = Shape::Hex;
+    bool deformed = false;
+    bool fp32 = false;
+    int nm = 0, nq[3] = {0, 0, 0}, nqtot = 0, ncoeff = 0, nelmt = 0;
+
+    double lambda = 1.0;
+    bool has_diff = false;
+    double diff[6] = {1, 0, 1, 0, 0, 1};
+
+    std::vector<double> jac, df;
+    bool have_jac = false, have_df = false, geo_dirty = true;
+
+    DeviceBuffer tab, geo;
+
+    MFOperator(const std::string &key, int nummodes, int numpoints0, int nelmt, int precision);
+
+    bool needs_jac() const { return op == OpType::IProduct || op == OpType::Helmholtz; }
+    bool needs_df() const { return op == OpType::PhysDeriv || op == OpType::Helmholtz; }
+    size_t geom_points() const { return deformed ? (size_t)nelmt * nqtot : (size_t)nelmt; }
+
+    void set_jac(const double *j);
+    void set_df(const double *d);
+    void set_lambda(double l) { lambda = l; }
+    void set_diffusion(const double *d6);
+
+    void apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s);
+
+    // Packed geometry exactly as the kernels read it (host FP64); exposed
+    // so solvers can reuse it.
+    std::vector<double> pack_geometry() const;
+    static std::vector<double> build_tables(Shape shape, int nm, int nq0);
+
+private:
+    void ensure_geometry();
+};
+
+// Parse "<Op>_<Shape>_<Regular|Deformed>" (MatrixFree::GetOpstring,
+// Operator.cpp:46-90).
+void parse_operator_key(const std::string &key, OpType &op, Shape &shape, bool &deformed);
+
+}  // namespace nektar_amd

@@ -1,0 +1,4 @@
+// hex operators, 4 modes per direction.
+#define OPS_NM 4
+#define OPS_SHAPE_HEX
+#include "ops_inst.inc"

@@ -1,0 +1,4 @@
+// tet operators, 3 modes per direction.
+#define OPS_NM 3
+#define OPS_SHAPE_TET
+#include "ops_inst.inc"

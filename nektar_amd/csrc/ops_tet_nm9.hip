@@ -1,0 +1,4 @@
+// tet operators, 9 modes per direction.
+#define OPS_NM 9
+#define OPS_SHAPE_TET
+#include "ops_inst.inc"

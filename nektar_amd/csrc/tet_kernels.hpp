@@ -1,0 +1,569 @@
+// Sum-factorised tetrahedron kernels for gfx950, collapsed coordinates.
+//
+// Points: NQ0 = NQ (GLL, Modified_A), NQ1 = NQ2 = NQ-1 (Gauss-Radau-Jacobi
+// alpha=1 / alpha=2 with Modified_B / Modified_C), the MODIFIED defaults.
+// Coefficients are ordered (p, q, r) with r fastest and triangular bounds
+// q < NM-p, r < NM-p-q (BwdTransKernels.hpp:244-266).  Each element owns
+// NQ0*NQ1 work-items; item (i, j) holds the k-column of the element's
+// quadrature values in registers.  Triangular contractions run one
+// work-item per (p,q) pair / (p,k) line, so every LDS read feeds a full
+// line of fused multiply-adds.  The singular top/bottom vertex and edge
+// corrections (BwdTransKernels.hpp:296-324, IProductKernels.hpp:456-513)
+// are separable and are evaluated from the p=0 / p=1 partial sums.
+//
+// Table buffer (T): B0[NM*NQ0] B1[NMB*NQ1] B2[NC*NQ2] D0[NQ0^2] D1[NQ1^2]
+//   D2[NQ2^2] W0[NQ0] W1[NQ1] W2[NQ2] H0[NQ0] H1[NQ1] H2[NQ1] H3[NQ2]
+//   (W1, W2 carry the 0.5 / 0.25 Radau pre-scaling, Operator.hpp:600-616;
+//    H* are the half-space factors of HelmholtzKernels.hpp:1368-1389.)
+// Geometry: IProduct jac[e] | jac[e*NQT+q]; PhysDeriv df[e*9+d] |
+//   df[(e*9+d)*NQT+q]; Helmholtz geo[e*10+c] | geo[(e*10+c)*NQT+q] with
+//   c = 0..8: df, c = 9: J.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nektar_amd {
+namespace tet {
+
+template <int NM, int NQ>
+struct Dims {
+    static constexpr int NQ0 = NQ, NQ1 = NQ - 1, NQ2 = NQ - 1;
+    static constexpr int NQT = NQ0 * NQ1 * NQ2;
+    static constexpr int NMB = NM * (NM + 1) / 2;
+    static constexpr int NC = NM * (NM + 1) * (NM + 2) / 6;
+    static constexpr int TPE = NQ0 * NQ1;
+    static constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;
+    static constexpr int oB0 = 0, oB1 = oB0 + NM * NQ0, oB2 = oB1 + NMB * NQ1;
+    static constexpr int oD0 = oB2 + NC * NQ2, oD1 = oD0 + NQ0 * NQ0, oD2 = oD1 + NQ1 * NQ1;
+    static constexpr int oW0 = oD2 + NQ2 * NQ2, oW1 = oW0 + NQ0, oW2 = oW1 + NQ1;
+    static constexpr int oH0 = oW2 + NQ2, oH1 = oH0 + NQ0, oH2 = oH1 + NQ1, oH3 = oH2 + NQ1;
+    static constexpr int TABSIZE = oH3 + NQ2;
+    // slab sizes (per element): quadrature slab and pair/line scratch
+    static constexpr int SLAB = NQT + 2 * NQ2;  // >= NMB*NQ2 and NQ1*NQ2*NM
+};
+
+template <int NM, int NQ>
+constexpr int block_size()
+{
+    return Dims<NM, NQ>::EPB * Dims<NM, NQ>::TPE;
+}
+
+// (p, q) pair index t -> p, q (pairs ordered p-major, q < NM - p)
+template <int NM>
+__device__ __forceinline__ void pair_of(int t, int &p, int &q)
+{
+    p = 0;
+    int rem = t;
+#pragma unroll
+    for (int pp = 0; pp < NM; ++pp) {
+        if (rem >= NM - p) {
+            rem -= NM - p;
+            ++p;
+        }
+    }
+    q = rem;
+}
+// first coefficient index of the (p, q) r-run
+template <int NM>
+__device__ __forceinline__ int coeff_base(int p, int q)
+{
+    int b = 0;
+#pragma unroll
+    for (int pp = 0; pp < NM; ++pp)
+        if (pp < p) b += (NM - pp) * (NM - pp + 1) / 2;
+    return b + q * (NM - p) - q * (q - 1) / 2;
+}
+// Modified_B mode index of (p, q)
+template <int NM>
+__device__ __forceinline__ int modeb(int p, int q)
+{
+    return p * NM - p * (p - 1) / 2 + q;
+}
+
+template <typename T, int NM, int NQ>
+struct Tab {
+    using Dm = Dims<NM, NQ>;
+    const T *__restrict__ b;
+    __device__ __forceinline__ T B0(int p, int i) const { return b[Dm::oB0 + p * Dm::NQ0 + i]; }
+    __device__ __forceinline__ T B1(int m, int j) const { return b[Dm::oB1 + m * Dm::NQ1 + j]; }
+    __device__ __forceinline__ T B2(int m, int k) const { return b[Dm::oB2 + m * Dm::NQ2 + k]; }
+    __device__ __forceinline__ T D0(int l, int i) const { return b[Dm::oD0 + l * Dm::NQ0 + i]; }
+    __device__ __forceinline__ T D1(int l, int j) const { return b[Dm::oD1 + l * Dm::NQ1 + j]; }
+    __device__ __forceinline__ T D2(int l, int k) const { return b[Dm::oD2 + l * Dm::NQ2 + k]; }
+    __device__ __forceinline__ T W0(int i) const { return b[Dm::oW0 + i]; }
+    __device__ __forceinline__ T W1(int j) const { return b[Dm::oW1 + j]; }
+    __device__ __forceinline__ T W2(int k) const { return b[Dm::oW2 + k]; }
+    __device__ __forceinline__ T H0(int i) const { return b[Dm::oH0 + i]; }
+    __device__ __forceinline__ T H1(int j) const { return b[Dm::oH1 + j]; }
+    __device__ __forceinline__ T H2(int j) const { return b[Dm::oH2 + j]; }
+    __device__ __forceinline__ T H3(int k) const { return b[Dm::oH3 + k]; }
+};
+
+// ---- BwdTrans: coefficients (global) -> k-column u of item (i,j)
+// Stage 1 (pairs): fpq[pair][k]      -> S1[pair*NQ2 + k]
+// Stage 2 (p,k)  : fp[k][j][p]       -> S2[(k*NQ1 + j)*NM + p]
+// Stage 3 (j,k)  : u(i,j,k) + corr.  -> S3[(k*NQ1 + j)*NQ0 + i]
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const T *__restrict__ xin, bool valid,
+                                              int t, T *S1, T *S2, T *S3, T (&u)[NQ - 1])
+{
+    using Dm = Dims<NM, NQ>;
+    constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
+    if (t < Dm::NMB) {
+        int p, q;
+        pair_of<NM>(t, p, q);
+        const int base = coeff_base<NM>(p, q);
+        const int len = NM - p - q;
+        T x[NM];
+#pragma unroll
+        for (int r = 0; r < NM; ++r) x[r] = (valid && r < len) ? xin[base + r] : T(0);
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) {
+            T s = T(0);
+#pragma unroll
+            for (int r = 0; r < NM; ++r)
+                if (r < len) s = fma(x[r], tab.B2(base + r, k), s);
+            S1[t * NQ2 + k] = s;
+        }
+    }
+    __syncthreads();
+    if (t < NM * NQ2) {
+        const int p = t % NM, k = t / NM;
+        const int pb = modeb<NM>(p, 0);
+        T a[NM];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) a[q] = (q < NM - p) ? S1[(pb + q) * NQ2 + k] : T(0);
+        const int mb = modeb<NM>(p, 0);
+#pragma unroll
+        for (int j = 0; j < NQ1; ++j) {
+            T s = T(0);
+#pragma unroll
+            for (int q = 0; q < NM; ++q)
+                if (q < NM - p) s = fma(a[q], tab.B1(mb + q, j), s);
+            S2[(k * NQ1 + j) * NM + p] = s;
+        }
+    }
+    __syncthreads();
+    // singular-mode coefficients (broadcast reads)
+    T c_top = T(0), c_bot = T(0), c_edge[NM > 2 ? NM - 2 : 1];
+    if (valid) {
+        c_top = xin[1];
+        c_bot = xin[NM];
+#pragma unroll
+        for (int r = 1; r < NM - 1; ++r) c_edge[r - 1] = xin[NM + r];
+    }
+    if (t < NQ1 * NQ2) {
+        const int j = t % NQ1, k = t / NQ1;
+        T f[NM];
+#pragma unroll
+        for (int p = 0; p < NM; ++p) f[p] = S2[(k * NQ1 + j) * NM + p];
+        // correction weight along (i): top uses B0_0 B1_1 + B0_1 (B1_0 + B1_1),
+        // bottom and edge use B0_1 B1_1 (separable in i)
+        const T b10 = tab.B1(0, j), b11 = tab.B1(1, j);
+        T edge = c_bot * tab.B2(0, k);
+#pragma unroll
+        for (int r = 1; r < NM - 1; ++r) edge = fma(c_edge[r - 1], tab.B2(r + 1, k), edge);
+        const T b21 = tab.B2(1, k);
+#pragma unroll
+        for (int i = 0; i < NQ0; ++i) {
+            T s = T(0);
+#pragma unroll
+            for (int p = 0; p < NM; ++p) s = fma(f[p], tab.B0(p, i), s);
+            T top = tab.B0(0, i) * b11;
+            top = fma(tab.B0(1, i), b10, top);
+            top = fma(tab.B0(1, i), b11, top);
+            s = fma(top * b21, c_top, s);
+            s = fma(tab.B0(1, i) * b11, edge, s);
+            S3[(k * NQ1 + j) * NQ0 + i] = s;
+        }
+    }
+    __syncthreads();
+    {
+        const int i = t % NQ0, j = t / NQ0;
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) u[k] = S3[(k * NQ1 + j) * NQ0 + i];
+    }
+}
+
+// ---- IProduct (unweighted) from the k-column y of item (i,j) (already
+// multiplied by W*J) -> coefficients (global).
+// Stage I1 (j,k): s[p] = sum_i B0(p,i) y  -> S2[(k*NQ1 + j)*NM + p]
+// corrections  : TOP(k), BOT(k)          -> S3[k], S3[NQ2 + k]
+// Stage I2 (p,k): sum_j B1 s            -> S1[pair*NQ2 + k]
+// Stage I3 (pairs): sum_k B2            -> out
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const T (&y)[NQ - 1], bool valid,
+                                                  int t, T *S1, T *S2, T *S3, T *__restrict__ xout)
+{
+    using Dm = Dims<NM, NQ>;
+    constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
+    {
+        const int i = t % NQ0, j = t / NQ0;
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) S3[(k * NQ1 + j) * NQ0 + i] = y[k];
+    }
+    __syncthreads();
+    if (t < NQ1 * NQ2) {
+        const int j = t % NQ1, k = t / NQ1;
+        T v[NQ0];
+#pragma unroll
+        for (int i = 0; i < NQ0; ++i) v[i] = S3[(k * NQ1 + j) * NQ0 + i];
+#pragma unroll
+        for (int p = 0; p < NM; ++p) {
+            T s = v[0] * tab.B0(p, 0);
+#pragma unroll
+            for (int i = 1; i < NQ0; ++i) s = fma(v[i], tab.B0(p, i), s);
+            S2[(k * NQ1 + j) * NM + p] = s;
+        }
+    }
+    __syncthreads();
+    if (t < NQ2) {
+        const int k = t;
+        T top = T(0), bot = T(0);
+#pragma unroll
+        for (int j = 0; j < NQ1; ++j) {
+            const T s0 = S2[(k * NQ1 + j) * NM + 0], s1 = S2[(k * NQ1 + j) * NM + 1];
+            const T b10 = tab.B1(0, j), b11 = tab.B1(1, j);
+            top = fma(s0, b11, top);
+            top = fma(s1, b10 + b11, top);
+            bot = fma(s1, b11, bot);
+        }
+        S3[Dm::NQT + k] = top;
+        S3[Dm::NQT + NQ2 + k] = bot;
+    }
+    if (t < NM * NQ2) {
+        const int p = t % NM, k = t / NM;
+        const int pb = modeb<NM>(p, 0);
+        T v[NQ1];
+#pragma unroll
+        for (int j = 0; j < NQ1; ++j) v[j] = S2[(k * NQ1 + j) * NM + p];
+        const int mb = modeb<NM>(p, 0);
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            if (q < NM - p) {
+                T s = v[0] * tab.B1(mb + q, 0);
+#pragma unroll
+                for (int j = 1; j < NQ1; ++j) s = fma(v[j], tab.B1(mb + q, j), s);
+                S1[(pb + q) * NQ2 + k] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < Dm::NMB && valid) {
+        int p, q;
+        pair_of<NM>(t, p, q);
+        const int base = coeff_base<NM>(p, q);
+        const int len = NM - p - q;
+        T v[NQ2];
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) v[k] = S1[t * NQ2 + k];
+        const bool p00 = (p == 0 && q == 0), p01 = (p == 0 && q == 1);
+#pragma unroll
+        for (int r = 0; r < NM; ++r) {
+            if (r < len) {
+                T s = v[0] * tab.B2(base + r, 0);
+#pragma unroll
+                for (int k = 1; k < NQ2; ++k) s = fma(v[k], tab.B2(base + r, k), s);
+                if (p00 && r == 1) {
+                    // top vertex: sum_k TOP(k) B2_1(k)
+#pragma unroll
+                    for (int k = 0; k < NQ2; ++k) s = fma(S3[Dm::NQT + k], tab.B2(1, k), s);
+                }
+                if (p01 && r < NM - 1) {
+                    // bottom vertex (r = 0) and singular edge (r >= 1): B2_{r+1}, B2_0
+                    const int m2 = (r == 0) ? 0 : r + 1;
+#pragma unroll
+                    for (int k = 0; k < NQ2; ++k) s = fma(S3[Dm::NQT + NQ2 + k], tab.B2(m2, k), s);
+                }
+                xout[base + r] = s;
+            }
+        }
+    }
+}
+
+// ===================================================================
+template <typename T, int NM, int NQ>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void bwdtrans_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ in, T *__restrict__ out)
+{
+    using Dm = Dims<NM, NQ>;
+    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
+    const int e = blockIdx.x * Dm::EPB + le;
+    const bool valid = e < nelmt;
+    T u[NQ - 1];
+    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * Dm::NC, valid, t, lds[le][1], lds[le][2], lds[le][0], u);
+    if (valid) {
+        const int i = t % Dm::NQ0, j = t / Dm::NQ0;
+#pragma unroll
+        for (int k = 0; k < Dm::NQ2; ++k) out[(size_t)e * Dm::NQT + (k * Dm::NQ1 + j) * Dm::NQ0 + i] = u[k];
+    }
+}
+
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ jac, const T *__restrict__ in,
+    T *__restrict__ out)
+{
+    using Dm = Dims<NM, NQ>;
+    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
+    const int e = blockIdx.x * Dm::EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % Dm::NQ0, j = t / Dm::NQ0;
+    const T wij = tab.W0(i) * tab.W1(j);
+    T jv = T(0);
+    if (!DEF && valid) jv = jac[e];
+    T y[NQ - 1];
+#pragma unroll
+    for (int k = 0; k < Dm::NQ2; ++k) {
+        const int c = (k * Dm::NQ1 + j) * Dm::NQ0 + i;
+        const T fv = valid ? in[(size_t)e * Dm::NQT + c] : T(0);
+        const T jq = DEF ? (valid ? jac[(size_t)e * Dm::NQT + c] : T(0)) : jv;
+        y[k] = fv * jq * (wij * tab.W2(k));
+    }
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, lds[le][1], lds[le][2], lds[le][0], out + (size_t)e * Dm::NC);
+}
+
+// collocation derivatives along i (lines (j,k)) and j (lines (i,k)) from A
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const T *A, T *Bs, T *C)
+{
+    using Dm = Dims<NM, NQ>;
+    constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
+    if (t < NQ1 * NQ2) {
+        const int j = t % NQ1, k = t / NQ1;
+        T v[NQ0];
+#pragma unroll
+        for (int l = 0; l < NQ0; ++l) v[l] = A[(k * NQ1 + j) * NQ0 + l];
+#pragma unroll
+        for (int i = 0; i < NQ0; ++i) {
+            T s = v[0] * tab.D0(0, i);
+#pragma unroll
+            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(l, i), s);
+            Bs[(k * NQ1 + j) * NQ0 + i] = s;
+        }
+    }
+    if (t < NQ0 * NQ2) {
+        const int i = t % NQ0, k = t / NQ0;
+        T v[NQ1];
+#pragma unroll
+        for (int l = 0; l < NQ1; ++l) v[l] = A[(k * NQ1 + l) * NQ0 + i];
+#pragma unroll
+        for (int j = 0; j < NQ1; ++j) {
+            T s = v[0] * tab.D1(0, j);
+#pragma unroll
+            for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(l, j), s);
+            C[(k * NQ1 + j) * NQ0 + i] = s;
+        }
+    }
+}
+
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void deriv2_column(const Tab<T, NM, NQ> &tab, const T (&u)[NQ - 1], T (&d2)[NQ - 1])
+{
+    constexpr int NQ2 = NQ - 1;
+#pragma unroll
+    for (int k = 0; k < NQ2; ++k) {
+        T s = u[0] * tab.D2(0, k);
+#pragma unroll
+        for (int l = 1; l < NQ2; ++l) s = fma(u[l], tab.D2(l, k), s);
+        d2[k] = s;
+    }
+}
+
+// PhysDeriv: PhysDerivKernels.hpp:436-507 (tensor) + :198-259 (collapsed
+// chain rule) + :326-340 (df)
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ df, const T *__restrict__ in,
+    T *__restrict__ out0, T *__restrict__ out1, T *__restrict__ out2)
+{
+    using Dm = Dims<NM, NQ>;
+    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
+    const int e = blockIdx.x * Dm::EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % Dm::NQ0, j = t / Dm::NQ0;
+    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
+    T u[NQ - 1], d2[NQ - 1];
+#pragma unroll
+    for (int k = 0; k < Dm::NQ2; ++k) {
+        const int c = (k * Dm::NQ1 + j) * Dm::NQ0 + i;
+        u[k] = valid ? in[(size_t)e * Dm::NQT + c] : T(0);
+        A[c] = u[k];
+    }
+    deriv2_column<T, NM, NQ>(tab, u, d2);
+    __syncthreads();
+    deriv01<T, NM, NQ>(tab, t, A, Bs, C);
+    __syncthreads();
+    if (!valid) return;
+    T g[9];
+    if (!DEF) {
+#pragma unroll
+        for (int d = 0; d < 9; ++d) g[d] = df[(size_t)e * 9 + d];
+    }
+    const T h0 = tab.H0(i), h1 = tab.H1(j), h2 = tab.H2(j);
+#pragma unroll
+    for (int k = 0; k < Dm::NQ2; ++k) {
+        const int c = (k * Dm::NQ1 + j) * Dm::NQ0 + i;
+        if (DEF) {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) g[d] = df[((size_t)e * 9 + d) * Dm::NQT + c];
+        }
+        const T h3 = tab.H3(k);
+        const T a0 = (h2 * h3) * Bs[c];
+        const T w0 = h0 * a0;
+        const T w1 = h3 * C[c];
+        const T a1 = w0 + w1;
+        const T a2 = fma(w1, h1, w0) + d2[k];
+        T o0 = a0 * g[0];
+        o0 = fma(a1, g[1], o0);
+        o0 = fma(a2, g[2], o0);
+        T o1 = a0 * g[3];
+        o1 = fma(a1, g[4], o1);
+        o1 = fma(a2, g[5], o1);
+        T o2 = a0 * g[6];
+        o2 = fma(a1, g[7], o2);
+        o2 = fma(a2, g[8], o2);
+        out0[(size_t)e * Dm::NQT + c] = o0;
+        out1[(size_t)e * Dm::NQT + c] = o1;
+        out2[(size_t)e * Dm::NQT + c] = o2;
+    }
+}
+
+// Fused Helmholtz (Helmholtz.h:549-599 with DiffusionCoeffTetKernel,
+// HelmholtzKernels.hpp:700-913).
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
+    T *__restrict__ out)
+{
+    using Dm = Dims<NM, NQ>;
+    constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
+    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
+    const int e = blockIdx.x * Dm::EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % NQ0, j = t / NQ0;
+    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
+
+    T u[NQ - 1], d2[NQ - 1];
+    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * Dm::NC, valid, t, Bs, C, A, u);
+    deriv2_column<T, NM, NQ>(tab, u, d2);
+    __syncthreads();
+    deriv01<T, NM, NQ>(tab, t, A, Bs, C);
+    __syncthreads();
+
+    T y[NQ - 1], v2[NQ - 1];
+    {
+        T g[10];
+        if (!DEF) {
+#pragma unroll
+            for (int c = 0; c < 10; ++c) g[c] = valid ? geo[(size_t)e * 10 + c] : T(0);
+        }
+        const T h0 = tab.H0(i), h1 = tab.H1(j), h2 = tab.H2(j);
+        const T wij = tab.W0(i) * tab.W1(j);
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) {
+            const int c = (k * NQ1 + j) * NQ0 + i;
+            if (DEF) {
+#pragma unroll
+                for (int cc = 0; cc < 10; ++cc) g[cc] = valid ? geo[((size_t)e * 10 + cc) * NQT + c] : T(0);
+            }
+            const T h3 = tab.H3(k);
+            const T h2h3 = h2 * h3, h1h3 = h1 * h3, h0h2h3 = h0 * h2h3;
+            T t1 = h0h2h3 * (g[1] + g[2]);
+            t1 = fma(g[0], h2h3, t1);
+            T t2 = h0h2h3 * (g[4] + g[5]);
+            t2 = fma(g[3], h2h3, t2);
+            T t3 = h0h2h3 * (g[7] + g[8]);
+            t3 = fma(g[6], h2h3, t3);
+            T t4 = g[1] * h3;
+            t4 = fma(g[2], h1h3, t4);
+            T t5 = g[4] * h3;
+            t5 = fma(g[5], h1h3, t5);
+            T t6 = g[7] * h3;
+            t6 = fma(g[8], h1h3, t6);
+            T g0 = t1 * t1;
+            g0 = fma(t2, t2, g0);
+            g0 = fma(t3, t3, g0);
+            T g4 = g[2] * t1;
+            g4 = fma(g[5], t2, g4);
+            g4 = fma(g[8], t3, g4);
+            T g3 = t1 * t4;
+            g3 = fma(t2, t5, g3);
+            g3 = fma(t3, t6, g3);
+            T g1 = t4 * t4;
+            g1 = fma(t5, t5, g1);
+            g1 = fma(t6, t6, g1);
+            T g5 = g[2] * t4;
+            g5 = fma(g[5], t5, g5);
+            g5 = fma(g[8], t6, g5);
+            T g2 = g[2] * g[2];
+            g2 = fma(g[5], g[5], g2);
+            g2 = fma(g[8], g[8], g2);
+            const T a0 = Bs[c], a1 = C[c], a2 = d2[k];
+            const T wj = wij * tab.W2(k) * g[9];
+            T o0 = g0 * a0;
+            o0 = fma(g3, a1, o0);
+            o0 = fma(g4, a2, o0);
+            T o1 = g3 * a0;
+            o1 = fma(g1, a1, o1);
+            o1 = fma(g5, a2, o1);
+            T o2 = g4 * a0;
+            o2 = fma(g5, a1, o2);
+            o2 = fma(g2, a2, o2);
+            Bs[c] = wj * o0;
+            C[c] = wj * o1;
+            v2[k] = wj * o2;
+            y[k] = (lambda * wj) * u[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NQ2; ++k) {
+        T s = y[k];
+#pragma unroll
+        for (int l = 0; l < NQ2; ++l) s = fma(tab.D2(k, l), v2[l], s);
+        y[k] = s;
+    }
+    __syncthreads();
+    if (t < NQ1 * NQ2) {  // D0^T along i, lines (j,k): A <- D0^T Bs
+        const int jj = t % NQ1, k = t / NQ1;
+        T v[NQ0];
+#pragma unroll
+        for (int l = 0; l < NQ0; ++l) v[l] = Bs[(k * NQ1 + jj) * NQ0 + l];
+#pragma unroll
+        for (int ii = 0; ii < NQ0; ++ii) {
+            T s = v[0] * tab.D0(ii, 0);
+#pragma unroll
+            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(ii, l), s);
+            A[(k * NQ1 + jj) * NQ0 + ii] = s;
+        }
+    }
+    __syncthreads();
+    if (t < NQ0 * NQ2) {  // D1^T along j, lines (i,k): A += D1^T C
+        const int ii = t % NQ0, k = t / NQ0;
+        T v[NQ1];
+#pragma unroll
+        for (int l = 0; l < NQ1; ++l) v[l] = C[(k * NQ1 + l) * NQ0 + ii];
+#pragma unroll
+        for (int jj = 0; jj < NQ1; ++jj) {
+            T s = A[(k * NQ1 + jj) * NQ0 + ii];
+#pragma unroll
+            for (int l = 0; l < NQ1; ++l) s = fma(v[l], tab.D1(jj, l), s);
+            A[(k * NQ1 + jj) * NQ0 + ii] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ2; ++k) y[k] += A[(k * NQ1 + j) * NQ0 + i];
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, A, out + (size_t)e * Dm::NC);
+}
+
+}  // namespace tet
+}  // namespace nektar_amd

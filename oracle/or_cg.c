@@ -1,0 +1,161 @@
+/*
+ * or_cg.c -- the reference's single-reduction preconditioned CG
+ * (LibUtilities/LinearAlgebra/NekLinSysIterCG.cpp:118-372, Demmel, Heath &
+ * van der Vorst 1993 reordering) with the Jacobi preconditioner
+ * (MultiRegions/PreconditionerDiagonal.cpp:93-125 build, :159-189 apply) and
+ * the C0 matrix-vector product of GlobalLinSysIterativeFull::v_DoMatrixMultiply
+ * (GlobalLinSysIterativeFull.cpp:180-241: GlobalToLocal -> Helmholtz ->
+ * Assemble).  Rhs magnitude: NekLinSysIter::Set_Rhs_Magnitude
+ * (NekLinSysIter.cpp:96-117) over all global rows, floored to 1 below 1e-6.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity oracle and CPU baseline).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef void (*matvec_fn)(void *ctx, const double *w_glob, double *s_glob);
+
+static double dot(int n, const double *a, const double *b)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* nd[]: the nNonDir non-Dirichlet global ids in ascending order. */
+static int cg_core(int nglob, int nnd, const int *nd, matvec_fn A, void *ctx,
+                   const double *b, const double *inv_diag, double tol,
+                   int maxit, double *x, or_cg_info *info)
+{
+    double *w = (double *)calloc(nglob, sizeof(double));
+    double *s = (double *)calloc(nglob, sizeof(double));
+    double *p = (double *)calloc(nnd, sizeof(double));
+    double *r = (double *)calloc(nnd, sizeof(double));
+    double *q = (double *)calloc(nnd, sizeof(double));
+    double *wn = (double *)calloc(nnd, sizeof(double));
+    double *sn = (double *)calloc(nnd, sizeof(double));
+    tol = fmax(tol, 1.0e-16); /* NekLinSysIter.cpp:57 */
+
+    for (int i = 0; i < nnd; ++i) { r[i] = b[nd[i]]; x[nd[i]] = 0.0; }
+    double eps = dot(nnd, r, r);
+    double rhs_mag = dot(nglob, b, b);
+    rhs_mag = rhs_mag > 1.0e-6 ? rhs_mag : 1.0;
+    int iters = 0;
+    if (eps < tol * tol * rhs_mag) goto done;
+
+#define PRECON()                                                              \
+    for (int i = 0; i < nnd; ++i) {                                           \
+        wn[i] = inv_diag ? r[i] * inv_diag[nd[i]] : r[i];                     \
+        w[nd[i]] = wn[i];                                                     \
+    }
+#define MATVEC()                                                              \
+    A(ctx, w, s);                                                             \
+    for (int i = 0; i < nnd; ++i) sn[i] = s[nd[i]];
+
+    PRECON();
+    MATVEC();
+    double rho = dot(nnd, r, wn), mu = dot(nnd, sn, wn);
+    double rho_star = 0.0, beta = 0.0, alpha = rho / mu, rho_new;
+    iters = 1;
+    while (1) {
+        if (iters > maxit) break;
+        for (int i = 0; i < nnd; ++i) p[i] = beta * p[i] + wn[i];
+        for (int i = 0; i < nnd; ++i) q[i] = beta * q[i] + sn[i];
+        for (int i = 0; i < nnd; ++i) x[nd[i]] = alpha * p[i] + x[nd[i]];
+        for (int i = 0; i < nnd; ++i) r[i] = -alpha * q[i] + r[i];
+        PRECON();
+        MATVEC();
+        rho_new = dot(nnd, r, wn);
+        mu = dot(nnd, sn, wn);
+        eps = dot(nnd, r, r);
+        iters++;
+        if (eps < tol * tol * rhs_mag) break;
+        beta = (rho_new - rho_star) / rho;
+        alpha = rho_new / (mu - rho_new * beta / alpha);
+        rho = rho_new;
+    }
+#undef PRECON
+#undef MATVEC
+done:
+    if (info) {
+        info->iterations = iters;
+        info->final_error = sqrt(eps / rhs_mag);
+        info->rhs_magnitude = rhs_mag;
+    }
+    free(w); free(s); free(p); free(r); free(q); free(wn); free(sn);
+    return iters;
+}
+
+/* ---- dense operator (tests) ---------------------------------------- */
+typedef struct { int n; const double *A; } dense_ctx;
+
+static void dense_mv(void *c, const double *w, double *s)
+{
+    const dense_ctx *d = (const dense_ctx *)c;
+    for (int i = 0; i < d->n; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < d->n; ++j) t += d->A[(size_t)i * d->n + j] * w[j];
+        s[i] = t;
+    }
+}
+
+int or_cg_dense(int n, int ndir, const double *A, const double *b,
+                const double *inv_diag, double tol, int maxit, double *x,
+                or_cg_info *info)
+{
+    int *nd = (int *)malloc((size_t)n * sizeof(int));
+    for (int i = ndir; i < n; ++i) nd[i - ndir] = i;
+    dense_ctx c = {n, A};
+    int it = cg_core(n, n - ndir, nd, dense_mv, &c, b, inv_diag, tol, maxit, x, info);
+    free(nd);
+    return it;
+}
+
+/* ---- Helmholtz on the structured hex mesh ---------------------------- */
+typedef struct {
+    int n, nm, nq, deformed, nelmt, nloc, nglob;
+    const double *jac, *df;
+    double lambda;
+    const int *map;
+    double *loc_in, *loc_out;
+} helm_ctx;
+
+static void helm_mv(void *c, const double *w, double *s)
+{
+    helm_ctx *h = (helm_ctx *)c;
+    or_global_to_local(h->nloc, h->map, w, h->loc_in);
+    or_helmholtz(OR_HEX, h->nm, h->nq, h->nelmt, h->deformed, h->jac, h->df, h->lambda, NULL,
+                 h->loc_in, h->loc_out);
+    or_assemble(h->nloc, h->map, h->loc_out, h->nglob, s);
+}
+
+int or_cg_helmholtz_hex(int n, int nm, int nq, int deformed,
+                        const double *jac, const double *df, double lambda,
+                        const double *b_glob, const double *inv_diag,
+                        double tol, int maxit, double *x_glob,
+                        or_cg_info *info)
+{
+    helm_ctx h;
+    h.n = n; h.nm = nm; h.nq = nq; h.deformed = deformed;
+    h.nelmt = n * n * n;
+    h.nloc = h.nelmt * nm * nm * nm;
+    h.nglob = or_hex_nglobal(n, nm);
+    h.jac = jac; h.df = df; h.lambda = lambda;
+    int *map = (int *)malloc((size_t)h.nloc * sizeof(int));
+    uint8_t *dir = (uint8_t *)malloc((size_t)h.nglob);
+    or_hex_l2g(n, nm, map, dir);
+    h.map = map;
+    h.loc_in = (double *)malloc((size_t)h.nloc * sizeof(double));
+    h.loc_out = (double *)malloc((size_t)h.nloc * sizeof(double));
+    int *nd = (int *)malloc((size_t)h.nglob * sizeof(int));
+    int nnd = 0;
+    for (int g = 0; g < h.nglob; ++g)
+        if (!dir[g]) nd[nnd++] = g;
+        else x_glob[g] = 0.0;
+    int it = cg_core(h.nglob, nnd, nd, helm_mv, &h, b_glob, inv_diag, tol, maxit, x_glob, info);
+    free(map); free(dir); free(h.loc_in); free(h.loc_out); free(nd);
+    return it;
+}

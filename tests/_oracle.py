@@ -1,0 +1,185 @@
+"""ctypes bindings of oracle/liboracle.so (the CPU FP64 restatement of the
+reference path).  Test infrastructure: used only by tests/, smoke() and
+bench.py's CPU-baseline leg, always as the checker."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_POLYLIB = os.path.join(ROOT, "oracle", "_ref", "libpolylib_ref.so")
+
+HEX, TET = 0, 1
+GLL, RADAU_A1, RADAU_A2 = 0, 1, 2
+MOD_A, MOD_B, MOD_C = 0, 1, 2
+
+_D = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class CGInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("final_error", ctypes.c_double),
+                ("rhs_magnitude", ctypes.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"{ORACLE_PATH} missing: run `make -C oracle` (or __graft_entry__.build())")
+        _lib = ctypes.CDLL(ORACLE_PATH)
+    return _lib
+
+
+def p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def set_threads(n: int):
+    lib().or_set_threads(ctypes.c_int(n))
+
+
+def ncoeffs(shape, nm):
+    return nm ** 3 if shape == HEX else nm * (nm + 1) * (nm + 2) // 6
+
+
+def default_nq(shape, nq0):
+    return (nq0, nq0, nq0) if shape == HEX else (nq0, nq0 - 1, nq0 - 1)
+
+
+def nqtot(shape, nq0):
+    a, b, c = default_nq(shape, nq0)
+    return a * b * c
+
+
+def points(ptype, nq):
+    z, w, D = np.zeros(nq), np.zeros(nq), np.zeros(nq * nq)
+    lib().or_points(ptype, nq, p(z), p(w), p(D))
+    return z, w, D
+
+
+def basis(btype, nm, nq):
+    n = lib().or_basis_nmodes(btype, nm)
+    b, db = np.zeros(n * nq), np.zeros(n * nq)
+    lib().or_basis(btype, nm, nq, p(b), p(db))
+    return b.reshape(n, nq), db.reshape(n, nq)
+
+
+def jacobfd(z, n, a, b):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    poly, polyd = np.zeros_like(z), np.zeros_like(z)
+    lib().or_jacobfd(len(z), p(z), p(poly), p(polyd), n, ctypes.c_double(a), ctypes.c_double(b))
+    return poly, polyd
+
+
+def bwdtrans(shape, nm, nq0, nelmt, x):
+    out = np.zeros(nelmt * nqtot(shape, nq0))
+    lib().or_bwdtrans(shape, nm, nq0, nelmt, p(np.ascontiguousarray(x)), p(out))
+    return out
+
+
+def iproduct(shape, nm, nq0, nelmt, deformed, jac, f):
+    out = np.zeros(nelmt * ncoeffs(shape, nm))
+    lib().or_iproduct(shape, nm, nq0, nelmt, int(deformed), p(jac), p(np.ascontiguousarray(f)), p(out))
+    return out
+
+
+def physderiv(shape, nm, nq0, nelmt, deformed, df, u):
+    n = nelmt * nqtot(shape, nq0)
+    o = [np.zeros(n) for _ in range(3)]
+    lib().or_physderiv(shape, nm, nq0, nelmt, int(deformed), p(df), p(np.ascontiguousarray(u)),
+                       p(o[0]), p(o[1]), p(o[2]))
+    return o
+
+
+def helmholtz(shape, nm, nq0, nelmt, deformed, jac, df, lam, x, diff6=None):
+    out = np.zeros(nelmt * ncoeffs(shape, nm))
+    d = None if diff6 is None else np.ascontiguousarray(diff6, dtype=np.float64)
+    lib().or_helmholtz(shape, nm, nq0, nelmt, int(deformed), p(jac), p(df), ctypes.c_double(lam),
+                       p(d), p(np.ascontiguousarray(x)), p(out))
+    return out
+
+
+def helmholtz_diag(shape, nm, nq0, nelmt, deformed, jac, df, lam):
+    out = np.zeros(nelmt * ncoeffs(shape, nm))
+    lib().or_helmholtz_diag_bruteforce(shape, nm, nq0, nelmt, int(deformed), p(jac), p(df),
+                                       ctypes.c_double(lam), p(out))
+    return out
+
+
+def hex_mesh(n, amp, seed):
+    v = np.zeros(n ** 3 * 24)
+    lib().or_hex_mesh(n, ctypes.c_double(amp), ctypes.c_uint64(seed), p(v))
+    return v
+
+
+def hex_is_regular(verts8x3):
+    v = np.ascontiguousarray(verts8x3, dtype=np.float64)
+    return bool(lib().or_hex_is_regular(p(v)))
+
+
+def hex_geom(verts, nelmt, nq, deformed):
+    gp = nelmt * (nq ** 3 if deformed else 1)
+    jac, df = np.zeros(gp), np.zeros(9 * gp)
+    lib().or_hex_geom(p(np.ascontiguousarray(verts)), nelmt, nq, int(deformed), p(jac), p(df))
+    return jac, df
+
+
+def hex_coords(verts, nelmt, nq):
+    xyz = np.zeros(nelmt * nq ** 3 * 3)
+    lib().or_hex_coords(p(np.ascontiguousarray(verts)), nelmt, nq, p(xyz))
+    return xyz.reshape(nelmt, nq ** 3, 3)
+
+
+def tet_geom(verts, nelmt):
+    jac, df = np.zeros(nelmt), np.zeros(9 * nelmt)
+    lib().or_tet_geom(p(np.ascontiguousarray(verts)), nelmt, p(jac), p(df))
+    return jac, df
+
+
+def hex_l2g(n, nm):
+    nloc = n ** 3 * nm ** 3
+    nglob = lib().or_hex_nglobal(n, nm)
+    m = np.zeros(nloc, dtype=np.int32)
+    d = np.zeros(nglob, dtype=np.uint8)
+    lib().or_hex_l2g(n, nm, m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                     d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return m, d.astype(bool)
+
+
+def cg_helmholtz_hex(n, nm, nq, deformed, jac, df, lam, b, inv_diag, tol, maxit):
+    x = np.zeros_like(b)
+    info = CGInfo()
+    lib().or_cg_helmholtz_hex(n, nm, nq, int(deformed), p(jac), p(df), ctypes.c_double(lam), p(b),
+                              p(inv_diag), ctypes.c_double(tol), maxit, p(x), ctypes.byref(info))
+    return x, info
+
+
+def cg_dense(A, b, ndir=0, inv_diag=None, tol=1e-10, maxit=5000):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    x = np.zeros(len(b))
+    info = CGInfo()
+    lib().or_cg_dense(len(b), ndir, p(A), p(np.ascontiguousarray(b, dtype=np.float64)),
+                      p(inv_diag), ctypes.c_double(tol), maxit, p(x), ctypes.byref(info))
+    return x, info
+
+
+def splitmix_uniform(seed: int, n: int) -> np.ndarray:
+    """SplitMix64 stream mapped to [-1, 1) as 2*(x>>11)*2^-53 - 1."""
+    out = np.empty(n)
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out[i] = 2.0 * (z >> 11) * 2.0 ** -53 - 1.0
+    return out

@@ -1,0 +1,192 @@
+"""Independent dense ('brute force') evaluation of the MODIFIED hex/tet
+expansions with numpy/scipy, used to pin the oracle.
+
+Nothing here calls the oracle: points come from numpy Legendre/Jacobi root
+finders, basis functions from scipy's Jacobi polynomials, derivatives from
+exact polynomial differentiation (numpy.polynomial), and tet integrals from a
+Duffy-transformed Gauss-Legendre rule of much higher order."""
+from __future__ import annotations
+
+import numpy as np
+from numpy.polynomial import Polynomial as Poly
+from numpy.polynomial import legendre as L
+from scipy.special import eval_jacobi, roots_jacobi
+
+
+# ---------------------------------------------------------------- points
+def gll(n):
+    """Gauss-Lobatto-Legendre points/weights (n points)."""
+    c = np.zeros(n)
+    c[-1] = 1.0
+    inner = np.sort(np.real(L.legroots(L.legder(c)))) if n > 2 else np.array([])
+    z = np.concatenate([[-1.0], inner, [1.0]])
+    Pn1 = L.legval(z, c)
+    w = 2.0 / (n * (n - 1) * Pn1 ** 2)
+    return z, w
+
+
+def radau_m(n, alpha):
+    """Gauss-Radau-Jacobi points with z=-1 for weight (1-z)^alpha, and the
+    weights of int_{-1}^{1} (1-z)^alpha f(z) dz (the un-prescaled Polylib
+    weights): exact for degree 2n-2."""
+    inner = np.sort(roots_jacobi(n - 1, alpha, 1.0)[0]) if n > 1 else np.array([])
+    z = np.concatenate([[-1.0], inner])
+    # weights from exactness on monomials (small n, well conditioned enough)
+    V = np.vander(z, n, increasing=True).T
+    mom = np.array([_moment(alpha, k) for k in range(n)])
+    w = np.linalg.solve(V, mom)
+    return z, w
+
+
+def _moment(alpha, k):
+    # int_{-1}^{1} (1-z)^alpha z^k dz, exact by high-order Gauss-Legendre
+    x, wx = L.leggauss(64)
+    return float(np.sum(wx * (1 - x) ** alpha * x ** k))
+
+
+def lagrange_deriv_matrix(z):
+    """D[k*n + i] = l_k'(z_i) by exact polynomial differentiation."""
+    n = len(z)
+    D = np.zeros((n, n))
+    for k in range(n):
+        others = np.delete(z, k)
+        p = Poly.fromroots(others)
+        p = p / p(z[k])
+        dp = p.deriv()
+        D[k] = dp(z)
+    return D.reshape(-1)
+
+
+# ---------------------------------------------------------------- bases
+def _poly_jacobi(n, a, b):
+    """P_n^{a,b} as a numpy Polynomial (exact coefficients via fitting)."""
+    if n < 0:
+        return Poly([0.0])
+    x = np.cos(np.linspace(0, np.pi, n + 1)) if n > 0 else np.array([0.0])
+    return Poly.fit(x, eval_jacobi(n, a, b, x), n, domain=[-1, 1], window=[-1, 1]) if n > 0 else Poly([1.0])
+
+
+HALF_M = Poly([0.5, -0.5])  # (1-z)/2
+HALF_P = Poly([0.5, 0.5])   # (1+z)/2
+
+
+def modified_a(p):
+    if p == 0:
+        return HALF_M
+    if p == 1:
+        return HALF_P
+    return HALF_M * HALF_P * _poly_jacobi(p - 2, 1.0, 1.0)
+
+
+def modified_b(p, q):
+    """Row p, column q of Modified_B (Basis.cpp:499-589 packing)."""
+    if p == 0:
+        return modified_a(q)
+    if p == 1:
+        return HALF_M if q == 0 else HALF_M * HALF_P * _poly_jacobi(q - 1, 1.0, 1.0)
+    mp = HALF_M ** p
+    return mp if q == 0 else mp * HALF_P * _poly_jacobi(q - 1, 2.0 * p - 1.0, 1.0)
+
+
+def hex_modes(nm):
+    return [(p, q, r) for r in range(nm) for q in range(nm) for p in range(nm)]
+
+
+def tet_modes(nm):
+    return [(p, q, r) for p in range(nm) for q in range(nm - p) for r in range(nm - p - q)]
+
+
+# ------------------------------------------------------- hex dense operator
+def hex_dense(nm, nq):
+    """Phi[c, m] and its reference-direction derivatives on GLL^3."""
+    z, w = gll(nq)
+    B = np.array([[modified_a(p)(zi) for p in range(nm)] for zi in z])       # [i, p]
+    Bd = np.array([[modified_a(p).deriv()(zi) for p in range(nm)] for zi in z])
+    Phi = np.kron(B, np.kron(B, B))
+    D0 = np.kron(B, np.kron(B, Bd))
+    D1 = np.kron(B, np.kron(Bd, B))
+    D2 = np.kron(Bd, np.kron(B, B))
+    W = np.kron(w, np.kron(w, w))
+    return Phi, (D0, D1, D2), W
+
+
+def hex_helmholtz_dense(nm, nq, jac, df, lam, diff6=None):
+    """Element matrix lam*M + L for one hex (jac scalar or per point, df 9 or
+    [nq^3, 9]); returns the dense matrix."""
+    Phi, Dd, W = hex_dense(nm, nq)
+    nqt = nq ** 3
+    jac = np.broadcast_to(np.asarray(jac, dtype=float).reshape(-1), (nqt,)) if np.size(jac) in (1, nqt) else None
+    df = np.asarray(df, dtype=float).reshape(-1, 9)
+    if df.shape[0] == 1:
+        df = np.repeat(df, nqt, axis=0)
+    K = np.eye(3)
+    if diff6 is not None:
+        d = diff6
+        K = np.array([[d[0], d[1], d[3]], [d[1], d[2], d[4]], [d[3], d[4], d[5]]])
+    A = lam * (Phi.T * (W * jac)) @ Phi
+    # physical gradient: grad_x u = sum_k df[j*3+k] d_k u  -> G_j = sum_k df[:, j*3+k] * Dk
+    G = [sum(df[:, j * 3 + k][:, None] * Dd[k] for k in range(3)) for j in range(3)]
+    for a in range(3):
+        for b in range(3):
+            if K[a, b] != 0.0:
+                A += K[a, b] * (G[a].T * (W * jac)) @ G[b]
+    return A
+
+
+# ------------------------------------------------------- tet dense operator
+def tet_basis_fn(nm, mode_index):
+    """Full tet basis function (with Nektar's singular-vertex treatment) and
+    its exact partial derivatives in collapsed coordinates (a, b, c).
+    Returns f(a, b, c) -> (value, d/da, d/db, d/dc)."""
+    modes = tet_modes(nm)
+    p, q, r = modes[mode_index]
+    fa, fb, fc = modified_a(p), modified_b(p, q), modified_b(p + q, r)
+    one = Poly([1.0])
+    if mode_index == 1:  # top vertex: (psi_a0 + psi_a1)(psi_b00 + psi_b01) psi_c001 = (1+c)/2
+        terms = [(one, one, HALF_P)]
+    elif p == 0 and q == 1:  # bottom vertex (r = 0) / singular edge (r >= 1)
+        terms = [(fa, fb, fc), (HALF_P, HALF_P, fc)]
+    else:
+        terms = [(fa, fb, fc)]
+
+    def f(a, b, c):
+        v = sum(ta(a) * tb(b) * tc(c) for ta, tb, tc in terms)
+        da = sum(ta.deriv()(a) * tb(b) * tc(c) for ta, tb, tc in terms)
+        db = sum(ta(a) * tb.deriv()(b) * tc(c) for ta, tb, tc in terms)
+        dc = sum(ta(a) * tb(b) * tc.deriv()(c) for ta, tb, tc in terms)
+        return v, da, db, dc
+    return f
+
+
+def duffy_points(n):
+    """Gauss-Legendre rule on the collapsed cube with the Duffy Jacobian
+    ((1-b)/2) ((1-c)/2)^2, integrating over the reference tet."""
+    x, w = L.leggauss(n)
+    A, Bb, C = np.meshgrid(x, x, x, indexing="ij")
+    Wt = (w[:, None, None] * w[None, :, None] * w[None, None, :])
+    Wt = Wt * (0.5 * (1 - Bb)) * (0.5 * (1 - C)) ** 2
+    return A.ravel(), Bb.ravel(), C.ravel(), Wt.ravel()
+
+
+def tet_dense_mass_stiff(nm, jac, df, lam, nquad=None):
+    """Exact (high-order Duffy) lam*M + L for one affine tet; basis gradients
+    w.r.t. xi via the collapsed chain rule from exact polynomial derivatives."""
+    nc = len(tet_modes(nm))
+    nquad = nquad or (nm + 4)
+    a, b, c, w = duffy_points(nquad)
+    vals = np.zeros((len(a), nc))
+    grads = np.zeros((3, len(a), nc))
+    for m in range(nc):
+        v, da, db, dc = tet_basis_fn(nm, m)(a, b, c)
+        vals[:, m] = v
+        # chain rule collapsed -> standard (PhysDerivKernels.hpp:198-259)
+        d0 = da * 4.0 / ((1 - b) * (1 - c))
+        d1 = 0.5 * (1 + a) * d0 + db * 2.0 / (1 - c)
+        d2 = 0.5 * (1 + a) * d0 + 0.5 * (1 + b) * db * 2.0 / (1 - c) + dc
+        grads[0, :, m], grads[1, :, m], grads[2, :, m] = d0, d1, d2
+    df = np.asarray(df).reshape(9)
+    G = [sum(df[j * 3 + k] * grads[k] for k in range(3)) for j in range(3)]
+    A = lam * (vals.T * (w * jac)) @ vals
+    for j in range(3):
+        A += (G[j].T * (w * jac)) @ G[j]
+    return A, vals, w
