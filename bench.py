@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the matrix-free Helmholtz path on one MI355X.
+
+Workload (one "step"): one FP64 Helmholtz operator apply y = (lambda M + L) x
+over the 32x32x32 regular hex mesh of [0,1]^3, P=6 (7 modes), GLL Q=8,
+lambda=1 (11,239,424 element-local DOFs), inputs resident in HBM.
+value = element-local GDOF/s of that apply.
+
+Also reported (same mesh): the FP32 apply used inside the mixed-precision
+solve, and the C0 Helmholtz solve -lap(u)+u=f with u=sin(pi x)sin(pi y)sin(pi z)
+to relative residual 1e-10 in FP64 CG and in mixed-precision
+iterative-refinement CG (iterations, solve time, solve GDOF/s, L2 error).
+
+roofline: achieved TFLOP/s of the dominant kernel (the fused Helmholtz
+apply) = algorithmic flops per launch / average launch time from HIP events
+on the launch stream; peak = MI355X FP64 dense 78.6 TFLOP/s.
+cpu_baseline: the oracle (CPU FP64 restatement of the reference kernels,
+vectorised across elements, OpenMP over element blocks) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Helmholtz apply GDOF/s (FP64 vs mixed); served solves/s + p99 at 1e-10 res"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def hex_flops_per_elem(nm, nq):
+    """Algorithmic flops of one sum-factorised hex Helmholtz apply: forward
+    transform T, 3-direction collocation gradient G, pointwise metric+mass,
+    transposes."""
+    T = 2 * (nm ** 3 * nq + nm ** 2 * nq ** 2 + nm * nq ** 3)
+    G = 6 * nq ** 4
+    return 2 * T + 2 * G + 18 * nq ** 3
+
+
+def hex_bytes_per_elem(nm, nq, deformed, sv):
+    g = 7 * (nq ** 3 if deformed else 1)
+    return 2 * nm ** 3 * sv + g * sv
+
+
+def read_traffic(kernel_tag):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", "helmholtz_hex_p6_q8_32cubed_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel_tag + " traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def time_apply(op, x, y, steps, warmup, torch):
+    for _ in range(warmup):
+        op(x, y)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    for _ in range(steps):
+        op(x, y)
+    e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return s.elapsed_time(e) / steps, wall
+
+
+def cpu_baseline(n_side, nm, nq, target_s=10.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import _oracle as O
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(cores, 64))
+    O.set_threads(cores)
+    ne = 4096  # sample: 4096 elements of the same mesh/order (1/8 of the mesh)
+    verts = O.hex_mesh(n_side, 0.0, 2)[: ne * 24]
+    jac, df = O.hex_geom(verts, ne, nq, False)
+    x = np.random.default_rng(2).uniform(-1, 1, ne * nm ** 3)
+    O.helmholtz(O.HEX, nm, nq, ne, False, jac, df, 1.0, x)  # warm-up
+    reps, t_total = 0, 0.0
+    while t_total < target_s:
+        t0 = time.perf_counter()
+        O.helmholtz(O.HEX, nm, nq, ne, False, jac, df, 1.0, x)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    t = t_total / reps
+    return {"value": ne * nm ** 3 / t / 1e9, "unit": "GDOF/s", "cores": cores, "kind": "port",
+            "sample": f"{ne} elements of the 32^3 P=6 Q=8 mesh, FP64 Helmholtz apply, {reps} reps "
+                      f"({t_total:.1f} s), AVX2 across elements x OpenMP over element blocks"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-solve", action="store_true", help="skip the CG solves")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 512 MiB ten times (PMC byte-counter calibration)")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": src.numel() * 8, "copy_bytes_written_per_launch": src.numel() * 8,
+                          "launches": 10}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build runs on a single MI355X); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nektar_amd as N
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n_side, nm, nq, lam = 32, 7, 8, 1.0
+    ne = n_side ** 3
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+    # geometry of the regular mesh: one J and df per element (axis-aligned cubes)
+    import numpy as np
+    h = 1.0 / n_side
+    jac = np.full(ne, (h / 2) ** 3)
+    df = np.tile(np.array([2 / h, 0, 0, 0, 2 / h, 0, 0, 0, 2 / h]), ne)
+
+    results = {}
+    for prec, tag in ((N.FP64, "fp64"), (N.FP32, "fp32")):
+        op = N.GetOperatorFactory().CreateInstance("Helmholtz_Hex_Regular", nm, nq, ne, prec)
+        op.SetJac(jac)
+        op.SetDF(df)
+        op.SetLambda(lam)
+        dt = torch.float64 if prec == N.FP64 else torch.float32
+        g = torch.Generator(device="cuda").manual_seed(2)
+        x = (torch.rand(op.input_size(), dtype=dt, device="cuda", generator=g) * 2 - 1).contiguous()
+        y = torch.zeros(op.output_size(), dtype=dt, device="cuda")
+        ms, wall = time_apply(op, x, y, args.steps, args.warmup, torch)
+        results[tag] = (ms, wall)
+        del op, x, y
+
+    ms64, wall64 = results["fp64"]
+    ms32, _ = results["fp32"]
+    ndof = ne * nm ** 3
+    F = hex_flops_per_elem(nm, nq) * ne
+    value = ndof / (ms64 * 1e-3) / 1e9
+    achieved = F / (ms64 * 1e-3) / 1e12
+    traffic = read_traffic("helmholtz_fp64")
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                "kernel": "hex::helmholtz_kernel<double,7,8,false>",
+                "flops_per_launch": F,
+                "alg_bytes_per_launch": hex_bytes_per_elem(nm, nq, False, 8) * ne}
+
+    extra = {
+        "apply_fp64_gdofs": round(value, 3),
+        "apply_fp32_gdofs": round(ndof / (ms32 * 1e-3) / 1e9, 3),
+        "apply_fp32_over_fp64": round(ms64 / ms32, 3),
+        "apply_fp64_us": round(ms64 * 1e3, 2),
+        "apply_fp32_us": round(ms32 * 1e3, 2),
+    }
+    if not args.no_solve:
+        s = N.HelmholtzSolver(n_side, nm, nq, lam)
+        s.set_forcing_manufactured(1)
+        nunk = s.nglobal - s.ndirichlet
+        for mixed, tag in ((False, "fp64"), (True, "mixed")):
+            s.solve(mixed=mixed, tol=1e-10)  # warm (graph capture, FP32 buffers)
+            info = s.solve(mixed=mixed, tol=1e-10)
+            l2 = s.l2_error_manufactured(1)
+            extra[f"solve_{tag}"] = {
+                "iterations": info.iterations, "outer": info.outer_iterations,
+                "converged": bool(info.converged), "final_rel_residual": info.final_error,
+                "solve_ms": round(info.solve_ms, 3),
+                "solve_gdofs": round(nunk / (info.solve_ms * 1e-3) / 1e9, 4),
+                "solves_per_s": round(1e3 / info.solve_ms, 3), "l2_error": l2}
+        extra["solve_mixed_speedup"] = round(extra["solve_fp64"]["solve_ms"] / extra["solve_mixed"]["solve_ms"], 3)
+        extra["unknowns"] = nunk
+        del s
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "GDOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms64, 6),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Helmholtz operator apply, 32^3 regular hex on [0,1]^3, P=6 (7 modes), GLL Q=8, "
+                               "lambda=1, element-local coefficients U[-1,1)",
+                   "elements": ne, "nummodes": nm, "numpoints": nq, "local_dofs": ndof},
+        "roofline": roofline,
+        "extra": extra,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n_side, nm, nq)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -119,6 +119,58 @@ int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in,
                                       double *out0, double *out1,
                                       double *out2);
 
+/* ---------------------------------------------------------------------
+ * C0 Helmholtz solve on a structured hex mesh of [0,1]^3
+ * (ContField::HelmSolve -> GlobalLinSysIterativeFull -> NekLinSysIterCG with
+ *  the Diagonal preconditioner; ContField.cpp:797-925,
+ *  GlobalLinSysIterativeFull.cpp:96-383, NekLinSysIterCG.cpp:118-372).
+ * Solves -lap(u) + lambda u = f with u = 0 on the boundary.  Global
+ * vectors hold all N^3 C0 modes (N = n*(nummodes-1)+1) in the order
+ * ((gz*N + gy)*N + gx); Dirichlet rows are zero in the solution.
+ * ------------------------------------------------------------------- */
+typedef struct {
+    int iterations;       /* CG iterations (total inner ones when mixed)  */
+    int outer_iterations; /* refinement steps (1 for FP64)                */
+    int converged;
+    double final_error;   /* sqrt(||r||^2/||b||^2), FP64 residual          */
+    double rhs_magnitude; /* ||b||^2 over all global rows                  */
+    double solve_ms;      /* device time of the solve                     */
+} nmf_solve_info;
+
+/* Structured n^3 hex mesh; interior vertices perturbed by U[-perturb*h,
+ * perturb*h] per coordinate from SplitMix64(seed) (perturb = 0: regular,
+ * else deformed geometry).  stream: hipStream_t to run on (NULL: own). */
+int nmf_solver_create_hex_box(int n, int nummodes, int numpoints0,
+                              double perturb, uint64_t seed, double lambda,
+                              void *stream, nmf_solver *out);
+int nmf_solver_destroy(nmf_solver s);
+int nmf_solver_sizes(nmf_solver s, int64_t *nglobal, int64_t *nlocal,
+                     int64_t *ndirichlet, int64_t *device_bytes);
+/* Device memory a solver of this shape needs (admission control). */
+int64_t nmf_solver_estimate_bytes(int n, int nummodes, int numpoints0,
+                                  int deformed, int mixed);
+/* Forcing f = (3 k^2 pi^2 + lambda) sin(k pi x) sin(k pi y) sin(k pi z)
+ * (manufactured solution sin sin sin). */
+int nmf_solver_set_forcing_manufactured(nmf_solver s, int k);
+/* Forcing at the quadrature points (device FP64, element-major, element
+ * e = ex + n*(ey + n*ez)). */
+int nmf_solver_set_forcing_phys(nmf_solver s, const double *f_dev);
+/* mixed = 0: FP64 CG; mixed = 1: FP32-inner iterative-refinement CG with
+ * FP64 residual, dots and diagonal.  Stops at ||r|| < tol ||b||. */
+int nmf_solver_solve(nmf_solver s, int mixed, double tol, int maxit,
+                     nmf_solve_info *info);
+/* L2 norm of u_h - sin(k pi x) sin(k pi y) sin(k pi z). */
+int nmf_solver_l2_error_manufactured(nmf_solver s, int k, double *l2);
+/* Host copies of the global solution / assembled rhs / inverse diagonal. */
+int nmf_solver_get_solution(nmf_solver s, double *x_host);
+int nmf_solver_get_rhs(nmf_solver s, double *b_host);
+int nmf_solver_get_inv_diag(nmf_solver s, double *d_host);
+/* y = A x for global device vectors (FP64), Dirichlet rows set to 0. */
+int nmf_solver_apply_global(nmf_solver s, const double *x_dev, double *y_dev);
+/* The mesh generator used by nmf_solver_create_hex_box: (n+1)^3*3
+ * vertex coordinates, vertex (ix,iy,iz) at ((iz*(n+1)+iy)*(n+1)+ix)*3. */
+int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz);
+
 #ifdef __cplusplus
 }
 #endif

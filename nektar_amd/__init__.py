@@ -28,7 +28,7 @@ import numpy as np
 
 __all__ = [
     "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring",
-    "basis_tables", "FP64", "FP32",
+    "basis_tables", "FP64", "FP32", "HelmholtzSolver", "SolveInfo", "hex_box_vertices",
 ]
 
 FP64, FP32 = 0, 1
@@ -52,6 +52,19 @@ _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 
+class SolveInfo(ctypes.Structure):
+    """nmf_solve_info: CG iterations, refinement steps, final relative
+    residual sqrt(||r||^2/||b||^2), ||b||^2 and device solve time."""
+    _fields_ = [("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int),
+                ("converged", ctypes.c_int), ("final_error", ctypes.c_double),
+                ("rhs_magnitude", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_L = ctypes.POINTER(ctypes.c_int64)
+
 _SIGNATURES = {
     "nmf_last_error": (ctypes.c_char_p, []),
     "nmf_version": (ctypes.c_int, []),
@@ -73,6 +86,21 @@ _SIGNATURES = {
     "nmf_operator_apply_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "nmf_operator_apply_host": (ctypes.c_int, [_P, _D, _D]),
     "nmf_operator_apply_physderiv_host": (ctypes.c_int, [_P, _D, _D, _D, _D]),
+    "nmf_solver_create_hex_box": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_uint64, ctypes.c_double, _P, ctypes.POINTER(_P)]),
+    "nmf_solver_destroy": (ctypes.c_int, [_P]),
+    "nmf_solver_sizes": (ctypes.c_int, [_P, _L, _L, _L, _L]),
+    "nmf_solver_estimate_bytes": (ctypes.c_int64, [ctypes.c_int] * 5),
+    "nmf_solver_set_forcing_manufactured": (ctypes.c_int, [_P, ctypes.c_int]),
+    "nmf_solver_set_forcing_phys": (ctypes.c_int, [_P, _P]),
+    "nmf_solver_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        ctypes.POINTER(SolveInfo)]),
+    "nmf_solver_l2_error_manufactured": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "nmf_solver_get_solution": (ctypes.c_int, [_P, _D]),
+    "nmf_solver_get_rhs": (ctypes.c_int, [_P, _D]),
+    "nmf_solver_get_inv_diag": (ctypes.c_int, [_P, _D]),
+    "nmf_solver_apply_global": (ctypes.c_int, [_P, _P, _P]),
+    "nmf_hex_box_vertices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
 }
 
 
@@ -241,3 +269,80 @@ _FACTORY = _Factory()
 
 def GetOperatorFactory() -> _Factory:
     return _FACTORY
+
+
+def hex_box_vertices(n: int, perturb: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Vertices of the structured n^3 mesh used by :class:`HelmholtzSolver`."""
+    out = np.zeros((n + 1) ** 3 * 3)
+    check(lib().nmf_hex_box_vertices(n, perturb, seed, _dptr(out)))
+    return out
+
+
+class HelmholtzSolver:
+    """C0 solve of -lap(u) + lambda u = f, u = 0 on the boundary, on a
+    structured n^3 hex mesh of [0,1]^3 (the ContField::HelmSolve path with
+    the iterative-full CG and Diagonal preconditioner)."""
+
+    def __init__(self, n: int, nummodes: int, numpoints: int, lam: float = 1.0,
+                 perturb: float = 0.0, seed: int = 0, stream=None):
+        self._h = None
+        h = ctypes.c_void_p()
+        check(lib().nmf_solver_create_hex_box(n, nummodes, numpoints, perturb, seed, lam,
+                                              stream, ctypes.byref(h)))
+        self._h = h
+        self.n, self.nummodes, self.numpoints, self.lam = n, nummodes, numpoints, lam
+        ng, nl, nd, nb = (ctypes.c_int64() for _ in range(4))
+        check(lib().nmf_solver_sizes(h, ctypes.byref(ng), ctypes.byref(nl), ctypes.byref(nd),
+                                     ctypes.byref(nb)))
+        self.nglobal, self.nlocal, self.ndirichlet = ng.value, nl.value, nd.value
+        self.device_bytes = nb.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nmf_solver_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def estimate_bytes(n, nummodes, numpoints, deformed=False, mixed=False) -> int:
+        return lib().nmf_solver_estimate_bytes(n, nummodes, numpoints, int(deformed), int(mixed))
+
+    def set_forcing_manufactured(self, k: int = 1) -> None:
+        check(lib().nmf_solver_set_forcing_manufactured(self._h, k))
+
+    def set_forcing_phys(self, f) -> None:
+        count = self.n ** 3 * self.numpoints ** 3
+        ptr = _device_ptr(f, torch.float64, count)
+        if torch is not None:
+            torch.cuda.current_stream().synchronize()  # the solver runs on its own stream
+        check(lib().nmf_solver_set_forcing_phys(self._h, ptr))
+
+    def solve(self, mixed: bool = False, tol: float = 1e-10, maxit: int = 5000) -> SolveInfo:
+        info = SolveInfo()
+        check(lib().nmf_solver_solve(self._h, int(mixed), tol, maxit, ctypes.byref(info)))
+        return info
+
+    def l2_error_manufactured(self, k: int = 1) -> float:
+        v = ctypes.c_double()
+        check(lib().nmf_solver_l2_error_manufactured(self._h, k, ctypes.byref(v)))
+        return v.value
+
+    def _global(self, fn):
+        out = np.zeros(self.nglobal)
+        check(fn(self._h, _dptr(out)))
+        return out
+
+    def solution(self) -> np.ndarray:
+        return self._global(lib().nmf_solver_get_solution)
+
+    def rhs(self) -> np.ndarray:
+        return self._global(lib().nmf_solver_get_rhs)
+
+    def inv_diag(self) -> np.ndarray:
+        return self._global(lib().nmf_solver_get_inv_diag)
+
+    def apply_global(self, x, y) -> None:
+        px = _device_ptr(x, torch.float64, self.nglobal)
+        py = _device_ptr(y, torch.float64, self.nglobal)
+        if torch is not None:
+            torch.cuda.current_stream().synchronize()  # the solver runs on its own stream
+        check(lib().nmf_solver_apply_global(self._h, px, py))

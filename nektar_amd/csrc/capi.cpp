@@ -6,9 +6,14 @@
 #include "basis.hpp"
 #include "common.hpp"
 #include "mf_operator.hpp"
+#include "solver.hpp"
 
 struct nmf_operator_s {
     std::unique_ptr<nektar_amd::MFOperator> impl;
+};
+
+struct nmf_solver_s {
+    std::unique_ptr<nektar_amd::HexSolver> impl;
 };
 
 namespace nektar_amd {
@@ -225,6 +230,140 @@ int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in, double 
         to_host(d0, out0, n, o.fp32);
         to_host(d1, out1, n, o.fp32);
         to_host(d2, out2, n, o.fp32);
+    });
+}
+
+#define NMF_SOLVER_CHECK() NMF_REQUIRE(s && s->impl, NMF_ERR_INVALID, "null solver handle")
+
+int nmf_solver_create_hex_box(int n, int nummodes, int numpoints0, double perturb, uint64_t seed, double lambda,
+                              void *stream, nmf_solver *out)
+{
+    return guard([&] {
+        NMF_REQUIRE(out, NMF_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        NMF_REQUIRE(perturb >= 0.0 && perturb < 0.5, NMF_ERR_INVALID, "perturb must be in [0, 0.5)");
+        auto mesh = HexBoxMesh::make(n, perturb, seed);
+        auto h = std::make_unique<nmf_solver_s>();
+        h->impl = std::make_unique<HexSolver>(mesh, nummodes, numpoints0, lambda, static_cast<hipStream_t>(stream));
+        *out = h.release();
+    });
+}
+
+int nmf_solver_destroy(nmf_solver s)
+{
+    return guard([&] { delete s; });
+}
+
+int nmf_solver_sizes(nmf_solver s, int64_t *nglobal, int64_t *nlocal, int64_t *ndirichlet, int64_t *device_bytes)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        if (nglobal) *nglobal = (int64_t)s->impl->nglobal();
+        if (nlocal) *nlocal = (int64_t)s->impl->nlocal();
+        if (ndirichlet) *ndirichlet = (int64_t)s->impl->ndirichlet();
+        if (device_bytes) *device_bytes = (int64_t)s->impl->device_bytes();
+    });
+}
+
+int64_t nmf_solver_estimate_bytes(int n, int nummodes, int numpoints0, int deformed, int mixed)
+{
+    if (n < 1 || nummodes < 2 || numpoints0 < nummodes) return -1;
+    return (int64_t)HexSolver::estimate_bytes(n, nummodes, numpoints0, deformed != 0, mixed != 0);
+}
+
+int nmf_solver_set_forcing_manufactured(nmf_solver s, int k)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        NMF_REQUIRE(k >= 1, NMF_ERR_INVALID, "frequency k must be >= 1");
+        s->impl->set_forcing_manufactured(k);
+    });
+}
+
+int nmf_solver_set_forcing_phys(nmf_solver s, const double *f_dev)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        NMF_REQUIRE(f_dev, NMF_ERR_INVALID, "null forcing pointer");
+        s->impl->set_forcing_phys(f_dev);
+    });
+}
+
+int nmf_solver_solve(nmf_solver s, int mixed, double tol, int maxit, nmf_solve_info *info)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        NMF_REQUIRE(tol > 0.0 && maxit > 0, NMF_ERR_INVALID, "tol and maxit must be positive");
+        SolveInfo r = s->impl->solve(mixed != 0, tol, maxit);
+        if (info) {
+            info->iterations = r.iterations;
+            info->outer_iterations = r.outer_iterations;
+            info->converged = r.converged;
+            info->final_error = r.final_error;
+            info->rhs_magnitude = r.rhs_magnitude;
+            info->solve_ms = r.solve_ms;
+        }
+    });
+}
+
+int nmf_solver_l2_error_manufactured(nmf_solver s, int k, double *l2)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        NMF_REQUIRE(l2, NMF_ERR_INVALID, "null output");
+        *l2 = s->impl->l2_error_manufactured(k);
+    });
+}
+
+namespace {
+void copy_global(nmf_solver s, const double *src, double *dst)
+{
+    NMF_REQUIRE(dst, NMF_ERR_INVALID, "null output");
+    NMF_HIP(hipStreamSynchronize(s->impl->stream()));
+    NMF_HIP(hipMemcpy(dst, src, s->impl->nglobal() * sizeof(double), hipMemcpyDeviceToHost));
+}
+}  // namespace
+
+int nmf_solver_get_solution(nmf_solver s, double *x_host)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        copy_global(s, s->impl->solution(), x_host);
+    });
+}
+
+int nmf_solver_get_rhs(nmf_solver s, double *b_host)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        copy_global(s, s->impl->rhs(), b_host);
+    });
+}
+
+int nmf_solver_get_inv_diag(nmf_solver s, double *d_host)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        copy_global(s, s->impl->inv_diag(), d_host);
+    });
+}
+
+int nmf_solver_apply_global(nmf_solver s, const double *x_dev, double *y_dev)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        NMF_REQUIRE(x_dev && y_dev, NMF_ERR_INVALID, "null vector");
+        s->impl->apply_global(x_dev, y_dev);
+        NMF_HIP(hipStreamSynchronize(s->impl->stream()));
+    });
+}
+
+int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz)
+{
+    return guard([&] {
+        NMF_REQUIRE(xyz, NMF_ERR_INVALID, "null output");
+        auto m = HexBoxMesh::make(n, perturb, seed);
+        std::memcpy(xyz, m.X.data(), m.X.size() * sizeof(double));
     });
 }
 

@@ -1,0 +1,367 @@
+// Host side of the structured-hex C0 Helmholtz solver.
+//
+// Reference flow (ContField::v_HelmSolve, ContField.cpp:797-925 ->
+// GlobalLinSysIterativeFull::v_Solve, :96-178 -> NekLinSysIterCG):
+//   rhs = IProductWRTBase(f) assembled; homogeneous Dirichlet rows held at 0;
+//   Jacobi preconditioner from the assembled diagonal; single-reduction PCG
+//   stopping at ||r||^2 < tol^2 ||b||^2 (b over all global rows).
+// The mixed-precision variant wraps an FP32 CG (FP32 operator, vectors and
+// geometry; FP64 dots and diagonal) in FP64 iterative refinement with the
+// residual recomputed by the FP64 operator (NekLinSysIterCG::v_DoIterate,
+// NekLinSysIterCG.cpp:85-116, is the reference's one-step analogue).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "basis.hpp"
+#include "mf_operator.hpp"
+
+namespace nektar_amd {
+
+using solver::CGState;
+
+namespace {
+
+uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+double uniform_pm1(uint64_t &s) { return 2.0 * (double)(splitmix64(s) >> 11) * 0x1.0p-53 - 1.0; }
+
+}  // namespace
+
+HexBoxMesh HexBoxMesh::make(int n, double perturb, uint64_t seed)
+{
+    NMF_REQUIRE(n >= 1 && n <= 1024, NMF_ERR_INVALID, "mesh size n must be in [1, 1024]");
+    HexBoxMesh m;
+    m.n = n;
+    m.deformed = perturb != 0.0 && n > 1;
+    const int nv = n + 1;
+    const double h = 1.0 / n;
+    m.X.resize((size_t)nv * nv * nv * 3);
+    uint64_t st = seed;
+    for (int iz = 0; iz < nv; ++iz)
+        for (int iy = 0; iy < nv; ++iy)
+            for (int ix = 0; ix < nv; ++ix) {
+                double *x = &m.X[((size_t)(iz * nv + iy) * nv + ix) * 3];
+                x[0] = ix * h;
+                x[1] = iy * h;
+                x[2] = iz * h;
+                const bool interior = ix > 0 && ix < n && iy > 0 && iy < n && iz > 0 && iz < n;
+                if (interior && perturb != 0.0)
+                    for (int d = 0; d < 3; ++d) x[d] += perturb * h * uniform_pm1(st);
+            }
+    return m;
+}
+
+size_t HexSolver::estimate_bytes(int n, int nm, int nq, bool deformed, bool mixed)
+{
+    const size_t N = (size_t)n * (nm - 1) + 1, NG = N * N * N, nel = (size_t)n * n * n;
+    const size_t nloc = nel * nm * nm * nm;
+    const size_t geo = nel * 7 * (deformed ? (size_t)nq * nq * nq : 1);
+    size_t b = 8 * (NG * (5 + 6) + nloc + geo);
+    if (mixed) b += 4 * (NG * 6 + nloc + geo);
+    b += 8 * nel * (size_t)nq * nq * nq;  // forcing / quadrature scratch
+    return b;
+}
+
+HexSolver::HexSolver(const HexBoxMesh &mesh, int nummodes, int numpoints, double lambda, hipStream_t stream)
+    : n_(mesh.n), nm_(nummodes), nq_(numpoints), deformed_(mesh.deformed), lambda_(lambda)
+{
+    NMF_REQUIRE(hex_supported(nummodes, numpoints), NMF_ERR_UNSUPPORTED,
+                "solver order (nummodes, numpoints) not instantiated");
+    NMF_REQUIRE(mesh.X.size() == (size_t)(n_ + 1) * (n_ + 1) * (n_ + 1) * 3, NMF_ERR_INVALID, "bad mesh vertices");
+    N_ = n_ * (nm_ - 1) + 1;
+    NG_ = (size_t)N_ * N_ * N_;
+    nelmt_ = n_ * n_ * n_;
+    nc_ = nm_ * nm_ * nm_;
+    own_stream_ = stream == nullptr;
+    if (own_stream_) NMF_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    else stream_ = stream;
+    nb_ = solver::reduction_blocks(NG_);
+
+    X_.upload(mesh.X, false);
+    Points1D P = make_points(PointsType::GLL, nq_);
+    z_.upload(P.z, false);
+    w_.upload(P.w, false);
+    const std::vector<double> tab = MFOperator::build_tables(Shape::Hex, nm_, nq_);
+    tab64_.upload(tab, false);
+    tab32_.upload(tab, true);
+    Basis1D B = make_basis(BasisType::ModifiedA, nm_, nq_);
+    std::vector<double> dt(3 * (size_t)nm_ * nq_);
+    for (size_t i = 0; i < (size_t)nm_ * nq_; ++i) {
+        dt[i] = B.bdata[i] * B.bdata[i];
+        dt[(size_t)nm_ * nq_ + i] = B.bdata[i] * B.dbdata[i];
+        dt[2 * (size_t)nm_ * nq_ + i] = B.dbdata[i] * B.dbdata[i];
+    }
+    dtab_.upload(dt, false);
+
+    const size_t nloc = (size_t)nelmt_ * nc_;
+    for (auto *buf : {&b_, &x_, &invd_, &tmp64_, &r64_}) buf->allocate(NG_ * sizeof(double));
+    sloc64_.allocate(nloc * sizeof(double));
+    for (auto &v : cg64_) v.allocate(NG_ * sizeof(double));
+    state_.allocate(sizeof(CGState));
+    pa_.allocate(3 * (size_t)nb_ * sizeof(double));
+    pmu_.allocate((size_t)nb_ * sizeof(double));
+    scalar_.allocate(8 * sizeof(double));
+    NMF_HIP(hipHostMalloc((void **)&host_state_, sizeof(CGState), hipHostMallocDefault));
+    NMF_HIP(hipHostMalloc((void **)&host_scalar_, 8 * sizeof(double), hipHostMallocDefault));
+    NMF_HIP(hipMemsetAsync(b_.get(), 0, NG_ * sizeof(double), stream_));
+    NMF_HIP(hipMemsetAsync(x_.get(), 0, NG_ * sizeof(double), stream_));
+
+    build_geometry();
+    build_diagonal();
+    NMF_HIP(hipStreamSynchronize(stream_));
+}
+
+HexSolver::~HexSolver()
+{
+    for (auto &g : graph_)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (host_state_) (void)hipHostFree(host_state_);
+    if (host_scalar_) (void)hipHostFree(host_scalar_);
+    if (own_stream_ && stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+size_t HexSolver::ndirichlet() const
+{
+    const size_t Ni = (size_t)N_ - 2;
+    return NG_ - Ni * Ni * Ni;
+}
+
+size_t HexSolver::device_bytes() const
+{
+    size_t s = 0;
+    for (const DeviceBuffer *b : {&X_, &z_, &w_, &tab64_, &tab32_, &dtab_, &geo64_, &geo32_, &b_, &x_, &invd_,
+                                  &sloc64_, &tmp64_, &r64_, &sloc32_, &state_, &pa_, &pmu_, &scalar_})
+        s += b->bytes();
+    for (const auto &v : cg64_) s += v.bytes();
+    for (const auto &v : cg32_) s += v.bytes();
+    return s;
+}
+
+void HexSolver::build_geometry()
+{
+    const size_t g = (size_t)nelmt_ * 7 * (deformed_ ? (size_t)nq_ * nq_ * nq_ : 1);
+    geo64_.allocate(g * sizeof(double));
+    NMF_HIP(solver::launch_geom(stream_, n_, nq_, X_.as<double>(), z_.as<double>(), w_.as<double>(), deformed_,
+                                geo64_.as<double>()));
+}
+
+void HexSolver::build_diagonal()
+{
+    SolverArgs a{SolverKernel::Diag, false, deformed_, n_, N_, tab64_.get(), dtab_.get(), geo64_.get(), lambda_,
+                 nullptr, sloc64_.get(), nullptr, stream_};
+    NMF_HIP(launch_hex_solver(nm_, nq_, a));
+    NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), invd_.as<double>(), nullptr,
+                                            nullptr, nullptr));
+    NMF_HIP(solver::launch_invert_diag(stream_, N_, invd_.as<double>()));
+}
+
+void HexSolver::set_forcing_phys(const double *f_dev)
+{
+    SolverArgs a{SolverKernel::Rhs, false, deformed_, n_, N_, tab64_.get(), nullptr, geo64_.get(), lambda_, f_dev,
+                 sloc64_.get(), nullptr, stream_};
+    NMF_HIP(launch_hex_solver(nm_, nq_, a));
+    // Full assembly (Dirichlet rows kept: they enter ||b||^2 as in the
+    // reference's Set_Rhs_Magnitude over all global rows).
+    NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), b_.as<double>(), nullptr,
+                                            nullptr, nullptr, true));
+    NMF_HIP(hipStreamSynchronize(stream_));
+}
+
+void HexSolver::set_forcing_manufactured(int k)
+{
+    DeviceBuffer f((size_t)nelmt_ * nq_ * nq_ * nq_ * sizeof(double));
+    NMF_HIP(solver::launch_forcing(stream_, n_, nq_, X_.as<double>(), z_.as<double>(), k, lambda_, f.as<double>()));
+    set_forcing_phys(f.as<double>());
+}
+
+void HexSolver::apply_local(bool fp32, const void *xg, void *sloc, const int *done)
+{
+    SolverArgs a{SolverKernel::Apply, fp32, deformed_, n_, N_, fp32 ? tab32_.get() : tab64_.get(), nullptr,
+                 fp32 ? geo32_.get() : geo64_.get(), lambda_, xg, sloc, done, stream_};
+    NMF_HIP(launch_hex_solver(nm_, nq_, a));
+}
+
+void HexSolver::apply_global(const double *x, double *y)
+{
+    apply_local(false, x, sloc64_.get(), nullptr);
+    NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), y, nullptr, nullptr, nullptr));
+}
+
+void HexSolver::capture_iterations(bool fp32)
+{
+    const int slot = fp32 ? 1 : 0;
+    if (graph_[slot]) return;
+    CGState *st = state_.as<CGState>();
+    const int *done = &st->done;
+    hipGraph_t g;
+    NMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < kItersPerGraph; ++it) {
+        if (fp32) {
+            float *x = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>(),
+                  *q = cg32_[3].as<float>(), *w = cg32_[4].as<float>(), *s = cg32_[5].as<float>();
+            NMF_HIP(solver::launch_cg_update<float>(stream_, N_, st, p, q, x, r, w, s, invd_.as<double>(),
+                                                    pa_.as<double>()));
+            apply_local(true, w, sloc32_.get(), done);
+            NMF_HIP(solver::launch_assemble<float>(stream_, n_, nm_, N_, sloc32_.as<float>(), s, w, pmu_.as<double>(),
+                                                   done));
+        } else {
+            double *x = cg64_[0].as<double>(), *r = cg64_[1].as<double>(), *p = cg64_[2].as<double>(),
+                   *q = cg64_[3].as<double>(), *w = cg64_[4].as<double>(), *s = cg64_[5].as<double>();
+            NMF_HIP(solver::launch_cg_update<double>(stream_, N_, st, p, q, x, r, w, s, invd_.as<double>(),
+                                                     pa_.as<double>()));
+            apply_local(false, w, sloc64_.get(), done);
+            NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), s, w,
+                                                    pmu_.as<double>(), done));
+        }
+        NMF_HIP(solver::launch_cg_scalar(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
+    }
+    NMF_HIP(hipStreamEndCapture(stream_, &g));
+    hipError_t e = hipGraphInstantiate(&graph_[slot], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+}
+
+CGState HexSolver::run_cg(bool fp32, const double *b, double tol, int maxit)
+{
+    if (fp32) {
+        if (!geo32_.get()) {
+            const size_t g = geo64_.bytes() / sizeof(double);
+            geo32_.allocate(g * sizeof(float));
+            NMF_HIP(solver::launch_cast(stream_, g, geo64_.as<double>(), geo32_.as<float>()));
+        }
+        if (!cg32_[0].get()) {
+            for (auto &v : cg32_) v.allocate(NG_ * sizeof(float));
+            sloc32_.allocate((size_t)nelmt_ * nc_ * sizeof(float));
+        }
+    }
+    CGState init{};
+    tol = std::max(tol, 1.0e-16);
+    init.tol2 = tol * tol;
+    init.maxit = maxit;
+    *host_state_ = init;
+    NMF_HIP(hipMemcpyAsync(state_.get(), host_state_, sizeof(CGState), hipMemcpyHostToDevice, stream_));
+    CGState *st = state_.as<CGState>();
+    if (fp32) {
+        float *x = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>(),
+              *q = cg32_[3].as<float>(), *w = cg32_[4].as<float>(), *s = cg32_[5].as<float>();
+        NMF_HIP(solver::launch_cg_init<float>(stream_, N_, b, invd_.as<double>(), x, r, p, q, w, pa_.as<double>()));
+        apply_local(true, w, sloc32_.get(), nullptr);
+        NMF_HIP(solver::launch_assemble<float>(stream_, n_, nm_, N_, sloc32_.as<float>(), s, w, pmu_.as<double>(),
+                                               nullptr));
+    } else {
+        double *x = cg64_[0].as<double>(), *r = cg64_[1].as<double>(), *p = cg64_[2].as<double>(),
+               *q = cg64_[3].as<double>(), *w = cg64_[4].as<double>(), *s = cg64_[5].as<double>();
+        NMF_HIP(solver::launch_cg_init<double>(stream_, N_, b, invd_.as<double>(), x, r, p, q, w, pa_.as<double>()));
+        apply_local(false, w, sloc64_.get(), nullptr);
+        NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), s, w, pmu_.as<double>(),
+                                                nullptr));
+    }
+    NMF_HIP(solver::launch_cg_scalar_init(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
+    capture_iterations(fp32);
+    // Each replay runs kItersPerGraph iterations; kernels of a finished solve
+    // return at once.  The host polls the state once per replay.
+    const int max_replays = maxit / kItersPerGraph + 2;
+    for (int rep = 0; rep <= max_replays; ++rep) {
+        NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        NMF_HIP(hipStreamSynchronize(stream_));
+        if (host_state_->done) break;
+        NMF_HIP(hipGraphLaunch(graph_[fp32 ? 1 : 0], stream_));
+    }
+    return *host_state_;
+}
+
+SolveInfo HexSolver::solve(bool mixed, double tol, int maxit, double inner_tol)
+{
+    SolveInfo info;
+    hipEvent_t e0, e1;
+    NMF_HIP(hipEventCreate(&e0));
+    NMF_HIP(hipEventCreate(&e1));
+    NMF_HIP(hipEventRecord(e0, stream_));
+    if (!mixed) {
+        CGState st = run_cg(false, b_.as<double>(), tol, maxit);
+        NMF_HIP(hipMemcpyAsync(x_.get(), cg64_[0].get(), NG_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        info.iterations = st.iter;
+        info.outer_iterations = 1;
+        info.converged = st.converged;
+        info.final_error = st.final_error;
+        info.rhs_magnitude = st.rhs_mag;
+    } else {
+        // ||b||^2 over all global rows (Set_Rhs_Magnitude, NekLinSysIter.cpp:96-117)
+        NMF_HIP(hipMemsetAsync(tmp64_.get(), 0, NG_ * sizeof(double), stream_));
+        NMF_HIP(hipMemsetAsync(x_.get(), 0, NG_ * sizeof(double), stream_));
+        double *dsc = scalar_.as<double>();
+        NMF_HIP(solver::launch_sumsq(stream_, NG_, b_.as<double>(), pmu_.as<double>()));
+        NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc));
+        // r64 = b on non-Dirichlet rows (x = 0), ||r64||^2
+        NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
+                                        pmu_.as<double>()));
+        NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
+        NMF_HIP(hipMemcpyAsync(host_scalar_, dsc, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        NMF_HIP(hipStreamSynchronize(stream_));
+        double rr = host_scalar_[1];
+        const double bmag = host_scalar_[0];
+        const double rhs_mag = bmag > 1.0e-6 ? bmag : 1.0;
+        const double tol2 = std::max(tol, 1e-16) * std::max(tol, 1e-16);
+        int total = 0, outer = 0;
+        while (rr >= tol2 * rhs_mag && total < maxit && outer < 50) {
+            // inner FP32 solve of A d = r64, no tighter than the outer target needs
+            const double need = 0.5 * std::sqrt(tol2 * rhs_mag / rr);
+            const double itol = std::max(inner_tol, need);
+            CGState st = run_cg(true, r64_.as<double>(), itol, maxit - total);
+            total += st.iter;
+            ++outer;
+            NMF_HIP(solver::launch_add<float>(stream_, NG_, x_.as<double>(), cg32_[0].as<float>()));
+            apply_global(x_.as<double>(), tmp64_.as<double>());
+            NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
+                                            pmu_.as<double>()));
+            NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
+            NMF_HIP(hipMemcpyAsync(host_scalar_ + 1, dsc + 1, sizeof(double), hipMemcpyDeviceToHost, stream_));
+            NMF_HIP(hipStreamSynchronize(stream_));
+            rr = host_scalar_[1];
+            if (st.iter == 0) break;
+        }
+        info.iterations = total;
+        info.outer_iterations = outer;
+        info.converged = rr < tol2 * rhs_mag;
+        info.final_error = std::sqrt(rr / rhs_mag);
+        info.rhs_magnitude = rhs_mag;
+    }
+    NMF_HIP(hipEventRecord(e1, stream_));
+    NMF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    NMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    info.solve_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return info;
+}
+
+double HexSolver::l2_error_manufactured(int k)
+{
+    const size_t nq3 = (size_t)nq_ * nq_ * nq_;
+    DeviceBuffer u((size_t)nelmt_ * nq3 * sizeof(double));
+    SolverArgs a{SolverKernel::BwdGather, false, deformed_, n_, N_, tab64_.get(), nullptr, nullptr, lambda_,
+                 x_.get(), u.get(), nullptr, stream_};
+    NMF_HIP(launch_hex_solver(nm_, nq_, a));
+    int nb = 0;
+    NMF_HIP(solver::launch_l2err(stream_, n_, nq_, X_.as<double>(), z_.as<double>(), w_.as<double>(),
+                                 geo64_.as<double>(), deformed_, k, u.as<double>(), pmu_.as<double>(), &nb));
+    NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb, scalar_.as<double>()));
+    NMF_HIP(hipMemcpyAsync(host_scalar_, scalar_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    NMF_HIP(hipStreamSynchronize(stream_));
+    return std::sqrt(host_scalar_[0]);
+}
+
+}  // namespace nektar_amd

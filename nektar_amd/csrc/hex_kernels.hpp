@@ -48,18 +48,42 @@ struct Tab {
     __device__ __forceinline__ T W(int i) const { return base[NM * NQ + NQ * NQ + i]; }
 };
 
-// ---- forward transform: coefficients (global) -> u column in registers,
-//      also written to slab A[(k*NQ+j)*NQ+i] when `keep` is set.
-template <typename T, int NM, int NQ>
-__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const T *__restrict__ xin,
-                                              bool valid, int t, T *A, T *Bs, T (&u)[NQ])
+// Coefficient fetch for an element stored contiguously (the Collections
+// layout): x(p, q, r) = xin[(r*NM + q)*NM + p].
+template <typename T, int NM>
+struct LocalFetch {
+    const T *__restrict__ xin;
+    __device__ __forceinline__ T operator()(int p, int q, int r) const { return xin[(r * NM + q) * NM + p]; }
+};
+
+// Coefficient fetch straight from a global C0 vector of the structured
+// n^3 mesh (GlobalToLocal fused into the operator, AssemblyMapCG.cpp:
+// 2804-2827): 1-D node of local mode p in element ex is ex*(NM-1) + {0 for
+// p=0, NM-1 for p=1, p-1 for bubbles}; nodes per direction N = n*(NM-1)+1.
+template <typename T, int NM>
+struct StructuredFetch {
+    const T *__restrict__ g;
+    int N;
+    int bx, by, bz;  // ex*(NM-1), ...
+    __device__ __forceinline__ static int off(int p) { return p == 0 ? 0 : (p == 1 ? NM - 1 : p - 1); }
+    __device__ __forceinline__ T operator()(int p, int q, int r) const
+    {
+        return g[((size_t)(bz + off(r)) * N + (by + off(q))) * N + (bx + off(p))];
+    }
+};
+
+// ---- forward transform: coefficients -> u column (k) of item (i,j) in
+//      registers.
+template <typename T, int NM, int NQ, typename Fetch>
+__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const Fetch &xin, bool valid, int t,
+                                              T *A, T *Bs, T (&u)[NQ])
 {
     // F1: contract p -> i along lines (q, r)
     if (t < NM * NM) {
         const int q = t % NM, r = t / NM;
         T x[NM];
 #pragma unroll
-        for (int p = 0; p < NM; ++p) x[p] = valid ? xin[(r * NM + q) * NM + p] : T(0);
+        for (int p = 0; p < NM; ++p) x[p] = valid ? xin(p, q, r) : T(0);
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             T s = x[0] * tab.B(0, i);
@@ -205,7 +229,8 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void bwdtrans_k
     const int e = blockIdx.x * EPB + le;
     const bool valid = e < nelmt;
     T u[NQ];
-    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * NM * NM * NM, valid, t, lds[le][0], lds[le][1], u);
+    bwd_to_column<T, NM, NQ>(tab, LocalFetch<T, NM>{in + (size_t)e * NM * NM * NM}, valid, t, lds[le][0],
+                             lds[le][1], u);
     if (valid) {
         const int i = t % NQ, j = t / NQ;
         T *o = out + (size_t)e * NQ3;
@@ -298,22 +323,18 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void physderiv_
 }
 
 // Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_kernel(
-    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
-    T *__restrict__ out)
+// Body shared by the Collections kernel (element-local input) and the
+// solver kernel (input gathered from the global vector).
+template <typename T, int NM, int NQ, bool DEF, typename Fetch>
+__device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, const Fetch &fetch, bool valid, int e,
+                                                  int t, const T *__restrict__ geo, T lambda, T *A, T *Bs, T *C,
+                                                  T *__restrict__ xout)
 {
-    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ T lds[EPB][3][NQ3];
-    const Tab<T, NM, NQ> tab{tabp};
-    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
-    const int e = blockIdx.x * EPB + le;
-    const bool valid = e < nelmt;
+    constexpr int NQ3 = NQ * NQ * NQ;
     const int i = t % NQ, j = t / NQ;
-    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
 
     T u[NQ];
-    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * NM * NM * NM, valid, t, A, Bs, u);
+    bwd_to_column<T, NM, NQ>(tab, fetch, valid, t, A, Bs, u);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) A[(k * NQ + j) * NQ + i] = u[k];
     T d2[NQ];
@@ -401,7 +422,44 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NQ; ++k) y[k] += A[(k * NQ + j) * NQ + i];
-    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, out + (size_t)e * NM * NM * NM);
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, xout);
+}
+
+// Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
+    T *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ T lds[EPB][3][NQ3];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    helmholtz_element<T, NM, NQ, DEF>(tab, LocalFetch<T, NM>{in + (size_t)e * NM * NM * NM}, valid, e, t, geo,
+                                      lambda, lds[le][0], lds[le][1], lds[le][2], out + (size_t)e * NM * NM * NM);
+}
+
+// Solver form: x gathered from the global vector of the structured n^3
+// mesh, element results written element-major (assembled afterwards).
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_gather_kernel(
+    int n, int N, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ xg,
+    T *__restrict__ out, const int *__restrict__ done)
+{
+    if (done && *done) return;
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ T lds[EPB][3][NQ3];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const int nelmt = n * n * n;
+    const bool valid = e < nelmt;
+    const int ex = e % n, ey = (e / n) % n, ez = e / (n * n);
+    const StructuredFetch<T, NM> f{xg, N, ex * (NM - 1), ey * (NM - 1), ez * (NM - 1)};
+    helmholtz_element<T, NM, NQ, DEF>(tab, f, valid, e, t, geo, lambda, lds[le][0], lds[le][1], lds[le][2],
+                                      out + (size_t)e * NM * NM * NM);
 }
 
 }  // namespace hex

@@ -1,0 +1,518 @@
+// Order-independent kernels of the structured-hex C0 Helmholtz solver.
+//
+// Global vectors hold every C0 degree of freedom of the n^3 mesh (N^3 with
+// N = n*(nm-1)+1), Dirichlet rows included and kept at zero, so the
+// local->global map is pure index arithmetic and never stored.
+//
+// CG (NekLinSysIterCG.cpp:118-372): every iteration is four launches -
+// update (p, q, x, r, w and the partial dots r.w, r.r), gathered operator
+// apply, assembly (with the partial dot s.w) and a one-block scalar step that
+// finishes the reductions in a fixed order, tests ||r||^2 < tol^2 ||b||^2 and
+// forms beta and alpha.  All state (alpha, beta, rho, ..., iteration count,
+// done flag) lives in device memory, so a fixed batch of iterations can be
+// launched (or replayed from a HIP graph) without host round trips; kernels
+// of a finished solve return immediately.  Reductions use a fixed grid and a
+// fixed combination order, so results are bitwise reproducible.
+#include "solver.hpp"
+
+#include <cmath>
+
+namespace nektar_amd {
+namespace solver {
+
+namespace {
+
+__device__ __forceinline__ bool on_boundary(int gx, int gy, int gz, int N)
+{
+    return gx == 0 || gy == 0 || gz == 0 || gx == N - 1 || gy == N - 1 || gz == N - 1;
+}
+
+// Block-wide deterministic sum of K values per thread; result in out[] of
+// thread 0.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *out)
+{
+    __shared__ double red[K][kBlock];
+#pragma unroll
+    for (int c = 0; c < K; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int c = 0; c < K; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) out[c] = red[c][0];
+    }
+}
+
+// Entries of 1-D global node g: element index and local mode (<= 2 entries).
+__device__ __forceinline__ int node_contrib(int g, int nm, int n, int (&e)[2], int (&p)[2])
+{
+    const int s = nm - 1;
+    if (g % s == 0) {
+        const int v = g / s;
+        int c = 0;
+        if (v > 0) { e[c] = v - 1; p[c] = 1; ++c; }
+        if (v < n) { e[c] = v; p[c] = 0; ++c; }
+        return c;
+    }
+    e[0] = g / s;
+    p[0] = g % s + 1;
+    return 1;
+}
+
+__device__ __forceinline__ void trilinear_deriv(const double *__restrict__ X, int n, int ex, int ey, int ez,
+                                                double x0, double x1, double x2, double M[3][3], double *pos)
+{
+    const int nv = n + 1;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) M[a][b] = 0.0;
+    if (pos) pos[0] = pos[1] = pos[2] = 0.0;
+    for (int v = 0; v < 8; ++v) {
+        const int sx = (v & 1) ? 1 : -1, sy = (v & 2) ? 1 : -1, sz = (v & 4) ? 1 : -1;
+        const int ix = ex + (v & 1), iy = ey + ((v >> 1) & 1), iz = ez + ((v >> 2) & 1);
+        const double *P = X + ((size_t)(iz * nv + iy) * nv + ix) * 3;
+        const double f0 = 1 + sx * x0, f1 = 1 + sy * x1, f2 = 1 + sz * x2;
+        const double dn[3] = {sx * f1 * f2 * 0.125, f0 * sy * f2 * 0.125, f0 * f1 * sz * 0.125};
+        for (int k = 0; k < 3; ++k) {
+            for (int i = 0; i < 3; ++i) M[k][i] += dn[i] * P[k];  // M[k][i] = dx_k / dxi_i
+            if (pos) pos[k] += f0 * f1 * f2 * 0.125 * P[k];
+        }
+    }
+}
+
+// J, and metric m_de = sum_j df[j*3+d] df[j*3+e] with df = (dx/dxi)^{-T}
+__device__ __forceinline__ void metric_from(const double M[3][3], double &J, double m[6])
+{
+    const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    const double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+    const double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    const double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+    const double id = 1.0 / det;
+    // inverse Minv[i][k] = d xi_i / d x_k ; df[j*3+d] = Minv[d][j]
+    double Mi[3][3];
+    Mi[0][0] = c00 * id;
+    Mi[1][0] = c01 * id;
+    Mi[2][0] = c02 * id;
+    Mi[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * id;
+    Mi[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * id;
+    Mi[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * id;
+    Mi[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * id;
+    Mi[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * id;
+    Mi[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * id;
+    const int idx[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int c = 0; c < 6; ++c) {
+        const int d = idx[c][0], e = idx[c][1];
+        m[c] = Mi[d][0] * Mi[e][0] + Mi[d][1] * Mi[e][1] + Mi[d][2] * Mi[e][2];
+    }
+    J = fabs(det);
+}
+
+}  // namespace
+
+// Packed Helmholtz geometry of the structured mesh from its vertices
+// (GeomFactors.cpp:150-477 for trilinear maps).
+__global__ void geom_kernel(int n, int NQ, const double *__restrict__ X, const double *__restrict__ z,
+                            const double *__restrict__ w, int deformed, double *__restrict__ geo)
+{
+    const size_t nelmt = (size_t)n * n * n, NQ3 = (size_t)NQ * NQ * NQ;
+    const size_t total = deformed ? nelmt * NQ3 : nelmt;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = deformed ? id / NQ3 : id;
+        const int q = deformed ? (int)(id % NQ3) : 0;
+        const int ex = (int)(e % n), ey = (int)((e / n) % n), ez = (int)(e / ((size_t)n * n));
+        const int i = q % NQ, j = (q / NQ) % NQ, k = q / (NQ * NQ);
+        double M[3][3], J, m[6];
+        trilinear_deriv(X, n, ex, ey, ez, deformed ? z[i] : 0.0, deformed ? z[j] : 0.0, deformed ? z[k] : 0.0, M,
+                        nullptr);
+        metric_from(M, J, m);
+        if (deformed) {
+            const double wj = w[i] * w[j] * w[k] * J;
+            for (int c = 0; c < 6; ++c) geo[(e * 7 + c) * NQ3 + q] = wj * m[c];
+            geo[(e * 7 + 6) * NQ3 + q] = wj;
+        } else {
+            for (int c = 0; c < 6; ++c) geo[e * 7 + c] = J * m[c];
+            geo[e * 7 + 6] = J;
+        }
+    }
+}
+
+// f = (3 k^2 pi^2 + lambda) prod_d sin(k pi x_d) at the quadrature points:
+// the forcing of -lap(u) + lambda u = f with u = prod sin(k pi x_d).
+__global__ void forcing_kernel(int n, int NQ, const double *__restrict__ X, const double *__restrict__ z,
+                               int kw, double lambda, double *__restrict__ f)
+{
+    const size_t NQ3 = (size_t)NQ * NQ * NQ, total = (size_t)n * n * n * NQ3;
+    const double kp = kw * M_PI, amp = 3.0 * kp * kp + lambda;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = id / NQ3;
+        const int q = (int)(id % NQ3);
+        const int ex = (int)(e % n), ey = (int)((e / n) % n), ez = (int)(e / ((size_t)n * n));
+        double M[3][3], x[3];
+        trilinear_deriv(X, n, ex, ey, ez, z[q % NQ], z[(q / NQ) % NQ], z[q / (NQ * NQ)], M, x);
+        f[id] = amp * sin(kp * x[0]) * sin(kp * x[1]) * sin(kp * x[2]);
+    }
+}
+
+// sum_e sum_q W J (u - u_exact)^2, u_exact = prod sin(k pi x_d); partials
+// per block (deterministic).
+__global__ void l2err_kernel(int n, int NQ, const double *__restrict__ X, const double *__restrict__ z,
+                             const double *__restrict__ w, const double *__restrict__ geo, int deformed, int kw,
+                             const double *__restrict__ u, double *__restrict__ partials)
+{
+    const size_t NQ3 = (size_t)NQ * NQ * NQ, total = (size_t)n * n * n * NQ3;
+    const double kp = kw * M_PI;
+    double acc[1] = {0.0};
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = id / NQ3;
+        const int q = (int)(id % NQ3);
+        const int i = q % NQ, j = (q / NQ) % NQ, k = q / (NQ * NQ);
+        const int ex = (int)(e % n), ey = (int)((e / n) % n), ez = (int)(e / ((size_t)n * n));
+        double M[3][3], x[3];
+        trilinear_deriv(X, n, ex, ey, ez, z[i], z[j], z[k], M, x);
+        const double ue = sin(kp * x[0]) * sin(kp * x[1]) * sin(kp * x[2]);
+        const double wj = deformed ? geo[(e * 7 + 6) * NQ3 + q] : geo[e * 7 + 6] * w[i] * w[j] * w[k];
+        const double d = u[id] - ue;
+        acc[0] += wj * d * d;
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+// Direct-stiffness summation of element results into the global vector
+// (AssemblyMapCG::Assemble, AssemblyMapCG.cpp:2860-2886) as a gather: each
+// global row sums its <= 8 element contributions in a fixed order.
+// Dirichlet rows are set to 0.  Optionally accumulates s . wdot.
+template <typename T>
+__global__ void assemble_kernel(int n, int nm, int N, const T *__restrict__ sloc, T *__restrict__ sg,
+                                const T *__restrict__ wdot, double *__restrict__ partials, const int *__restrict__ done,
+                                int keep_dirichlet)
+{
+    if (done && *done) return;
+    const size_t NG = (size_t)N * N * N;
+    const size_t nc = (size_t)nm * nm * nm;
+    double acc[1] = {0.0};
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        const int gx = (int)(G % N), gy = (int)((G / N) % N), gz = (int)(G / ((size_t)N * N));
+        if (!keep_dirichlet && on_boundary(gx, gy, gz, N)) {
+            sg[G] = T(0);
+            continue;
+        }
+        int ex[2], px[2], ey[2], py[2], ez[2], pz[2];
+        const int cx = node_contrib(gx, nm, n, ex, px), cy = node_contrib(gy, nm, n, ey, py),
+                  cz = node_contrib(gz, nm, n, ez, pz);
+        double s = 0.0;
+        for (int c = 0; c < cz; ++c)
+            for (int b = 0; b < cy; ++b)
+                for (int a = 0; a < cx; ++a) {
+                    const size_t e = ((size_t)ez[c] * n + ey[b]) * n + ex[a];
+                    s += (double)sloc[e * nc + ((size_t)pz[c] * nm + py[b]) * nm + px[a]];
+                }
+        sg[G] = T(s);
+        if (wdot) acc[0] += (double)T(s) * (double)wdot[G];
+    }
+    if (partials) block_sum<1>(acc, partials + blockIdx.x);
+}
+
+// r = b (non-Dirichlet rows), x = p = q = 0, w = M^{-1} r; partials
+// {r.r, r.w, b.b(all rows)}.
+template <typename T>
+__global__ void cg_init_kernel(int N, const double *__restrict__ b, const double *__restrict__ inv_diag,
+                               T *__restrict__ x, T *__restrict__ r, T *__restrict__ p, T *__restrict__ q,
+                               T *__restrict__ w, double *__restrict__ partials)
+{
+    const size_t NG = (size_t)N * N * N;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        const int gx = (int)(G % N), gy = (int)((G / N) % N), gz = (int)(G / ((size_t)N * N));
+        const double bv = b[G];
+        acc[2] += bv * bv;
+        const T rv = on_boundary(gx, gy, gz, N) ? T(0) : T(bv);
+        const T wv = T(inv_diag[G] * (double)rv);
+        x[G] = T(0);
+        p[G] = T(0);
+        q[G] = T(0);
+        r[G] = rv;
+        w[G] = wv;
+        acc[0] += (double)rv * (double)rv;
+        acc[1] += (double)rv * (double)wv;
+    }
+    block_sum<3>(acc, partials + 3 * blockIdx.x);
+}
+
+// NekLinSysIterCG.cpp:240-249 (search directions, solution, residual) and
+// :264-307 (preconditioner, r.w, r.r).
+template <typename T>
+__global__ void cg_update_kernel(size_t NG, const CGState *__restrict__ st, T *__restrict__ p, T *__restrict__ q,
+                                 T *__restrict__ x, T *__restrict__ r, T *__restrict__ w, const T *__restrict__ s,
+                                 const double *__restrict__ inv_diag, double *__restrict__ partials)
+{
+    if (st->done) return;
+    const T alpha = T(st->alpha), beta = T(st->beta);
+    double acc[2] = {0.0, 0.0};
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        const T pv = beta * p[G] + w[G];
+        const T qv = beta * q[G] + s[G];
+        x[G] = alpha * pv + x[G];
+        const T rv = -alpha * qv + r[G];
+        const T wv = T(inv_diag[G] * (double)rv);
+        p[G] = pv;
+        q[G] = qv;
+        r[G] = rv;
+        w[G] = wv;
+        acc[0] += (double)rv * (double)wv;
+        acc[1] += (double)rv * (double)rv;
+    }
+    block_sum<2>(acc, partials + 2 * blockIdx.x);
+}
+
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ pa, int stride, int comp, int nb,
+                                                double *out)
+{
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += kBlock) v[0] += pa[(size_t)b * stride + comp];
+    block_sum<1>(v, out);
+}
+
+__global__ void cg_scalar_init_kernel(CGState *st, const double *__restrict__ pa, const double *__restrict__ pmu,
+                                      int nb)
+{
+    __shared__ double res[4];
+    reduce_partials(pa, 3, 0, nb, &res[0]);
+    __syncthreads();
+    reduce_partials(pa, 3, 1, nb, &res[1]);
+    __syncthreads();
+    reduce_partials(pa, 3, 2, nb, &res[2]);
+    __syncthreads();
+    reduce_partials(pmu, 1, 0, nb, &res[3]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double eps = res[0], rho = res[1], bb = res[2], mu = res[3];
+    st->rhs_mag = bb > 1.0e-6 ? bb : 1.0;  // NekLinSysIter.cpp:116
+    st->eps = eps;
+    st->iter = 0;
+    st->converged = 0;
+    st->done = 0;
+    st->final_error = sqrt(eps / st->rhs_mag);
+    if (eps < st->tol2 * st->rhs_mag) {  // NekLinSysIterCG.cpp:169
+        st->done = 1;
+        st->converged = 1;
+        return;
+    }
+    st->rho = rho;
+    st->mu = mu;
+    st->beta = 0.0;
+    st->alpha = rho / mu;
+    st->iter = 1;
+}
+
+__global__ void cg_scalar_kernel(CGState *st, const double *__restrict__ pa, const double *__restrict__ pmu, int nb)
+{
+    if (st->done) return;
+    __shared__ double res[3];
+    reduce_partials(pa, 2, 0, nb, &res[0]);
+    __syncthreads();
+    reduce_partials(pa, 2, 1, nb, &res[1]);
+    __syncthreads();
+    reduce_partials(pmu, 1, 0, nb, &res[2]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double rho_new = res[0], eps = res[1], mu = res[2];
+    st->iter += 1;
+    st->eps = eps;
+    st->final_error = sqrt(eps / st->rhs_mag);
+    if (eps < st->tol2 * st->rhs_mag) {  // NekLinSysIterCG.cpp:340
+        st->done = 1;
+        st->converged = 1;
+        return;
+    }
+    const double beta = rho_new / st->rho;  // :358 (rho_star = 0)
+    st->alpha = rho_new / (mu - rho_new * beta / st->alpha);
+    st->beta = beta;
+    st->rho = rho_new;
+    st->mu = mu;
+    if (st->iter > st->maxit) st->done = 1;  // :215 (checked at the loop top)
+}
+
+// r64 = b - s on non-Dirichlet rows (0 elsewhere); partial r.r
+__global__ void residual_kernel(int N, const double *__restrict__ b, const double *__restrict__ s,
+                                double *__restrict__ r, double *__restrict__ partials)
+{
+    const size_t NG = (size_t)N * N * N;
+    double acc[1] = {0.0};
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        const int gx = (int)(G % N), gy = (int)((G / N) % N), gz = (int)(G / ((size_t)N * N));
+        const double v = on_boundary(gx, gy, gz, N) ? 0.0 : b[G] - s[G];
+        r[G] = v;
+        acc[0] += v * v;
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+// x64 += d
+template <typename T>
+__global__ void add_kernel(size_t NG, double *__restrict__ x, const T *__restrict__ d)
+{
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x)
+        x[G] += (double)d[G];
+}
+
+// inv_diag = 1/diag on non-Dirichlet rows, 0 on Dirichlet rows
+__global__ void invert_diag_kernel(int N, double *__restrict__ d)
+{
+    const size_t NG = (size_t)N * N * N;
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        const int gx = (int)(G % N), gy = (int)((G / N) % N), gz = (int)(G / ((size_t)N * N));
+        d[G] = on_boundary(gx, gy, gz, N) ? 0.0 : 1.0 / d[G];
+    }
+}
+
+__global__ void sum_partials_kernel(const double *__restrict__ pa, int nb, double *out)
+{
+    reduce_partials(pa, 1, 0, nb, out);
+}
+
+__global__ void sumsq_kernel(size_t n, const double *__restrict__ a, double *__restrict__ partials)
+{
+    double acc[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc[0] += a[i] * a[i];
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+__global__ void cast_kernel(size_t n, const double *__restrict__ a, float *__restrict__ b)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------------- launchers
+#define NMF_LAUNCH(kernel, grid, ...) \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__)
+
+static int grid_for(size_t n)
+{
+    size_t g = (n + kBlock - 1) / kBlock;
+    return (int)(g < (size_t)kRedBlocks ? (g > 0 ? g : 1) : kRedBlocks);
+}
+
+hipError_t launch_geom(hipStream_t stream, int n, int NQ, const double *X, const double *z, const double *w,
+                       bool deformed, double *geo)
+{
+    const size_t total = (size_t)n * n * n * (deformed ? (size_t)NQ * NQ * NQ : 1);
+    NMF_LAUNCH(geom_kernel, grid_for(total), n, NQ, X, z, w, deformed ? 1 : 0, geo);
+    return hipGetLastError();
+}
+
+hipError_t launch_forcing(hipStream_t stream, int n, int NQ, const double *X, const double *z, int k, double lambda,
+                          double *f)
+{
+    NMF_LAUNCH(forcing_kernel, grid_for((size_t)n * n * n * NQ * NQ * NQ), n, NQ, X, z, k, lambda, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_l2err(hipStream_t stream, int n, int NQ, const double *X, const double *z, const double *w,
+                        const double *geo, bool deformed, int k, const double *u, double *partials, int *nb)
+{
+    *nb = grid_for((size_t)n * n * n * NQ * NQ * NQ);
+    NMF_LAUNCH(l2err_kernel, *nb, n, NQ, X, z, w, geo, deformed ? 1 : 0, k, u, partials);
+    return hipGetLastError();
+}
+
+int reduction_blocks(size_t NG) { return grid_for(NG); }
+
+template <typename T>
+hipError_t launch_assemble(hipStream_t stream, int n, int nm, int N, const T *sloc, T *sg, const T *wdot,
+                           double *partials, const int *done, bool keep_dirichlet)
+{
+    NMF_LAUNCH((assemble_kernel<T>), grid_for((size_t)N * N * N), n, nm, N, sloc, sg, wdot, partials, done,
+               keep_dirichlet ? 1 : 0);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_cg_init(hipStream_t stream, int N, const double *b, const double *inv_diag, T *x, T *r, T *p, T *q,
+                          T *w, double *partials)
+{
+    NMF_LAUNCH((cg_init_kernel<T>), grid_for((size_t)N * N * N), N, b, inv_diag, x, r, p, q, w, partials);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_cg_update(hipStream_t stream, int N, const CGState *st, T *p, T *q, T *x, T *r, T *w, const T *s,
+                            const double *inv_diag, double *partials)
+{
+    const size_t NG = (size_t)N * N * N;
+    NMF_LAUNCH((cg_update_kernel<T>), grid_for(NG), NG, st, p, q, x, r, w, s, inv_diag, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_scalar_init(hipStream_t stream, CGState *st, const double *pa, const double *pmu, int nb)
+{
+    NMF_LAUNCH(cg_scalar_init_kernel, 1, st, pa, pmu, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_scalar(hipStream_t stream, CGState *st, const double *pa, const double *pmu, int nb)
+{
+    NMF_LAUNCH(cg_scalar_kernel, 1, st, pa, pmu, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(hipStream_t stream, int N, const double *b, const double *s, double *r, double *partials)
+{
+    NMF_LAUNCH(residual_kernel, grid_for((size_t)N * N * N), N, b, s, r, partials);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_add(hipStream_t stream, size_t NG, double *x, const T *d)
+{
+    NMF_LAUNCH((add_kernel<T>), grid_for(NG), NG, x, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_invert_diag(hipStream_t stream, int N, double *d)
+{
+    NMF_LAUNCH(invert_diag_kernel, grid_for((size_t)N * N * N), N, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(hipStream_t stream, const double *pa, int nb, double *out)
+{
+    NMF_LAUNCH(sum_partials_kernel, 1, pa, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sumsq(hipStream_t stream, size_t n, const double *a, double *partials)
+{
+    NMF_LAUNCH(sumsq_kernel, grid_for(n), n, a, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_cast(hipStream_t stream, size_t n, const double *a, float *b)
+{
+    NMF_LAUNCH(cast_kernel, grid_for(n), n, a, b);
+    return hipGetLastError();
+}
+
+template hipError_t launch_assemble<double>(hipStream_t, int, int, int, const double *, double *, const double *,
+                                            double *, const int *, bool);
+template hipError_t launch_assemble<float>(hipStream_t, int, int, int, const float *, float *, const float *, double *,
+                                           const int *, bool);
+template hipError_t launch_cg_init<double>(hipStream_t, int, const double *, const double *, double *, double *,
+                                           double *, double *, double *, double *);
+template hipError_t launch_cg_init<float>(hipStream_t, int, const double *, const double *, float *, float *, float *,
+                                          float *, float *, double *);
+template hipError_t launch_cg_update<double>(hipStream_t, int, const CGState *, double *, double *, double *, double *,
+                                             double *, const double *, const double *, double *);
+template hipError_t launch_cg_update<float>(hipStream_t, int, const CGState *, float *, float *, float *, float *,
+                                            float *, const float *, const double *, double *);
+template hipError_t launch_add<float>(hipStream_t, size_t, double *, const float *);
+template hipError_t launch_add<double>(hipStream_t, size_t, double *, const double *);
+
+}  // namespace solver
+}  // namespace nektar_amd

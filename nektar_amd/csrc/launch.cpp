@@ -5,7 +5,8 @@ namespace nektar_amd {
 
 #define NMF_DECL(n)                                              \
     hipError_t launch_hex_nm##n(int nq, const KernelArgs &a);    \
-    hipError_t launch_tet_nm##n(int nq, const KernelArgs &a);
+    hipError_t launch_tet_nm##n(int nq, const KernelArgs &a);    \
+    hipError_t launch_hex_solver_nm##n(int nq, const SolverArgs &a);
 NMF_DECL(2) NMF_DECL(3) NMF_DECL(4) NMF_DECL(5) NMF_DECL(6) NMF_DECL(7) NMF_DECL(8) NMF_DECL(9) NMF_DECL(10)
 #undef NMF_DECL
 
@@ -29,6 +30,16 @@ hipError_t launch_tet(int nm, int nq, const KernelArgs &a)
     switch (nm) {
         NMF_CASE(tet, 2) NMF_CASE(tet, 3) NMF_CASE(tet, 4) NMF_CASE(tet, 5) NMF_CASE(tet, 6)
         NMF_CASE(tet, 7) NMF_CASE(tet, 8) NMF_CASE(tet, 9) NMF_CASE(tet, 10)
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_hex_solver(int nm, int nq, const SolverArgs &a)
+{
+    switch (nm) {
+        NMF_CASE(hex_solver, 2) NMF_CASE(hex_solver, 3) NMF_CASE(hex_solver, 4) NMF_CASE(hex_solver, 5)
+        NMF_CASE(hex_solver, 6) NMF_CASE(hex_solver, 7) NMF_CASE(hex_solver, 8) NMF_CASE(hex_solver, 9)
+        NMF_CASE(hex_solver, 10)
     default: return hipErrorInvalidValue;
     }
 }

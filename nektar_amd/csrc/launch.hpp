@@ -22,11 +22,30 @@ struct KernelArgs {
     hipStream_t stream;
 };
 
+// Order-templated kernels of the structured-hex C0 solver.
+enum class SolverKernel { Apply = 0, Diag = 1, Rhs = 2, BwdGather = 3 };
+
+struct SolverArgs {
+    SolverKernel kind;
+    bool fp32;      // Apply only: FP32 operator (vectors, tables, geometry)
+    bool deformed;
+    int n, N;       // elements and global nodes per direction
+    const void *tab;
+    const void *dtab;  // Diag: {B^2, B B', B'^2} (FP64)
+    const void *geo;
+    double lambda;
+    const void *in;
+    void *out;
+    const int *done;   // Apply: early exit flag (NULL = always run)
+    hipStream_t stream;
+};
+
 constexpr int kMinModes = 2, kMaxModes = 10;
 
 bool hex_supported(int nm, int nq);
 bool tet_supported(int nm, int nq0);
 hipError_t launch_hex(int nm, int nq, const KernelArgs &a);
 hipError_t launch_tet(int nm, int nq0, const KernelArgs &a);
+hipError_t launch_hex_solver(int nm, int nq, const SolverArgs &a);
 
 }  // namespace nektar_amd

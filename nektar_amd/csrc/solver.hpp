@@ -1,0 +1,136 @@
+// C0 Helmholtz solver on structured hex meshes: the GPU counterpart of
+// ContField::HelmSolve -> GlobalLinSysIterativeFull -> NekLinSysIterCG with
+// the Diagonal (Jacobi) preconditioner, plus a mixed-precision
+// iterative-refinement variant (FP32 inner CG, FP64 residuals, dots and
+// diagonal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace nektar_amd {
+namespace solver {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 2048;
+
+struct CGState {
+    double alpha, beta, rho, mu, eps, rhs_mag, tol2, final_error;
+    int iter, maxit, done, converged;
+};
+
+int reduction_blocks(size_t NG);
+hipError_t launch_geom(hipStream_t, int n, int NQ, const double *X, const double *z, const double *w, bool deformed,
+                       double *geo);
+hipError_t launch_forcing(hipStream_t, int n, int NQ, const double *X, const double *z, int k, double lambda,
+                          double *f);
+hipError_t launch_l2err(hipStream_t, int n, int NQ, const double *X, const double *z, const double *w,
+                        const double *geo, bool deformed, int k, const double *u, double *partials, int *nb);
+template <typename T>
+hipError_t launch_assemble(hipStream_t, int n, int nm, int N, const T *sloc, T *sg, const T *wdot, double *partials,
+                           const int *done, bool keep_dirichlet = false);
+template <typename T>
+hipError_t launch_cg_init(hipStream_t, int N, const double *b, const double *inv_diag, T *x, T *r, T *p, T *q, T *w,
+                          double *partials);
+template <typename T>
+hipError_t launch_cg_update(hipStream_t, int N, const CGState *st, T *p, T *q, T *x, T *r, T *w, const T *s,
+                            const double *inv_diag, double *partials);
+hipError_t launch_cg_scalar_init(hipStream_t, CGState *st, const double *pa, const double *pmu, int nb);
+hipError_t launch_cg_scalar(hipStream_t, CGState *st, const double *pa, const double *pmu, int nb);
+hipError_t launch_residual(hipStream_t, int N, const double *b, const double *s, double *r, double *partials);
+template <typename T>
+hipError_t launch_add(hipStream_t, size_t NG, double *x, const T *d);
+hipError_t launch_invert_diag(hipStream_t, int N, double *d);
+hipError_t launch_sum_partials(hipStream_t, const double *pa, int nb, double *out);
+// partial sums of a[i]^2 over i < n (grid of reduction_blocks(n))
+hipError_t launch_sumsq(hipStream_t, size_t n, const double *a, double *partials);
+hipError_t launch_cast(hipStream_t, size_t n, const double *a, float *b);
+
+}  // namespace solver
+
+struct SolveInfo {
+    int iterations = 0;        // total CG iterations (inner iterations for mixed)
+    int outer_iterations = 0;  // refinement steps (1 for plain FP64)
+    int converged = 0;
+    double final_error = 0.0;  // sqrt(||r||^2 / ||b||^2) of the FP64 residual
+    double rhs_magnitude = 0.0;
+    double solve_ms = 0.0;
+};
+
+// Structured n^3 hex mesh on [0,1]^3 with per-vertex coordinates.
+struct HexBoxMesh {
+    int n = 0;
+    std::vector<double> X;  // (n+1)^3 * 3, vertex (ix,iy,iz) at ((iz*(n+1)+iy)*(n+1)+ix)*3
+    bool deformed = false;
+    static HexBoxMesh make(int n, double perturb, uint64_t seed);
+};
+
+class HexSolver {
+public:
+    HexSolver(const HexBoxMesh &mesh, int nummodes, int numpoints, double lambda, hipStream_t stream);
+    ~HexSolver();
+    HexSolver(const HexSolver &) = delete;
+    HexSolver &operator=(const HexSolver &) = delete;
+
+    // b = Assemble(IProductWRTBase(f)), f = (3 k^2 pi^2 + lambda) prod sin(k pi x)
+    void set_forcing_manufactured(int k);
+    // b = Assemble(IProductWRTBase(f)) for f at quadrature points (device, FP64)
+    void set_forcing_phys(const double *f_dev);
+    // mixed: FP32 inner CG inside FP64 iterative refinement
+    SolveInfo solve(bool mixed, double tol, int maxit, double inner_tol = 1e-5);
+    // L2 error of the current solution against prod sin(k pi x)
+    double l2_error_manufactured(int k);
+    // y = A x on global vectors (device, FP64), Dirichlet rows zeroed
+    void apply_global(const double *x, double *y);
+
+    size_t nglobal() const { return NG_; }
+    size_t nlocal() const { return (size_t)nelmt_ * nc_; }
+    size_t ndirichlet() const;
+    int nelmt() const { return nelmt_; }
+    int nm() const { return nm_; }
+    int nq() const { return nq_; }
+    bool deformed() const { return deformed_; }
+    double lambda() const { return lambda_; }
+    hipStream_t stream() const { return stream_; }
+    const double *solution() const { return x_.as<double>(); }
+    const double *rhs() const { return b_.as<double>(); }
+    const double *inv_diag() const { return invd_.as<double>(); }
+    size_t device_bytes() const;
+    static size_t estimate_bytes(int n, int nm, int nq, bool deformed, bool mixed);
+
+    // CG on the current rhs_, stopping at tol; result in x (FP64 or FP32
+    // buffers); returns the final state.  Public for the serving loop.
+    solver::CGState run_cg(bool fp32, const double *b, double tol, int maxit);
+
+private:
+    void build_geometry();
+    void build_diagonal();
+    void apply_local(bool fp32, const void *xg, void *sloc, const int *done);
+    void capture_iterations(bool fp32);
+
+    int n_, nm_, nq_, N_, nelmt_, nc_;
+    size_t NG_;
+    bool deformed_;
+    double lambda_;
+    hipStream_t stream_;
+    bool own_stream_;
+    int nb_;  // reduction blocks
+
+    DeviceBuffer X_, z_, w_, tab64_, tab32_, dtab_, geo64_, geo32_;
+    DeviceBuffer b_, x_, invd_, sloc64_, tmp64_, r64_;
+    // CG work vectors per precision
+    DeviceBuffer cg64_[6], cg32_[6], sloc32_;  // x r p q w s
+    DeviceBuffer state_, pa_, pmu_, scalar_;
+    solver::CGState *host_state_ = nullptr;
+    double *host_scalar_ = nullptr;
+
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    static constexpr int kItersPerGraph = 8;
+};
+
+}  // namespace nektar_amd

@@ -1,0 +1,145 @@
+// Order-templated kernels used by the C0 Helmholtz solver on structured hex
+// meshes (in addition to hex::helmholtz_gather_kernel):
+//   * diag_kernel      : diagonal of each element's Helmholtz matrix,
+//                        sum-factorised over the seven metric/mass terms
+//                        (the numbers PreconditionerDiagonal.cpp:93-125 reads
+//                        off the dense local matrices);
+//   * rhs_kernel       : IProductWRTBase of a forcing at quadrature points
+//                        with the packed geometry (ContField.cpp:811-817);
+//   * bwd_gather_kernel: BwdTrans of a global C0 vector (ExpList::BwdTrans
+//                        after GlobalToLocal, ExpList.cpp:3105).
+#pragma once
+
+#include "hex_kernels.hpp"
+
+namespace nektar_amd {
+namespace hex {
+
+// Geometry weight of component c at point (i,j,k) of element e, from the
+// packed Helmholtz geometry (c = 0..5 metric, 6 mass/Jacobian).
+template <typename T, int NM, int NQ, bool DEF>
+__device__ __forceinline__ double packed_geo(const Tab<T, NM, NQ> &tab, const double *__restrict__ geo, int e,
+                                             int c, int i, int j, int k)
+{
+    constexpr int NQ3 = NQ * NQ * NQ;
+    if (DEF) return geo[((size_t)e * 7 + c) * NQ3 + (k * NQ + j) * NQ + i];
+    return geo[(size_t)e * 7 + c] * ((double)tab.W(i) * (double)tab.W(j) * (double)tab.W(k));
+}
+
+// dtab: [3][NM*NQ] = { B^2, B*B', B'^2 } (B' = dbdata), FP64.
+template <int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void diag_kernel(
+    int nelmt, const double *__restrict__ tabp, const double *__restrict__ dtab, const double *__restrict__ geo,
+    double lambda, double *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ;
+    __shared__ double lds[EPB][2][NQ * NQ * NQ];
+    const Tab<double, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    double *S1 = lds[le][0], *S2 = lds[le][1];
+    // (X, Y, Z) table index per component and multiplicity
+    constexpr int XT[7] = {2, 1, 1, 0, 0, 0, 0}, YT[7] = {0, 1, 0, 2, 1, 0, 0}, ZT[7] = {0, 0, 1, 0, 1, 2, 0};
+    constexpr int GC[7] = {0, 1, 2, 3, 4, 5, 6};
+    const double FAC[7] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0, lambda};
+    double acc[NM];
+#pragma unroll
+    for (int p = 0; p < NM; ++p) acc[p] = 0.0;
+    const int i0 = t % NQ, j0 = t / NQ;
+#pragma unroll 1
+    for (int c = 0; c < 7; ++c) {
+        const double *X = dtab + XT[c] * NM * NQ, *Y = dtab + YT[c] * NM * NQ, *Z = dtab + ZT[c] * NM * NQ;
+        // k -> r on column (i0, j0)
+        double g[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) g[k] = valid ? packed_geo<double, NM, NQ, DEF>(tab, geo, e, GC[c], i0, j0, k) : 0.0;
+#pragma unroll
+        for (int r = 0; r < NM; ++r) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) s = fma(g[k], Z[r * NQ + k], s);
+            S1[(r * NQ + j0) * NQ + i0] = s;
+        }
+        __syncthreads();
+        if (t < NQ * NM) {  // j -> q on line (i, r)
+            const int i = t % NQ, r = t / NQ;
+            double v[NQ];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) v[j] = S1[(r * NQ + j) * NQ + i];
+#pragma unroll
+            for (int q = 0; q < NM; ++q) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NQ; ++j) s = fma(v[j], Y[q * NQ + j], s);
+                S2[(r * NM + q) * NQ + i] = s;
+            }
+        }
+        __syncthreads();
+        if (t < NM * NM) {  // i -> p on line (q, r)
+            const int q = t % NM, r = t / NM;
+            double v[NQ];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) v[i] = S2[(r * NM + q) * NQ + i];
+#pragma unroll
+            for (int p = 0; p < NM; ++p) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) s = fma(v[i], X[p * NQ + i], s);
+                acc[p] = fma(FAC[c], s, acc[p]);
+            }
+        }
+        __syncthreads();
+    }
+    if (valid && t < NM * NM) {
+        const int q = t % NM, r = t / NM;
+#pragma unroll
+        for (int p = 0; p < NM; ++p) out[(size_t)e * NM * NM * NM + (r * NM + q) * NM + p] = acc[p];
+    }
+}
+
+// b_loc = B^T (W J f) with W J from the packed geometry (FP64).
+template <int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void rhs_kernel(
+    int nelmt, const double *__restrict__ tabp, const double *__restrict__ geo, const double *__restrict__ f,
+    double *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ double lds[EPB][2][NQ3];
+    const Tab<double, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % NQ, j = t / NQ;
+    double y[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k)
+        y[k] = valid ? f[(size_t)e * NQ3 + (k * NQ + j) * NQ + i] * packed_geo<double, NM, NQ, DEF>(tab, geo, e, 6, i, j, k)
+                     : 0.0;
+    iprod_from_column<double, NM, NQ>(tab, y, valid, t, lds[le][0], lds[le][1], out + (size_t)e * NM * NM * NM);
+}
+
+// u (quadrature points, element-major) = BwdTrans(GlobalToLocal(xg)), FP64.
+template <int NM, int NQ>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void bwd_gather_kernel(
+    int n, int N, const double *__restrict__ tabp, const double *__restrict__ xg, double *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
+    __shared__ double lds[EPB][2][NQ3];
+    const Tab<double, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < n * n * n;
+    const int ex = e % n, ey = (e / n) % n, ez = e / (n * n);
+    const StructuredFetch<double, NM> fch{xg, N, ex * (NM - 1), ey * (NM - 1), ez * (NM - 1)};
+    double u[NQ];
+    bwd_to_column<double, NM, NQ>(tab, fch, valid, t, lds[le][0], lds[le][1], u);
+    if (valid) {
+        const int i = t % NQ, j = t / NQ;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) out[(size_t)e * NQ3 + (k * NQ + j) * NQ + i] = u[k];
+    }
+}
+
+}  // namespace hex
+}  // namespace nektar_amd

@@ -1,0 +1,119 @@
+"""GPU parity of the C0 Helmholtz solver against the oracle.
+
+Pieces (rhs assembly, Jacobi diagonal, assembled operator) are compared at
+<= 1e-12 relative; CG solutions, which amplify rounding by the condition
+number, at <= 1e-8 relative with iteration counts within 2 of the
+oracle's restatement of the reference CG."""
+import numpy as np
+import pytest
+
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def nmf():
+    assert torch.cuda.is_available()
+    import nektar_amd
+    nektar_amd.lib()
+    return nektar_amd
+
+
+def _oracle_problem(n, nm, nq, lam, amp, seed, k=1):
+    deformed = amp > 0
+    verts = O.hex_mesh(n, amp, seed)
+    jac, df = O.hex_geom(verts, n ** 3, nq, deformed)
+    xyz = O.hex_coords(verts, n ** 3, nq)
+    f = (3 * (k * np.pi) ** 2 + lam) * np.prod(np.sin(k * np.pi * xyz), axis=2).ravel()
+    m, dirichlet = O.hex_l2g(n, nm)
+    bl = O.iproduct(O.HEX, nm, nq, n ** 3, deformed, jac, f)
+    b = np.zeros(len(dirichlet))
+    np.add.at(b, m, bl)
+    dl = O.helmholtz_diag(O.HEX, nm, nq, n ** 3, deformed, jac, df, lam)
+    dg = np.zeros(len(dirichlet))
+    np.add.at(dg, m, dl)
+    invd = np.where(dirichlet, 0.0, 1.0 / dg)
+    return dict(verts=verts, jac=jac, df=df, m=m, dirichlet=dirichlet, b=b, invd=invd, deformed=deformed)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+CASES = [(3, 5, 6, 1.0, 0.0, 1), (4, 4, 5, 2.5, 0.15, 3), (2, 7, 8, 0.3, 0.0, 2), (3, 6, 7, 1.0, 0.2, 5)]
+
+
+@pytest.mark.parametrize("n,nm,nq,lam,amp,seed", CASES)
+def test_solver_pieces_match_oracle(nmf, n, nm, nq, lam, amp, seed):
+    P = _oracle_problem(n, nm, nq, lam, amp, seed)
+    s = nmf.HelmholtzSolver(n, nm, nq, lam, amp, seed)
+    assert s.nglobal == len(P["dirichlet"])
+    assert s.ndirichlet == int(P["dirichlet"].sum())
+    assert np.array_equal(nmf.hex_box_vertices(n, amp, seed).reshape(-1, 3).min(0), [0, 0, 0])
+    s.set_forcing_manufactured(1)
+    assert _rel(s.rhs(), P["b"]) <= 1e-12
+    assert _rel(s.inv_diag(), P["invd"]) <= 1e-12
+    # assembled operator on a random global vector (Dirichlet rows zero)
+    x = np.random.default_rng(seed).uniform(-1, 1, s.nglobal)
+    x[P["dirichlet"]] = 0.0
+    y = torch.zeros(s.nglobal, dtype=torch.float64, device="cuda")
+    s.apply_global(torch.from_numpy(x).cuda(), y)
+    yl = O.helmholtz(O.HEX, nm, nq, n ** 3, P["deformed"], P["jac"], P["df"], lam, x[P["m"]])
+    yo = np.zeros(s.nglobal)
+    np.add.at(yo, P["m"], yl)
+    yo[P["dirichlet"]] = 0.0
+    assert _rel(y.cpu().numpy(), yo) <= 1e-12
+
+
+@pytest.mark.parametrize("n,nm,nq,lam,amp,seed", CASES)
+def test_fp64_cg_matches_oracle(nmf, n, nm, nq, lam, amp, seed):
+    P = _oracle_problem(n, nm, nq, lam, amp, seed)
+    xo, info_o = O.cg_helmholtz_hex(n, nm, nq, P["deformed"], P["jac"], P["df"], lam, P["b"], P["invd"],
+                                    1e-10, 5000)
+    s = nmf.HelmholtzSolver(n, nm, nq, lam, amp, seed)
+    s.set_forcing_manufactured(1)
+    info = s.solve(mixed=False, tol=1e-10)
+    assert info.converged and info.final_error < 1e-10
+    assert abs(info.iterations - info_o.iterations) <= 2, (info.iterations, info_o.iterations)
+    assert info.rhs_magnitude == pytest.approx(info_o.rhs_magnitude, rel=1e-12)
+    assert _rel(s.solution(), xo) <= 1e-8
+
+
+@pytest.mark.parametrize("n,nm,nq,lam,amp,seed", CASES)
+def test_mixed_precision_cg(nmf, n, nm, nq, lam, amp, seed):
+    s = nmf.HelmholtzSolver(n, nm, nq, lam, amp, seed)
+    s.set_forcing_manufactured(1)
+    i64 = s.solve(mixed=False, tol=1e-10)
+    x64 = s.solution()
+    imx = s.solve(mixed=True, tol=1e-10)
+    assert imx.converged and imx.final_error < 1e-10
+    assert imx.outer_iterations >= 2
+    assert _rel(s.solution(), x64) <= 1e-8
+
+
+def test_solve_is_bitwise_reproducible(nmf):
+    s = nmf.HelmholtzSolver(3, 5, 6, 1.0, 0.1, 7)
+    s.set_forcing_manufactured(2)
+    a = s.solve(False, 1e-10)
+    xa = s.solution()
+    b = s.solve(False, 1e-10)
+    assert a.iterations == b.iterations and np.array_equal(xa, s.solution())
+
+
+@pytest.mark.parametrize("nm", [3, 5, 7])
+def test_manufactured_solution_spectral_convergence(nmf, nm):
+    s = nmf.HelmholtzSolver(4, nm, nm + 1, 1.0)
+    s.set_forcing_manufactured(1)
+    info = s.solve(False, 1e-12)
+    l2 = s.l2_error_manufactured(1)
+    assert info.converged
+    assert l2 < {3: 1e-3, 5: 1e-5, 7: 1e-7}[nm], l2
+
+
+def test_maxit_reports_not_converged(nmf):
+    s = nmf.HelmholtzSolver(3, 6, 7, 1.0)
+    s.set_forcing_manufactured(1)
+    info = s.solve(False, 1e-12, maxit=5)
+    assert not info.converged and info.iterations == 6
