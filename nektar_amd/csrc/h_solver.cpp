@@ -107,8 +107,9 @@ HexSolver::HexSolver(const HexBoxMesh &mesh, int nummodes, int numpoints, double
     sloc64_.allocate(nloc * sizeof(double));
     for (auto &v : cg64_) v.allocate(NG_ * sizeof(double));
     state_.allocate(sizeof(CGState));
-    pa_.allocate(3 * (size_t)nb_ * sizeof(double));
-    pmu_.allocate((size_t)nb_ * sizeof(double));
+    // partial-sum buffers sized for the largest reduction grid any kernel uses
+    pa_.allocate(3 * (size_t)solver::kRedBlocks * sizeof(double));
+    pmu_.allocate((size_t)solver::kRedBlocks * sizeof(double));
     scalar_.allocate(8 * sizeof(double));
     NMF_HIP(hipHostMalloc((void **)&host_state_, sizeof(CGState), hipHostMallocDefault));
     NMF_HIP(hipHostMalloc((void **)&host_scalar_, 8 * sizeof(double), hipHostMallocDefault));
