@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "basis.hpp"
@@ -23,6 +25,17 @@ namespace nektar_amd {
 using solver::CGState;
 
 namespace {
+
+// NMF_VERBOSE=1 prints CG progress like the reference's --verbose
+// (NekLinSysIterCG.cpp:178-190, 323-355).
+bool verbose()
+{
+    static const bool v = [] {
+        const char *e = std::getenv("NMF_VERBOSE");
+        return e && *e && *e != '0';
+    }();
+    return v;
+}
 
 uint64_t splitmix64(uint64_t &s)
 {
@@ -234,19 +247,71 @@ void HexSolver::capture_iterations(bool fp32)
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
 }
 
+void HexSolver::ensure_fp32()
+{
+    if (!geo32_.get()) {
+        const size_t g = geo64_.bytes() / sizeof(double);
+        geo32_.allocate(g * sizeof(float));
+        NMF_HIP(solver::launch_cast(stream_, g, geo64_.as<double>(), geo32_.as<float>()));
+    }
+    if (!cg32_[0].get()) {
+        for (auto &v : cg32_) v.allocate(NG_ * sizeof(float));
+        sloc32_.allocate((size_t)nelmt_ * nc_ * sizeof(float));
+    }
+}
+
+void HexSolver::capture_pcg32()
+{
+    if (graph_[2]) return;
+    CGState *st = state_.as<CGState>();
+    const int *done = &st->done;
+    float *x = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>(), *q = cg32_[3].as<float>();
+    hipGraph_t g;
+    NMF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < kItersPerGraph; ++it) {
+        apply_local(true, p, sloc32_.get(), done);
+        NMF_HIP(solver::launch_assemble<float>(stream_, n_, nm_, N_, sloc32_.as<float>(), q, p, pmu_.as<double>(),
+                                               done));
+        NMF_HIP(solver::launch_pcg_alpha(stream_, st, pmu_.as<double>(), nb_));
+        NMF_HIP(solver::launch_pcg_update_xr<float>(stream_, N_, st, x, r, p, q, invd_.as<double>(),
+                                                    pa_.as<double>()));
+        NMF_HIP(solver::launch_pcg_beta(stream_, st, pa_.as<double>(), nb_));
+        NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd_.as<double>()));
+    }
+    NMF_HIP(hipStreamEndCapture(stream_, &g));
+    hipError_t e = hipGraphInstantiate(&graph_[2], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+}
+
+CGState HexSolver::run_pcg32(const double *b, double tol, int maxit, double rhs_mag)
+{
+    ensure_fp32();
+    CGState init{};
+    tol = std::max(tol, 1.0e-16);
+    init.tol2 = tol * tol;
+    init.maxit = maxit;
+    init.rhs_mag_override = rhs_mag;
+    *host_state_ = init;
+    NMF_HIP(hipMemcpyAsync(state_.get(), host_state_, sizeof(CGState), hipMemcpyHostToDevice, stream_));
+    CGState *st = state_.as<CGState>();
+    NMF_HIP(solver::launch_pcg_init<float>(stream_, N_, b, invd_.as<double>(), cg32_[0].as<float>(),
+                                           cg32_[1].as<float>(), cg32_[2].as<float>(), pa_.as<double>()));
+    NMF_HIP(solver::launch_pcg_scalar_init(stream_, st, pa_.as<double>(), nb_));
+    capture_pcg32();
+    const int max_replays = maxit / kItersPerGraph + 2;
+    for (int rep = 0; rep <= max_replays; ++rep) {
+        NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        NMF_HIP(hipStreamSynchronize(stream_));
+        if (host_state_->done) break;
+        NMF_HIP(hipGraphLaunch(graph_[2], stream_));
+    }
+    return *host_state_;
+}
+
 CGState HexSolver::run_cg(bool fp32, const double *b, double tol, int maxit)
 {
-    if (fp32) {
-        if (!geo32_.get()) {
-            const size_t g = geo64_.bytes() / sizeof(double);
-            geo32_.allocate(g * sizeof(float));
-            NMF_HIP(solver::launch_cast(stream_, g, geo64_.as<double>(), geo32_.as<float>()));
-        }
-        if (!cg32_[0].get()) {
-            for (auto &v : cg32_) v.allocate(NG_ * sizeof(float));
-            sloc32_.allocate((size_t)nelmt_ * nc_ * sizeof(float));
-        }
-    }
+    if (fp32) ensure_fp32();
     CGState init{};
     tol = std::max(tol, 1.0e-16);
     init.tol2 = tol * tol;
@@ -293,6 +358,9 @@ SolveInfo HexSolver::solve(bool mixed, double tol, int maxit, double inner_tol)
     if (!mixed) {
         CGState st = run_cg(false, b_.as<double>(), tol, maxit);
         NMF_HIP(hipMemcpyAsync(x_.get(), cg64_[0].get(), NG_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        if (verbose())
+            std::fprintf(stderr, "CG iterations made = %d using tolerance of %.1e (error = %.6e, rhs_mag = %.6e)\n",
+                         st.iter, tol, st.final_error, std::sqrt(st.rhs_mag));
         info.iterations = st.iter;
         info.outer_iterations = 1;
         info.converged = st.converged;
@@ -320,7 +388,7 @@ SolveInfo HexSolver::solve(bool mixed, double tol, int maxit, double inner_tol)
             // inner FP32 solve of A d = r64, no tighter than the outer target needs
             const double need = 0.5 * std::sqrt(tol2 * rhs_mag / rr);
             const double itol = std::max(inner_tol, need);
-            CGState st = run_cg(true, r64_.as<double>(), itol, maxit - total);
+            CGState st = run_pcg32(r64_.as<double>(), itol, maxit - total, rr);
             total += st.iter;
             ++outer;
             NMF_HIP(solver::launch_add<float>(stream_, NG_, x_.as<double>(), cg32_[0].as<float>()));
@@ -331,6 +399,11 @@ SolveInfo HexSolver::solve(bool mixed, double tol, int maxit, double inner_tol)
             NMF_HIP(hipMemcpyAsync(host_scalar_ + 1, dsc + 1, sizeof(double), hipMemcpyDeviceToHost, stream_));
             NMF_HIP(hipStreamSynchronize(stream_));
             rr = host_scalar_[1];
+            if (verbose())
+                std::fprintf(stderr,
+                             "IR-CG step %d: inner iterations %d (tol %.1e, inner error %.3e, converged %d), "
+                             "FP64 error = %.6e\n",
+                             outer, st.iter, itol, st.final_error, st.converged, std::sqrt(rr / rhs_mag));
             if (st.iter == 0) break;
         }
         info.iterations = total;

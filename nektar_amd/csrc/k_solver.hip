@@ -291,7 +291,7 @@ __global__ void cg_scalar_init_kernel(CGState *st, const double *__restrict__ pa
     __syncthreads();
     if (threadIdx.x != 0) return;
     const double eps = res[0], rho = res[1], bb = res[2], mu = res[3];
-    st->rhs_mag = bb > 1.0e-6 ? bb : 1.0;  // NekLinSysIter.cpp:116
+    st->rhs_mag = st->rhs_mag_override > 0.0 ? st->rhs_mag_override : (bb > 1.0e-6 ? bb : 1.0);  // NekLinSysIter.cpp:116
     st->eps = eps;
     st->iter = 0;
     st->converged = 0;
@@ -335,6 +335,117 @@ __global__ void cg_scalar_kernel(CGState *st, const double *__restrict__ pa, con
     st->rho = rho_new;
     st->mu = mu;
     if (st->iter > st->maxit) st->done = 1;  // :215 (checked at the loop top)
+}
+
+// ---- Hestenes-Stiefel PCG ------------------------------------------
+// r = b (non-Dirichlet), x = 0, p = z = M^{-1} r; partials {r.z, r.r, b.b}
+template <typename T>
+__global__ void pcg_init_kernel(int N, const double *__restrict__ b, const double *__restrict__ inv_diag,
+                                T *__restrict__ x, T *__restrict__ r, T *__restrict__ p, double *__restrict__ partials)
+{
+    const size_t NG = (size_t)N * N * N;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        const int gx = (int)(G % N), gy = (int)((G / N) % N), gz = (int)(G / ((size_t)N * N));
+        const double bv = b[G];
+        acc[2] += bv * bv;
+        const T rv = on_boundary(gx, gy, gz, N) ? T(0) : T(bv);
+        const double zv = inv_diag[G] * (double)rv;
+        x[G] = T(0);
+        r[G] = rv;
+        p[G] = T(zv);
+        acc[0] += (double)rv * zv;
+        acc[1] += (double)rv * (double)rv;
+    }
+    block_sum<3>(acc, partials + 3 * blockIdx.x);
+}
+
+__global__ void pcg_scalar_init_kernel(CGState *st, const double *__restrict__ pa, int nb)
+{
+    __shared__ double res[3];
+    reduce_partials(pa, 3, 0, nb, &res[0]);
+    __syncthreads();
+    reduce_partials(pa, 3, 1, nb, &res[1]);
+    __syncthreads();
+    reduce_partials(pa, 3, 2, nb, &res[2]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double rz = res[0], eps = res[1], bb = res[2];
+    st->rhs_mag = st->rhs_mag_override > 0.0 ? st->rhs_mag_override : (bb > 1.0e-6 ? bb : 1.0);
+    st->eps = eps;
+    st->rho = rz;
+    st->iter = 0;
+    st->converged = 0;
+    st->done = 0;
+    st->final_error = sqrt(eps / st->rhs_mag);
+    if (eps < st->tol2 * st->rhs_mag) {
+        st->done = 1;
+        st->converged = 1;
+    }
+}
+
+__global__ void pcg_alpha_kernel(CGState *st, const double *__restrict__ pmu, int nb)
+{
+    if (st->done) return;
+    __shared__ double res[1];
+    reduce_partials(pmu, 1, 0, nb, &res[0]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    st->mu = res[0];
+    st->alpha = st->rho / res[0];
+}
+
+// x += alpha p, r -= alpha q, partials {r.z, r.r} with z = M^{-1} r
+template <typename T>
+__global__ void pcg_update_xr_kernel(size_t NG, const CGState *__restrict__ st, T *__restrict__ x, T *__restrict__ r,
+                                     const T *__restrict__ p, const T *__restrict__ q,
+                                     const double *__restrict__ inv_diag, double *__restrict__ partials)
+{
+    if (st->done) return;
+    const T alpha = T(st->alpha);
+    double acc[2] = {0.0, 0.0};
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        x[G] = alpha * p[G] + x[G];
+        const T rv = -alpha * q[G] + r[G];
+        r[G] = rv;
+        acc[0] += (double)rv * (inv_diag[G] * (double)rv);
+        acc[1] += (double)rv * (double)rv;
+    }
+    block_sum<2>(acc, partials + 2 * blockIdx.x);
+}
+
+__global__ void pcg_beta_kernel(CGState *st, const double *__restrict__ pa, int nb)
+{
+    if (st->done) return;
+    __shared__ double res[2];
+    reduce_partials(pa, 2, 0, nb, &res[0]);
+    __syncthreads();
+    reduce_partials(pa, 2, 1, nb, &res[1]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double rz = res[0], eps = res[1];
+    st->iter += 1;
+    st->eps = eps;
+    st->final_error = sqrt(eps / st->rhs_mag);
+    if (eps < st->tol2 * st->rhs_mag) {
+        st->done = 1;
+        st->converged = 1;
+        return;
+    }
+    st->beta = rz / st->rho;
+    st->rho = rz;
+    if (st->iter >= st->maxit) st->done = 1;
+}
+
+// p = M^{-1} r + beta p
+template <typename T>
+__global__ void pcg_update_p_kernel(size_t NG, const CGState *__restrict__ st, const T *__restrict__ r,
+                                    T *__restrict__ p, const double *__restrict__ inv_diag)
+{
+    if (st->done) return;
+    const double beta = st->beta;
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x)
+        p[G] = T(inv_diag[G] * (double)r[G] + beta * (double)p[G]);
 }
 
 // r64 = b - s on non-Dirichlet rows (0 elsewhere); partial r.r
@@ -461,6 +572,62 @@ hipError_t launch_cg_scalar(hipStream_t stream, CGState *st, const double *pa, c
     NMF_LAUNCH(cg_scalar_kernel, 1, st, pa, pmu, nb);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_pcg_init(hipStream_t stream, int N, const double *b, const double *inv_diag, T *x, T *r, T *p,
+                           double *partials)
+{
+    NMF_LAUNCH((pcg_init_kernel<T>), grid_for((size_t)N * N * N), N, b, inv_diag, x, r, p, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_scalar_init(hipStream_t stream, CGState *st, const double *pa, int nb)
+{
+    NMF_LAUNCH(pcg_scalar_init_kernel, 1, st, pa, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_alpha(hipStream_t stream, CGState *st, const double *pmu, int nb)
+{
+    NMF_LAUNCH(pcg_alpha_kernel, 1, st, pmu, nb);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pcg_update_xr(hipStream_t stream, int N, const CGState *st, T *x, T *r, const T *p, const T *q,
+                                const double *inv_diag, double *partials)
+{
+    const size_t NG = (size_t)N * N * N;
+    NMF_LAUNCH((pcg_update_xr_kernel<T>), grid_for(NG), NG, st, x, r, p, q, inv_diag, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_beta(hipStream_t stream, CGState *st, const double *pa, int nb)
+{
+    NMF_LAUNCH(pcg_beta_kernel, 1, st, pa, nb);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pcg_update_p(hipStream_t stream, int N, const CGState *st, const T *r, T *p, const double *inv_diag)
+{
+    const size_t NG = (size_t)N * N * N;
+    NMF_LAUNCH((pcg_update_p_kernel<T>), grid_for(NG), NG, st, r, p, inv_diag);
+    return hipGetLastError();
+}
+
+template hipError_t launch_pcg_init<float>(hipStream_t, int, const double *, const double *, float *, float *, float *,
+                                           double *);
+template hipError_t launch_pcg_init<double>(hipStream_t, int, const double *, const double *, double *, double *,
+                                            double *, double *);
+template hipError_t launch_pcg_update_xr<float>(hipStream_t, int, const CGState *, float *, float *, const float *,
+                                                const float *, const double *, double *);
+template hipError_t launch_pcg_update_xr<double>(hipStream_t, int, const CGState *, double *, double *, const double *,
+                                                 const double *, const double *, double *);
+template hipError_t launch_pcg_update_p<float>(hipStream_t, int, const CGState *, const float *, float *,
+                                               const double *);
+template hipError_t launch_pcg_update_p<double>(hipStream_t, int, const CGState *, const double *, double *,
+                                                const double *);
 
 hipError_t launch_residual(hipStream_t stream, int N, const double *b, const double *s, double *r, double *partials)
 {

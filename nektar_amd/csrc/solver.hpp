@@ -21,6 +21,7 @@ constexpr int kRedBlocks = 2048;
 
 struct CGState {
     double alpha, beta, rho, mu, eps, rhs_mag, tol2, final_error;
+    double rhs_mag_override;  // > 0: use instead of ||b||^2 (inner solves)
     int iter, maxit, done, converged;
 };
 
@@ -43,6 +44,18 @@ hipError_t launch_cg_update(hipStream_t, int N, const CGState *st, T *p, T *q, T
 hipError_t launch_cg_scalar_init(hipStream_t, CGState *st, const double *pa, const double *pmu, int nb);
 hipError_t launch_cg_scalar(hipStream_t, CGState *st, const double *pa, const double *pmu, int nb);
 hipError_t launch_residual(hipStream_t, int N, const double *b, const double *s, double *r, double *partials);
+// Hestenes-Stiefel PCG (used for the FP32 inner solve of the mixed path)
+template <typename T>
+hipError_t launch_pcg_init(hipStream_t, int N, const double *b, const double *inv_diag, T *x, T *r, T *p,
+                           double *partials);
+hipError_t launch_pcg_scalar_init(hipStream_t, CGState *st, const double *pa, int nb);
+hipError_t launch_pcg_alpha(hipStream_t, CGState *st, const double *pmu, int nb);
+template <typename T>
+hipError_t launch_pcg_update_xr(hipStream_t, int N, const CGState *st, T *x, T *r, const T *p, const T *q,
+                                const double *inv_diag, double *partials);
+hipError_t launch_pcg_beta(hipStream_t, CGState *st, const double *pa, int nb);
+template <typename T>
+hipError_t launch_pcg_update_p(hipStream_t, int N, const CGState *st, const T *r, T *p, const double *inv_diag);
 template <typename T>
 hipError_t launch_add(hipStream_t, size_t NG, double *x, const T *d);
 hipError_t launch_invert_diag(hipStream_t, int N, double *d);
@@ -106,6 +119,10 @@ public:
     // CG on the current rhs_, stopping at tol; result in x (FP64 or FP32
     // buffers); returns the final state.  Public for the serving loop.
     solver::CGState run_cg(bool fp32, const double *b, double tol, int maxit);
+    // Hestenes-Stiefel PCG in FP32 (FP64 dots and diagonal) on rhs b, with
+    // the relative tolerance taken against rhs_mag; solution in the FP32
+    // x buffer.
+    solver::CGState run_pcg32(const double *b, double tol, int maxit, double rhs_mag);
 
 private:
     void build_geometry();
@@ -129,7 +146,9 @@ private:
     solver::CGState *host_state_ = nullptr;
     double *host_scalar_ = nullptr;
 
-    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_[3] = {nullptr, nullptr, nullptr};
+    void capture_pcg32();
+    void ensure_fp32();
     static constexpr int kItersPerGraph = 8;
 };
 
