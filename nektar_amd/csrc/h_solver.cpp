@@ -367,50 +367,7 @@ SolveInfo HexSolver::solve(bool mixed, double tol, int maxit, double inner_tol)
         info.final_error = st.final_error;
         info.rhs_magnitude = st.rhs_mag;
     } else {
-        // ||b||^2 over all global rows (Set_Rhs_Magnitude, NekLinSysIter.cpp:96-117)
-        NMF_HIP(hipMemsetAsync(tmp64_.get(), 0, NG_ * sizeof(double), stream_));
-        NMF_HIP(hipMemsetAsync(x_.get(), 0, NG_ * sizeof(double), stream_));
-        double *dsc = scalar_.as<double>();
-        NMF_HIP(solver::launch_sumsq(stream_, NG_, b_.as<double>(), pmu_.as<double>()));
-        NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc));
-        // r64 = b on non-Dirichlet rows (x = 0), ||r64||^2
-        NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
-                                        pmu_.as<double>()));
-        NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
-        NMF_HIP(hipMemcpyAsync(host_scalar_, dsc, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
-        NMF_HIP(hipStreamSynchronize(stream_));
-        double rr = host_scalar_[1];
-        const double bmag = host_scalar_[0];
-        const double rhs_mag = bmag > 1.0e-6 ? bmag : 1.0;
-        const double tol2 = std::max(tol, 1e-16) * std::max(tol, 1e-16);
-        int total = 0, outer = 0;
-        while (rr >= tol2 * rhs_mag && total < maxit && outer < 50) {
-            // inner FP32 solve of A d = r64, no tighter than the outer target needs
-            const double need = 0.5 * std::sqrt(tol2 * rhs_mag / rr);
-            const double itol = std::max(inner_tol, need);
-            CGState st = run_pcg32(r64_.as<double>(), itol, maxit - total, rr);
-            total += st.iter;
-            ++outer;
-            NMF_HIP(solver::launch_add<float>(stream_, NG_, x_.as<double>(), cg32_[0].as<float>()));
-            apply_global(x_.as<double>(), tmp64_.as<double>());
-            NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
-                                            pmu_.as<double>()));
-            NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
-            NMF_HIP(hipMemcpyAsync(host_scalar_ + 1, dsc + 1, sizeof(double), hipMemcpyDeviceToHost, stream_));
-            NMF_HIP(hipStreamSynchronize(stream_));
-            rr = host_scalar_[1];
-            if (verbose())
-                std::fprintf(stderr,
-                             "IR-CG step %d: inner iterations %d (tol %.1e, inner error %.3e, converged %d), "
-                             "FP64 error = %.6e\n",
-                             outer, st.iter, itol, st.final_error, st.converged, std::sqrt(rr / rhs_mag));
-            if (st.iter == 0) break;
-        }
-        info.iterations = total;
-        info.outer_iterations = outer;
-        info.converged = rr < tol2 * rhs_mag;
-        info.final_error = std::sqrt(rr / rhs_mag);
-        info.rhs_magnitude = rhs_mag;
+        info = solve_mixed(tol, maxit, 0.1);
     }
     NMF_HIP(hipEventRecord(e1, stream_));
     NMF_HIP(hipEventSynchronize(e1));
@@ -419,6 +376,74 @@ SolveInfo HexSolver::solve(bool mixed, double tol, int maxit, double inner_tol)
     info.solve_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return info;
+}
+
+SolveInfo HexSolver::solve_mixed(double tol, int maxit, double delta)
+{
+    SolveInfo info;
+    ensure_fp32();
+    float *y = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>();
+    double *dsc = scalar_.as<double>();
+    // x64 = 0, r64 = b on non-Dirichlet rows; ||b||^2 over all global rows
+    // (Set_Rhs_Magnitude, NekLinSysIter.cpp:96-117) and ||r64||^2.
+    NMF_HIP(solver::launch_zero<double>(stream_, NG_, x_.as<double>()));
+    NMF_HIP(solver::launch_zero<double>(stream_, NG_, tmp64_.as<double>()));
+    NMF_HIP(solver::launch_sumsq(stream_, NG_, b_.as<double>(), pmu_.as<double>()));
+    NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc));
+    NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
+                                    pmu_.as<double>()));
+    NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
+    NMF_HIP(hipMemcpyAsync(host_scalar_, dsc, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    NMF_HIP(hipStreamSynchronize(stream_));
+    const double bmag = host_scalar_[0], rr0 = host_scalar_[1];
+    const double rhs_mag = bmag > 1.0e-6 ? bmag : 1.0;
+    tol = std::max(tol, 1.0e-16);
+
+    CGState init{};
+    init.tol2 = tol * tol;
+    init.maxit = maxit;
+    init.rhs_mag_override = rhs_mag;
+    init.delta2 = delta * delta;
+    init.rr_ref = rr0;
+    *host_state_ = init;
+    NMF_HIP(hipMemcpyAsync(state_.get(), host_state_, sizeof(CGState), hipMemcpyHostToDevice, stream_));
+    CGState *st = state_.as<CGState>();
+    NMF_HIP(solver::launch_pcg_init<float>(stream_, N_, r64_.as<double>(), invd_.as<double>(), y, r, p,
+                                           pa_.as<double>()));
+    NMF_HIP(solver::launch_pcg_scalar_init(stream_, st, pa_.as<double>(), nb_));
+    capture_pcg32();
+    const int max_rounds = maxit / kItersPerGraph + 64 + maxit;
+    for (int round = 0; round < max_rounds; ++round) {
+        NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+        NMF_HIP(hipStreamSynchronize(stream_));
+        if (host_state_->done) break;
+        if (!host_state_->pause) {
+            NMF_HIP(hipGraphLaunch(graph_[2], stream_));
+            continue;
+        }
+        // reliable update: accumulate in FP64, recompute the true residual
+        // with the FP64 operator, restart the recurrence residual from it
+        NMF_HIP(solver::launch_add<float>(stream_, NG_, x_.as<double>(), y));
+        NMF_HIP(solver::launch_zero<float>(stream_, NG_, y));
+        apply_global(x_.as<double>(), tmp64_.as<double>());
+        NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
+                                        pmu_.as<double>()));
+        NMF_HIP(solver::launch_ru_resume(stream_, N_, r64_.as<double>(), r, invd_.as<double>(), pa_.as<double>()));
+        NMF_HIP(solver::launch_ru_scalar(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
+        NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd_.as<double>()));
+        if (verbose()) {
+            NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+            NMF_HIP(hipStreamSynchronize(stream_));
+            std::fprintf(stderr, "mixed CG reliable update %d at iteration %d: FP64 error = %.6e\n",
+                         host_state_->updates, host_state_->iter, host_state_->final_error);
+        }
+    }
+    info.iterations = host_state_->iter;
+    info.outer_iterations = host_state_->updates;
+    info.converged = host_state_->converged;
+    info.final_error = host_state_->final_error;
+    info.rhs_magnitude = rhs_mag;
     return info;
 }
 

@@ -386,7 +386,7 @@ __global__ void pcg_scalar_init_kernel(CGState *st, const double *__restrict__ p
 
 __global__ void pcg_alpha_kernel(CGState *st, const double *__restrict__ pmu, int nb)
 {
-    if (st->done) return;
+    if (st->done || st->pause) return;
     __shared__ double res[1];
     reduce_partials(pmu, 1, 0, nb, &res[0]);
     __syncthreads();
@@ -401,7 +401,7 @@ __global__ void pcg_update_xr_kernel(size_t NG, const CGState *__restrict__ st, 
                                      const T *__restrict__ p, const T *__restrict__ q,
                                      const double *__restrict__ inv_diag, double *__restrict__ partials)
 {
-    if (st->done) return;
+    if (st->done || st->pause) return;
     const T alpha = T(st->alpha);
     double acc[2] = {0.0, 0.0};
     for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
@@ -416,7 +416,7 @@ __global__ void pcg_update_xr_kernel(size_t NG, const CGState *__restrict__ st, 
 
 __global__ void pcg_beta_kernel(CGState *st, const double *__restrict__ pa, int nb)
 {
-    if (st->done) return;
+    if (st->done || st->pause) return;
     __shared__ double res[2];
     reduce_partials(pa, 2, 0, nb, &res[0]);
     __syncthreads();
@@ -427,6 +427,17 @@ __global__ void pcg_beta_kernel(CGState *st, const double *__restrict__ pa, int 
     st->iter += 1;
     st->eps = eps;
     st->final_error = sqrt(eps / st->rhs_mag);
+    if (st->delta2 > 0.0) {
+        // reliable-update mode: convergence is only declared on a true
+        // residual, so a small recurrence residual suspends the iterations
+        if (eps < st->tol2 * st->rhs_mag || eps < st->delta2 * st->rr_ref || st->iter >= st->maxit) {
+            st->pause = 1;  // beta and rho are then formed from the true residual
+        } else {
+            st->beta = rz / st->rho;
+            st->rho = rz;
+        }
+        return;
+    }
     if (eps < st->tol2 * st->rhs_mag) {
         st->done = 1;
         st->converged = 1;
@@ -442,10 +453,63 @@ template <typename T>
 __global__ void pcg_update_p_kernel(size_t NG, const CGState *__restrict__ st, const T *__restrict__ r,
                                     T *__restrict__ p, const double *__restrict__ inv_diag)
 {
-    if (st->done) return;
+    if (st->done || st->pause) return;
     const double beta = st->beta;
     for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x)
         p[G] = T(inv_diag[G] * (double)r[G] + beta * (double)p[G]);
+}
+
+// reliable update, part 1: r32 = r64 on non-Dirichlet rows; partial r.z
+__global__ void ru_resume_kernel(int N, const double *__restrict__ r64, float *__restrict__ r32,
+                                 const double *__restrict__ inv_diag, double *__restrict__ partials)
+{
+    const size_t NG = (size_t)N * N * N;
+    double acc[1] = {0.0};
+    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
+        const float rv = (float)r64[G];  // r64 is 0 on Dirichlet rows
+        r32[G] = rv;
+        acc[0] += (double)rv * (inv_diag[G] * (double)rv);
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+// reliable update, part 2: true residual test; beta restarted from r.z of
+// the true residual (p = z + beta p follows in pcg_update_p).
+__global__ void ru_scalar_kernel(CGState *st, const double *__restrict__ pa, const double *__restrict__ prr, int nb)
+{
+    __shared__ double res[2];
+    reduce_partials(pa, 1, 0, nb, &res[0]);
+    __syncthreads();
+    reduce_partials(prr, 1, 0, nb, &res[1]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double rz = res[0], rr = res[1];
+    st->rr_true = rr;
+    st->rr_ref = rr;
+    st->eps = rr;
+    st->updates += 1;
+    st->final_error = sqrt(rr / st->rhs_mag);
+    if (rr < st->tol2 * st->rhs_mag) {
+        st->done = 1;
+        st->converged = 1;
+        st->pause = 0;
+        return;
+    }
+    if (st->iter >= st->maxit) {
+        st->done = 1;
+        st->pause = 0;
+        return;
+    }
+    st->beta = rz / st->rho;
+    st->rho = rz;
+    st->pause = 0;
+}
+
+template <typename T>
+__global__ void zero_kernel(size_t n, T *__restrict__ a)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = T(0);
 }
 
 // r64 = b - s on non-Dirichlet rows (0 elsewhere); partial r.r
@@ -628,6 +692,28 @@ template hipError_t launch_pcg_update_p<float>(hipStream_t, int, const CGState *
                                                const double *);
 template hipError_t launch_pcg_update_p<double>(hipStream_t, int, const CGState *, const double *, double *,
                                                 const double *);
+
+hipError_t launch_ru_resume(hipStream_t stream, int N, const double *r64, float *r32, const double *inv_diag,
+                            double *partials)
+{
+    NMF_LAUNCH(ru_resume_kernel, grid_for((size_t)N * N * N), N, r64, r32, inv_diag, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_ru_scalar(hipStream_t stream, CGState *st, const double *pa, const double *prr, int nb)
+{
+    NMF_LAUNCH(ru_scalar_kernel, 1, st, pa, prr, nb);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_zero(hipStream_t stream, size_t n, T *a)
+{
+    NMF_LAUNCH((zero_kernel<T>), grid_for(n), n, a);
+    return hipGetLastError();
+}
+template hipError_t launch_zero<float>(hipStream_t, size_t, float *);
+template hipError_t launch_zero<double>(hipStream_t, size_t, double *);
 
 hipError_t launch_residual(hipStream_t stream, int N, const double *b, const double *s, double *r, double *partials)
 {

@@ -21,8 +21,13 @@ constexpr int kRedBlocks = 2048;
 
 struct CGState {
     double alpha, beta, rho, mu, eps, rhs_mag, tol2, final_error;
-    double rhs_mag_override;  // > 0: use instead of ||b||^2 (inner solves)
+    double rhs_mag_override;  // > 0: use instead of ||b||^2
+    double delta2;            // reliable update when ||r||^2 < delta2 * rr_ref (0: never)
+    double rr_ref;            // ||r||^2 of the true residual at the last update
+    double rr_true;           // scratch: true residual norm^2 from the FP64 recomputation
     int iter, maxit, done, converged;
+    int pause;                // 1: iterations suspended for a reliable update
+    int updates;              // number of reliable updates performed
 };
 
 int reduction_blocks(size_t NG);
@@ -56,6 +61,13 @@ hipError_t launch_pcg_update_xr(hipStream_t, int N, const CGState *st, T *x, T *
 hipError_t launch_pcg_beta(hipStream_t, CGState *st, const double *pa, int nb);
 template <typename T>
 hipError_t launch_pcg_update_p(hipStream_t, int N, const CGState *st, const T *r, T *p, const double *inv_diag);
+// reliable update: r32 = r64 (non-Dirichlet), partial r.z; then the scalar
+// step that tests the true residual and restarts beta from it
+hipError_t launch_ru_resume(hipStream_t, int N, const double *r64, float *r32, const double *inv_diag,
+                            double *partials);
+hipError_t launch_ru_scalar(hipStream_t, CGState *st, const double *pa, const double *prr, int nb);
+template <typename T>
+hipError_t launch_zero(hipStream_t, size_t n, T *a);
 template <typename T>
 hipError_t launch_add(hipStream_t, size_t NG, double *x, const T *d);
 hipError_t launch_invert_diag(hipStream_t, int N, double *d);
@@ -123,6 +135,9 @@ public:
     // the relative tolerance taken against rhs_mag; solution in the FP32
     // x buffer.
     solver::CGState run_pcg32(const double *b, double tol, int maxit, double rhs_mag);
+    // mixed precision: FP32 PCG with reliable updates (FP64 residual
+    // recomputation and FP64 solution accumulation)
+    SolveInfo solve_mixed(double tol, int maxit, double delta);
 
 private:
     void build_geometry();

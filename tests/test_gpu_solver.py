@@ -83,14 +83,21 @@ def test_fp64_cg_matches_oracle(nmf, n, nm, nq, lam, amp, seed):
 
 @pytest.mark.parametrize("n,nm,nq,lam,amp,seed", CASES)
 def test_mixed_precision_cg(nmf, n, nm, nq, lam, amp, seed):
+    """Mixed FP32/FP64 solve reaches the same 1e-10 FP64 residual; both it
+    and the FP64 CG solution lie within the residual-implied distance of a
+    1e-13 FP64 solve."""
     s = nmf.HelmholtzSolver(n, nm, nq, lam, amp, seed)
     s.set_forcing_manufactured(1)
+    s.solve(mixed=False, tol=1e-13)
+    x_tight = s.solution()
     i64 = s.solve(mixed=False, tol=1e-10)
     x64 = s.solution()
     imx = s.solve(mixed=True, tol=1e-10)
     assert imx.converged and imx.final_error < 1e-10
-    assert imx.outer_iterations >= 2
-    assert _rel(s.solution(), x64) <= 1e-8
+    assert imx.outer_iterations >= 2  # reliable updates (FP64 residual recomputations)
+    assert _rel(s.solution(), x_tight) <= 1e-7
+    assert _rel(x64, x_tight) <= 1e-7
+    assert imx.iterations <= 2 * i64.iterations + 10
 
 
 def test_solve_is_bitwise_reproducible(nmf):
