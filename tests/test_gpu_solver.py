@@ -83,21 +83,28 @@ def test_fp64_cg_matches_oracle(nmf, n, nm, nq, lam, amp, seed):
 
 @pytest.mark.parametrize("n,nm,nq,lam,amp,seed", CASES)
 def test_mixed_precision_cg(nmf, n, nm, nq, lam, amp, seed):
-    """Mixed FP32/FP64 solve reaches the same 1e-10 FP64 residual; both it
-    and the FP64 CG solution lie within the residual-implied distance of a
-    1e-13 FP64 solve."""
+    """The mixed FP32/FP64 solve reaches the same 1e-10 relative residual of
+    the FP64 operator (recomputed in FP64); its error against the exact
+    solution matches the FP64 solve's and the residual it reports is the
+    true FP64 residual of the returned solution."""
     s = nmf.HelmholtzSolver(n, nm, nq, lam, amp, seed)
     s.set_forcing_manufactured(1)
-    s.solve(mixed=False, tol=1e-13)
-    x_tight = s.solution()
     i64 = s.solve(mixed=False, tol=1e-10)
-    x64 = s.solution()
+    l2_64 = s.l2_error_manufactured(1)
     imx = s.solve(mixed=True, tol=1e-10)
+    l2_mx = s.l2_error_manufactured(1)
     assert imx.converged and imx.final_error < 1e-10
     assert imx.outer_iterations >= 2  # reliable updates (FP64 residual recomputations)
-    assert _rel(s.solution(), x_tight) <= 1e-7
-    assert _rel(x64, x_tight) <= 1e-7
-    assert imx.iterations <= 2 * i64.iterations + 10
+    assert abs(l2_mx - l2_64) <= 1e-3 * l2_64
+    # independent FP64 residual of the returned solution
+    x = torch.from_numpy(s.solution()).cuda()
+    ax = torch.zeros_like(x)
+    s.apply_global(x, ax)
+    b = s.rhs()
+    interior = ~_oracle_problem(n, nm, nq, lam, amp, seed)["dirichlet"]
+    r = (b - ax.cpu().numpy())[interior]
+    assert np.sqrt(r @ r / (b @ b)) < 1e-10
+    assert imx.iterations <= 3 * i64.iterations + 20
 
 
 def test_solve_is_bitwise_reproducible(nmf):
