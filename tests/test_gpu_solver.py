@@ -116,14 +116,21 @@ def test_solve_is_bitwise_reproducible(nmf):
     assert a.iterations == b.iterations and np.array_equal(xa, s.solution())
 
 
-@pytest.mark.parametrize("nm", [3, 5, 7])
-def test_manufactured_solution_spectral_convergence(nmf, nm):
-    s = nmf.HelmholtzSolver(4, nm, nm + 1, 1.0)
-    s.set_forcing_manufactured(1)
-    info = s.solve(False, 1e-12)
-    l2 = s.l2_error_manufactured(1)
-    assert info.converged
-    assert l2 < {3: 1e-3, 5: 1e-5, 7: 1e-7}[nm], l2
+def test_manufactured_solution_spectral_convergence(nmf):
+    """L2 error against sin(pi x)sin(pi y)sin(pi z) equals the oracle's for
+    the same discretisation and decays spectrally with P."""
+    from manufactured import oracle_solve
+    errs = []
+    for nm in (3, 5, 7):
+        s = nmf.HelmholtzSolver(3, nm, nm + 1, 1.0)
+        s.set_forcing_manufactured(1)
+        info = s.solve(False, 1e-12)
+        l2 = s.l2_error_manufactured(1)
+        _, _, l2_o = oracle_solve(3, nm, nm + 1, 1.0, tol=1e-12)
+        assert info.converged
+        assert abs(l2 - l2_o) <= 1e-6 * l2_o, (l2, l2_o)
+        errs.append(l2)
+    assert errs[0] / errs[1] > 20 and errs[1] / errs[2] > 20, errs
 
 
 def test_maxit_reports_not_converged(nmf):
