@@ -144,16 +144,17 @@ def main():
         op.SetJac(jac)
         op.SetDF(df)
         op.SetLambda(lam)
+        tile, tile_us = op.autotune()  # elements per workgroup, measured on this mesh
         dt = torch.float64 if prec == N.FP64 else torch.float32
         g = torch.Generator(device="cuda").manual_seed(2)
         x = (torch.rand(op.input_size(), dtype=dt, device="cuda", generator=g) * 2 - 1).contiguous()
         y = torch.zeros(op.output_size(), dtype=dt, device="cuda")
         ms, wall = time_apply(op, x, y, args.steps, args.warmup, torch)
-        results[tag] = (ms, wall)
+        results[tag] = (ms, wall, tile, tile_us)
         del op, x, y
 
-    ms64, wall64 = results["fp64"]
-    ms32, _ = results["fp32"]
+    ms64, wall64, tile64, tiles64 = results["fp64"]
+    ms32, _, tile32, tiles32 = results["fp32"]
     ndof = ne * nm ** 3
     F = hex_flops_per_elem(nm, nq) * ne
     value = ndof / (ms64 * 1e-3) / 1e9
@@ -161,7 +162,7 @@ def main():
     traffic = read_traffic("helmholtz_fp64")
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "kernel": "hex::helmholtz_kernel<double,7,8,false>",
+                "kernel": f"hex::helmholtz_kernel<double,7,8,false,{tile64}>",
                 "flops_per_launch": F,
                 "alg_bytes_per_launch": hex_bytes_per_elem(nm, nq, False, 8) * ne}
 
@@ -171,10 +172,14 @@ def main():
         "apply_fp32_over_fp64": round(ms64 / ms32, 3),
         "apply_fp64_us": round(ms64 * 1e3, 2),
         "apply_fp32_us": round(ms32 * 1e3, 2),
+        "tile_fp64": tile64, "tile_fp32": tile32,
+        "autotune_us_fp64": {str(k): round(v, 2) for k, v in tiles64.items()},
+        "autotune_us_fp32": {str(k): round(v, 2) for k, v in tiles32.items()},
     }
     if not args.no_solve:
         s = N.HelmholtzSolver(n_side, nm, nq, lam)
         s.set_forcing_manufactured(1)
+        extra["solver_tiles"] = list(s.autotune())
         nunk = s.nglobal - s.ndirichlet
         for mixed, tag in ((False, "fp64"), (True, "mixed")):
             s.solve(mixed=mixed, tol=1e-10)  # warm (graph capture, FP32 buffers)

@@ -120,6 +120,27 @@ int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in,
                                       double *out2);
 
 /* ---------------------------------------------------------------------
+ * Workgroup tiles and the autotuner.  No counterpart in the reference,
+ * whose kernels have one shape per vector width (VecDataType in
+ * MatrixFreeOps/Operator.hpp:56-57); here the Helmholtz kernels are
+ * instantiated for several tiles (elements per workgroup) and the
+ * fastest is measured per (key, nummodes, numpoints, precision) on the
+ * caller's data.  The choice is remembered for the rest of the process:
+ * operators and solvers created later for the same kind start from it.
+ * tile 0 = the library default.  Results do not depend on the tile.
+ * ------------------------------------------------------------------- */
+/* Tiles available to op (count 0 when its kernel has a single shape). */
+int nmf_operator_tile_candidates(nmf_operator op, int *tiles, int max,
+                                 int *count);
+int nmf_operator_set_tile(nmf_operator op, int tile);
+int nmf_operator_get_tile(nmf_operator op, int *tile);
+/* Time every candidate on op's geometry (allocates scratch arrays of the
+ * operator's size), keep the fastest; times_us[i] (may be NULL, else
+ * `max` entries) receives the mean launch time of candidate i. */
+int nmf_operator_autotune(nmf_operator op, void *stream, int *tile,
+                          double *times_us, int max);
+
+/* ---------------------------------------------------------------------
  * C0 Helmholtz solve on a structured hex mesh of [0,1]^3
  * (ContField::HelmSolve -> GlobalLinSysIterativeFull -> NekLinSysIterCG with
  *  the Diagonal preconditioner; ContField.cpp:797-925,
@@ -167,6 +188,11 @@ int nmf_solver_get_rhs(nmf_solver s, double *b_host);
 int nmf_solver_get_inv_diag(nmf_solver s, double *d_host);
 /* y = A x for global device vectors (FP64), Dirichlet rows set to 0. */
 int nmf_solver_apply_global(nmf_solver s, const double *x_dev, double *y_dev);
+/* Autotune the solver's gather-apply kernel (FP64 and FP32 iterations);
+ * iteration graphs are re-captured on the next solve. */
+int nmf_solver_autotune(nmf_solver s, int *tile64, int *tile32);
+/* Forget every tile the autotuner recorded in this process. */
+int nmf_clear_tuning(void);
 /* The mesh generator used by nmf_solver_create_hex_box: (n+1)^3*3
  * vertex coordinates, vertex (ix,iy,iz) at ((iz*(n+1)+iy)*(n+1)+ix)*3. */
 int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz);

@@ -27,7 +27,7 @@ except ImportError:  # pragma: no cover - torch is optional for host-only use
 import numpy as np
 
 __all__ = [
-    "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring",
+    "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring", "clear_tuning",
     "basis_tables", "FP64", "FP32", "HelmholtzSolver", "SolveInfo", "hex_box_vertices",
 ]
 
@@ -86,6 +86,12 @@ _SIGNATURES = {
     "nmf_operator_apply_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "nmf_operator_apply_host": (ctypes.c_int, [_P, _D, _D]),
     "nmf_operator_apply_physderiv_host": (ctypes.c_int, [_P, _D, _D, _D, _D]),
+    "nmf_operator_tile_candidates": (ctypes.c_int, [_P, _I, ctypes.c_int, _I]),
+    "nmf_operator_set_tile": (ctypes.c_int, [_P, ctypes.c_int]),
+    "nmf_operator_get_tile": (ctypes.c_int, [_P, _I]),
+    "nmf_operator_autotune": (ctypes.c_int, [_P, _P, _I, _D, ctypes.c_int]),
+    "nmf_solver_autotune": (ctypes.c_int, [_P, _I, _I]),
+    "nmf_clear_tuning": (ctypes.c_int, []),
     "nmf_solver_create_hex_box": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                  ctypes.c_uint64, ctypes.c_double, _P, ctypes.POINTER(_P)]),
     "nmf_solver_destroy": (ctypes.c_int, [_P]),
@@ -242,6 +248,35 @@ class Operator:
         else:
             check(lib().nmf_operator_apply(self._h, pin, _device_ptr(out, dt, self.output_size()), s))
 
+    # --- workgroup tiles ---------------------------------------------------
+    def tile_candidates(self) -> list:
+        buf = (ctypes.c_int * 16)()
+        n = ctypes.c_int()
+        check(lib().nmf_operator_tile_candidates(self._h, buf, 16, ctypes.byref(n)))
+        return list(buf[:min(n.value, 16)])
+
+    @property
+    def tile(self) -> int:
+        t = ctypes.c_int()
+        check(lib().nmf_operator_get_tile(self._h, ctypes.byref(t)))
+        return t.value
+
+    @tile.setter
+    def tile(self, t: int) -> None:
+        check(lib().nmf_operator_set_tile(self._h, int(t)))
+
+    def autotune(self, stream=None):
+        """Time every tile on this operator's geometry and keep the fastest;
+        returns (tile, {tile: mean microseconds})."""
+        cand = self.tile_candidates()
+        times = (ctypes.c_double * max(1, len(cand)))()
+        t = ctypes.c_int()
+        s = stream
+        if s is None and torch is not None:
+            s = torch.cuda.current_stream().cuda_stream
+        check(lib().nmf_operator_autotune(self._h, s, ctypes.byref(t), times, len(cand)))
+        return t.value, {c: times[i] for i, c in enumerate(cand)}
+
     def apply_host(self, inp: np.ndarray):
         """Host FP64 in / out (synchronous); returns ndarray(s)."""
         x = np.ascontiguousarray(inp, dtype=np.float64)
@@ -269,6 +304,11 @@ _FACTORY = _Factory()
 
 def GetOperatorFactory() -> _Factory:
     return _FACTORY
+
+
+def clear_tuning() -> None:
+    """Forget every tile the autotuner recorded in this process."""
+    check(lib().nmf_clear_tuning())
 
 
 def hex_box_vertices(n: int, perturb: float = 0.0, seed: int = 0) -> np.ndarray:
@@ -320,6 +360,13 @@ class HelmholtzSolver:
         info = SolveInfo()
         check(lib().nmf_solver_solve(self._h, int(mixed), tol, maxit, ctypes.byref(info)))
         return info
+
+    def autotune(self):
+        """Choose the gather-apply tile for FP64 and FP32 iterations;
+        returns (tile64, tile32)."""
+        t64, t32 = ctypes.c_int(), ctypes.c_int()
+        check(lib().nmf_solver_autotune(self._h, ctypes.byref(t64), ctypes.byref(t32)))
+        return t64.value, t32.value
 
     def l2_error_manufactured(self, k: int = 1) -> float:
         v = ctypes.c_double()

@@ -2,6 +2,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "basis.hpp"
 #include "common.hpp"
@@ -235,6 +236,48 @@ int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in, double 
 
 #define NMF_SOLVER_CHECK() NMF_REQUIRE(s && s->impl, NMF_ERR_INVALID, "null solver handle")
 
+int nmf_operator_tile_candidates(nmf_operator op, int *tiles, int max, int *count)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(count, NMF_ERR_INVALID, "null count");
+        NMF_REQUIRE(max >= 0 && (tiles || max == 0), NMF_ERR_INVALID, "null tiles array");
+        const auto c = op->impl->tile_candidates();
+        *count = (int)c.size();
+        for (int i = 0; i < (int)c.size() && i < max; ++i) tiles[i] = c[i];
+    });
+}
+
+int nmf_operator_set_tile(nmf_operator op, int tile)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        op->impl->set_tile(tile);
+    });
+}
+
+int nmf_operator_get_tile(nmf_operator op, int *tile)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(tile, NMF_ERR_INVALID, "null tile");
+        *tile = op->impl->tile;
+    });
+}
+
+int nmf_operator_autotune(nmf_operator op, void *stream, int *tile, double *times_us, int max)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(!times_us || max >= 0, NMF_ERR_INVALID, "negative max");
+        std::vector<double> t;
+        const int best = op->impl->autotune(static_cast<hipStream_t>(stream), &t);
+        if (tile) *tile = best;
+        if (times_us)
+            for (int i = 0; i < (int)t.size() && i < max; ++i) times_us[i] = t[i];
+    });
+}
+
 int nmf_solver_create_hex_box(int n, int nummodes, int numpoints0, double perturb, uint64_t seed, double lambda,
                               void *stream, nmf_solver *out)
 {
@@ -356,6 +399,19 @@ int nmf_solver_apply_global(nmf_solver s, const double *x_dev, double *y_dev)
         s->impl->apply_global(x_dev, y_dev);
         NMF_HIP(hipStreamSynchronize(s->impl->stream()));
     });
+}
+
+int nmf_solver_autotune(nmf_solver s, int *tile64, int *tile32)
+{
+    return guard([&] {
+        NMF_REQUIRE(s && s->impl, NMF_ERR_INVALID, "null solver handle");
+        s->impl->autotune(tile64, tile32);
+    });
+}
+
+int nmf_clear_tuning(void)
+{
+    return guard([&] { clear_tuned_tiles(); });
 }
 
 int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz)

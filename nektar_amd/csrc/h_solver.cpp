@@ -131,6 +131,8 @@ HexSolver::HexSolver(const HexBoxMesh &mesh, int nummodes, int numpoints, double
 
     build_geometry();
     build_diagonal();
+    tile64_ = tuned_tile(TuneKey{(int)Shape::Hex, (int)OpType::Helmholtz, nm_, nq_, 0, deformed_ ? 1 : 0, 1});
+    tile32_ = tuned_tile(TuneKey{(int)Shape::Hex, (int)OpType::Helmholtz, nm_, nq_, 1, deformed_ ? 1 : 0, 1});
     NMF_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -203,8 +205,41 @@ void HexSolver::set_forcing_manufactured(int k)
 void HexSolver::apply_local(bool fp32, const void *xg, void *sloc, const int *done)
 {
     SolverArgs a{SolverKernel::Apply, fp32, deformed_, n_, N_, fp32 ? tab32_.get() : tab64_.get(), nullptr,
-                 fp32 ? geo32_.get() : geo64_.get(), lambda_, xg, sloc, done, stream_};
+                 fp32 ? geo32_.get() : geo64_.get(), lambda_, xg, sloc, done, stream_, fp32 ? tile32_ : tile64_};
     NMF_HIP(launch_hex_solver(nm_, nq_, a));
+}
+
+void HexSolver::autotune(int *tile64, int *tile32, std::vector<double> *times64, std::vector<double> *times32)
+{
+    ensure_fp32();
+    for (int prec = 0; prec < 2; ++prec) {
+        const bool fp32 = prec == 1;
+        int buf[8];
+        const int nc = hex_tile_candidates(nm_, nq_, fp32, buf, 8);
+        std::vector<int> cand(buf, buf + nc);
+        // a zeroed global vector of the right precision serves as input
+        void *xg = fp32 ? cg32_[2].get() : tmp64_.get();
+        NMF_HIP(hipMemsetAsync(xg, 0, NG_ * (fp32 ? sizeof(float) : sizeof(double)), stream_));
+        void *sloc = fp32 ? sloc32_.get() : sloc64_.get();
+        SolverArgs a{SolverKernel::Apply, fp32, deformed_, n_, N_, fp32 ? tab32_.get() : tab64_.get(), nullptr,
+                     fp32 ? geo32_.get() : geo64_.get(), lambda_, xg, sloc, nullptr, stream_, 0};
+        const int best = time_tiles(
+            stream_, cand,
+            [&](int t) {
+                a.tile = t;
+                return launch_hex_solver(nm_, nq_, a);
+            },
+            10, fp32 ? times32 : times64);
+        (fp32 ? tile32_ : tile64_) = best;
+        record_tile(TuneKey{(int)Shape::Hex, (int)OpType::Helmholtz, nm_, nq_, prec, deformed_ ? 1 : 0, 1}, best);
+    }
+    for (auto &g : graph_)
+        if (g) {
+            NMF_HIP(hipGraphExecDestroy(g));
+            g = nullptr;
+        }
+    if (tile64) *tile64 = tile64_;
+    if (tile32) *tile32 = tile32_;
 }
 
 void HexSolver::apply_global(const double *x, double *y)

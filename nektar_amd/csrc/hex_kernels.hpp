@@ -40,6 +40,22 @@ constexpr int elements_per_block()
     return (256 / (NQ * NQ)) > 0 ? (256 / (NQ * NQ)) : 1;
 }
 
+// LDS row pitch (in values) of the per-element slabs.  Rows of an even
+// number of doubles put the lanes of a line-wise access (lane stride = one
+// row) on the same few banks; an odd pitch spreads them over all banks.
+template <int NQ>
+constexpr int pitch()
+{
+    return NQ % 2 == 0 ? NQ + 1 : NQ;
+}
+
+// values per slab: NQ*NQ rows of pitch<NQ>() (rows index (k,j), (r,q), ...)
+template <int NQ>
+constexpr int slab()
+{
+    return NQ * NQ * pitch<NQ>();
+}
+
 template <typename T, int NM, int NQ>
 struct Tab {
     const T *__restrict__ base;
@@ -78,6 +94,7 @@ template <typename T, int NM, int NQ, typename Fetch>
 __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const Fetch &xin, bool valid, int t,
                                               T *A, T *Bs, T (&u)[NQ])
 {
+    constexpr int P = pitch<NQ>();
     // F1: contract p -> i along lines (q, r)
     if (t < NM * NM) {
         const int q = t % NM, r = t / NM;
@@ -89,7 +106,7 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
             T s = x[0] * tab.B(0, i);
 #pragma unroll
             for (int p = 1; p < NM; ++p) s = fma(x[p], tab.B(p, i), s);
-            A[(r * NM + q) * NQ + i] = s;
+            A[(r * NM + q) * P + i] = s;
         }
     }
     __syncthreads();
@@ -98,13 +115,13 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
         const int i = t % NQ, r = t / NQ;
         T a[NM];
 #pragma unroll
-        for (int q = 0; q < NM; ++q) a[q] = A[(r * NM + q) * NQ + i];
+        for (int q = 0; q < NM; ++q) a[q] = A[(r * NM + q) * P + i];
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
             T s = a[0] * tab.B(0, j);
 #pragma unroll
             for (int q = 1; q < NM; ++q) s = fma(a[q], tab.B(q, j), s);
-            Bs[(r * NQ + j) * NQ + i] = s;
+            Bs[(r * NQ + j) * P + i] = s;
         }
     }
     __syncthreads();
@@ -113,7 +130,7 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
         const int i = t % NQ, j = t / NQ;
         T b[NM];
 #pragma unroll
-        for (int r = 0; r < NM; ++r) b[r] = Bs[(r * NQ + j) * NQ + i];
+        for (int r = 0; r < NM; ++r) b[r] = Bs[(r * NQ + j) * P + i];
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
             T s = b[0] * tab.B(0, k);
@@ -130,6 +147,7 @@ template <typename T, int NM, int NQ>
 __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const T (&y)[NQ], bool valid,
                                                   int t, T *Bs, T *C, T *__restrict__ xout)
 {
+    constexpr int P = pitch<NQ>();
     {
         const int i = t % NQ, j = t / NQ;
 #pragma unroll
@@ -137,7 +155,7 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
             T s = y[0] * tab.B(r, 0);
 #pragma unroll
             for (int k = 1; k < NQ; ++k) s = fma(y[k], tab.B(r, k), s);
-            Bs[(r * NQ + j) * NQ + i] = s;
+            Bs[(r * NQ + j) * P + i] = s;
         }
     }
     __syncthreads();
@@ -145,13 +163,13 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
         const int i = t % NQ, r = t / NQ;
         T z[NQ];
 #pragma unroll
-        for (int j = 0; j < NQ; ++j) z[j] = Bs[(r * NQ + j) * NQ + i];
+        for (int j = 0; j < NQ; ++j) z[j] = Bs[(r * NQ + j) * P + i];
 #pragma unroll
         for (int q = 0; q < NM; ++q) {
             T s = z[0] * tab.B(q, 0);
 #pragma unroll
             for (int j = 1; j < NQ; ++j) s = fma(z[j], tab.B(q, j), s);
-            C[(r * NM + q) * NQ + i] = s;
+            C[(r * NM + q) * P + i] = s;
         }
     }
     __syncthreads();
@@ -159,7 +177,7 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
         const int q = t % NM, r = t / NM;
         T z[NQ];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) z[i] = C[(r * NM + q) * NQ + i];
+        for (int i = 0; i < NQ; ++i) z[i] = C[(r * NM + q) * P + i];
         if (valid) {
 #pragma unroll
             for (int p = 0; p < NM; ++p) {
@@ -177,29 +195,30 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
 template <typename T, int NM, int NQ>
 __device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const T *A, T *Bs, T *C)
 {
+    constexpr int P = pitch<NQ>();
     const int l0 = t % NQ, k = t / NQ;
     {   // line (j = l0, k) along i
         T v[NQ];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l0) * NQ + l];
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l0) * P + l];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             T s = v[0] * tab.D(0, i);
 #pragma unroll
             for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(l, i), s);
-            Bs[(k * NQ + l0) * NQ + i] = s;
+            Bs[(k * NQ + l0) * P + i] = s;
         }
     }
     {   // line (i = l0, k) along j
         T v[NQ];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l) * NQ + l0];
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l) * P + l0];
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
             T s = v[0] * tab.D(0, j);
 #pragma unroll
             for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(l, j), s);
-            C[(k * NQ + j) * NQ + l0] = s;
+            C[(k * NQ + j) * P + l0] = s;
         }
     }
 }
@@ -223,7 +242,7 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void bwdtrans_k
     int nelmt, const T *__restrict__ tabp, const T *__restrict__ in, T *__restrict__ out)
 {
     constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ T lds[EPB][2][NQ3];
+    __shared__ T lds[EPB][2][slab<NQ>()];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
@@ -246,7 +265,7 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void iproduct_k
     T *__restrict__ out)
 {
     constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ T lds[EPB][2][NQ3];
+    __shared__ T lds[EPB][2][slab<NQ>()];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
@@ -274,7 +293,7 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void physderiv_
     T *__restrict__ out0, T *__restrict__ out1, T *__restrict__ out2)
 {
     constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ T lds[EPB][3][NQ3];
+    __shared__ T lds[EPB][3][slab<NQ>()];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
@@ -287,7 +306,7 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void physderiv_
     for (int k = 0; k < NQ; ++k) {
         const int c = (k * NQ + j) * NQ + i;
         u[k] = valid ? f[c] : T(0);
-        A[c] = u[k];
+        A[(k * NQ + j) * pitch<NQ>() + i] = u[k];
     }
     deriv2_column<T, NM, NQ>(tab, u, d2);
     __syncthreads();
@@ -306,7 +325,8 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void physderiv_
 #pragma unroll
             for (int d = 0; d < 9; ++d) g[d] = df[((size_t)e * 9 + d) * NQ3 + c];
         }
-        const T a0 = Bs[c], a1 = C[c], a2 = d2[k];
+        const int lc = (k * NQ + j) * pitch<NQ>() + i;
+        const T a0 = Bs[lc], a1 = C[lc], a2 = d2[k];
         T o0 = a0 * g[0];
         o0 = fma(a1, g[1], o0);
         o0 = fma(a2, g[2], o0);
@@ -330,13 +350,13 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
                                                   int t, const T *__restrict__ geo, T lambda, T *A, T *Bs, T *C,
                                                   T *__restrict__ xout)
 {
-    constexpr int NQ3 = NQ * NQ * NQ;
+    constexpr int NQ3 = NQ * NQ * NQ, P = pitch<NQ>();
     const int i = t % NQ, j = t / NQ;
 
     T u[NQ];
     bwd_to_column<T, NM, NQ>(tab, fetch, valid, t, A, Bs, u);
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) A[(k * NQ + j) * NQ + i] = u[k];
+    for (int k = 0; k < NQ; ++k) A[(k * NQ + j) * P + i] = u[k];
     T d2[NQ];
     deriv2_column<T, NM, NQ>(tab, u, d2);
     __syncthreads();
@@ -362,7 +382,8 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
             } else {
                 w = wij * tab.W(k);
             }
-            const T a0 = Bs[c], a1 = C[c], a2 = d2[k];
+            const int lc = (k * NQ + j) * P + i;
+            const T a0 = Bs[lc], a1 = C[lc], a2 = d2[k];
             T t0 = g[0] * a0;
             t0 = fma(g[1], a1, t0);
             t0 = fma(g[2], a2, t0);
@@ -377,8 +398,8 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
                 t1 *= w;
                 t2 *= w;
             }
-            Bs[c] = t0;
-            C[c] = t1;
+            Bs[lc] = t0;
+            C[lc] = t1;
             v2[k] = t2;
             y[k] = (lambda * g[6]) * (DEF ? u[k] : w * u[k]);
         }
@@ -396,13 +417,13 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
         const int l0 = t % NQ, k = t / NQ;
         T v[NQ];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = Bs[(k * NQ + l0) * NQ + l];
+        for (int l = 0; l < NQ; ++l) v[l] = Bs[(k * NQ + l0) * P + l];
 #pragma unroll
         for (int ii = 0; ii < NQ; ++ii) {
             T s = v[0] * tab.D(ii, 0);
 #pragma unroll
             for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(ii, l), s);
-            A[(k * NQ + l0) * NQ + ii] = s;
+            A[(k * NQ + l0) * P + ii] = s;
         }
     }
     __syncthreads();
@@ -410,29 +431,42 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
         const int l0 = t % NQ, k = t / NQ;
         T v[NQ];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = C[(k * NQ + l) * NQ + l0];
+        for (int l = 0; l < NQ; ++l) v[l] = C[(k * NQ + l) * P + l0];
 #pragma unroll
         for (int jj = 0; jj < NQ; ++jj) {
-            T s = A[(k * NQ + jj) * NQ + l0];
+            T s = A[(k * NQ + jj) * P + l0];
 #pragma unroll
             for (int l = 0; l < NQ; ++l) s = fma(v[l], tab.D(jj, l), s);
-            A[(k * NQ + jj) * NQ + l0] = s;
+            A[(k * NQ + jj) * P + l0] = s;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) y[k] += A[(k * NQ + j) * NQ + i];
+    for (int k = 0; k < NQ; ++k) y[k] += A[(k * NQ + j) * P + i];
     iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, xout);
 }
 
-// Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_kernel(
-    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
-    T *__restrict__ out)
+// Tile shapes (elements per workgroup) instantiated for the Helmholtz
+// kernels; the autotuner times them per (shape, order, precision).
+template <int NQ>
+constexpr int tile_fill()
 {
-    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ T lds[EPB][3][NQ3];
+    return elements_per_block<NQ>();
+}
+template <typename T, int NQ, int EPB>
+constexpr bool tile_fits()
+{
+    return EPB * NQ * NQ <= 1024 && EPB * 3 * slab<NQ>() * (int)sizeof(T) <= 65536;
+}
+
+// Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
+template <typename T, int NM, int NQ, bool DEF, int EPB>
+__global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_kernel(int nelmt, const T *__restrict__ tabp,
+                                                                  const T *__restrict__ geo, T lambda,
+                                                                  const T *__restrict__ in, T *__restrict__ out)
+{
+    constexpr int TPE = NQ * NQ;
+    __shared__ T lds[EPB][3][slab<NQ>()];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
@@ -443,14 +477,16 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_
 
 // Solver form: x gathered from the global vector of the structured n^3
 // mesh, element results written element-major (assembled afterwards).
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void helmholtz_gather_kernel(
-    int n, int N, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ xg,
-    T *__restrict__ out, const int *__restrict__ done)
+template <typename T, int NM, int NQ, bool DEF, int EPB>
+__global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_gather_kernel(int n, int N, const T *__restrict__ tabp,
+                                                                         const T *__restrict__ geo, T lambda,
+                                                                         const T *__restrict__ xg,
+                                                                         T *__restrict__ out,
+                                                                         const int *__restrict__ done)
 {
     if (done && *done) return;
-    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ T lds[EPB][3][NQ3];
+    constexpr int TPE = NQ * NQ;
+    __shared__ T lds[EPB][3][slab<NQ>()];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;

@@ -6,7 +6,8 @@ namespace nektar_amd {
 #define NMF_DECL(n)                                              \
     hipError_t launch_hex_nm##n(int nq, const KernelArgs &a);    \
     hipError_t launch_tet_nm##n(int nq, const KernelArgs &a);    \
-    hipError_t launch_hex_solver_nm##n(int nq, const SolverArgs &a);
+    hipError_t launch_hex_solver_nm##n(int nq, const SolverArgs &a); \
+    int hex_tiles_nm##n(int nq, bool fp32, int *out, int max);
 NMF_DECL(2) NMF_DECL(3) NMF_DECL(4) NMF_DECL(5) NMF_DECL(6) NMF_DECL(7) NMF_DECL(8) NMF_DECL(9) NMF_DECL(10)
 #undef NMF_DECL
 
@@ -31,6 +32,18 @@ hipError_t launch_tet(int nm, int nq, const KernelArgs &a)
         NMF_CASE(tet, 2) NMF_CASE(tet, 3) NMF_CASE(tet, 4) NMF_CASE(tet, 5) NMF_CASE(tet, 6)
         NMF_CASE(tet, 7) NMF_CASE(tet, 8) NMF_CASE(tet, 9) NMF_CASE(tet, 10)
     default: return hipErrorInvalidValue;
+    }
+}
+
+int hex_tile_candidates(int nm, int nq, bool fp32, int *out, int max)
+{
+    switch (nm) {
+#define NMF_TILES(n) \
+    case n: return hex_tiles_nm##n(nq, fp32, out, max);
+        NMF_TILES(2) NMF_TILES(3) NMF_TILES(4) NMF_TILES(5) NMF_TILES(6) NMF_TILES(7) NMF_TILES(8) NMF_TILES(9)
+        NMF_TILES(10)
+#undef NMF_TILES
+    default: return 0;
     }
 }
 

@@ -20,6 +20,7 @@ struct KernelArgs {
     const void *in;
     void *out0, *out1, *out2;
     hipStream_t stream;
+    int tile = 0;  // Helmholtz: elements per workgroup (0 = default)
 };
 
 // Order-templated kernels of the structured-hex C0 solver.
@@ -38,7 +39,12 @@ struct SolverArgs {
     void *out;
     const int *done;   // Apply: early exit flag (NULL = always run)
     hipStream_t stream;
+    int tile = 0;      // Apply: elements per workgroup (0 = default)
 };
+
+// Tile shapes (elements per workgroup) available for the hex Helmholtz
+// kernels of a given order, in the order the autotuner tries them.
+int hex_tile_candidates(int nm, int nq, bool fp32, int *out, int max);
 
 constexpr int kMinModes = 2, kMaxModes = 10;
 

@@ -73,6 +73,7 @@ MFOperator::MFOperator(const std::string &key, int nummodes, int numpoints0, int
     ncoeff = num_coeffs(shape, nummodes);
     tab.upload(build_tables(shape, nm, numpoints0), fp32);
     if (op == OpType::BwdTrans) deformed = false;  // BwdTrans needs no geometry
+    tile = tuned_tile(tune_key());
 }
 
 void MFOperator::set_jac(const double *j)
@@ -233,9 +234,57 @@ void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipSt
     NMF_REQUIRE(nelmt == 0 || (in && out0), NMF_ERR_INVALID, "null input/output pointer");
     if (op == OpType::PhysDeriv) NMF_REQUIRE(nelmt == 0 || (out1 && out2), NMF_ERR_INVALID, "PhysDeriv needs three outputs");
     ensure_geometry();
-    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in, out0, out1, out2, s};
+    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in, out0, out1, out2, s, tile};
     hipError_t e = shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+}
+
+TuneKey MFOperator::tune_key() const
+{
+    return TuneKey{(int)shape, (int)op, nm, nq[0], fp32 ? 1 : 0, deformed ? 1 : 0, 0};
+}
+
+std::vector<int> MFOperator::tile_candidates() const
+{
+    std::vector<int> c;
+    if (shape != Shape::Hex || op != OpType::Helmholtz) return c;
+    int buf[8];
+    const int n = hex_tile_candidates(nm, nq[0], fp32, buf, 8);
+    c.assign(buf, buf + n);
+    return c;
+}
+
+void MFOperator::set_tile(int t)
+{
+    if (t != 0) {
+        const auto c = tile_candidates();
+        bool ok = false;
+        for (int x : c) ok |= x == t;
+        NMF_REQUIRE(ok, NMF_ERR_INVALID, "tile " + std::to_string(t) + " is not instantiated for this operator");
+    }
+    tile = t;
+}
+
+int MFOperator::autotune(hipStream_t s, std::vector<double> *times_us)
+{
+    const auto cand = tile_candidates();
+    if (times_us) times_us->clear();
+    if (cand.empty() || nelmt == 0) return 0;
+    ensure_geometry();
+    const size_t es = fp32 ? sizeof(float) : sizeof(double);
+    DeviceBuffer in((size_t)nelmt * ncoeff * es), out((size_t)nelmt * ncoeff * es);
+    NMF_HIP(hipMemsetAsync(in.get(), 0, (size_t)nelmt * ncoeff * es, s));
+    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in.get(), out.get(), nullptr, nullptr, s, 0};
+    const int best = time_tiles(
+        s, cand,
+        [&](int t) {
+            a.tile = t;
+            return launch_hex(nm, nq[0], a);
+        },
+        10, times_us);
+    tile = best;
+    record_tile(tune_key(), best);
+    return best;
 }
 
 }  // namespace nektar_amd

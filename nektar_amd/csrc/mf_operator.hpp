@@ -11,6 +11,7 @@
 #include "basis.hpp"
 #include "common.hpp"
 #include "launch.hpp"
+#include "tune.hpp"
 
 namespace nektar_amd {
 
@@ -29,6 +30,7 @@ struct MFOperator {
     bool have_jac = false, have_df = false, geo_dirty = true;
 
     DeviceBuffer tab, geo;
+    int tile = 0;  // elements per workgroup of the Helmholtz kernel (0 = default)
 
     MFOperator(const std::string &key, int nummodes, int numpoints0, int nelmt, int precision);
 
@@ -43,10 +45,22 @@ struct MFOperator {
 
     void apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s);
 
+    // Tiles this operator's kernel is instantiated with (empty when the
+    // kernel has a single shape).
+    std::vector<int> tile_candidates() const;
+    void set_tile(int t);
+    // Time every candidate on this operator's geometry (scratch input and
+    // output of the operator's size), keep the fastest and record it for
+    // later operators of the same kind.  Returns the chosen tile (0 when
+    // there is nothing to choose).
+    int autotune(hipStream_t s, std::vector<double> *times_us = nullptr);
+
     // Packed geometry exactly as the kernels read it (host FP64); exposed
     // so solvers can reuse it.
     std::vector<double> pack_geometry() const;
     static std::vector<double> build_tables(Shape shape, int nm, int nq0);
+
+    TuneKey tune_key() const;
 
 private:
     void ensure_geometry();

@@ -12,6 +12,7 @@
 
 #include "common.hpp"
 #include "launch.hpp"
+#include "tune.hpp"
 
 namespace nektar_amd {
 namespace solver {
@@ -112,6 +113,13 @@ public:
     double l2_error_manufactured(int k);
     // y = A x on global vectors (device, FP64), Dirichlet rows zeroed
     void apply_global(const double *x, double *y);
+    // Choose the workgroup tile of the gather-apply kernel in FP64 and
+    // FP32 by timing each candidate on this mesh; the choice is recorded
+    // for later solvers of the same kind.  Captured iteration graphs are
+    // rebuilt on the next solve.
+    void autotune(int *tile64, int *tile32, std::vector<double> *times64 = nullptr,
+                  std::vector<double> *times32 = nullptr);
+    int tile(bool fp32) const { return fp32 ? tile32_ : tile64_; }
 
     size_t nglobal() const { return NG_; }
     size_t nlocal() const { return (size_t)nelmt_ * nc_; }
@@ -152,6 +160,7 @@ private:
     hipStream_t stream_;
     bool own_stream_;
     int nb_;  // reduction blocks
+    int tile64_ = 0, tile32_ = 0;  // gather-apply workgroup tiles (0 = default)
 
     DeviceBuffer X_, z_, w_, tab64_, tab32_, dtab_, geo64_, geo32_;
     DeviceBuffer b_, x_, invd_, sloc64_, tmp64_, r64_;

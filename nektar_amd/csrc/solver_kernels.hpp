@@ -32,8 +32,8 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void diag_kerne
     int nelmt, const double *__restrict__ tabp, const double *__restrict__ dtab, const double *__restrict__ geo,
     double lambda, double *__restrict__ out)
 {
-    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ;
-    __shared__ double lds[EPB][2][NQ * NQ * NQ];
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, P = pitch<NQ>();
+    __shared__ double lds[EPB][2][slab<NQ>()];
     const Tab<double, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
@@ -59,20 +59,20 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void diag_kerne
             double s = 0.0;
 #pragma unroll
             for (int k = 0; k < NQ; ++k) s = fma(g[k], Z[r * NQ + k], s);
-            S1[(r * NQ + j0) * NQ + i0] = s;
+            S1[(r * NQ + j0) * P + i0] = s;
         }
         __syncthreads();
         if (t < NQ * NM) {  // j -> q on line (i, r)
             const int i = t % NQ, r = t / NQ;
             double v[NQ];
 #pragma unroll
-            for (int j = 0; j < NQ; ++j) v[j] = S1[(r * NQ + j) * NQ + i];
+            for (int j = 0; j < NQ; ++j) v[j] = S1[(r * NQ + j) * P + i];
 #pragma unroll
             for (int q = 0; q < NM; ++q) {
                 double s = 0.0;
 #pragma unroll
                 for (int j = 0; j < NQ; ++j) s = fma(v[j], Y[q * NQ + j], s);
-                S2[(r * NM + q) * NQ + i] = s;
+                S2[(r * NM + q) * P + i] = s;
             }
         }
         __syncthreads();
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void diag_kerne
             const int q = t % NM, r = t / NM;
             double v[NQ];
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) v[i] = S2[(r * NM + q) * NQ + i];
+            for (int i = 0; i < NQ; ++i) v[i] = S2[(r * NM + q) * P + i];
 #pragma unroll
             for (int p = 0; p < NM; ++p) {
                 double s = 0.0;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void rhs_kernel
     double *__restrict__ out)
 {
     constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ double lds[EPB][2][NQ3];
+    __shared__ double lds[EPB][2][slab<NQ>()];
     const Tab<double, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void bwd_gather
     int n, int N, const double *__restrict__ tabp, const double *__restrict__ xg, double *__restrict__ out)
 {
     constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ;
-    __shared__ double lds[EPB][2][NQ3];
+    __shared__ double lds[EPB][2][slab<NQ>()];
     const Tab<double, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
