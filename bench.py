@@ -74,6 +74,75 @@ def time_apply(op, x, y, steps, warmup, torch):
     return s.elapsed_time(e) / steps, wall
 
 
+def batch_workload(N, seed=4, count=64):
+    """64 Helmholtz applies: hex/tet 50/50, P ~ U{3..9}, Q = P+2, E
+    log-uniform in [1e3, 1e5], affine elements with per-element scale
+    U[0.5, 2], coefficients U[-1, 1].  Returns [(shape, nm, nq, E, jac, df,
+    lam)]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        shape = "Hex" if i % 2 == 0 else "Tet"
+        nm = int(rng.integers(4, 11))
+        E = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+        s = rng.uniform(0.5, 2.0, E)[:, None, None]
+        A = np.eye(3)[None] + 0.25 * rng.uniform(-1, 1, (E, 3, 3))
+        M = s * A * 0.05  # element size ~ 0.1
+        jac = np.abs(np.linalg.det(M)) / (1.0 if shape == "Hex" else 8.0)
+        Minv = np.linalg.inv(M if shape == "Hex" else M / 2)
+        df = np.ascontiguousarray(np.transpose(Minv, (0, 2, 1))).reshape(-1)  # df[j*3+k] = dxi_k/dx_j
+        out.append((shape, nm, nm + 1, E, jac, df, float(rng.uniform(0.1, 10))))
+    return out
+
+
+def bench_batch(N, torch, steps, warmup):
+    """Batched uneven workload: Sum of element-local DOFs / time of one
+    batch submission (graph of grouped launches), FP64 and FP32, against
+    the entry-by-entry loop."""
+    work = batch_workload(N)
+    res = {"items": len(work)}
+    for prec, tag in ((N.FP64, "fp64"), (N.FP32, "fp32")):
+        dt = torch.float64 if prec == N.FP64 else torch.float32
+        ops, arrays = [], []
+        for (shape, nm, nq, E, jac, df, lam) in work:
+            op = N.GetOperatorFactory().CreateInstance(f"Helmholtz_{shape}_Regular", nm, nq, E, prec)
+            op.SetJac(jac)
+            op.SetDF(df)
+            op.SetLambda(lam)
+            g = torch.Generator(device="cuda").manual_seed(4)
+            x = torch.rand(op.input_size(), dtype=dt, device="cuda", generator=g) * 2 - 1
+            y = torch.zeros(op.output_size(), dtype=dt, device="cuda")
+            ops.append(op)
+            arrays.append((x, y))
+        b = N.Batch(mode=N.Batch.GRAPH, streams=4)
+        for op, (x, y) in zip(ops, arrays):
+            b.add(op, x, y)
+        b.autotune()
+        info = b.info()
+        r = {"kinds": info["groups"], "dofs": info["dofs"]}
+        for mode, mtag in ((N.Batch.GRAPH, "graph"), (N.Batch.GROUPED, "grouped"), (N.Batch.PER_ITEM, "per_item")):
+            b.set_mode(mode)
+            for _ in range(warmup):
+                b.run()
+            torch.cuda.synchronize()
+            s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s0.record()
+            for _ in range(steps):
+                b.run()
+            s1.record()
+            torch.cuda.synchronize()
+            ms = s0.elapsed_time(s1) / steps
+            r[f"{mtag}_us"] = round(ms * 1e3, 1)
+            r[f"{mtag}_gdofs"] = round(info["dofs"] / (ms * 1e-3) / 1e9, 2)
+        r["graph_over_per_item"] = round(r["per_item_us"] / r["graph_us"], 3)
+        res[tag] = r
+        del b, ops, arrays
+    res["fp32_over_fp64"] = round(res["fp64"]["graph_us"] / res["fp32"]["graph_us"], 3)
+    return res
+
+
 def cpu_baseline(n_side, nm, nq, target_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
@@ -106,6 +175,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the CG solves")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 512 MiB ten times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -194,6 +264,9 @@ def main():
         extra["solve_mixed_speedup"] = round(extra["solve_fp64"]["solve_ms"] / extra["solve_mixed"]["solve_ms"], 3)
         extra["unknowns"] = nunk
         del s
+
+    if not args.no_batch:
+        extra["batch"] = bench_batch(N, torch, steps=10, warmup=2)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "GDOF/s", "n_gpus": 1,

@@ -56,6 +56,7 @@ extern "C" {
 
 typedef struct nmf_operator_s *nmf_operator;
 typedef struct nmf_solver_s *nmf_solver;
+typedef struct nmf_batch_s *nmf_batch;
 
 /* Last error message of the calling thread ("" if none). */
 const char *nmf_last_error(void);
@@ -139,6 +140,41 @@ int nmf_operator_get_tile(nmf_operator op, int *tile);
  * `max` entries) receives the mean launch time of candidate i. */
 int nmf_operator_autotune(nmf_operator op, void *stream, int *tile,
                           double *times_us, int max);
+
+/* ---------------------------------------------------------------------
+ * Batches of operators of uneven kinds and sizes.  Replaces the loop over
+ * collections in ExpList::GeneralMatrixOp (MultiRegions/ExpList.cpp:
+ * 2666-2757: one Collection::ApplyOperator per shape/order collection)
+ * with one submission: Helmholtz entries of one kind (shape, nummodes,
+ * numpoints, precision, geometry type, tile) run as a single grouped
+ * launch, kinds are spread over `streams` streams, and the whole batch is
+ * captured once into a hipGraph that every later run replays
+ * (NMF_BATCH_GRAPH, the default).  NMF_BATCH_GROUPED launches the same
+ * grouped kernels without a graph; NMF_BATCH_PER_ITEM applies entry by
+ * entry on the given stream (the reference's loop).  Results are identical
+ * in every mode.  Operators must outlive the batch; in/out arrays are
+ * fixed when added.  A change to an operator (geometry, lambda, diffusion,
+ * tile) is picked up by the next run, which then rebuilds the item tables
+ * and graph (synchronising the device).
+ * ------------------------------------------------------------------- */
+#define NMF_BATCH_GRAPH 0
+#define NMF_BATCH_GROUPED 1
+#define NMF_BATCH_PER_ITEM 2
+int nmf_batch_create(nmf_batch *out);
+int nmf_batch_destroy(nmf_batch b);
+/* operator()(in, out) of op, as nmf_operator_apply */
+int nmf_batch_add(nmf_batch b, nmf_operator op, const void *in, void *out);
+int nmf_batch_add_physderiv(nmf_batch b, nmf_operator op, const void *in,
+                            void *out0, void *out1, void *out2);
+int nmf_batch_set_mode(nmf_batch b, int mode);
+/* streams 1..8 (default 4) */
+int nmf_batch_set_streams(nmf_batch b, int streams);
+/* Autotune each hex Helmholtz kind on its largest member; every member of
+ * the kind takes the chosen tile. */
+int nmf_batch_autotune(nmf_batch b, void *stream);
+int nmf_batch_run(nmf_batch b, void *stream);
+/* entries, grouped launches, and total element-local input DOFs */
+int nmf_batch_info(nmf_batch b, int *items, int *groups, int64_t *dofs);
 
 /* ---------------------------------------------------------------------
  * C0 Helmholtz solve on a structured hex mesh of [0,1]^3

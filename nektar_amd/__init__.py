@@ -27,7 +27,7 @@ except ImportError:  # pragma: no cover - torch is optional for host-only use
 import numpy as np
 
 __all__ = [
-    "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring", "clear_tuning",
+    "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring", "clear_tuning", "Batch",
     "basis_tables", "FP64", "FP32", "HelmholtzSolver", "SolveInfo", "hex_box_vertices",
 ]
 
@@ -91,6 +91,15 @@ _SIGNATURES = {
     "nmf_operator_get_tile": (ctypes.c_int, [_P, _I]),
     "nmf_operator_autotune": (ctypes.c_int, [_P, _P, _I, _D, ctypes.c_int]),
     "nmf_solver_autotune": (ctypes.c_int, [_P, _I, _I]),
+    "nmf_batch_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "nmf_batch_destroy": (ctypes.c_int, [_P]),
+    "nmf_batch_add": (ctypes.c_int, [_P, _P, _P, _P]),
+    "nmf_batch_add_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "nmf_batch_set_mode": (ctypes.c_int, [_P, ctypes.c_int]),
+    "nmf_batch_set_streams": (ctypes.c_int, [_P, ctypes.c_int]),
+    "nmf_batch_autotune": (ctypes.c_int, [_P, _P]),
+    "nmf_batch_run": (ctypes.c_int, [_P, _P]),
+    "nmf_batch_info": (ctypes.c_int, [_P, _I, _I, _L]),
     "nmf_clear_tuning": (ctypes.c_int, []),
     "nmf_solver_create_hex_box": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                  ctypes.c_uint64, ctypes.c_double, _P, ctypes.POINTER(_P)]),
@@ -304,6 +313,62 @@ _FACTORY = _Factory()
 
 def GetOperatorFactory() -> _Factory:
     return _FACTORY
+
+
+class Batch:
+    """Many operators of uneven kinds and sizes applied as one submission
+    (grouped launches per kind, spread over streams, captured once into a
+    hipGraph).  Keeps references to its operators and arrays."""
+
+    GRAPH, GROUPED, PER_ITEM = 0, 1, 2
+
+    def __init__(self, mode: int = 0, streams: int = 4):
+        self._h = None
+        h = ctypes.c_void_p()
+        check(lib().nmf_batch_create(ctypes.byref(h)))
+        self._h = h
+        self._keep = []
+        self.set_mode(mode)
+        self.set_streams(streams)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nmf_batch_destroy(self._h)
+            self._h = None
+
+    def add(self, op: "Operator", inp, out) -> None:
+        dt = op.torch_dtype() if torch is not None else None
+        check(lib().nmf_batch_add(self._h, op._h, _device_ptr(inp, dt, op.input_size()),
+                                  _device_ptr(out, dt, op.output_size())))
+        self._keep.append((op, inp, out))
+
+    def add_physderiv(self, op: "Operator", inp, out0, out1, out2) -> None:
+        dt = op.torch_dtype() if torch is not None else None
+        ptrs = [_device_ptr(o, dt, op.output_size()) for o in (out0, out1, out2)]
+        check(lib().nmf_batch_add_physderiv(self._h, op._h, _device_ptr(inp, dt, op.input_size()), *ptrs))
+        self._keep.append((op, inp, out0, out1, out2))
+
+    def set_mode(self, mode: int) -> None:
+        check(lib().nmf_batch_set_mode(self._h, int(mode)))
+
+    def set_streams(self, n: int) -> None:
+        check(lib().nmf_batch_set_streams(self._h, int(n)))
+
+    def _stream(self, stream):
+        if stream is None and torch is not None:
+            return torch.cuda.current_stream().cuda_stream
+        return stream
+
+    def autotune(self, stream=None) -> None:
+        check(lib().nmf_batch_autotune(self._h, self._stream(stream)))
+
+    def run(self, stream=None) -> None:
+        check(lib().nmf_batch_run(self._h, self._stream(stream)))
+
+    def info(self) -> dict:
+        n, g, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        check(lib().nmf_batch_info(self._h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(d)))
+        return {"items": n.value, "groups": g.value, "dofs": d.value}
 
 
 def clear_tuning() -> None:

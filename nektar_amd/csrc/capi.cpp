@@ -5,12 +5,17 @@
 #include <vector>
 
 #include "basis.hpp"
+#include "batch.hpp"
 #include "common.hpp"
 #include "mf_operator.hpp"
 #include "solver.hpp"
 
 struct nmf_operator_s {
     std::unique_ptr<nektar_amd::MFOperator> impl;
+};
+
+struct nmf_batch_s {
+    std::unique_ptr<nektar_amd::Batch> impl;
 };
 
 struct nmf_solver_s {
@@ -275,6 +280,87 @@ int nmf_operator_autotune(nmf_operator op, void *stream, int *tile, double *time
         if (tile) *tile = best;
         if (times_us)
             for (int i = 0; i < (int)t.size() && i < max; ++i) times_us[i] = t[i];
+    });
+}
+
+int nmf_batch_create(nmf_batch *out)
+{
+    return guard([&] {
+        NMF_REQUIRE(out, NMF_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        auto h = std::make_unique<nmf_batch_s>();
+        h->impl = std::make_unique<Batch>();
+        *out = h.release();
+    });
+}
+
+int nmf_batch_destroy(nmf_batch b)
+{
+    return guard([&] { delete b; });
+}
+
+#define NMF_BATCH_CHECK() NMF_REQUIRE(b && b->impl, NMF_ERR_INVALID, "null batch handle")
+
+int nmf_batch_add(nmf_batch b, nmf_operator op, const void *in, void *out)
+{
+    return guard([&] {
+        NMF_BATCH_CHECK();
+        NMF_OP_CHECK();
+        NMF_REQUIRE(op->impl->op != OpType::PhysDeriv, NMF_ERR_INVALID,
+                    "PhysDeriv has three outputs: use nmf_batch_add_physderiv");
+        b->impl->add(op->impl.get(), in, out, nullptr, nullptr);
+    });
+}
+
+int nmf_batch_add_physderiv(nmf_batch b, nmf_operator op, const void *in, void *out0, void *out1, void *out2)
+{
+    return guard([&] {
+        NMF_BATCH_CHECK();
+        NMF_OP_CHECK();
+        NMF_REQUIRE(op->impl->op == OpType::PhysDeriv, NMF_ERR_INVALID, "not a PhysDeriv operator");
+        b->impl->add(op->impl.get(), in, out0, out1, out2);
+    });
+}
+
+int nmf_batch_set_mode(nmf_batch b, int mode)
+{
+    return guard([&] {
+        NMF_BATCH_CHECK();
+        b->impl->set_mode(mode);
+    });
+}
+
+int nmf_batch_set_streams(nmf_batch b, int streams)
+{
+    return guard([&] {
+        NMF_BATCH_CHECK();
+        b->impl->set_streams(streams);
+    });
+}
+
+int nmf_batch_autotune(nmf_batch b, void *stream)
+{
+    return guard([&] {
+        NMF_BATCH_CHECK();
+        b->impl->autotune(static_cast<hipStream_t>(stream));
+    });
+}
+
+int nmf_batch_run(nmf_batch b, void *stream)
+{
+    return guard([&] {
+        NMF_BATCH_CHECK();
+        b->impl->run(static_cast<hipStream_t>(stream));
+    });
+}
+
+int nmf_batch_info(nmf_batch b, int *items, int *groups, int64_t *dofs)
+{
+    return guard([&] {
+        NMF_BATCH_CHECK();
+        if (items) *items = b->impl->items();
+        if (groups) *groups = b->impl->groups();
+        if (dofs) *dofs = b->impl->dofs();
     });
 }
 

@@ -31,6 +31,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "launch.hpp"
+
 namespace nektar_amd {
 namespace hex {
 
@@ -473,6 +475,28 @@ __global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_kernel(int nelmt, con
     const bool valid = e < nelmt;
     helmholtz_element<T, NM, NQ, DEF>(tab, LocalFetch<T, NM>{in + (size_t)e * NM * NM * NM}, valid, e, t, geo,
                                       lambda, lds[le][0], lds[le][1], lds[le][2], out + (size_t)e * NM * NM * NM);
+}
+
+// Grouped form: several operators of the same kind in one launch; block b
+// belongs to the item whose [block0, next block0) range holds it.
+template <typename T, int NM, int NQ, bool DEF, int EPB>
+__global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_grouped_kernel(int count,
+                                                                          const GroupItem<T> *__restrict__ items,
+                                                                          const T *__restrict__ tabp)
+{
+    constexpr int TPE = NQ * NQ;
+    __shared__ T lds[EPB][3][slab<NQ>()];
+    const int b = blockIdx.x;
+    int it = 0;
+    while (it + 1 < count && b >= items[it + 1].block0) ++it;
+    const GroupItem<T> I = items[it];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = (b - I.block0) * EPB + le;
+    const bool valid = e < I.nelmt;
+    helmholtz_element<T, NM, NQ, DEF>(tab, LocalFetch<T, NM>{I.in + (size_t)e * NM * NM * NM}, valid, e, t, I.geo,
+                                      I.lambda, lds[le][0], lds[le][1], lds[le][2],
+                                      I.out + (size_t)e * NM * NM * NM);
 }
 
 // Solver form: x gathered from the global vector of the structured n^3

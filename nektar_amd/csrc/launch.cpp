@@ -7,7 +7,9 @@ namespace nektar_amd {
     hipError_t launch_hex_nm##n(int nq, const KernelArgs &a);    \
     hipError_t launch_tet_nm##n(int nq, const KernelArgs &a);    \
     hipError_t launch_hex_solver_nm##n(int nq, const SolverArgs &a); \
-    int hex_tiles_nm##n(int nq, bool fp32, int *out, int max);
+    int hex_tiles_nm##n(int nq, bool fp32, int *out, int max);        \
+    hipError_t launch_hex_grouped_nm##n(int nq, const GroupArgs &a);  \
+    hipError_t launch_tet_grouped_nm##n(int nq, const GroupArgs &a);
 NMF_DECL(2) NMF_DECL(3) NMF_DECL(4) NMF_DECL(5) NMF_DECL(6) NMF_DECL(7) NMF_DECL(8) NMF_DECL(9) NMF_DECL(10)
 #undef NMF_DECL
 
@@ -44,6 +46,26 @@ int hex_tile_candidates(int nm, int nq, bool fp32, int *out, int max)
         NMF_TILES(10)
 #undef NMF_TILES
     default: return 0;
+    }
+}
+
+hipError_t launch_hex_grouped(int nm, int nq, const GroupArgs &a)
+{
+    switch (nm) {
+        NMF_CASE(hex_grouped, 2) NMF_CASE(hex_grouped, 3) NMF_CASE(hex_grouped, 4) NMF_CASE(hex_grouped, 5)
+        NMF_CASE(hex_grouped, 6) NMF_CASE(hex_grouped, 7) NMF_CASE(hex_grouped, 8) NMF_CASE(hex_grouped, 9)
+        NMF_CASE(hex_grouped, 10)
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tet_grouped(int nm, int nq, const GroupArgs &a)
+{
+    switch (nm) {
+        NMF_CASE(tet_grouped, 2) NMF_CASE(tet_grouped, 3) NMF_CASE(tet_grouped, 4) NMF_CASE(tet_grouped, 5)
+        NMF_CASE(tet_grouped, 6) NMF_CASE(tet_grouped, 7) NMF_CASE(tet_grouped, 8) NMF_CASE(tet_grouped, 9)
+        NMF_CASE(tet_grouped, 10)
+    default: return hipErrorInvalidValue;
     }
 }
 

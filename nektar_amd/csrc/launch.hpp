@@ -46,6 +46,39 @@ struct SolverArgs {
 // kernels of a given order, in the order the autotuner tries them.
 int hex_tile_candidates(int nm, int nq, bool fp32, int *out, int max);
 
+// Grouped Helmholtz: several operators of one kind (shape, modes, points,
+// precision, geometry type) in one launch.  Items are ordered with
+// block0 ascending from 0; item i owns workgroups [block0_i, block0_{i+1}).
+template <typename T>
+struct GroupItem {
+    const T *geo;
+    const T *in;
+    T *out;
+    T lambda;
+    int nelmt;
+    int block0;
+};
+
+struct GroupArgs {
+    bool fp32;
+    bool deformed;
+    int count;          // items
+    const void *items;  // device array of GroupItem<float|double>
+    int nblocks;        // total workgroups of the launch
+    const void *tab;    // tables shared by every item of the kind
+    hipStream_t stream;
+    int tile = 0;       // hex: elements per workgroup (0 = default)
+};
+
+// Elements per workgroup: hex default fill (tile 0) or the given tile;
+// tets have one shape.  The kernels assert they use the same numbers.
+constexpr int hex_fill_elems(int nq) { return 256 / (nq * nq) > 0 ? 256 / (nq * nq) : 1; }
+constexpr int tet_block_elems(int nq0) { return 256 / (nq0 * (nq0 - 1)) > 0 ? 256 / (nq0 * (nq0 - 1)) : 1; }
+inline int hex_tile_elems(int nq, int tile) { return tile > 0 ? tile : hex_fill_elems(nq); }
+
+hipError_t launch_hex_grouped(int nm, int nq, const GroupArgs &a);
+hipError_t launch_tet_grouped(int nm, int nq0, const GroupArgs &a);
+
 constexpr int kMinModes = 2, kMaxModes = 10;
 
 bool hex_supported(int nm, int nq);

@@ -82,6 +82,7 @@ void MFOperator::set_jac(const double *j)
     jac.assign(j, j + geom_points());
     have_jac = true;
     geo_dirty = true;
+    ++version;
 }
 
 void MFOperator::set_df(const double *d)
@@ -90,6 +91,7 @@ void MFOperator::set_df(const double *d)
     df.assign(d, d + 9 * geom_points());
     have_df = true;
     geo_dirty = true;
+    ++version;
 }
 
 void MFOperator::set_diffusion(const double *d6)
@@ -103,6 +105,7 @@ void MFOperator::set_diffusion(const double *d6)
         std::memcpy(diff, d6, sizeof(diff));
     }
     geo_dirty = true;
+    ++version;
 }
 
 namespace {
@@ -263,6 +266,7 @@ void MFOperator::set_tile(int t)
         NMF_REQUIRE(ok, NMF_ERR_INVALID, "tile " + std::to_string(t) + " is not instantiated for this operator");
     }
     tile = t;
+    ++version;
 }
 
 int MFOperator::autotune(hipStream_t s, std::vector<double> *times_us)
@@ -283,6 +287,7 @@ int MFOperator::autotune(hipStream_t s, std::vector<double> *times_us)
         },
         10, times_us);
     tile = best;
+    ++version;
     record_tile(tune_key(), best);
     return best;
 }

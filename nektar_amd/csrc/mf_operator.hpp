@@ -5,6 +5,7 @@
 // holds the 1-D tables and the geometry packed for the kernels.
 #pragma once
 
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,7 @@ struct MFOperator {
 
     DeviceBuffer tab, geo;
     int tile = 0;  // elements per workgroup of the Helmholtz kernel (0 = default)
+    uint64_t version = 0;  // bumped by every change of geometry, factors or tile
 
     MFOperator(const std::string &key, int nummodes, int numpoints0, int nelmt, int precision);
 
@@ -40,7 +42,11 @@ struct MFOperator {
 
     void set_jac(const double *j);
     void set_df(const double *d);
-    void set_lambda(double l) { lambda = l; }
+    void set_lambda(double l)
+    {
+        lambda = l;
+        ++version;
+    }
     void set_diffusion(const double *d6);
 
     void apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s);
@@ -61,6 +67,9 @@ struct MFOperator {
     static std::vector<double> build_tables(Shape shape, int nm, int nq0);
 
     TuneKey tune_key() const;
+    // Upload geometry if it changed (done by apply; batches call it before
+    // they take the device pointers).
+    void prepare() { ensure_geometry(); }
 
 private:
     void ensure_geometry();

@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "launch.hpp"
+
 namespace nektar_amd {
 namespace tet {
 
@@ -437,17 +439,18 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
 
 // Fused Helmholtz (Helmholtz.h:549-599 with DiffusionCoeffTetKernel,
 // HelmholtzKernels.hpp:700-913).
+// One workgroup of the fused Helmholtz apply: elements blk*EPB ..
+// blk*EPB+EPB-1 of an operator with nelmt elements.
 template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
-    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
-    T *__restrict__ out)
+__device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__restrict__ tabp,
+                                                const T *__restrict__ geo, T lambda, const T *__restrict__ in,
+                                                T *__restrict__ out, T (*lds)[3][Dims<NM, NQ>::SLAB])
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
-    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
-    const int e = blockIdx.x * Dm::EPB + le;
+    const int e = blk * Dm::EPB + le;
     const bool valid = e < nelmt;
     const int i = t % NQ0, j = t / NQ0;
     T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
@@ -563,6 +566,31 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
 #pragma unroll
     for (int k = 0; k < NQ2; ++k) y[k] += A[(k * NQ1 + j) * NQ0 + i];
     iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, A, out + (size_t)e * Dm::NC);
+}
+
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
+    T *__restrict__ out)
+{
+    using Dm = Dims<NM, NQ>;
+    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    helmholtz_block<T, NM, NQ, DEF>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds);
+}
+
+// Grouped form: several operators of the same kind in one launch; block b
+// belongs to the item whose [block0, next block0) range holds it.
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_grouped_kernel(
+    int count, const GroupItem<T> *__restrict__ items, const T *__restrict__ tabp)
+{
+    using Dm = Dims<NM, NQ>;
+    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    const int b = blockIdx.x;
+    int it = 0;
+    while (it + 1 < count && b >= items[it + 1].block0) ++it;
+    const GroupItem<T> I = items[it];
+    helmholtz_block<T, NM, NQ, DEF>(b - I.block0, I.nelmt, tabp, I.geo, I.lambda, I.in, I.out, lds);
 }
 
 }  // namespace tet
