@@ -217,9 +217,10 @@ void HexSolver::autotune(int *tile64, int *tile32, std::vector<double> *times64,
         int buf[8];
         const int nc = hex_tile_candidates(nm_, nq_, fp32, buf, 8);
         std::vector<int> cand(buf, buf + nc);
-        // a zeroed global vector of the right precision serves as input
+        // a scratch global vector of non-zero normal values (every byte
+        // 0x3F) serves as input, so the timed arithmetic draws real power
         void *xg = fp32 ? cg32_[2].get() : tmp64_.get();
-        NMF_HIP(hipMemsetAsync(xg, 0, NG_ * (fp32 ? sizeof(float) : sizeof(double)), stream_));
+        NMF_HIP(hipMemsetAsync(xg, 0x3F, NG_ * (fp32 ? sizeof(float) : sizeof(double)), stream_));
         void *sloc = fp32 ? sloc32_.get() : sloc64_.get();
         SolverArgs a{SolverKernel::Apply, fp32, deformed_, n_, N_, fp32 ? tab32_.get() : tab64_.get(), nullptr,
                      fp32 ? geo32_.get() : geo64_.get(), lambda_, xg, sloc, nullptr, stream_, 0};

@@ -58,12 +58,39 @@ constexpr int slab()
     return NQ * NQ * pitch<NQ>();
 }
 
+// Hides a (wave-uniform) table pointer from the optimiser: the asm emits
+// nothing, but loads through the returned pointer can neither be merged
+// with loads of the same entries elsewhere in the kernel nor be hoisted
+// above this point.  Without it the compiler keeps whole tables live in
+// scalar registers across all contractions and spills them to vector
+// lanes (v_writelane/v_readlane), doubling the vector instruction count.
+template <typename T>
+using const_ptr = __attribute__((address_space(4))) const T *;
+
+template <typename T>
+__device__ __forceinline__ const_ptr<T> opaque(const T *p)
+{
+    const_ptr<T> q = (const_ptr<T>)p;
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" : "+s"(q));
+    return q;
+}
+
+// 1-D tables of one direction: B[p*NQ+i] = phi_p(z_i), D[l*NQ+i] =
+// l'_l(z_i) (column-major D(i,l)), W[i], and transposed copies BT, DT so
+// that every contraction reads one contiguous row per output.
 template <typename T, int NM, int NQ>
 struct Tab {
+    static constexpr int OFF_D = NM * NQ, OFF_W = OFF_D + NQ * NQ, OFF_BT = OFF_W + NQ, OFF_DT = OFF_BT + NQ * NM;
     const T *__restrict__ base;
     __device__ __forceinline__ T B(int p, int i) const { return base[p * NQ + i]; }
-    __device__ __forceinline__ T D(int l, int i) const { return base[NM * NQ + l * NQ + i]; }
-    __device__ __forceinline__ T W(int i) const { return base[NM * NQ + NQ * NQ + i]; }
+    __device__ __forceinline__ T D(int l, int i) const { return base[OFF_D + l * NQ + i]; }
+    __device__ __forceinline__ T W(int i) const { return base[OFF_W + i]; }
+    // B(p, 0..NQ-1), B(0..NM-1, i), D(l, 0..NQ-1), D(0..NQ-1, i)
+    __device__ __forceinline__ const_ptr<T> Brow(int p) const { return opaque(base + p * NQ); }
+    __device__ __forceinline__ const_ptr<T> BTrow(int i) const { return opaque(base + OFF_BT + i * NM); }
+    __device__ __forceinline__ const_ptr<T> Drow(int l) const { return opaque(base + OFF_D + l * NQ); }
+    __device__ __forceinline__ const_ptr<T> DTrow(int i) const { return opaque(base + OFF_DT + i * NQ); }
 };
 
 // Coefficient fetch for an element stored contiguously (the Collections
@@ -105,9 +132,10 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
         for (int p = 0; p < NM; ++p) x[p] = valid ? xin(p, q, r) : T(0);
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
-            T s = x[0] * tab.B(0, i);
+            const auto b = tab.BTrow(i);
+            T s = x[0] * b[0];
 #pragma unroll
-            for (int p = 1; p < NM; ++p) s = fma(x[p], tab.B(p, i), s);
+            for (int p = 1; p < NM; ++p) s = fma(x[p], b[p], s);
             A[(r * NM + q) * P + i] = s;
         }
     }
@@ -120,9 +148,10 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
         for (int q = 0; q < NM; ++q) a[q] = A[(r * NM + q) * P + i];
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
-            T s = a[0] * tab.B(0, j);
+            const auto bt = tab.BTrow(j);
+            T s = a[0] * bt[0];
 #pragma unroll
-            for (int q = 1; q < NM; ++q) s = fma(a[q], tab.B(q, j), s);
+            for (int q = 1; q < NM; ++q) s = fma(a[q], bt[q], s);
             Bs[(r * NQ + j) * P + i] = s;
         }
     }
@@ -135,9 +164,10 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
         for (int r = 0; r < NM; ++r) b[r] = Bs[(r * NQ + j) * P + i];
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            T s = b[0] * tab.B(0, k);
+            const auto bt = tab.BTrow(k);
+            T s = b[0] * bt[0];
 #pragma unroll
-            for (int r = 1; r < NM; ++r) s = fma(b[r], tab.B(r, k), s);
+            for (int r = 1; r < NM; ++r) s = fma(b[r], bt[r], s);
             u[k] = s;
         }
     }
@@ -154,9 +184,10 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
         const int i = t % NQ, j = t / NQ;
 #pragma unroll
         for (int r = 0; r < NM; ++r) {
-            T s = y[0] * tab.B(r, 0);
+            const auto b = tab.Brow(r);
+            T s = y[0] * b[0];
 #pragma unroll
-            for (int k = 1; k < NQ; ++k) s = fma(y[k], tab.B(r, k), s);
+            for (int k = 1; k < NQ; ++k) s = fma(y[k], b[k], s);
             Bs[(r * NQ + j) * P + i] = s;
         }
     }
@@ -168,9 +199,10 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
         for (int j = 0; j < NQ; ++j) z[j] = Bs[(r * NQ + j) * P + i];
 #pragma unroll
         for (int q = 0; q < NM; ++q) {
-            T s = z[0] * tab.B(q, 0);
+            const auto b = tab.Brow(q);
+            T s = z[0] * b[0];
 #pragma unroll
-            for (int j = 1; j < NQ; ++j) s = fma(z[j], tab.B(q, j), s);
+            for (int j = 1; j < NQ; ++j) s = fma(z[j], b[j], s);
             C[(r * NM + q) * P + i] = s;
         }
     }
@@ -183,9 +215,10 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
         if (valid) {
 #pragma unroll
             for (int p = 0; p < NM; ++p) {
-                T s = z[0] * tab.B(p, 0);
+                const auto b = tab.Brow(p);
+                T s = z[0] * b[0];
 #pragma unroll
-                for (int i = 1; i < NQ; ++i) s = fma(z[i], tab.B(p, i), s);
+                for (int i = 1; i < NQ; ++i) s = fma(z[i], b[i], s);
                 xout[(r * NM + q) * NM + p] = s;
             }
         }
@@ -205,9 +238,10 @@ __device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const 
         for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l0) * P + l];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
-            T s = v[0] * tab.D(0, i);
+            const auto d = tab.DTrow(i);
+            T s = v[0] * d[0];
 #pragma unroll
-            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(l, i), s);
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], d[l], s);
             Bs[(k * NQ + l0) * P + i] = s;
         }
     }
@@ -217,9 +251,10 @@ __device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const 
         for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l) * P + l0];
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
-            T s = v[0] * tab.D(0, j);
+            const auto d = tab.DTrow(j);
+            T s = v[0] * d[0];
 #pragma unroll
-            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(l, j), s);
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], d[l], s);
             C[(k * NQ + j) * P + l0] = s;
         }
     }
@@ -230,9 +265,10 @@ __device__ __forceinline__ void deriv2_column(const Tab<T, NM, NQ> &tab, const T
 {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-        T s = u[0] * tab.D(0, k);
+        const auto d = tab.DTrow(k);
+        T s = u[0] * d[0];
 #pragma unroll
-        for (int l = 1; l < NQ; ++l) s = fma(u[l], tab.D(l, k), s);
+        for (int l = 1; l < NQ; ++l) s = fma(u[l], d[l], s);
         d2[k] = s;
     }
 }
@@ -359,8 +395,6 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
     bwd_to_column<T, NM, NQ>(tab, fetch, valid, t, A, Bs, u);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) A[(k * NQ + j) * P + i] = u[k];
-    T d2[NQ];
-    deriv2_column<T, NM, NQ>(tab, u, d2);
     __syncthreads();
     deriv01<T, NM, NQ>(tab, t, A, Bs, C);
     __syncthreads();
@@ -385,7 +419,13 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
                 w = wij * tab.W(k);
             }
             const int lc = (k * NQ + j) * P + i;
-            const T a0 = Bs[lc], a1 = C[lc], a2 = d2[k];
+            // direction-2 derivative of the register column, formed where
+            // it is used so no table row stays live across the barriers
+            const auto d = tab.DTrow(k);
+            T a2 = u[0] * d[0];
+#pragma unroll
+            for (int l = 1; l < NQ; ++l) a2 = fma(u[l], d[l], a2);
+            const T a0 = Bs[lc], a1 = C[lc];
             T t0 = g[0] * a0;
             t0 = fma(g[1], a1, t0);
             t0 = fma(g[2], a2, t0);
@@ -409,9 +449,10 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
     // D2^T v2 (in registers): (D^T v)(k) = sum_l D(l,k) v(l), D(out,in)=tab.D(in,out)
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
+        const auto d = tab.Drow(k);
         T s = y[k];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) s = fma(tab.D(k, l), v2[l], s);
+        for (int l = 0; l < NQ; ++l) s = fma(d[l], v2[l], s);
         y[k] = s;
     }
     __syncthreads();
@@ -422,9 +463,10 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
         for (int l = 0; l < NQ; ++l) v[l] = Bs[(k * NQ + l0) * P + l];
 #pragma unroll
         for (int ii = 0; ii < NQ; ++ii) {
-            T s = v[0] * tab.D(ii, 0);
+            const auto d = tab.Drow(ii);
+            T s = v[0] * d[0];
 #pragma unroll
-            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D(ii, l), s);
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], d[l], s);
             A[(k * NQ + l0) * P + ii] = s;
         }
     }
@@ -436,9 +478,10 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
         for (int l = 0; l < NQ; ++l) v[l] = C[(k * NQ + l) * P + l0];
 #pragma unroll
         for (int jj = 0; jj < NQ; ++jj) {
+            const auto d = tab.Drow(jj);
             T s = A[(k * NQ + jj) * P + l0];
 #pragma unroll
-            for (int l = 0; l < NQ; ++l) s = fma(v[l], tab.D(jj, l), s);
+            for (int l = 0; l < NQ; ++l) s = fma(v[l], d[l], s);
             A[(k * NQ + jj) * P + l0] = s;
         }
     }
