@@ -57,6 +57,7 @@ extern "C" {
 typedef struct nmf_operator_s *nmf_operator;
 typedef struct nmf_solver_s *nmf_solver;
 typedef struct nmf_batch_s *nmf_batch;
+typedef struct nmf_server_s *nmf_server;
 
 /* Last error message of the calling thread ("" if none). */
 const char *nmf_last_error(void);
@@ -229,6 +230,60 @@ int nmf_solver_apply_global(nmf_solver s, const double *x_dev, double *y_dev);
 int nmf_solver_autotune(nmf_solver s, int *tile64, int *tile32);
 /* Forget every tile the autotuner recorded in this process. */
 int nmf_clear_tuning(void);
+/* ---------------------------------------------------------------------
+ * Serving loop for Helmholtz solves.  No counterpart in the reference
+ * (which solves one system per process, ContField::HelmSolve); it admits
+ * many solves of different meshes and orders as they arrive.  Requests
+ * are structured hex boxes as nmf_solver_create_hex_box (perturb 0) with
+ * the manufactured forcing of frequency k.  One host loop queues
+ * arrivals (FIFO), admits while fewer than max_active solves run and the
+ * request's device memory fits the HBM budget (solvers -- geometry,
+ * diagonal, Krylov vectors, captured iterations -- stay resident, pooled
+ * by mesh shape and reused with a new lambda; idle ones are evicted
+ * least-recently-used), and advances every admitted solve chunk by chunk
+ * (8 iterations) without blocking, on `streams` streams, so the work of
+ * different requests overlaps on the device.  Results do not depend on
+ * what runs beside a solve.
+ * ------------------------------------------------------------------- */
+typedef struct {
+    int n, nummodes, numpoints; /* mesh n^3, P+1, Q                         */
+    int k;                      /* forcing frequency                        */
+    int mixed;                  /* 0: FP64 CG, 1: mixed-precision PCG        */
+    int maxit;
+    double lambda, tol;
+    double arrival_s;           /* seconds after the start of the run       */
+} nmf_request;
+
+typedef struct {
+    int64_t unknowns;           /* non-Dirichlet global DOFs                */
+    int iterations, outer_iterations, converged;
+    int reused_solver;          /* 1: served by a pooled solver              */
+    int stream;
+    double final_error;         /* relative FP64 residual                   */
+    double l2_error;            /* vs the manufactured solution (validate)  */
+    double solution_norm;       /* ||x|| over all global rows (validate)    */
+    double arrival_s, admit_s, finish_s;
+    double latency_ms;          /* finish - arrival                         */
+    double queue_ms;            /* admit - arrival                          */
+    double device_ms;           /* first to last device event of the solve  */
+} nmf_result;
+
+typedef struct {
+    int64_t peak_bytes;
+    int peak_active, solvers_created, solvers_reused, evictions;
+    double makespan_s;
+} nmf_server_stats;
+
+int nmf_server_create(int64_t hbm_budget_bytes, int streams, int max_active,
+                      nmf_server *out);
+int nmf_server_destroy(nmf_server s);
+/* Replays the arrival times and returns when every request has finished;
+ * results[i] belongs to reqs[i].  validate != 0 also fills l2_error and
+ * solution_norm (computed after finish_s is taken).  stats may be NULL. */
+int nmf_server_run(nmf_server s, const nmf_request *reqs, int count,
+                   int validate, nmf_result *results,
+                   nmf_server_stats *stats);
+
 /* The mesh generator used by nmf_solver_create_hex_box: (n+1)^3*3
  * vertex coordinates, vertex (ix,iy,iz) at ((iz*(n+1)+iy)*(n+1)+ix)*3. */
 int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz);

@@ -27,7 +27,7 @@ except ImportError:  # pragma: no cover - torch is optional for host-only use
 import numpy as np
 
 __all__ = [
-    "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring", "clear_tuning", "Batch",
+    "NektarError", "lib", "Operator", "GetOperatorFactory", "GetOpstring", "clear_tuning", "Batch", "Server",
     "basis_tables", "FP64", "FP32", "HelmholtzSolver", "SolveInfo", "hex_box_vertices",
 ]
 
@@ -91,6 +91,9 @@ _SIGNATURES = {
     "nmf_operator_get_tile": (ctypes.c_int, [_P, _I]),
     "nmf_operator_autotune": (ctypes.c_int, [_P, _P, _I, _D, ctypes.c_int]),
     "nmf_solver_autotune": (ctypes.c_int, [_P, _I, _I]),
+    "nmf_server_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
+    "nmf_server_destroy": (ctypes.c_int, [_P]),
+    "nmf_server_run": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _P, _P]),
     "nmf_batch_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
     "nmf_batch_destroy": (ctypes.c_int, [_P]),
     "nmf_batch_add": (ctypes.c_int, [_P, _P, _P, _P]),
@@ -369,6 +372,60 @@ class Batch:
         n, g, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
         check(lib().nmf_batch_info(self._h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(d)))
         return {"items": n.value, "groups": g.value, "dofs": d.value}
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nummodes", ctypes.c_int), ("numpoints", ctypes.c_int),
+                ("k", ctypes.c_int), ("mixed", ctypes.c_int), ("maxit", ctypes.c_int),
+                ("lam", ctypes.c_double), ("tol", ctypes.c_double), ("arrival_s", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("unknowns", ctypes.c_int64), ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int),
+                ("converged", ctypes.c_int), ("reused_solver", ctypes.c_int), ("stream", ctypes.c_int),
+                ("final_error", ctypes.c_double), ("l2_error", ctypes.c_double), ("solution_norm", ctypes.c_double),
+                ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("peak_bytes", ctypes.c_int64), ("peak_active", ctypes.c_int), ("solvers_created", ctypes.c_int),
+                ("solvers_reused", ctypes.c_int), ("evictions", ctypes.c_int), ("makespan_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Server:
+    """Serving loop for Helmholtz solves (admission by HBM budget, pooled
+    resident solvers, solves advanced chunk by chunk on several streams)."""
+
+    def __init__(self, hbm_budget_bytes: int, streams: int = 4, max_active: int = 16):
+        self._h = None
+        h = ctypes.c_void_p()
+        check(lib().nmf_server_create(int(hbm_budget_bytes), int(streams), int(max_active), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nmf_server_destroy(self._h)
+            self._h = None
+
+    def run(self, requests, validate: bool = False):
+        """requests: dicts with n, nummodes, numpoints, k, mixed, maxit, lam,
+        tol, arrival_s.  Returns (list of result dicts, stats dict)."""
+        count = len(requests)
+        reqs = (Request * max(1, count))()
+        for i, r in enumerate(requests):
+            reqs[i] = Request(r["n"], r["nummodes"], r["numpoints"], r.get("k", 1), int(r.get("mixed", 0)),
+                              r.get("maxit", 5000), r.get("lam", 1.0), r.get("tol", 1e-10), r.get("arrival_s", 0.0))
+        res = (Result * max(1, count))()
+        st = ServerStats()
+        check(lib().nmf_server_run(self._h, reqs, count, int(validate), res, ctypes.byref(st)))
+        return [res[i].as_dict() for i in range(count)], st.as_dict()
 
 
 def clear_tuning() -> None:

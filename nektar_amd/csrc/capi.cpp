@@ -8,6 +8,7 @@
 #include "batch.hpp"
 #include "common.hpp"
 #include "mf_operator.hpp"
+#include "server.hpp"
 #include "solver.hpp"
 
 struct nmf_operator_s {
@@ -16,6 +17,10 @@ struct nmf_operator_s {
 
 struct nmf_batch_s {
     std::unique_ptr<nektar_amd::Batch> impl;
+};
+
+struct nmf_server_s {
+    std::unique_ptr<nektar_amd::Server> impl;
 };
 
 struct nmf_solver_s {
@@ -498,6 +503,65 @@ int nmf_solver_autotune(nmf_solver s, int *tile64, int *tile32)
 int nmf_clear_tuning(void)
 {
     return guard([&] { clear_tuned_tiles(); });
+}
+
+int nmf_server_create(int64_t hbm_budget_bytes, int streams, int max_active, nmf_server *out)
+{
+    return guard([&] {
+        NMF_REQUIRE(out, NMF_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        auto h = std::make_unique<nmf_server_s>();
+        h->impl = std::make_unique<Server>(hbm_budget_bytes, streams, max_active);
+        *out = h.release();
+    });
+}
+
+int nmf_server_destroy(nmf_server s)
+{
+    return guard([&] { delete s; });
+}
+
+int nmf_server_run(nmf_server s, const nmf_request *reqs, int count, int validate, nmf_result *results,
+                   nmf_server_stats *stats)
+{
+    return guard([&] {
+        NMF_REQUIRE(s && s->impl, NMF_ERR_INVALID, "null server handle");
+        NMF_REQUIRE(count >= 0 && (count == 0 || (reqs && results)), NMF_ERR_INVALID, "null request/result array");
+        std::vector<ServeRequest> rq(count);
+        for (int i = 0; i < count; ++i) {
+            const nmf_request &r = reqs[i];
+            rq[i] = ServeRequest{r.n, r.nummodes, r.numpoints, r.k, r.mixed, r.maxit, r.lambda, r.tol, r.arrival_s};
+        }
+        ServeStats st;
+        const std::vector<ServeResult> out = s->impl->run(rq, validate != 0, &st);
+        for (int i = 0; i < count; ++i) {
+            const ServeResult &o = out[i];
+            nmf_result &r = results[i];
+            r.unknowns = o.unknowns;
+            r.iterations = o.iterations;
+            r.outer_iterations = o.outer_iterations;
+            r.converged = o.converged;
+            r.reused_solver = o.reused_solver;
+            r.stream = o.stream;
+            r.final_error = o.final_error;
+            r.l2_error = o.l2_error;
+            r.solution_norm = o.solution_norm;
+            r.arrival_s = o.arrival_s;
+            r.admit_s = o.admit_s;
+            r.finish_s = o.finish_s;
+            r.latency_ms = o.latency_ms;
+            r.queue_ms = o.queue_ms;
+            r.device_ms = o.device_ms;
+        }
+        if (stats) {
+            stats->peak_bytes = st.peak_bytes;
+            stats->peak_active = st.peak_active;
+            stats->solvers_created = st.solvers_created;
+            stats->solvers_reused = st.solvers_reused;
+            stats->evictions = st.evictions;
+            stats->makespan_s = st.makespan_s;
+        }
+    });
 }
 
 int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz)

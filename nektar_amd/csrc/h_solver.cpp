@@ -129,6 +129,9 @@ HexSolver::HexSolver(const HexBoxMesh &mesh, int nummodes, int numpoints, double
     NMF_HIP(hipMemsetAsync(b_.get(), 0, NG_ * sizeof(double), stream_));
     NMF_HIP(hipMemsetAsync(x_.get(), 0, NG_ * sizeof(double), stream_));
 
+    NMF_HIP(hipEventCreateWithFlags(&ev_state_, hipEventDisableTiming));
+    NMF_HIP(hipEventCreate(&ev_t0_));
+    NMF_HIP(hipEventCreate(&ev_t1_));
     build_geometry();
     build_diagonal();
     tile64_ = tuned_tile(TuneKey{(int)Shape::Hex, (int)OpType::Helmholtz, nm_, nq_, 0, deformed_ ? 1 : 0, 1});
@@ -140,6 +143,9 @@ HexSolver::~HexSolver()
 {
     for (auto &g : graph_)
         if (g) (void)hipGraphExecDestroy(g);
+    if (own_stream_ || job_.running) (void)hipStreamSynchronize(stream_);
+    for (auto ev : {ev_state_, ev_t0_, ev_t1_})
+        if (ev) (void)hipEventDestroy(ev);
     if (host_state_) (void)hipHostFree(host_state_);
     if (host_scalar_) (void)hipHostFree(host_scalar_);
     if (own_stream_ && stream_) {
@@ -185,6 +191,27 @@ void HexSolver::build_diagonal()
 
 void HexSolver::set_forcing_phys(const double *f_dev)
 {
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "set_forcing during a solve");
+    assemble_rhs(f_dev);
+    NMF_HIP(hipStreamSynchronize(stream_));  // f_dev is the caller's
+}
+
+void HexSolver::set_forcing_manufactured(int k)
+{
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "set_forcing during a solve");
+    // stream-ordered scratch: no device-wide synchronisation, so solves on
+    // other streams keep running while a new request is set up
+    double *f = nullptr;
+    const size_t bytes = (size_t)nelmt_ * nq_ * nq_ * nq_ * sizeof(double);
+    hipError_t e = hipMallocAsync((void **)&f, bytes, stream_);
+    NMF_REQUIRE(e == hipSuccess, NMF_ERR_MEMORY, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    NMF_HIP(solver::launch_forcing(stream_, n_, nq_, X_.as<double>(), z_.as<double>(), k, lambda_, f));
+    assemble_rhs(f);
+    NMF_HIP(hipFreeAsync(f, stream_));
+}
+
+void HexSolver::assemble_rhs(const double *f_dev)
+{
     SolverArgs a{SolverKernel::Rhs, false, deformed_, n_, N_, tab64_.get(), nullptr, geo64_.get(), lambda_, f_dev,
                  sloc64_.get(), nullptr, stream_};
     NMF_HIP(launch_hex_solver(nm_, nq_, a));
@@ -192,14 +219,6 @@ void HexSolver::set_forcing_phys(const double *f_dev)
     // reference's Set_Rhs_Magnitude over all global rows).
     NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), b_.as<double>(), nullptr,
                                             nullptr, nullptr, true));
-    NMF_HIP(hipStreamSynchronize(stream_));
-}
-
-void HexSolver::set_forcing_manufactured(int k)
-{
-    DeviceBuffer f((size_t)nelmt_ * nq_ * nq_ * nq_ * sizeof(double));
-    NMF_HIP(solver::launch_forcing(stream_, n_, nq_, X_.as<double>(), z_.as<double>(), k, lambda_, f.as<double>()));
-    set_forcing_phys(f.as<double>());
 }
 
 void HexSolver::apply_local(bool fp32, const void *xg, void *sloc, const int *done)
@@ -320,167 +339,184 @@ void HexSolver::capture_pcg32()
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
 }
 
-CGState HexSolver::run_pcg32(const double *b, double tol, int maxit, double rhs_mag)
+void HexSolver::enqueue_state_read()
 {
-    ensure_fp32();
-    CGState init{};
-    tol = std::max(tol, 1.0e-16);
-    init.tol2 = tol * tol;
-    init.maxit = maxit;
-    init.rhs_mag_override = rhs_mag;
-    *host_state_ = init;
-    NMF_HIP(hipMemcpyAsync(state_.get(), host_state_, sizeof(CGState), hipMemcpyHostToDevice, stream_));
-    CGState *st = state_.as<CGState>();
-    NMF_HIP(solver::launch_pcg_init<float>(stream_, N_, b, invd_.as<double>(), cg32_[0].as<float>(),
-                                           cg32_[1].as<float>(), cg32_[2].as<float>(), pa_.as<double>()));
-    NMF_HIP(solver::launch_pcg_scalar_init(stream_, st, pa_.as<double>(), nb_));
-    capture_pcg32();
-    const int max_replays = maxit / kItersPerGraph + 2;
-    for (int rep = 0; rep <= max_replays; ++rep) {
-        NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
-        NMF_HIP(hipStreamSynchronize(stream_));
-        if (host_state_->done) break;
-        NMF_HIP(hipGraphLaunch(graph_[2], stream_));
-    }
-    return *host_state_;
+    NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
+    NMF_HIP(hipEventRecord(ev_state_, stream_));
+    job_.pending = true;
 }
 
-CGState HexSolver::run_cg(bool fp32, const double *b, double tol, int maxit)
+void HexSolver::begin_solve(bool mixed, double tol, int maxit)
 {
-    if (fp32) ensure_fp32();
-    CGState init{};
-    tol = std::max(tol, 1.0e-16);
-    init.tol2 = tol * tol;
-    init.maxit = maxit;
-    *host_state_ = init;
-    NMF_HIP(hipMemcpyAsync(state_.get(), host_state_, sizeof(CGState), hipMemcpyHostToDevice, stream_));
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "a solve is already in progress on this solver");
+    NMF_REQUIRE(maxit >= 0, NMF_ERR_INVALID, "maxit must be >= 0");
+    job_ = Job{};
+    job_.running = true;
+    job_.mixed = mixed;
+    tol = std::max(tol, 1.0e-16);  // NekLinSysIter.cpp:57
+    job_.tol = tol;
+    NMF_HIP(hipEventRecord(ev_t0_, stream_));
     CGState *st = state_.as<CGState>();
-    if (fp32) {
-        float *x = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>(),
-              *q = cg32_[3].as<float>(), *w = cg32_[4].as<float>(), *s = cg32_[5].as<float>();
-        NMF_HIP(solver::launch_cg_init<float>(stream_, N_, b, invd_.as<double>(), x, r, p, q, w, pa_.as<double>()));
-        apply_local(true, w, sloc32_.get(), nullptr);
-        NMF_HIP(solver::launch_assemble<float>(stream_, n_, nm_, N_, sloc32_.as<float>(), s, w, pmu_.as<double>(),
-                                               nullptr));
-    } else {
+    if (!mixed) {
+        CGState init{};
+        init.tol2 = tol * tol;
+        init.maxit = maxit;
+        *host_state_ = init;
+        NMF_HIP(hipMemcpyAsync(state_.get(), host_state_, sizeof(CGState), hipMemcpyHostToDevice, stream_));
         double *x = cg64_[0].as<double>(), *r = cg64_[1].as<double>(), *p = cg64_[2].as<double>(),
                *q = cg64_[3].as<double>(), *w = cg64_[4].as<double>(), *s = cg64_[5].as<double>();
-        NMF_HIP(solver::launch_cg_init<double>(stream_, N_, b, invd_.as<double>(), x, r, p, q, w, pa_.as<double>()));
+        NMF_HIP(solver::launch_cg_init<double>(stream_, N_, b_.as<double>(), invd_.as<double>(), x, r, p, q, w,
+                                               pa_.as<double>()));
         apply_local(false, w, sloc64_.get(), nullptr);
         NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), s, w, pmu_.as<double>(),
                                                 nullptr));
-    }
-    NMF_HIP(solver::launch_cg_scalar_init(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
-    capture_iterations(fp32);
-    // Each replay runs kItersPerGraph iterations; kernels of a finished solve
-    // return at once.  The host polls the state once per replay.
-    const int max_replays = maxit / kItersPerGraph + 2;
-    for (int rep = 0; rep <= max_replays; ++rep) {
-        NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
-        NMF_HIP(hipStreamSynchronize(stream_));
-        if (host_state_->done) break;
-        NMF_HIP(hipGraphLaunch(graph_[fp32 ? 1 : 0], stream_));
-    }
-    return *host_state_;
-}
-
-SolveInfo HexSolver::solve(bool mixed, double tol, int maxit, double inner_tol)
-{
-    SolveInfo info;
-    hipEvent_t e0, e1;
-    NMF_HIP(hipEventCreate(&e0));
-    NMF_HIP(hipEventCreate(&e1));
-    NMF_HIP(hipEventRecord(e0, stream_));
-    if (!mixed) {
-        CGState st = run_cg(false, b_.as<double>(), tol, maxit);
-        NMF_HIP(hipMemcpyAsync(x_.get(), cg64_[0].get(), NG_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        if (verbose())
-            std::fprintf(stderr, "CG iterations made = %d using tolerance of %.1e (error = %.6e, rhs_mag = %.6e)\n",
-                         st.iter, tol, st.final_error, std::sqrt(st.rhs_mag));
-        info.iterations = st.iter;
-        info.outer_iterations = 1;
-        info.converged = st.converged;
-        info.final_error = st.final_error;
-        info.rhs_magnitude = st.rhs_mag;
+        NMF_HIP(solver::launch_cg_scalar_init(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
+        capture_iterations(false);
+        job_.max_rounds = maxit / kItersPerGraph + 3;
     } else {
-        info = solve_mixed(tol, maxit, 0.1);
-    }
-    NMF_HIP(hipEventRecord(e1, stream_));
-    NMF_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    NMF_HIP(hipEventElapsedTime(&ms, e0, e1));
-    info.solve_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return info;
-}
-
-SolveInfo HexSolver::solve_mixed(double tol, int maxit, double delta)
-{
-    SolveInfo info;
-    ensure_fp32();
-    float *y = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>();
-    double *dsc = scalar_.as<double>();
-    // x64 = 0, r64 = b on non-Dirichlet rows; ||b||^2 over all global rows
-    // (Set_Rhs_Magnitude, NekLinSysIter.cpp:96-117) and ||r64||^2.
-    NMF_HIP(solver::launch_zero<double>(stream_, NG_, x_.as<double>()));
-    NMF_HIP(solver::launch_zero<double>(stream_, NG_, tmp64_.as<double>()));
-    NMF_HIP(solver::launch_sumsq(stream_, NG_, b_.as<double>(), pmu_.as<double>()));
-    NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc));
-    NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
-                                    pmu_.as<double>()));
-    NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
-    NMF_HIP(hipMemcpyAsync(host_scalar_, dsc, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
-    NMF_HIP(hipStreamSynchronize(stream_));
-    const double bmag = host_scalar_[0], rr0 = host_scalar_[1];
-    const double rhs_mag = bmag > 1.0e-6 ? bmag : 1.0;
-    tol = std::max(tol, 1.0e-16);
-
-    CGState init{};
-    init.tol2 = tol * tol;
-    init.maxit = maxit;
-    init.rhs_mag_override = rhs_mag;
-    init.delta2 = delta * delta;
-    init.rr_ref = rr0;
-    *host_state_ = init;
-    NMF_HIP(hipMemcpyAsync(state_.get(), host_state_, sizeof(CGState), hipMemcpyHostToDevice, stream_));
-    CGState *st = state_.as<CGState>();
-    NMF_HIP(solver::launch_pcg_init<float>(stream_, N_, r64_.as<double>(), invd_.as<double>(), y, r, p,
-                                           pa_.as<double>()));
-    NMF_HIP(solver::launch_pcg_scalar_init(stream_, st, pa_.as<double>(), nb_));
-    capture_pcg32();
-    const int max_rounds = maxit / kItersPerGraph + 64 + maxit;
-    for (int round = 0; round < max_rounds; ++round) {
-        NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
-        NMF_HIP(hipStreamSynchronize(stream_));
-        if (host_state_->done) break;
-        if (!host_state_->pause) {
-            NMF_HIP(hipGraphLaunch(graph_[2], stream_));
-            continue;
-        }
-        // reliable update: accumulate in FP64, recompute the true residual
-        // with the FP64 operator, restart the recurrence residual from it
-        NMF_HIP(solver::launch_add<float>(stream_, NG_, x_.as<double>(), y));
-        NMF_HIP(solver::launch_zero<float>(stream_, NG_, y));
-        apply_global(x_.as<double>(), tmp64_.as<double>());
+        ensure_fp32();
+        float *y = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>();
+        double *dsc = scalar_.as<double>();
+        // x64 = 0, r64 = b on non-Dirichlet rows; ||b||^2 over all global
+        // rows (Set_Rhs_Magnitude, NekLinSysIter.cpp:96-117) and ||r64||^2
+        // go to the device state without a host round trip.
+        NMF_HIP(solver::launch_zero<double>(stream_, NG_, x_.as<double>()));
+        NMF_HIP(solver::launch_zero<double>(stream_, NG_, tmp64_.as<double>()));
+        NMF_HIP(solver::launch_sumsq(stream_, NG_, b_.as<double>(), pmu_.as<double>()));
+        NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc));
         NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
                                         pmu_.as<double>()));
-        NMF_HIP(solver::launch_ru_resume(stream_, N_, r64_.as<double>(), r, invd_.as<double>(), pa_.as<double>()));
-        NMF_HIP(solver::launch_ru_scalar(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
-        NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd_.as<double>()));
-        if (verbose()) {
-            NMF_HIP(hipMemcpyAsync(host_state_, state_.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream_));
-            NMF_HIP(hipStreamSynchronize(stream_));
-            std::fprintf(stderr, "mixed CG reliable update %d at iteration %d: FP64 error = %.6e\n",
-                         host_state_->updates, host_state_->iter, host_state_->final_error);
-        }
+        NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
+        NMF_HIP(solver::launch_mixed_state_init(stream_, st, dsc, tol * tol, maxit, kReliableDelta * kReliableDelta));
+        NMF_HIP(solver::launch_pcg_init<float>(stream_, N_, r64_.as<double>(), invd_.as<double>(), y, r, p,
+                                               pa_.as<double>()));
+        NMF_HIP(solver::launch_pcg_scalar_init(stream_, st, pa_.as<double>(), nb_));
+        capture_pcg32();
+        job_.max_rounds = maxit / kItersPerGraph + 64 + maxit;
     }
-    info.iterations = host_state_->iter;
-    info.outer_iterations = host_state_->updates;
-    info.converged = host_state_->converged;
-    info.final_error = host_state_->final_error;
-    info.rhs_magnitude = rhs_mag;
-    return info;
+    enqueue_state_read();
+}
+
+void HexSolver::enqueue_reliable_update()
+{
+    // accumulate the FP32 correction in FP64, recompute the true residual
+    // with the FP64 operator, restart the recurrence residual from it
+    CGState *st = state_.as<CGState>();
+    float *y = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>();
+    NMF_HIP(solver::launch_add<float>(stream_, NG_, x_.as<double>(), y));
+    NMF_HIP(solver::launch_zero<float>(stream_, NG_, y));
+    apply_global(x_.as<double>(), tmp64_.as<double>());
+    NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
+                                    pmu_.as<double>()));
+    NMF_HIP(solver::launch_ru_resume(stream_, N_, r64_.as<double>(), r, invd_.as<double>(), pa_.as<double>()));
+    NMF_HIP(solver::launch_ru_scalar(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
+    NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd_.as<double>()));
+}
+
+bool HexSolver::advance(bool wait)
+{
+    NMF_REQUIRE(job_.running, NMF_ERR_STATE, "no solve in progress on this solver");
+    if (job_.finished) return true;
+    if (job_.finalizing) {
+        if (wait) {
+            NMF_HIP(hipEventSynchronize(ev_t1_));
+        } else {
+            const hipError_t q = hipEventQuery(ev_t1_);
+            if (q == hipErrorNotReady) return false;
+            NMF_HIP(q);
+        }
+        float ms = 0.f;
+        NMF_HIP(hipEventElapsedTime(&ms, ev_t0_, ev_t1_));
+        job_.info.solve_ms = ms;
+        job_.finished = true;
+        return true;
+    }
+    if (job_.pending) {
+        if (wait) {
+            NMF_HIP(hipEventSynchronize(ev_state_));
+        } else {
+            const hipError_t q = hipEventQuery(ev_state_);
+            if (q == hipErrorNotReady) return false;
+            NMF_HIP(q);
+        }
+        job_.pending = false;
+    }
+    const CGState st = *host_state_;
+    if (job_.mixed && verbose() && st.updates > job_.reported_updates) {
+        job_.reported_updates = st.updates;
+        std::fprintf(stderr, "mixed CG reliable update %d at iteration %d: FP64 error = %.6e\n", st.updates, st.iter,
+                     st.final_error);
+    }
+    if (st.done || job_.rounds >= job_.max_rounds) {
+        SolveInfo &info = job_.info;
+        info.iterations = st.iter;
+        info.converged = st.converged;
+        info.final_error = st.final_error;
+        if (job_.mixed) {
+            info.outer_iterations = st.updates;
+            info.rhs_magnitude = st.rhs_mag_override;
+        } else {
+            info.outer_iterations = 1;
+            info.rhs_magnitude = st.rhs_mag;
+            NMF_HIP(hipMemcpyAsync(x_.get(), cg64_[0].get(), NG_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            if (verbose())
+                std::fprintf(stderr, "CG iterations made = %d using tolerance of %.1e (error = %.6e, rhs_mag = %.6e)\n",
+                             st.iter, job_.tol, st.final_error, std::sqrt(st.rhs_mag));
+        }
+        NMF_HIP(hipEventRecord(ev_t1_, stream_));
+        job_.finalizing = true;
+        return advance(wait);
+    }
+    if (job_.mixed && st.pause)
+        enqueue_reliable_update();
+    else
+        NMF_HIP(hipGraphLaunch(graph_[job_.mixed ? 2 : 0], stream_));
+    ++job_.rounds;
+    enqueue_state_read();
+    return false;
+}
+
+SolveInfo HexSolver::finish()
+{
+    NMF_REQUIRE(job_.running && job_.finished, NMF_ERR_STATE, "solve not finished");
+    job_.running = false;
+    return job_.info;
+}
+
+SolveInfo HexSolver::solve(bool mixed, double tol, int maxit)
+{
+    begin_solve(mixed, tol, maxit);
+    try {
+        while (!advance(true)) {
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        job_ = Job{};
+        throw;
+    }
+    return finish();
+}
+
+void HexSolver::set_lambda(double lambda)
+{
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "set_lambda during a solve");
+    lambda_ = lambda;
+    build_diagonal();
+    // the captured iterations carry lambda as a kernel argument
+    for (auto &g : graph_)
+        if (g) {
+            NMF_HIP(hipGraphExecDestroy(g));
+            g = nullptr;
+        }
+}
+
+double HexSolver::solution_norm()
+{
+    NMF_HIP(solver::launch_sumsq(stream_, NG_, x_.as<double>(), pmu_.as<double>()));
+    NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, scalar_.as<double>()));
+    NMF_HIP(hipMemcpyAsync(host_scalar_, scalar_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    NMF_HIP(hipStreamSynchronize(stream_));
+    return std::sqrt(host_scalar_[0]);
 }
 
 double HexSolver::l2_error_manufactured(int k)

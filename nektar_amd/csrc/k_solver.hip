@@ -550,6 +550,22 @@ __global__ void sum_partials_kernel(const double *__restrict__ pa, int nb, doubl
     reduce_partials(pa, 1, 0, nb, out);
 }
 
+// Initial state of the mixed solve: relative tolerance against ||b||^2
+// with the rhs-magnitude floor of NekLinSysIter::Set_Rhs_Magnitude
+// (NekLinSysIter.cpp:96-117), reliable updates against ||r0||^2.
+__global__ void mixed_state_init_kernel(CGState *st, const double *__restrict__ dsc, double tol2, int maxit,
+                                        double delta2)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    CGState s{};
+    s.tol2 = tol2;
+    s.maxit = maxit;
+    s.rhs_mag_override = dsc[0] > 1.0e-6 ? dsc[0] : 1.0;
+    s.delta2 = delta2;
+    s.rr_ref = dsc[1];
+    *st = s;
+}
+
 __global__ void sumsq_kernel(size_t n, const double *__restrict__ a, double *__restrict__ partials)
 {
     double acc[1] = {0.0};
@@ -737,6 +753,13 @@ hipError_t launch_invert_diag(hipStream_t stream, int N, double *d)
 hipError_t launch_sum_partials(hipStream_t stream, const double *pa, int nb, double *out)
 {
     NMF_LAUNCH(sum_partials_kernel, 1, pa, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mixed_state_init(hipStream_t stream, CGState *st, const double *dsc, double tol2, int maxit,
+                                  double delta2)
+{
+    NMF_LAUNCH(mixed_state_init_kernel, 1, st, dsc, tol2, maxit, delta2);
     return hipGetLastError();
 }
 

@@ -73,6 +73,8 @@ template <typename T>
 hipError_t launch_add(hipStream_t, size_t NG, double *x, const T *d);
 hipError_t launch_invert_diag(hipStream_t, int N, double *d);
 hipError_t launch_sum_partials(hipStream_t, const double *pa, int nb, double *out);
+// state of a mixed solve from dsc[0] = ||b||^2 and dsc[1] = ||r0||^2
+hipError_t launch_mixed_state_init(hipStream_t, CGState *st, const double *dsc, double tol2, int maxit, double delta2);
 // partial sums of a[i]^2 over i < n (grid of reduction_blocks(n))
 hipError_t launch_sumsq(hipStream_t, size_t n, const double *a, double *partials);
 hipError_t launch_cast(hipStream_t, size_t n, const double *a, float *b);
@@ -107,10 +109,25 @@ public:
     void set_forcing_manufactured(int k);
     // b = Assemble(IProductWRTBase(f)) for f at quadrature points (device, FP64)
     void set_forcing_phys(const double *f_dev);
-    // mixed: FP32 inner CG inside FP64 iterative refinement
-    SolveInfo solve(bool mixed, double tol, int maxit, double inner_tol = 1e-5);
+    // mixed: FP32 PCG with FP64 reliable updates (residual, accumulation)
+    SolveInfo solve(bool mixed, double tol, int maxit);
+    // The same solve as a non-blocking sequence, for serving loops that
+    // interleave many solves: begin_solve enqueues the set-up; advance
+    // enqueues the next 8 iterations (or a reliable update) once the state
+    // of the previous chunk is on the host and returns true when the solve
+    // has finished (wait = block for the chunk in flight instead of
+    // returning false); finish returns the result.
+    void begin_solve(bool mixed, double tol, int maxit);
+    bool advance(bool wait);
+    SolveInfo finish();
+    bool solving() const { return job_.running; }
+    // New lambda: rebuilds the diagonal and drops captured iterations; the
+    // forcing must be set again afterwards (it depends on lambda).
+    void set_lambda(double lambda);
     // L2 error of the current solution against prod sin(k pi x)
     double l2_error_manufactured(int k);
+    // ||x|| over all global rows of the current solution
+    double solution_norm();
     // y = A x on global vectors (device, FP64), Dirichlet rows zeroed
     void apply_global(const double *x, double *y);
     // Choose the workgroup tile of the gather-apply kernel in FP64 and
@@ -136,22 +153,26 @@ public:
     size_t device_bytes() const;
     static size_t estimate_bytes(int n, int nm, int nq, bool deformed, bool mixed);
 
-    // CG on the current rhs_, stopping at tol; result in x (FP64 or FP32
-    // buffers); returns the final state.  Public for the serving loop.
-    solver::CGState run_cg(bool fp32, const double *b, double tol, int maxit);
-    // Hestenes-Stiefel PCG in FP32 (FP64 dots and diagonal) on rhs b, with
-    // the relative tolerance taken against rhs_mag; solution in the FP32
-    // x buffer.
-    solver::CGState run_pcg32(const double *b, double tol, int maxit, double rhs_mag);
-    // mixed precision: FP32 PCG with reliable updates (FP64 residual
-    // recomputation and FP64 solution accumulation)
-    SolveInfo solve_mixed(double tol, int maxit, double delta);
 
 private:
     void build_geometry();
     void build_diagonal();
     void apply_local(bool fp32, const void *xg, void *sloc, const int *done);
     void capture_iterations(bool fp32);
+    void assemble_rhs(const double *f_dev);
+    void enqueue_state_read();
+    void enqueue_reliable_update();
+
+    struct Job {
+        bool running = false, mixed = false, pending = false, finalizing = false, finished = false;
+        double tol = 0.0;
+        int max_rounds = 0, rounds = 0, reported_updates = 0;
+        SolveInfo info;
+    };
+    Job job_;
+    hipEvent_t ev_state_ = nullptr, ev_t0_ = nullptr, ev_t1_ = nullptr;
+    // reliable update once ||r||^2 drops below kReliableDelta^2 * ||r_ref||^2
+    static constexpr double kReliableDelta = 0.1;
 
     int n_, nm_, nq_, N_, nelmt_, nc_;
     size_t NG_;

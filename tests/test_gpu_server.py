@@ -1,0 +1,117 @@
+"""Serving loop: many Helmholtz solves admitted as they arrive.
+
+Each request's result must be the same alone and in the mix: iterations,
+final residual, L2 error against the manufactured solution and the norm of
+the solution vector are compared for exact equality with a plain solve of
+the same request on a fresh solver (the solve's arithmetic does not depend
+on what runs beside it; a reused pooled solver gets a new lambda)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def nmf():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import nektar_amd
+    nektar_amd.lib()
+    return nektar_amd
+
+
+def mixed_requests():
+    # (n, nummodes, lambda, k, mixed); repeated shapes exercise pooled reuse
+    spec = [(3, 4, 1.0, 1, 0), (4, 5, 2.5, 2, 1), (3, 4, 7.0, 3, 0), (5, 4, 0.3, 1, 0),
+            (4, 6, 1.7, 2, 0), (4, 5, 9.1, 1, 0), (2, 7, 4.0, 1, 1), (3, 4, 0.1, 2, 1),
+            (6, 4, 3.3, 3, 0), (4, 6, 5.5, 1, 1)]
+    return [dict(n=n, nummodes=nm, numpoints=nm + 1, lam=lam, k=k, mixed=mx, tol=1e-10, maxit=5000,
+                 arrival_s=0.0) for (n, nm, lam, k, mx) in spec]
+
+
+def alone(nmf, r):
+    s = nmf.HelmholtzSolver(r["n"], r["nummodes"], r["numpoints"], r["lam"])
+    s.set_forcing_manufactured(r["k"])
+    info = s.solve(mixed=bool(r["mixed"]), tol=r["tol"], maxit=r["maxit"])
+    x = s.solution()
+    return info, s.l2_error_manufactured(r["k"]), float(np.sqrt(np.sum(x * x)))
+
+
+def check_same(nmf, reqs, results):
+    for r, got in zip(reqs, results):
+        info, l2, xnorm = alone(nmf, r)
+        assert got["converged"] == 1 and info.converged, r
+        assert got["iterations"] == info.iterations, (r, got["iterations"], info.iterations)
+        assert got["outer_iterations"] == info.outer_iterations, r
+        assert got["final_error"] == info.final_error, r
+        assert got["l2_error"] == l2, (r, got["l2_error"], l2)
+        # the server's norm is a device reduction, the alone one a host sum
+        assert abs(got["solution_norm"] - xnorm) <= 1e-12 * xnorm, r
+
+
+def test_same_alone_and_in_the_mix(nmf):
+    reqs = mixed_requests()
+    srv = nmf.Server(hbm_budget_bytes=8 << 30, streams=4, max_active=8)
+    results, stats = srv.run(reqs, validate=True)
+    assert stats["peak_active"] >= 4  # solves really overlapped
+    for r, got in zip(reqs, results):
+        assert got["unknowns"] == (r["n"] * (r["nummodes"] - 1) - 1) ** 3
+        assert got["finish_s"] >= got["admit_s"] >= got["arrival_s"]
+        # discretisation error (up to ~0.03 for k=3 on a 3^3 P=3 mesh); the
+        # exact solution has L2 norm 0.35
+        assert 0.0 < got["l2_error"] < 0.1
+    check_same(nmf, reqs, results)
+
+
+def test_budget_admission_and_eviction(nmf):
+    reqs = mixed_requests()
+    est = [nmf.HelmholtzSolver.estimate_bytes(r["n"], r["nummodes"], r["numpoints"], False, bool(r["mixed"]))
+           for r in reqs]
+    budget = int(max(est) * 1.5)  # room for one large request (plus change) at a time
+    srv = nmf.Server(hbm_budget_bytes=budget, streams=2, max_active=8)
+    results, stats = srv.run(reqs, validate=True)
+    assert stats["peak_bytes"] <= budget
+    assert stats["evictions"] >= 1
+    assert stats["peak_active"] < len(reqs)
+    check_same(nmf, reqs, results)
+
+
+def test_timed_arrivals(nmf):
+    reqs = mixed_requests()[:6]
+    for i, r in enumerate(reqs):
+        r["arrival_s"] = 0.02 * i
+    srv = nmf.Server(hbm_budget_bytes=8 << 30, streams=4, max_active=4)
+    results, stats = srv.run(reqs, validate=False)
+    for r, got in zip(reqs, results):
+        assert got["converged"] == 1
+        assert got["admit_s"] >= r["arrival_s"]
+        assert got["latency_ms"] == pytest.approx(1e3 * (got["finish_s"] - r["arrival_s"]))
+        assert got["queue_ms"] >= 0.0 and got["device_ms"] > 0.0
+    assert stats["makespan_s"] >= reqs[-1]["arrival_s"]
+
+
+def test_one_at_a_time_matches(nmf):
+    reqs = mixed_requests()[:4]
+    srv = nmf.Server(hbm_budget_bytes=8 << 30, streams=1, max_active=1)
+    results, stats = srv.run(reqs, validate=True)
+    assert stats["peak_active"] == 1
+    # requests 0 and 2 have the same mesh: the second reuses the pooled
+    # solver with a new lambda
+    assert stats["solvers_reused"] >= 1 and results[2]["reused_solver"] == 1
+    check_same(nmf, reqs, results)
+
+
+def test_server_errors(nmf):
+    with pytest.raises(nmf.NektarError):
+        nmf.Server(0, 1, 1)
+    srv = nmf.Server(1 << 20, 1, 1)
+    big = dict(n=8, nummodes=6, numpoints=7, lam=1.0, k=1, mixed=0, tol=1e-10, maxit=100, arrival_s=0.0)
+    with pytest.raises(nmf.NektarError):
+        srv.run([big])
+    srv2 = nmf.Server(8 << 30, 1, 1)
+    bad = dict(big, nummodes=12, numpoints=13)
+    with pytest.raises(nmf.NektarError):
+        srv2.run([bad])
+    results, stats = srv2.run([])
+    assert results == [] and stats["solvers_created"] == 0
