@@ -42,7 +42,10 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-CASES = [(3, 5, 6, 1.0, 0.0, 1), (4, 4, 5, 2.5, 0.15, 3), (2, 7, 8, 0.3, 0.0, 2), (3, 6, 7, 1.0, 0.2, 5)]
+# the last case is the curved P=7 Q=9 mesh (interior vertices +-10 % of the
+# element width, per-point Jacobians) at a size the oracle solves quickly
+CASES = [(3, 5, 6, 1.0, 0.0, 1), (4, 4, 5, 2.5, 0.15, 3), (2, 7, 8, 0.3, 0.0, 2), (3, 6, 7, 1.0, 0.2, 5),
+         (6, 8, 9, 1.0, 0.1, 3)]
 
 
 @pytest.mark.parametrize("n,nm,nq,lam,amp,seed", CASES)
@@ -95,7 +98,10 @@ def test_mixed_precision_cg(nmf, n, nm, nq, lam, amp, seed):
     l2_mx = s.l2_error_manufactured(1)
     assert imx.converged and imx.final_error < 1e-10
     assert imx.outer_iterations >= 2  # reliable updates (FP64 residual recomputations)
-    assert abs(l2_mx - l2_64) <= 1e-3 * l2_64
+    # L2 errors agree to 1e-3 relative, or to 1e-12 absolute where the
+    # discretisation error is itself ~1e-11 (P=7) and the algebraic error
+    # left at residual 1e-10 shows
+    assert abs(l2_mx - l2_64) <= max(1e-3 * l2_64, 1e-12)
     # independent FP64 residual of the returned solution
     x = torch.from_numpy(s.solution()).cuda()
     ax = torch.zeros_like(x)
@@ -138,3 +144,39 @@ def test_maxit_reports_not_converged(nmf):
     s.set_forcing_manufactured(1)
     info = s.solve(False, 1e-12, maxit=5)
     assert not info.converged and info.iterations == 6
+
+
+def test_curved_48cubed_p7_mixed_against_fp64(nmf):
+    """The curved 48^3 P=7 Q=9 problem (38.3 M global DOFs, per-point
+    Jacobians held in FP64 and cast to FP32 for the FP32 apply): the mixed
+    solve reaches the 1e-10 relative residual of the FP64 operator
+    (recomputed here independently), and its solution differs from the FP64
+    GPU solve's by at most 1e-6 relative in the max norm, with the same
+    error against the exact solution to 1e-3 relative or 1e-12 absolute."""
+    n, nm, nq = 48, 8, 9
+    s = nmf.HelmholtzSolver(n, nm, nq, 1.0, 0.1, 3)
+    assert s.nglobal == 337 ** 3
+    s.set_forcing_manufactured(1)
+    i64 = s.solve(mixed=False, tol=1e-10)
+    assert i64.converged and i64.final_error < 1e-10
+    x64 = s.solution()
+    l2_64 = s.l2_error_manufactured(1)
+    imx = s.solve(mixed=True, tol=1e-10)
+    assert imx.converged and imx.final_error < 1e-10 and imx.outer_iterations >= 2
+    xmx = s.solution()
+    l2_mx = s.l2_error_manufactured(1)
+    assert _rel(xmx, x64) <= 1e-6
+    assert abs(l2_mx - l2_64) <= max(1e-3 * l2_64, 1e-12)
+    x = torch.from_numpy(xmx).cuda()
+    ax = torch.zeros_like(x)
+    s.apply_global(x, ax)
+    b = torch.from_numpy(s.rhs()).cuda()
+    N = 337
+    g = torch.arange(N, device="cuda")
+    edge = (g == 0) | (g == N - 1)
+    bnd = (edge[:, None, None] | edge[None, :, None] | edge[None, None, :]).reshape(-1)
+    r = torch.where(bnd, torch.zeros_like(b), b - ax)
+    assert float(torch.sqrt((r @ r) / (b @ b))) < 1e-10
+    print(f"48^3 P=7 curved: FP64 {i64.iterations} it {i64.solve_ms:.1f} ms L2 {l2_64:.3e}; "
+          f"mixed {imx.iterations} it ({imx.outer_iterations} updates) {imx.solve_ms:.1f} ms L2 {l2_mx:.3e}; "
+          f"max rel diff {_rel(xmx, x64):.2e}", flush=True)
