@@ -143,6 +143,95 @@ def bench_batch(N, torch, steps, warmup):
     return res
 
 
+def bench_curved(N):
+    """Curved 48^3 hex mesh (interior vertices +-10 % of the element width,
+    seed 3), P=7, Q=9, per-point geometry: FP64 CG against the mixed
+    FP32/FP64 reliable-update PCG, both to relative residual 1e-10."""
+    import numpy as np
+
+    s = N.HelmholtzSolver(48, 8, 9, 1.0, 0.1, 3)
+    s.set_forcing_manufactured(1)
+    tiles = s.autotune()
+    nunk = s.nglobal - s.ndirichlet
+    out = {"mesh": "48^3 curved hex, P=7, Q=9", "unknowns": nunk, "solver_tiles": list(tiles)}
+    sol = {}
+    for mixed, tag in ((False, "fp64"), (True, "mixed")):
+        s.solve(mixed=mixed, tol=1e-10)  # warm (graph capture, FP32 buffers)
+        info = s.solve(mixed=mixed, tol=1e-10)
+        sol[tag] = s.solution()
+        out[tag] = {"iterations": info.iterations, "outer": info.outer_iterations, "converged": bool(info.converged),
+                    "final_rel_residual": info.final_error, "solve_ms": round(info.solve_ms, 2),
+                    "solve_gdofs": round(nunk / (info.solve_ms * 1e-3) / 1e9, 4),
+                    "l2_error": s.l2_error_manufactured(1)}
+    out["mixed_speedup"] = round(out["fp64"]["solve_ms"] / out["mixed"]["solve_ms"], 3)
+    out["mixed_vs_fp64_max_rel_diff"] = float(np.max(np.abs(sol["mixed"] - sol["fp64"])) / np.max(np.abs(sol["fp64"])))
+    return out
+
+
+def serve_requests(count, seed=5):
+    """Helmholtz solve requests: hex box with E log-uniform in [2^10, 2^17]
+    (n = round(E^(1/3))), P ~ U{3..8}, Q = P+2, lambda ~ U[0.1, 10], k ~
+    U{1,2,3}, manufactured sine forcing, tol 1e-10."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        E = float(np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 17))))
+        P = int(rng.integers(3, 9))
+        out.append(dict(n=max(1, int(round(E ** (1 / 3)))), nummodes=P + 1, numpoints=P + 2,
+                        lam=float(rng.uniform(0.1, 10.0)), k=int(rng.integers(1, 4)), tol=1e-10, maxit=5000))
+    return out
+
+
+def bench_serve(N, count, budget_gib, streams, max_active):
+    """Serving stream: capacity of plain FP64 one-at-a-time solves, Poisson
+    arrivals at 80 % of that capacity served one at a time and by the
+    serving loop (FP64 and mixed), and the serving loop's own capacity."""
+    import numpy as np
+
+    reqs = serve_requests(count)
+    budget = int(budget_gib * (1 << 30))
+    # warm-up: load every order's kernels once
+    warm = [dict(n=2, nummodes=nm, numpoints=nm + 1, lam=1.0, k=1, tol=1e-10, maxit=5000, mixed=m, arrival_s=0.0)
+            for nm in range(4, 10) for m in (0, 1)]
+    N.Server(budget, streams, max_active).run(warm)
+
+    def run(arrivals, mixed, streams_, active_):
+        rq = [dict(r, mixed=mixed, arrival_s=float(a)) for r, a in zip(reqs, arrivals)]
+        res, st = N.Server(budget, streams_, active_).run(rq)
+        lat = np.array([x["latency_ms"] for x in res])
+        unk = float(sum(x["unknowns"] for x in res))
+        span = st["makespan_s"]
+        return {"solves_per_s": round(count / span, 3), "solve_gdofs": round(unk / span / 1e9, 4),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "makespan_s": round(span, 3), "all_converged": all(x["converged"] for x in res),
+                "max_final_error": max(x["final_error"] for x in res),
+                "iterations": int(sum(x["iterations"] for x in res)),
+                "peak_active": st["peak_active"], "peak_gib": round(st["peak_bytes"] / (1 << 30), 2),
+                "solvers_reused": st["solvers_reused"], "evictions": st["evictions"]}
+
+    zero = np.zeros(count)
+    out = {"requests": count, "hbm_budget_gib": budget_gib, "streams": streams, "max_active": max_active,
+           "unknowns_total": None}
+    cap1 = run(zero, 0, 1, 1)
+    rate = 0.8 * cap1["solves_per_s"]
+    arr = np.cumsum(np.random.default_rng(55).exponential(1.0 / rate, count))
+    arr -= arr[0]
+    out["poisson_rate_per_s"] = round(rate, 3)
+    out["fp64_one_at_a_time_capacity"] = cap1
+    out["fp64_one_at_a_time_poisson"] = run(arr, 0, 1, 1)
+    out["served_fp64_poisson"] = run(arr, 0, streams, max_active)
+    out["served_mixed_poisson"] = run(arr, 1, streams, max_active)
+    out["served_fp64_capacity"] = run(zero, 0, streams, max_active)
+    out["served_mixed_capacity"] = run(zero, 1, streams, max_active)
+    out["unknowns_total"] = int(sum((r["n"] * (r["nummodes"] - 1) - 1) ** 3 for r in reqs))
+    base = cap1["solve_gdofs"]
+    out["served_fp64_over_one_at_a_time"] = round(out["served_fp64_capacity"]["solve_gdofs"] / base, 3)
+    out["served_mixed_over_one_at_a_time"] = round(out["served_mixed_capacity"]["solve_gdofs"] / base, 3)
+    return out
+
+
 def cpu_baseline(n_side, nm, nq, target_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
@@ -176,6 +265,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the CG solves")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-curved", action="store_true", help="skip the curved 48^3 P=7 solves")
+    ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 512 MiB ten times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -265,8 +357,12 @@ def main():
         extra["unknowns"] = nunk
         del s
 
+    if not args.no_curved:
+        extra["curved_48_p7"] = bench_curved(N)
     if not args.no_batch:
         extra["batch"] = bench_batch(N, torch, steps=10, warmup=2)
+    if not args.no_serve:
+        extra["serve"] = bench_serve(N, args.serve_requests, budget_gib=64, streams=4, max_active=16)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "GDOF/s", "n_gpus": 1,

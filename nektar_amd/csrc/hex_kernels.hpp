@@ -76,6 +76,59 @@ __device__ __forceinline__ const_ptr<T> opaque(const T *p)
     return q;
 }
 
+// Marks values as needed here: the compiler waits for their loads at this
+// point (and not later, behind loads issued afterwards).
+template <typename T, int L>
+__device__ __forceinline__ void ready_s(const T (&x)[L])
+{
+#pragma unroll
+    for (int l = 0; l < L; ++l) asm volatile("" ::"s"(x[l]));
+}
+template <typename T, int L>
+__device__ __forceinline__ void ready_v(const T (&x)[L])
+{
+#pragma unroll
+    for (int l = 0; l < L; ++l) asm volatile("" ::"v"(x[l]));
+}
+
+// out[o] (+)= sum_l v[l] * row(o)[l] for o < NO, row(o) a table row of L
+// wave-uniform values.  Row o+1's scalar loads are issued once row o's
+// values have arrived and stay in flight while row o's FMAs issue; a wait
+// for scalar loads drains every outstanding LDS and scalar operation, so
+// the loop itself touches no LDS (callers store `out` afterwards).
+template <bool INIT, typename T, int NO, int L, typename RowFn>
+__device__ __forceinline__ void contract(const T (&v)[L], RowFn row, T (&out)[NO])
+{
+    ready_v(v);
+    T cur[L];
+    {
+        const auto r = row(0);
+#pragma unroll
+        for (int l = 0; l < L; ++l) cur[l] = r[l];
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+        ready_s(cur);
+        T nxt[L];
+        if (o + 1 < NO) {
+            const auto r = row(o + 1);
+#pragma unroll
+            for (int l = 0; l < L; ++l) nxt[l] = r[l];
+        }
+        T s = INIT ? fma(v[0], cur[0], out[o]) : v[0] * cur[0];
+#pragma unroll
+        for (int l = 1; l < L; ++l) s = fma(v[l], cur[l], s);
+        out[o] = s;
+        if (o + 1 < NO) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) cur[l] = nxt[l];
+        }
+    }
+    // the FMAs stay here (not sunk towards the uses of out, which would
+    // keep the table rows live in scalar registers)
+    ready_v(out);
+}
+
 // 1-D tables of one direction: B[p*NQ+i] = phi_p(z_i), D[l*NQ+i] =
 // l'_l(z_i) (column-major D(i,l)), W[i], and transposed copies BT, DT so
 // that every contraction reads one contiguous row per output.
@@ -127,33 +180,23 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
     // F1: contract p -> i along lines (q, r)
     if (t < NM * NM) {
         const int q = t % NM, r = t / NM;
-        T x[NM];
+        T x[NM], o[NQ];
 #pragma unroll
         for (int p = 0; p < NM; ++p) x[p] = valid ? xin(p, q, r) : T(0);
+        contract<false>(x, [&](int i) { return tab.BTrow(i); }, o);
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const auto b = tab.BTrow(i);
-            T s = x[0] * b[0];
-#pragma unroll
-            for (int p = 1; p < NM; ++p) s = fma(x[p], b[p], s);
-            A[(r * NM + q) * P + i] = s;
-        }
+        for (int i = 0; i < NQ; ++i) A[(r * NM + q) * P + i] = o[i];
     }
     __syncthreads();
     // F2: contract q -> j along lines (i, r)
     if (t < NQ * NM) {
         const int i = t % NQ, r = t / NQ;
-        T a[NM];
+        T a[NM], o[NQ];
 #pragma unroll
         for (int q = 0; q < NM; ++q) a[q] = A[(r * NM + q) * P + i];
+        contract<false>(a, [&](int j) { return tab.BTrow(j); }, o);
 #pragma unroll
-        for (int j = 0; j < NQ; ++j) {
-            const auto bt = tab.BTrow(j);
-            T s = a[0] * bt[0];
-#pragma unroll
-            for (int q = 1; q < NM; ++q) s = fma(a[q], bt[q], s);
-            Bs[(r * NQ + j) * P + i] = s;
-        }
+        for (int j = 0; j < NQ; ++j) Bs[(r * NQ + j) * P + i] = o[j];
     }
     __syncthreads();
     // F3: contract r -> k along lines (i, j), result stays in registers
@@ -162,14 +205,7 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
         T b[NM];
 #pragma unroll
         for (int r = 0; r < NM; ++r) b[r] = Bs[(r * NQ + j) * P + i];
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            const auto bt = tab.BTrow(k);
-            T s = b[0] * bt[0];
-#pragma unroll
-            for (int r = 1; r < NM; ++r) s = fma(b[r], bt[r], s);
-            u[k] = s;
-        }
+        contract<false>(b, [&](int k) { return tab.BTrow(k); }, u);
     }
 }
 
@@ -182,45 +218,31 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
     constexpr int P = pitch<NQ>();
     {
         const int i = t % NQ, j = t / NQ;
+        T o[NM];
+        contract<false>(y, [&](int r) { return tab.Brow(r); }, o);
 #pragma unroll
-        for (int r = 0; r < NM; ++r) {
-            const auto b = tab.Brow(r);
-            T s = y[0] * b[0];
-#pragma unroll
-            for (int k = 1; k < NQ; ++k) s = fma(y[k], b[k], s);
-            Bs[(r * NQ + j) * P + i] = s;
-        }
+        for (int r = 0; r < NM; ++r) Bs[(r * NQ + j) * P + i] = o[r];
     }
     __syncthreads();
     if (t < NQ * NM) {
         const int i = t % NQ, r = t / NQ;
-        T z[NQ];
+        T z[NQ], o[NM];
 #pragma unroll
         for (int j = 0; j < NQ; ++j) z[j] = Bs[(r * NQ + j) * P + i];
+        contract<false>(z, [&](int q) { return tab.Brow(q); }, o);
 #pragma unroll
-        for (int q = 0; q < NM; ++q) {
-            const auto b = tab.Brow(q);
-            T s = z[0] * b[0];
-#pragma unroll
-            for (int j = 1; j < NQ; ++j) s = fma(z[j], b[j], s);
-            C[(r * NM + q) * P + i] = s;
-        }
+        for (int q = 0; q < NM; ++q) C[(r * NM + q) * P + i] = o[q];
     }
     __syncthreads();
     if (t < NM * NM) {
         const int q = t % NM, r = t / NM;
-        T z[NQ];
+        T z[NQ], o[NM];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) z[i] = C[(r * NM + q) * P + i];
+        contract<false>(z, [&](int p) { return tab.Brow(p); }, o);
         if (valid) {
 #pragma unroll
-            for (int p = 0; p < NM; ++p) {
-                const auto b = tab.Brow(p);
-                T s = z[0] * b[0];
-#pragma unroll
-                for (int i = 1; i < NQ; ++i) s = fma(z[i], b[i], s);
-                xout[(r * NM + q) * NM + p] = s;
-            }
+            for (int p = 0; p < NM; ++p) xout[(r * NM + q) * NM + p] = o[p];
         }
     }
 }
@@ -233,44 +255,27 @@ __device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const 
     constexpr int P = pitch<NQ>();
     const int l0 = t % NQ, k = t / NQ;
     {   // line (j = l0, k) along i
-        T v[NQ];
+        T v[NQ], o[NQ];
 #pragma unroll
         for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l0) * P + l];
+        contract<false>(v, [&](int i) { return tab.DTrow(i); }, o);
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const auto d = tab.DTrow(i);
-            T s = v[0] * d[0];
-#pragma unroll
-            for (int l = 1; l < NQ; ++l) s = fma(v[l], d[l], s);
-            Bs[(k * NQ + l0) * P + i] = s;
-        }
+        for (int i = 0; i < NQ; ++i) Bs[(k * NQ + l0) * P + i] = o[i];
     }
     {   // line (i = l0, k) along j
-        T v[NQ];
+        T v[NQ], o[NQ];
 #pragma unroll
         for (int l = 0; l < NQ; ++l) v[l] = A[(k * NQ + l) * P + l0];
+        contract<false>(v, [&](int j) { return tab.DTrow(j); }, o);
 #pragma unroll
-        for (int j = 0; j < NQ; ++j) {
-            const auto d = tab.DTrow(j);
-            T s = v[0] * d[0];
-#pragma unroll
-            for (int l = 1; l < NQ; ++l) s = fma(v[l], d[l], s);
-            C[(k * NQ + j) * P + l0] = s;
-        }
+        for (int j = 0; j < NQ; ++j) C[(k * NQ + j) * P + l0] = o[j];
     }
 }
 
 template <typename T, int NM, int NQ>
 __device__ __forceinline__ void deriv2_column(const Tab<T, NM, NQ> &tab, const T (&u)[NQ], T (&d2)[NQ])
 {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-        const auto d = tab.DTrow(k);
-        T s = u[0] * d[0];
-#pragma unroll
-        for (int l = 1; l < NQ; ++l) s = fma(u[l], d[l], s);
-        d2[k] = s;
-    }
+    contract<false>(u, [&](int k) { return tab.DTrow(k); }, d2);
 }
 
 // ===================================================================
@@ -398,6 +403,10 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
     __syncthreads();
     deriv01<T, NM, NQ>(tab, t, A, Bs, C);
     __syncthreads();
+    // direction-2 derivative of the register column (formed after the
+    // barrier, next to its use, so no table row is held across barriers)
+    T d2[NQ];
+    deriv2_column<T, NM, NQ>(tab, u, d2);
 
     // pointwise: metric, Jacobian, weights, mass term
     T y[NQ], v2[NQ];
@@ -419,13 +428,7 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
                 w = wij * tab.W(k);
             }
             const int lc = (k * NQ + j) * P + i;
-            // direction-2 derivative of the register column, formed where
-            // it is used so no table row stays live across the barriers
-            const auto d = tab.DTrow(k);
-            T a2 = u[0] * d[0];
-#pragma unroll
-            for (int l = 1; l < NQ; ++l) a2 = fma(u[l], d[l], a2);
-            const T a0 = Bs[lc], a1 = C[lc];
+            const T a0 = Bs[lc], a1 = C[lc], a2 = d2[k];
             T t0 = g[0] * a0;
             t0 = fma(g[1], a1, t0);
             t0 = fma(g[2], a2, t0);
@@ -447,43 +450,29 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
         }
     }
     // D2^T v2 (in registers): (D^T v)(k) = sum_l D(l,k) v(l), D(out,in)=tab.D(in,out)
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-        const auto d = tab.Drow(k);
-        T s = y[k];
-#pragma unroll
-        for (int l = 0; l < NQ; ++l) s = fma(d[l], v2[l], s);
-        y[k] = s;
-    }
+    contract<true>(v2, [&](int k) { return tab.Drow(k); }, y);
     __syncthreads();
     {   // D0^T along i on line (j = l0, k): A <- D0^T Bs
         const int l0 = t % NQ, k = t / NQ;
-        T v[NQ];
+        T v[NQ], o[NQ];
 #pragma unroll
         for (int l = 0; l < NQ; ++l) v[l] = Bs[(k * NQ + l0) * P + l];
+        contract<false>(v, [&](int ii) { return tab.Drow(ii); }, o);
 #pragma unroll
-        for (int ii = 0; ii < NQ; ++ii) {
-            const auto d = tab.Drow(ii);
-            T s = v[0] * d[0];
-#pragma unroll
-            for (int l = 1; l < NQ; ++l) s = fma(v[l], d[l], s);
-            A[(k * NQ + l0) * P + ii] = s;
-        }
+        for (int ii = 0; ii < NQ; ++ii) A[(k * NQ + l0) * P + ii] = o[ii];
     }
     __syncthreads();
     {   // D1^T along j on line (i = l0, k): A += D1^T C
         const int l0 = t % NQ, k = t / NQ;
-        T v[NQ];
+        T v[NQ], o[NQ];
 #pragma unroll
         for (int l = 0; l < NQ; ++l) v[l] = C[(k * NQ + l) * P + l0];
 #pragma unroll
-        for (int jj = 0; jj < NQ; ++jj) {
-            const auto d = tab.Drow(jj);
-            T s = A[(k * NQ + jj) * P + l0];
+        for (int jj = 0; jj < NQ; ++jj) o[jj] = A[(k * NQ + jj) * P + l0];
+        ready_v(o);
+        contract<true>(v, [&](int jj) { return tab.Drow(jj); }, o);
 #pragma unroll
-            for (int l = 0; l < NQ; ++l) s = fma(v[l], d[l], s);
-            A[(k * NQ + jj) * P + l0] = s;
-        }
+        for (int jj = 0; jj < NQ; ++jj) A[(k * NQ + jj) * P + l0] = o[jj];
     }
     __syncthreads();
 #pragma unroll
