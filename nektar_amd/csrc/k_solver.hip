@@ -49,19 +49,30 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *out)
     }
 }
 
-// Entries of 1-D global node g: element index and local mode (<= 2 entries).
-__device__ __forceinline__ int node_contrib(int g, int nm, int n, int (&e)[2], int (&p)[2])
+// Entries of 1-D global node g: element index and local mode (<= 2
+// entries).  NM is a compile-time constant, so the divisions by NM-1 are
+// multiplications.
+template <int NM>
+__device__ __forceinline__ int node_contrib(int g, int n, int (&e)[2], int (&p)[2])
 {
-    const int s = nm - 1;
-    if (g % s == 0) {
-        const int v = g / s;
+    constexpr int s = NM - 1;
+    const int v = g / s, m = g - v * s;
+    if (m == 0) {
         int c = 0;
-        if (v > 0) { e[c] = v - 1; p[c] = 1; ++c; }
-        if (v < n) { e[c] = v; p[c] = 0; ++c; }
+        if (v > 0) {
+            e[c] = v - 1;
+            p[c] = 1;
+            ++c;
+        }
+        if (v < n) {
+            e[c] = v;
+            p[c] = 0;
+            ++c;
+        }
         return c;
     }
-    e[0] = g / s;
-    p[0] = g % s + 1;
+    e[0] = v;
+    p[0] = m + 1;
     return 1;
 }
 
@@ -186,33 +197,47 @@ __global__ void l2err_kernel(int n, int NQ, const double *__restrict__ X, const 
 // (AssemblyMapCG::Assemble, AssemblyMapCG.cpp:2860-2886) as a gather: each
 // global row sums its <= 8 element contributions in a fixed order.
 // Dirichlet rows are set to 0.  Optionally accumulates s . wdot.
-template <typename T>
-__global__ void assemble_kernel(int n, int nm, int N, const T *__restrict__ sloc, T *__restrict__ sg,
-                                const T *__restrict__ wdot, double *__restrict__ partials, const int *__restrict__ done,
-                                int keep_dirichlet)
+template <typename T, int NM>
+__global__ __launch_bounds__(kBlock) void assemble_kernel(int n, int N, const T *__restrict__ sloc,
+                                                          T *__restrict__ sg, const T *__restrict__ wdot,
+                                                          double *__restrict__ partials,
+                                                          const int *__restrict__ done, int keep_dirichlet)
 {
     if (done && *done) return;
-    const size_t NG = (size_t)N * N * N;
-    const size_t nc = (size_t)nm * nm * nm;
+    // One wave per grid line (gy, gz) of global nodes: the y and z
+    // contributions and the element-row offsets are wave-uniform, lanes
+    // run along gx (consecutive gx are consecutive modes of one element
+    // row, so the reads of sloc are nearly contiguous).
+    constexpr int WAVES = kBlock / 64;
+    constexpr size_t NC = (size_t)NM * NM * NM;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const long lines = (long)N * N;
     double acc[1] = {0.0};
-    for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
-        const int gx = (int)(G % N), gy = (int)((G / N) % N), gz = (int)(G / ((size_t)N * N));
-        if (!keep_dirichlet && on_boundary(gx, gy, gz, N)) {
-            sg[G] = T(0);
-            continue;
-        }
-        int ex[2], px[2], ey[2], py[2], ez[2], pz[2];
-        const int cx = node_contrib(gx, nm, n, ex, px), cy = node_contrib(gy, nm, n, ey, py),
-                  cz = node_contrib(gz, nm, n, ez, pz);
-        double s = 0.0;
+    for (long L = (long)blockIdx.x * WAVES + wave; L < lines; L += (long)gridDim.x * WAVES) {
+        const int gz = (int)(L / N), gy = (int)(L - (long)gz * N);
+        int ey[2], py[2], ez[2], pz[2];
+        const int cy = node_contrib<NM>(gy, n, ey, py), cz = node_contrib<NM>(gz, n, ez, pz);
+        const bool edge_line = gy == 0 || gz == 0 || gy == N - 1 || gz == N - 1;
+        size_t base[4];
+        int nrow = 0;
         for (int c = 0; c < cz; ++c)
             for (int b = 0; b < cy; ++b)
-                for (int a = 0; a < cx; ++a) {
-                    const size_t e = ((size_t)ez[c] * n + ey[b]) * n + ex[a];
-                    s += (double)sloc[e * nc + ((size_t)pz[c] * nm + py[b]) * nm + px[a]];
-                }
-        sg[G] = T(s);
-        if (wdot) acc[0] += (double)T(s) * (double)wdot[G];
+                base[nrow++] = ((size_t)ez[c] * n + ey[b]) * n * NC + ((size_t)pz[c] * NM + py[b]) * NM;
+        T *out = sg + (size_t)L * N;
+        const T *wl = wdot ? wdot + (size_t)L * N : nullptr;
+        for (int gx = lane; gx < N; gx += 64) {
+            if (!keep_dirichlet && (edge_line || gx == 0 || gx == N - 1)) {
+                out[gx] = T(0);
+                continue;
+            }
+            int ex[2], px[2];
+            const int cx = node_contrib<NM>(gx, n, ex, px);
+            double s = 0.0;
+            for (int r = 0; r < nrow; ++r)
+                for (int a = 0; a < cx; ++a) s += (double)sloc[base[r] + (size_t)ex[a] * NC + px[a]];
+            out[gx] = T(s);
+            if (wl) acc[0] += (double)T(s) * (double)wl[gx];
+        }
     }
     if (partials) block_sum<1>(acc, partials + blockIdx.x);
 }
@@ -619,8 +644,14 @@ template <typename T>
 hipError_t launch_assemble(hipStream_t stream, int n, int nm, int N, const T *sloc, T *sg, const T *wdot,
                            double *partials, const int *done, bool keep_dirichlet)
 {
-    NMF_LAUNCH((assemble_kernel<T>), grid_for((size_t)N * N * N), n, nm, N, sloc, sg, wdot, partials, done,
-               keep_dirichlet ? 1 : 0);
+    const int grid = grid_for((size_t)N * N * N), keep = keep_dirichlet ? 1 : 0;
+    switch (nm) {
+#define NMF_ASM(M) \
+    case M: NMF_LAUNCH((assemble_kernel<T, M>), grid, n, N, sloc, sg, wdot, partials, done, keep); break;
+        NMF_ASM(2) NMF_ASM(3) NMF_ASM(4) NMF_ASM(5) NMF_ASM(6) NMF_ASM(7) NMF_ASM(8) NMF_ASM(9) NMF_ASM(10)
+#undef NMF_ASM
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
