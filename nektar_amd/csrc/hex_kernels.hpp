@@ -317,28 +317,49 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void physderiv_
 
 // Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
 // Body shared by the Collections kernel (element-local input) and the
-// solver kernel (input gathered from the global vector).
+// solver kernel (input gathered from the global vector).  Two LDS slabs
+// per element: the direction-1 passes work in place on lines that belong
+// to one work-item each, so the third slab of a straightforward staging is
+// not needed (more elements resident per CU).
 template <typename T, int NM, int NQ, bool DEF, typename Fetch>
 __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, const Fetch &fetch, bool valid, int e,
-                                                  int t, const T *__restrict__ geo, T lambda, T *A, T *Bs, T *C,
+                                                  int t, const T *__restrict__ geo, T lambda, T *A, T *B,
                                                   T *__restrict__ xout)
 {
     constexpr int NQ3 = NQ * NQ * NQ, P = pitch<NQ>();
+    // column (i, j) of this work-item; as a line index: (l0, k0) = (i, j)
     const int i = t % NQ, j = t / NQ;
+    const int l0 = i, k0 = j;
 
     T u[NQ];
-    bwd_to_column<T, NM, NQ>(tab, fetch, valid, t, A, Bs, u);
+    bwd_to_column<T, NM, NQ>(tab, fetch, valid, t, A, B, u);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) A[(k * NQ + j) * P + i] = u[k];
     __syncthreads();
-    deriv01<T, NM, NQ>(tab, t, A, Bs, C);
+    {   // d0 along i on line (j = l0, k = k0): A -> B
+        T v[NQ], o[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k0 * NQ + l0) * P + l];
+        contract<false>(v, [&](int ii) { return tab.DTrow(ii); }, o);
+#pragma unroll
+        for (int ii = 0; ii < NQ; ++ii) B[(k0 * NQ + l0) * P + ii] = o[ii];
+    }
+    __syncthreads();
+    {   // d1 along j on line (i = l0, k = k0), in place in A
+        T v[NQ], o[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k0 * NQ + l) * P + l0];
+        contract<false>(v, [&](int jj) { return tab.DTrow(jj); }, o);
+#pragma unroll
+        for (int jj = 0; jj < NQ; ++jj) A[(k0 * NQ + jj) * P + l0] = o[jj];
+    }
     __syncthreads();
     // direction-2 derivative of the register column (formed after the
     // barrier, next to its use, so no table row is held across barriers)
     T d2[NQ];
     deriv2_column<T, NM, NQ>(tab, u, d2);
 
-    // pointwise: metric, Jacobian, weights, mass term
+    // pointwise: metric, Jacobian, weights, mass term (d0 in B, d1 in A)
     T y[NQ], v2[NQ];
     {
         T g[7];
@@ -358,7 +379,7 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
                 w = wij * tab.W(k);
             }
             const int lc = (k * NQ + j) * P + i;
-            const T a0 = Bs[lc], a1 = C[lc], a2 = d2[k];
+            const T a0 = B[lc], a1 = A[lc], a2 = d2[k];
             T t0 = g[0] * a0;
             t0 = fma(g[1], a1, t0);
             t0 = fma(g[2], a2, t0);
@@ -373,8 +394,8 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
                 t1 *= w;
                 t2 *= w;
             }
-            Bs[lc] = t0;
-            C[lc] = t1;
+            B[lc] = t0;
+            A[lc] = t1;
             v2[k] = t2;
             y[k] = (lambda * g[6]) * (DEF ? u[k] : w * u[k]);
         }
@@ -382,32 +403,30 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
     // D2^T v2 (in registers): (D^T v)(k) = sum_l D(l,k) v(l), D(out,in)=tab.D(in,out)
     contract<true>(v2, [&](int k) { return tab.Drow(k); }, y);
     __syncthreads();
-    {   // D0^T along i on line (j = l0, k): A <- D0^T Bs
-        const int l0 = t % NQ, k = t / NQ;
+    {   // D0^T along i on line (j = l0, k = k0), in place in B
         T v[NQ], o[NQ];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = Bs[(k * NQ + l0) * P + l];
+        for (int l = 0; l < NQ; ++l) v[l] = B[(k0 * NQ + l0) * P + l];
         contract<false>(v, [&](int ii) { return tab.Drow(ii); }, o);
 #pragma unroll
-        for (int ii = 0; ii < NQ; ++ii) A[(k * NQ + l0) * P + ii] = o[ii];
+        for (int ii = 0; ii < NQ; ++ii) B[(k0 * NQ + l0) * P + ii] = o[ii];
     }
     __syncthreads();
-    {   // D1^T along j on line (i = l0, k): A += D1^T C
-        const int l0 = t % NQ, k = t / NQ;
+    {   // D1^T along j on line (i = l0, k = k0): A <- B + D1^T A, in place in A
         T v[NQ], o[NQ];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = C[(k * NQ + l) * P + l0];
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k0 * NQ + l) * P + l0];
 #pragma unroll
-        for (int jj = 0; jj < NQ; ++jj) o[jj] = A[(k * NQ + jj) * P + l0];
+        for (int jj = 0; jj < NQ; ++jj) o[jj] = B[(k0 * NQ + jj) * P + l0];
         ready_v(o);
         contract<true>(v, [&](int jj) { return tab.Drow(jj); }, o);
 #pragma unroll
-        for (int jj = 0; jj < NQ; ++jj) A[(k * NQ + jj) * P + l0] = o[jj];
+        for (int jj = 0; jj < NQ; ++jj) A[(k0 * NQ + jj) * P + l0] = o[jj];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NQ; ++k) y[k] += A[(k * NQ + j) * P + i];
-    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, xout);
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, B, A, xout);
 }
 
 // Tile shapes (elements per workgroup) instantiated for the Helmholtz
@@ -420,7 +439,7 @@ constexpr int tile_fill()
 template <typename T, int NQ, int EPB>
 constexpr bool tile_fits()
 {
-    return EPB * NQ * NQ <= 1024 && EPB * 3 * slab<NQ>() * (int)sizeof(T) <= 65536;
+    return EPB * NQ * NQ <= 1024 && EPB * 2 * slab<NQ>() * (int)sizeof(T) <= 65536;
 }
 
 // Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
@@ -430,13 +449,13 @@ __global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_kernel(int nelmt, con
                                                                   const T *__restrict__ in, T *__restrict__ out)
 {
     constexpr int TPE = NQ * NQ;
-    __shared__ T lds[EPB][3][slab<NQ>()];
+    __shared__ T lds[EPB][2][slab<NQ>()];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
     const bool valid = e < nelmt;
     helmholtz_element<T, NM, NQ, DEF>(tab, LocalFetch<T, NM>{in + (size_t)e * NM * NM * NM}, valid, e, t, geo,
-                                      lambda, lds[le][0], lds[le][1], lds[le][2], out + (size_t)e * NM * NM * NM);
+                                      lambda, lds[le][0], lds[le][1], out + (size_t)e * NM * NM * NM);
 }
 
 // Grouped form: several operators of the same kind in one launch; block b
@@ -447,7 +466,7 @@ __global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_grouped_kernel(int co
                                                                           const T *__restrict__ tabp)
 {
     constexpr int TPE = NQ * NQ;
-    __shared__ T lds[EPB][3][slab<NQ>()];
+    __shared__ T lds[EPB][2][slab<NQ>()];
     const int b = blockIdx.x;
     int it = 0;
     while (it + 1 < count && b >= items[it + 1].block0) ++it;
@@ -457,8 +476,7 @@ __global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_grouped_kernel(int co
     const int e = (b - I.block0) * EPB + le;
     const bool valid = e < I.nelmt;
     helmholtz_element<T, NM, NQ, DEF>(tab, LocalFetch<T, NM>{I.in + (size_t)e * NM * NM * NM}, valid, e, t, I.geo,
-                                      I.lambda, lds[le][0], lds[le][1], lds[le][2],
-                                      I.out + (size_t)e * NM * NM * NM);
+                                      I.lambda, lds[le][0], lds[le][1], I.out + (size_t)e * NM * NM * NM);
 }
 
 // Solver form: x gathered from the global vector of the structured n^3
@@ -472,7 +490,7 @@ __global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_gather_kernel(int n, 
 {
     if (done && *done) return;
     constexpr int TPE = NQ * NQ;
-    __shared__ T lds[EPB][3][slab<NQ>()];
+    __shared__ T lds[EPB][2][slab<NQ>()];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * EPB + le;
@@ -480,7 +498,7 @@ __global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_gather_kernel(int n, 
     const bool valid = e < nelmt;
     const int ex = e % n, ey = (e / n) % n, ez = e / (n * n);
     const StructuredFetch<T, NM> f{xg, N, ex * (NM - 1), ey * (NM - 1), ez * (NM - 1)};
-    helmholtz_element<T, NM, NQ, DEF>(tab, f, valid, e, t, geo, lambda, lds[le][0], lds[le][1], lds[le][2],
+    helmholtz_element<T, NM, NQ, DEF>(tab, f, valid, e, t, geo, lambda, lds[le][0], lds[le][1],
                                       out + (size_t)e * NM * NM * NM);
 }
 
