@@ -3,6 +3,7 @@
 // MatrixFree / Collections classes can switch to the MI355X library.
 //
 //   Nektar::MatrixFree::Operator            <- MatrixFreeOps/Operator.hpp:70-366
+//   Nektar::MatrixFree::IProductWRTDerivBase <- MatrixFreeOps/Operator.hpp:192-217
 //   Nektar::MatrixFree::GetOperatorFactory  <- MatrixFreeOps/Operator.cpp:40-44
 //   Nektar::MatrixFree::GetOpstring         <- MatrixFreeOps/Operator.cpp:46-90
 //   Nektar::Collections::Helmholtz_MatrixFree::UpdateFactors
@@ -74,7 +75,11 @@ public:
     Operator(const Operator &) = delete;
     Operator &operator=(const Operator &) = delete;
 
-    bool NeedsDF() const { return m_key.rfind("PhysDeriv", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0; }
+    bool NeedsDF() const
+    {
+        return m_key.rfind("PhysDeriv", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0 ||
+               m_key.rfind("IProductWRTDerivBase", 0) == 0;
+    }
     bool NeedsJac() const { return m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0; }
 
     // SetJac / SetDF take per-element (not SIMD-interleaved) factors.
@@ -111,7 +116,7 @@ protected:
     }
     size_t OutputSize() const
     {
-        const bool coeffs = m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0;
+        const bool coeffs = m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0;  // incl. WRTDerivBase
         return (size_t)m_nElmt * (coeffs ? m_ncoeff : m_nqtot);
     }
     static void CheckSize(size_t got, size_t want, const char *what)
@@ -141,6 +146,20 @@ public:
     }
 };
 
+class IProductWRTDerivBase : public Operator {
+public:
+    using Operator::Operator;
+    // operator()(input[3], output) (Operator.hpp:213-215)
+    void operator()(const std::vector<std::vector<double>> &input, std::vector<double> &output)
+    {
+        if (input.size() != 3) throw NekError(NMF_ERR_INVALID, "IProductWRTDerivBase takes three inputs");
+        for (const auto &in : input) CheckSize(in.size(), (size_t)m_nElmt * m_nqtot, "IProductWRTDerivBase");
+        output.resize(OutputSize());
+        NmfCheck(nmf_operator_apply_iproduct_deriv_host(m_op, input[0].data(), input[1].data(), input[2].data(),
+                                                       output.data()));
+    }
+};
+
 class Helmholtz : public Operator {
 public:
     using Operator::Operator;
@@ -160,6 +179,8 @@ public:
                                              int nElmt, int precision = NMF_FP64) const
     {
         if (key.rfind("PhysDeriv", 0) == 0) return std::make_shared<PhysDeriv>(key, basis, nElmt, precision);
+        if (key.rfind("IProductWRTDerivBase", 0) == 0)
+            return std::make_shared<IProductWRTDerivBase>(key, basis, nElmt, precision);
         if (key.rfind("Helmholtz", 0) == 0) return std::make_shared<Helmholtz>(key, basis, nElmt, precision);
         return std::make_shared<Operator>(key, basis, nElmt, precision);
     }

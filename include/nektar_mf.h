@@ -90,7 +90,8 @@ int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0,
  * ------------------------------------------------------------------- */
 /* MatrixFree::GetOperatorFactory().CreateInstance(key, basis, nElmt)
  * (Operator.cpp:40-90).  key = "<Op>_<Shape>_<Regular|Deformed>" as built
- * by GetOpstring, Op in {BwdTrans, IProduct, PhysDeriv, Helmholtz},
+ * by GetOpstring, Op in {BwdTrans, IProduct, IProductWRTDerivBase,
+ * PhysDeriv, Helmholtz},
  * Shape in {Hex, Tet}.  nummodes = P+1 per direction, numpoints0 = Q in
  * direction 0 (tets use Q-1 Gauss-Radau points in directions 1, 2). */
 int nmf_operator_create(const char *key, int nummodes, int numpoints0,
@@ -115,6 +116,17 @@ int nmf_operator_apply(nmf_operator op, const void *in, void *out,
 /* PhysDeriv operator()(input, output[3]) (Operator.hpp:238-240) */
 int nmf_operator_apply_physderiv(nmf_operator op, const void *in, void *out0,
                                  void *out1, void *out2, void *stream);
+/* IProductWRTDerivBase operator()(input[3], output) (Operator.hpp:192-217,
+ * IProductWRTDerivBase.h:592-825): out = sum_j IProduct w.r.t. dphi/dx_j of
+ * in_j; inputs [e][nqtot] at the quadrature points, output [e][ncoeff].
+ * Key "IProductWRTDerivBase_<Hex|Tet>_<Regular|Deformed>"; needs SetJac and
+ * SetDF. */
+int nmf_operator_apply_iproduct_deriv(nmf_operator op, const void *in0,
+                                      const void *in1, const void *in2,
+                                      void *out, void *stream);
+int nmf_operator_apply_iproduct_deriv_host(nmf_operator op, const double *in0,
+                                           const double *in1,
+                                           const double *in2, double *out);
 /* Host FP64 convenience forms (copy in, apply, copy out, synchronise). */
 int nmf_operator_apply_host(nmf_operator op, const double *in, double *out);
 int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in,

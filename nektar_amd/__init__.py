@@ -86,6 +86,8 @@ _SIGNATURES = {
     "nmf_operator_apply_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "nmf_operator_apply_host": (ctypes.c_int, [_P, _D, _D]),
     "nmf_operator_apply_physderiv_host": (ctypes.c_int, [_P, _D, _D, _D, _D]),
+    "nmf_operator_apply_iproduct_deriv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "nmf_operator_apply_iproduct_deriv_host": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "nmf_operator_tile_candidates": (ctypes.c_int, [_P, _I, ctypes.c_int, _I]),
     "nmf_operator_set_tile": (ctypes.c_int, [_P, ctypes.c_int]),
     "nmf_operator_get_tile": (ctypes.c_int, [_P, _I]),
@@ -241,18 +243,27 @@ class Operator:
         return self.nelmt * (self.ncoeff if self.op in ("BwdTrans", "Helmholtz") else self.nqtot)
 
     def output_size(self) -> int:
-        return self.nelmt * (self.ncoeff if self.op in ("IProduct", "Helmholtz") else self.nqtot)
+        return self.nelmt * (self.ncoeff if self.op in ("IProduct", "Helmholtz", "IProductWRTDerivBase")
+                             else self.nqtot)
 
     def torch_dtype(self):
         return torch.float32 if self.precision == FP32 else torch.float64
 
     # --- operator() -------------------------------------------------------
     def __call__(self, inp, out, out1=None, out2=None, stream=None):
-        """Apply on device arrays (torch tensors or raw device addresses)."""
+        """Apply on device arrays (torch tensors or raw device addresses).
+        IProductWRTDerivBase takes a sequence of three inputs."""
         dt = self.torch_dtype() if torch is not None else None
         s = stream
         if s is None and torch is not None:
             s = torch.cuda.current_stream().cuda_stream
+        if self.op == "IProductWRTDerivBase":
+            if len(inp) != 3:
+                raise NektarError(1, "IProductWRTDerivBase takes three input arrays")
+            ptrs = [_device_ptr(a, dt, self.input_size()) for a in inp]
+            check(lib().nmf_operator_apply_iproduct_deriv(self._h, *ptrs, _device_ptr(out, dt, self.output_size()),
+                                                          s))
+            return
         pin = _device_ptr(inp, dt, self.input_size())
         if self.op == "PhysDeriv":
             ptrs = [_device_ptr(o, dt, self.output_size()) for o in (out, out1, out2)]
@@ -289,8 +300,16 @@ class Operator:
         check(lib().nmf_operator_autotune(self._h, s, ctypes.byref(t), times, len(cand)))
         return t.value, {c: times[i] for i, c in enumerate(cand)}
 
-    def apply_host(self, inp: np.ndarray):
-        """Host FP64 in / out (synchronous); returns ndarray(s)."""
+    def apply_host(self, inp):
+        """Host FP64 in / out (synchronous); returns ndarray(s).
+        IProductWRTDerivBase takes a sequence of three arrays."""
+        if self.op == "IProductWRTDerivBase":
+            xs = [np.ascontiguousarray(a, dtype=np.float64) for a in inp]
+            if len(xs) != 3 or any(a.size != self.input_size() for a in xs):
+                raise NektarError(1, f"expected three inputs of {self.input_size()} values")
+            y = np.zeros(self.output_size())
+            check(lib().nmf_operator_apply_iproduct_deriv_host(self._h, *map(_dptr, xs), _dptr(y)))
+            return y
         x = np.ascontiguousarray(inp, dtype=np.float64)
         if x.size != self.input_size():
             raise NektarError(1, f"input has {x.size} values, expected {self.input_size()}")

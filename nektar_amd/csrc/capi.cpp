@@ -171,6 +171,8 @@ int nmf_operator_apply(nmf_operator op, const void *in, void *out, void *stream)
         NMF_OP_CHECK();
         NMF_REQUIRE(op->impl->op != OpType::PhysDeriv, NMF_ERR_INVALID,
                     "PhysDeriv has three outputs: use nmf_operator_apply_physderiv");
+        NMF_REQUIRE(op->impl->op != OpType::IProductDeriv, NMF_ERR_INVALID,
+                    "IProductWRTDerivBase has three inputs: use nmf_operator_apply_iproduct_deriv");
         op->impl->apply(in, out, nullptr, nullptr, static_cast<hipStream_t>(stream));
     });
 }
@@ -191,7 +193,9 @@ size_t in_count(const MFOperator &o)
 }
 size_t out_count(const MFOperator &o)
 {
-    return (size_t)o.nelmt * (o.op == OpType::IProduct || o.op == OpType::Helmholtz ? o.ncoeff : o.nqtot);
+    return (size_t)o.nelmt *
+           (o.op == OpType::IProduct || o.op == OpType::Helmholtz || o.op == OpType::IProductDeriv ? o.ncoeff
+                                                                                                   : o.nqtot);
 }
 void to_device(DeviceBuffer &b, const double *h, size_t n, bool fp32)
 {
@@ -217,11 +221,39 @@ int nmf_operator_apply_host(nmf_operator op, const double *in, double *out)
         NMF_OP_CHECK();
         MFOperator &o = *op->impl;
         NMF_REQUIRE(o.op != OpType::PhysDeriv, NMF_ERR_INVALID, "use nmf_operator_apply_physderiv_host");
+        NMF_REQUIRE(o.op != OpType::IProductDeriv, NMF_ERR_INVALID, "use nmf_operator_apply_iproduct_deriv_host");
         const size_t ni = in_count(o), no = out_count(o);
         DeviceBuffer di, dout;
         to_device(di, in, ni, o.fp32);
         dout.allocate(no * (o.fp32 ? sizeof(float) : sizeof(double)));
         o.apply(di.get(), dout.get(), nullptr, nullptr, nullptr);
+        NMF_HIP(hipDeviceSynchronize());
+        to_host(dout, out, no, o.fp32);
+    });
+}
+
+int nmf_operator_apply_iproduct_deriv(nmf_operator op, const void *in0, const void *in1, const void *in2, void *out,
+                                      void *stream)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        op->impl->apply_iproduct_deriv(in0, in1, in2, out, static_cast<hipStream_t>(stream));
+    });
+}
+
+int nmf_operator_apply_iproduct_deriv_host(nmf_operator op, const double *in0, const double *in1,
+                                           const double *in2, double *out)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        MFOperator &o = *op->impl;
+        NMF_REQUIRE(o.op == OpType::IProductDeriv, NMF_ERR_INVALID, "not an IProductWRTDerivBase operator");
+        const size_t n = (size_t)o.nelmt * o.nqtot, no = out_count(o), sz = o.fp32 ? sizeof(float) : sizeof(double);
+        DeviceBuffer d0, d1, d2, dout(no * sz);
+        to_device(d0, in0, n, o.fp32);
+        to_device(d1, in1, n, o.fp32);
+        to_device(d2, in2, n, o.fp32);
+        o.apply_iproduct_deriv(d0.get(), d1.get(), d2.get(), dout.get(), nullptr);
         NMF_HIP(hipDeviceSynchronize());
         to_host(dout, out, no, o.fp32);
     });
@@ -313,6 +345,8 @@ int nmf_batch_add(nmf_batch b, nmf_operator op, const void *in, void *out)
         NMF_OP_CHECK();
         NMF_REQUIRE(op->impl->op != OpType::PhysDeriv, NMF_ERR_INVALID,
                     "PhysDeriv has three outputs: use nmf_batch_add_physderiv");
+        NMF_REQUIRE(op->impl->op != OpType::IProductDeriv, NMF_ERR_UNSUPPORTED,
+                    "IProductWRTDerivBase operators are not batched");
         b->impl->add(op->impl.get(), in, out, nullptr, nullptr);
     });
 }

@@ -315,6 +315,51 @@ __global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void physderiv_
     }
 }
 
+// Second half shared by the fused Helmholtz apply and IProductWRTDerivBase:
+// the weighted integrands of the three reference directions are in B
+// (direction 0), A (direction 1) and the register column v2 (direction 2)
+// at the quadrature points; y (register column) gets sum_d D_d^T of them
+// (IProduct w.r.t. dbdata_d = D_d bdata_d, Basis.cpp:493-495), and the
+// unweighted transpose transform of y gives the coefficients.  The caller's
+// writes of A and B must be followed by no barrier yet (one is taken here).
+template <typename T, int NM, int NQ, typename Store>
+__device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &tab, bool valid, int t,
+                                                         T (&y)[NQ], const T (&v2)[NQ], T *A, T *B,
+                                                         const Store &xout)
+{
+    constexpr int P = pitch<NQ>();
+    const int i = t % NQ, j = t / NQ;
+    const int l0 = i, k0 = j;
+    // D2^T v2 (in registers): (D^T v)(k) = sum_l D(l,k) v(l), D(out,in)=tab.D(in,out)
+    contract<true>(v2, [&](int k) { return tab.Drow(k); }, y);
+    __syncthreads();
+    {   // D0^T along i on line (j = l0, k = k0), in place in B
+        T v[NQ], o[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = B[(k0 * NQ + l0) * P + l];
+        contract<false>(v, [&](int ii) { return tab.Drow(ii); }, o);
+#pragma unroll
+        for (int ii = 0; ii < NQ; ++ii) B[(k0 * NQ + l0) * P + ii] = o[ii];
+    }
+    __syncthreads();
+    {   // D1^T along j on line (i = l0, k = k0): A <- B + D1^T A, in place in A
+        T v[NQ], o[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = A[(k0 * NQ + l) * P + l0];
+#pragma unroll
+        for (int jj = 0; jj < NQ; ++jj) o[jj] = B[(k0 * NQ + jj) * P + l0];
+        ready_v(o);
+        contract<true>(v, [&](int jj) { return tab.Drow(jj); }, o);
+#pragma unroll
+        for (int jj = 0; jj < NQ; ++jj) A[(k0 * NQ + jj) * P + l0] = o[jj];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) y[k] += A[(k * NQ + j) * P + i];
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, B, A, xout);
+}
+
+
 // Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)
 // Body shared by the Collections kernel (element-local input) and the
 // solver kernel (input gathered from the global vector).  Two LDS slabs
@@ -400,33 +445,65 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
             y[k] = (lambda * g[6]) * (DEF ? u[k] : w * u[k]);
         }
     }
-    // D2^T v2 (in registers): (D^T v)(k) = sum_l D(l,k) v(l), D(out,in)=tab.D(in,out)
-    contract<true>(v2, [&](int k) { return tab.Drow(k); }, y);
-    __syncthreads();
-    {   // D0^T along i on line (j = l0, k = k0), in place in B
-        T v[NQ], o[NQ];
+    grad_transpose_to_coeffs<T, NM, NQ>(tab, valid, t, y, v2, A, B, xout);
+}
+
+// IProductWRTDerivBase (IProductWRTDerivBase.h:751-788, hex pointwise step
+// IProductWRTDerivBaseKernels.hpp:194-247): out = sum_d IProduct(dbdata_d,
+// t_d) with t_d = sum_j df[j*3+d] in_j.  Packed geometry: regular
+// g[e*9 + j*3+d] = J df[j*3+d] (weights applied here); deformed
+// g[(e*9 + j*3+d)*Q^3 + q] = W_q J_q df_q[j*3+d].
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(elements_per_block<NQ>() * NQ * NQ) void iproduct_deriv_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, const T *__restrict__ in0,
+    const T *__restrict__ in1, const T *__restrict__ in2, T *__restrict__ out)
+{
+    constexpr int EPB = elements_per_block<NQ>(), TPE = NQ * NQ, NQ3 = NQ * NQ * NQ, P = pitch<NQ>();
+    __shared__ T lds[EPB][2][slab<NQ>()];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % NQ, j = t / NQ;
+    T *A = lds[le][0], *B = lds[le][1];
+    T g[9];
+    if (!DEF) {
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = B[(k0 * NQ + l0) * P + l];
-        contract<false>(v, [&](int ii) { return tab.Drow(ii); }, o);
-#pragma unroll
-        for (int ii = 0; ii < NQ; ++ii) B[(k0 * NQ + l0) * P + ii] = o[ii];
+        for (int c = 0; c < 9; ++c) g[c] = valid ? geo[(size_t)e * 9 + c] : T(0);
     }
-    __syncthreads();
-    {   // D1^T along j on line (i = l0, k = k0): A <- B + D1^T A, in place in A
-        T v[NQ], o[NQ];
+    const T wij = tab.W(i) * tab.W(j);
+    T y[NQ], v2[NQ];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) v[l] = A[(k0 * NQ + l) * P + l0];
+    for (int k = 0; k < NQ; ++k) {
+        const int c = (k * NQ + j) * NQ + i;
+        if (DEF) {
 #pragma unroll
-        for (int jj = 0; jj < NQ; ++jj) o[jj] = B[(k0 * NQ + jj) * P + l0];
-        ready_v(o);
-        contract<true>(v, [&](int jj) { return tab.Drow(jj); }, o);
-#pragma unroll
-        for (int jj = 0; jj < NQ; ++jj) A[(k0 * NQ + jj) * P + l0] = o[jj];
+            for (int cc = 0; cc < 9; ++cc) g[cc] = valid ? geo[((size_t)e * 9 + cc) * NQ3 + c] : T(0);
+        }
+        const size_t q = (size_t)e * NQ3 + c;
+        const T f0 = valid ? in0[q] : T(0), f1 = valid ? in1[q] : T(0), f2 = valid ? in2[q] : T(0);
+        T t0 = g[0] * f0;
+        t0 = fma(g[3], f1, t0);
+        t0 = fma(g[6], f2, t0);
+        T t1 = g[1] * f0;
+        t1 = fma(g[4], f1, t1);
+        t1 = fma(g[7], f2, t1);
+        T t2 = g[2] * f0;
+        t2 = fma(g[5], f1, t2);
+        t2 = fma(g[8], f2, t2);
+        if (!DEF) {
+            const T w = wij * tab.W(k);
+            t0 *= w;
+            t1 *= w;
+            t2 *= w;
+        }
+        const int lc = (k * NQ + j) * P + i;
+        B[lc] = t0;
+        A[lc] = t1;
+        v2[k] = t2;
+        y[k] = T(0);
     }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) y[k] += A[(k * NQ + j) * P + i];
-    iprod_from_column<T, NM, NQ>(tab, y, valid, t, B, A, xout);
+    grad_transpose_to_coeffs<T, NM, NQ>(tab, valid, t, y, v2, A, B, out + (size_t)e * NM * NM * NM);
 }
 
 // Tile shapes (elements per workgroup) instantiated for the Helmholtz

@@ -21,7 +21,11 @@ void parse_operator_key(const std::string &key, OpType &op, Shape &shape, bool &
     else if (o == "IProduct") op = OpType::IProduct;
     else if (o == "PhysDeriv") op = OpType::PhysDeriv;
     else if (o == "Helmholtz") op = OpType::Helmholtz;
-    else throw Error(NMF_ERR_UNSUPPORTED, "operator '" + o + "' is not provided (BwdTrans, IProduct, PhysDeriv, Helmholtz)");
+    else if (o == "IProductWRTDerivBase") op = OpType::IProductDeriv;
+    else
+        throw Error(NMF_ERR_UNSUPPORTED, "operator '" + o +
+                                             "' is not provided (BwdTrans, IProduct, IProductWRTDerivBase, PhysDeriv, "
+                                             "Helmholtz)");
     if (s == "Hex") shape = Shape::Hex;
     else if (s == "Tet") shape = Shape::Tet;
     else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet)");
@@ -175,7 +179,27 @@ std::vector<double> MFOperator::pack_geometry() const
                     g[((size_t)e * 9 + d) * nqtot + q] = df[((size_t)e * nqtot + q) * 9 + d];
         return g;
     }
-    // Helmholtz
+    if (op == OpType::IProductDeriv && shape == Shape::Hex) {
+        // J df per element (weights applied in the kernel), or W J df per point
+        g.resize(gp * 9);
+        if (!deformed) {
+            for (size_t e = 0; e < gp; ++e)
+                for (int c = 0; c < 9; ++c) g[e * 9 + c] = jac[e] * df[e * 9 + c];
+        } else {
+            Points1D P = make_points(PointsType::GLL, nq[0]);
+            const int Q = nq[0];
+            for (int e = 0; e < nelmt; ++e)
+                for (int k = 0, q = 0; k < Q; ++k)
+                    for (int j = 0; j < Q; ++j)
+                        for (int i = 0; i < Q; ++i, ++q) {
+                            const size_t pt = (size_t)e * nqtot + q;
+                            const double wj = P.w[i] * P.w[j] * P.w[k] * jac[pt];
+                            for (int c = 0; c < 9; ++c) g[((size_t)e * 9 + c) * nqtot + q] = wj * df[pt * 9 + c];
+                        }
+        }
+        return g;
+    }
+    // Helmholtz (and the tet IProductWRTDerivBase: df and J, as for Helmholtz)
     double L[3][3];
     if (has_diff) cholesky3(diff, L);
     auto fetch_df = [&](size_t pt, double *f) {
@@ -240,8 +264,19 @@ void MFOperator::ensure_geometry()
     geo_dirty = false;
 }
 
+void MFOperator::apply_iproduct_deriv(const void *in0, const void *in1, const void *in2, void *out, hipStream_t s)
+{
+    NMF_REQUIRE(op == OpType::IProductDeriv, NMF_ERR_INVALID, "not an IProductWRTDerivBase operator");
+    NMF_REQUIRE(nelmt == 0 || (in0 && in1 && in2 && out), NMF_ERR_INVALID, "null input/output pointer");
+    ensure_geometry();
+    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in0, out, nullptr, nullptr, s, 0, in1, in2};
+    hipError_t e = shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);
+    NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+}
+
 void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s)
 {
+    NMF_REQUIRE(op != OpType::IProductDeriv, NMF_ERR_INVALID, "IProductWRTDerivBase has three inputs");
     NMF_REQUIRE(nelmt == 0 || (in && out0), NMF_ERR_INVALID, "null input/output pointer");
     if (op == OpType::PhysDeriv) NMF_REQUIRE(nelmt == 0 || (out1 && out2), NMF_ERR_INVALID, "PhysDeriv needs three outputs");
     ensure_geometry();

@@ -36,8 +36,14 @@ struct MFOperator {
 
     MFOperator(const std::string &key, int nummodes, int numpoints0, int nelmt, int precision);
 
-    bool needs_jac() const { return op == OpType::IProduct || op == OpType::Helmholtz; }
-    bool needs_df() const { return op == OpType::PhysDeriv || op == OpType::Helmholtz; }
+    bool needs_jac() const
+    {
+        return op == OpType::IProduct || op == OpType::Helmholtz || op == OpType::IProductDeriv;
+    }
+    bool needs_df() const
+    {
+        return op == OpType::PhysDeriv || op == OpType::Helmholtz || op == OpType::IProductDeriv;
+    }
     size_t geom_points() const { return deformed ? (size_t)nelmt * nqtot : (size_t)nelmt; }
 
     void set_jac(const double *j);
@@ -50,6 +56,8 @@ struct MFOperator {
     void set_diffusion(const double *d6);
 
     void apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s);
+    // IProductWRTDerivBase: three inputs at the quadrature points, one output
+    void apply_iproduct_deriv(const void *in0, const void *in1, const void *in2, void *out, hipStream_t s);
 
     // Tiles this operator's kernel is instantiated with (empty when the
     // kernel has a single shape).

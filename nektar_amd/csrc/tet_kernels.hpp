@@ -439,6 +439,61 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
 
 // Fused Helmholtz (Helmholtz.h:549-599 with DiffusionCoeffTetKernel,
 // HelmholtzKernels.hpp:700-913).
+// Second half shared by the fused Helmholtz apply and IProductWRTDerivBase:
+// the weighted integrands of the collapsed directions 0 (in Bs), 1 (in C)
+// and 2 (register column v2) at the quadrature points; y (register column)
+// gets sum_d D_d^T of them, then the unweighted transpose transform (with
+// the collapsed-vertex corrections) gives the coefficients.  The caller's
+// writes of Bs and C must not be followed by a barrier yet.
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &tab, bool valid, int t, int e,
+                                                         T (&y)[NQ - 1], const T (&v2)[NQ - 1], T *A, T *Bs,
+                                                         T *C, T *__restrict__ out)
+{
+    using Dm = Dims<NM, NQ>;
+    constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
+    const int i = t % NQ0, j = t / NQ0;
+#pragma unroll
+    for (int k = 0; k < NQ2; ++k) {
+        T s = y[k];
+#pragma unroll
+        for (int l = 0; l < NQ2; ++l) s = fma(tab.D2(k, l), v2[l], s);
+        y[k] = s;
+    }
+    __syncthreads();
+    if (t < NQ1 * NQ2) {  // D0^T along i, lines (j,k): A <- D0^T Bs
+        const int jj = t % NQ1, k = t / NQ1;
+        T v[NQ0];
+#pragma unroll
+        for (int l = 0; l < NQ0; ++l) v[l] = Bs[(k * NQ1 + jj) * NQ0 + l];
+#pragma unroll
+        for (int ii = 0; ii < NQ0; ++ii) {
+            T s = v[0] * tab.D0(ii, 0);
+#pragma unroll
+            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(ii, l), s);
+            A[(k * NQ1 + jj) * NQ0 + ii] = s;
+        }
+    }
+    __syncthreads();
+    if (t < NQ0 * NQ2) {  // D1^T along j, lines (i,k): A += D1^T C
+        const int ii = t % NQ0, k = t / NQ0;
+        T v[NQ1];
+#pragma unroll
+        for (int l = 0; l < NQ1; ++l) v[l] = C[(k * NQ1 + l) * NQ0 + ii];
+#pragma unroll
+        for (int jj = 0; jj < NQ1; ++jj) {
+            T s = A[(k * NQ1 + jj) * NQ0 + ii];
+#pragma unroll
+            for (int l = 0; l < NQ1; ++l) s = fma(v[l], tab.D1(jj, l), s);
+            A[(k * NQ1 + jj) * NQ0 + ii] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ2; ++k) y[k] += A[(k * NQ1 + j) * NQ0 + i];
+    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, A, out + (size_t)e * Dm::NC);
+}
+
 // One workgroup of the fused Helmholtz apply: elements blk*EPB ..
 // blk*EPB+EPB-1 of an operator with nelmt elements.
 template <typename T, int NM, int NQ, bool DEF>
@@ -527,45 +582,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             y[k] = (lambda * wj) * u[k];
         }
     }
-#pragma unroll
-    for (int k = 0; k < NQ2; ++k) {
-        T s = y[k];
-#pragma unroll
-        for (int l = 0; l < NQ2; ++l) s = fma(tab.D2(k, l), v2[l], s);
-        y[k] = s;
-    }
-    __syncthreads();
-    if (t < NQ1 * NQ2) {  // D0^T along i, lines (j,k): A <- D0^T Bs
-        const int jj = t % NQ1, k = t / NQ1;
-        T v[NQ0];
-#pragma unroll
-        for (int l = 0; l < NQ0; ++l) v[l] = Bs[(k * NQ1 + jj) * NQ0 + l];
-#pragma unroll
-        for (int ii = 0; ii < NQ0; ++ii) {
-            T s = v[0] * tab.D0(ii, 0);
-#pragma unroll
-            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(ii, l), s);
-            A[(k * NQ1 + jj) * NQ0 + ii] = s;
-        }
-    }
-    __syncthreads();
-    if (t < NQ0 * NQ2) {  // D1^T along j, lines (i,k): A += D1^T C
-        const int ii = t % NQ0, k = t / NQ0;
-        T v[NQ1];
-#pragma unroll
-        for (int l = 0; l < NQ1; ++l) v[l] = C[(k * NQ1 + l) * NQ0 + ii];
-#pragma unroll
-        for (int jj = 0; jj < NQ1; ++jj) {
-            T s = A[(k * NQ1 + jj) * NQ0 + ii];
-#pragma unroll
-            for (int l = 0; l < NQ1; ++l) s = fma(v[l], tab.D1(jj, l), s);
-            A[(k * NQ1 + jj) * NQ0 + ii] = s;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NQ2; ++k) y[k] += A[(k * NQ1 + j) * NQ0 + i];
-    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, A, out + (size_t)e * Dm::NC);
+    grad_transpose_to_coeffs<T, NM, NQ>(tab, valid, t, e, y, v2, A, Bs, C, out);
 }
 
 template <typename T, int NM, int NQ, bool DEF>
@@ -591,6 +608,63 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_grouped_kern
     while (it + 1 < count && b >= items[it + 1].block0) ++it;
     const GroupItem<T> I = items[it];
     helmholtz_block<T, NM, NQ, DEF>(b - I.block0, I.nelmt, tabp, I.geo, I.lambda, I.in, I.out, lds);
+}
+
+// IProductWRTDerivBase for tets (IProductWRTDerivBase.h:751-788, pointwise
+// step IProductWRTDerivBaseKernels.hpp:249-335): t_d = sum_j df[j*3+d] in_j,
+// mapped to the collapsed directions t0 <- f0 (t0 + f1 (t1 + t2)),
+// t1 <- f2 (t1 + f3 t2) with f0 = h2 h3, f1 = h0, f2 = h3, f3 = h1, then
+// weighted by W J.  Geometry packed as for the Helmholtz kernel (df, J).
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, const T *__restrict__ in0,
+    const T *__restrict__ in1, const T *__restrict__ in2, T *__restrict__ out)
+{
+    using Dm = Dims<NM, NQ>;
+    constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
+    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
+    const int e = blockIdx.x * Dm::EPB + le;
+    const bool valid = e < nelmt;
+    const int i = t % NQ0, j = t / NQ0;
+    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
+    T g[10];
+    if (!DEF) {
+#pragma unroll
+        for (int c = 0; c < 10; ++c) g[c] = valid ? geo[(size_t)e * 10 + c] : T(0);
+    }
+    const T h0 = tab.H0(i), h1 = tab.H1(j), h2 = tab.H2(j);
+    const T wij = tab.W0(i) * tab.W1(j);
+    T y[NQ - 1], v2[NQ - 1];
+#pragma unroll
+    for (int k = 0; k < NQ2; ++k) {
+        const int c = (k * NQ1 + j) * NQ0 + i;
+        if (DEF) {
+#pragma unroll
+            for (int cc = 0; cc < 10; ++cc) g[cc] = valid ? geo[((size_t)e * 10 + cc) * NQT + c] : T(0);
+        }
+        const size_t q = (size_t)e * NQT + c;
+        const T f0 = valid ? in0[q] : T(0), f1 = valid ? in1[q] : T(0), f2 = valid ? in2[q] : T(0);
+        T t0 = g[0] * f0;
+        t0 = fma(g[3], f1, t0);
+        t0 = fma(g[6], f2, t0);
+        T t1 = g[1] * f0;
+        t1 = fma(g[4], f1, t1);
+        t1 = fma(g[7], f2, t1);
+        T t2 = g[2] * f0;
+        t2 = fma(g[5], f1, t2);
+        t2 = fma(g[8], f2, t2);
+        const T h3 = tab.H3(k);
+        t0 = fma(t1 + t2, h0, t0) * (h2 * h3);
+        t1 = fma(t2, h1, t1) * h3;
+        const T wj = wij * tab.W2(k) * g[9];
+        Bs[c] = wj * t0;
+        C[c] = wj * t1;
+        v2[k] = wj * t2;
+        y[k] = T(0);
+    }
+    grad_transpose_to_coeffs<T, NM, NQ>(tab, valid, t, e, y, v2, A, Bs, C, out);
 }
 
 }  // namespace tet

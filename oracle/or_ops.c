@@ -695,6 +695,79 @@ void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed,
     tables_free(&T);
 }
 
+/* IProductWRTDerivBaseKernels.hpp:194-247 (hex) and :249-335 (tet): the
+ * integrand of direction d is t_d = sum_j df[j*3+d] in_j at every point;
+ * for tets it is then mapped to the collapsed directions:
+ * t0 <- f0 (t0 + f1 (t1 + t2)), t1 <- f2 (t1 + f3 t2). */
+static void iprod_deriv_pointwise(const tables_t *T, int DEF, const vd *df,
+                                  const vd *in0, const vd *in1, const vd *in2,
+                                  vd *o0, vd *o1, vd *o2)
+{
+    const int nq0 = T->nq[0], nq1 = T->nq[1], nq2 = T->nq[2];
+    vd d[9];
+    if (!DEF)
+        for (int c = 0; c < 9; ++c) d[c] = df[c];
+    for (int k = 0, cnt = 0; k < nq2; ++k) {
+        const double f2 = 2.0 / (1.0 - T->Z[2][k]);
+        for (int j = 0; j < nq1; ++j) {
+            const double z1 = T->Z[1][j];
+            const double f3 = 0.5 * (1.0 + z1);
+            const double f0 = 2.0 * f2 / (1.0 - z1);
+            for (int i = 0; i < nq0; ++i, ++cnt) {
+                if (DEF)
+                    for (int c = 0; c < 9; ++c) d[c] = df[(size_t)cnt * 9 + c];
+                const vd a = in0[cnt], b = in1[cnt], e = in2[cnt];
+                vd t0 = vadd(vadd(vmul(d[0], a), vmul(d[3], b)), vmul(d[6], e));
+                vd t1 = vadd(vadd(vmul(d[1], a), vmul(d[4], b)), vmul(d[7], e));
+                vd t2 = vadd(vadd(vmul(d[2], a), vmul(d[5], b)), vmul(d[8], e));
+                if (T->shape == OR_TET) {
+                    const double f1 = 0.5 * (1.0 + T->Z[0][i]);
+                    t0 = vfmas(t0, vadd(t1, t2), f1);
+                    t0 = vmuls(t0, f0);
+                    t1 = vfmas(t1, t2, f3);
+                    t1 = vmuls(t1, f2);
+                }
+                o0[cnt] = t0;
+                o1[cnt] = t1;
+                o2[cnt] = t2;
+            }
+        }
+    }
+}
+
+/* IProductWRTDerivBase.h:751-788 (fusedKernel3D): pointwise step, then
+ * IProduct w.r.t. (db0 b1 b2), += (b0 db1 b2), += (b0 b1 db2). */
+void or_iproduct_deriv(int shape, int nm, int nq0, int nelmt, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, const double *in2, double *out)
+{
+    tables_t T;
+    tables_init(&T, shape, nm, nq0);
+    const int nblk = (nelmt + VW - 1) / VW;
+#pragma omp parallel num_threads(g_threads)
+    {
+        ws_t w;
+        ws_alloc(&w, &T, deformed);
+        vd *a1 = (vd *)calloc(T.nqtot, sizeof(vd)), *a2 = (vd *)calloc(T.nqtot, sizeof(vd));
+#pragma omp for schedule(static)
+        for (int b = 0; b < nblk; ++b) {
+            load_block(in0, nelmt, b, T.nqtot, w.in);
+            load_block(in1, nelmt, b, T.nqtot, a1);
+            load_block(in2, nelmt, b, T.nqtot, a2);
+            load_geom(&T, nelmt, b, deformed, jac, df, &w);
+            iprod_deriv_pointwise(&T, deformed, w.df, w.in, a1, a2, w.d0, w.d1, w.d2);
+            iprod_block(&T, 0, 0, deformed, w.d0, T.db[0], T.b[1], T.b[2], w.jac, &w, w.out, 1.0);
+            iprod_block(&T, 0, 1, deformed, w.d1, T.b[0], T.db[1], T.b[2], w.jac, &w, w.out, 1.0);
+            iprod_block(&T, 0, 1, deformed, w.d2, T.b[0], T.b[1], T.db[2], w.jac, &w, w.out, 1.0);
+            store_block(w.out, nelmt, b, T.ncoeff, out);
+        }
+        free(a1);
+        free(a2);
+        ws_free(&w);
+    }
+    tables_free(&T);
+}
+
 void or_helmholtz_diag_bruteforce(int shape, int nm, int nq0, int nelmt,
                                   int deformed, const double *jac,
                                   const double *df, double lambda,

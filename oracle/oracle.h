@@ -73,6 +73,11 @@ void or_physderiv(int shape, int nm, int nq0, int nelmt, int deformed,
                   const double *df, const double *in, double *out0,
                   double *out1, double *out2);
 /* diff6 = {d00,d01,d11,d02,d12,d22} or NULL for the identity tensor. */
+/* IProductWRTDerivBase: out = sum_d IProduct(dphi/dxi_d, sum_j df[j*3+d] in_j)
+ * (IProductWRTDerivBase.h:592-825, IProductWRTDerivBaseKernels.hpp:194-335). */
+void or_iproduct_deriv(int shape, int nm, int nq0, int nelmt, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, const double *in2, double *out);
 void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed,
                   const double *jac, const double *df, double lambda,
                   const double *diff6, const double *in, double *out);

@@ -98,6 +98,14 @@ def physderiv(shape, nm, nq0, nelmt, deformed, df, u):
     return o
 
 
+def iproduct_deriv(shape, nm, nq0, nelmt, deformed, jac, df, f0, f1, f2):
+    out = np.zeros(nelmt * ncoeffs(shape, nm))
+    c = [np.ascontiguousarray(f, dtype=np.float64) for f in (f0, f1, f2)]
+    lib().or_iproduct_deriv(shape, nm, nq0, nelmt, int(deformed), p(jac), p(df), p(c[0]), p(c[1]), p(c[2]),
+                            p(out))
+    return out
+
+
 def helmholtz(shape, nm, nq0, nelmt, deformed, jac, df, lam, x, diff6=None):
     out = np.zeros(nelmt * ncoeffs(shape, nm))
     d = None if diff6 is None else np.ascontiguousarray(diff6, dtype=np.float64)
