@@ -229,6 +229,17 @@ int nmf_solver_set_forcing_phys(nmf_solver s, const double *f_dev);
  * FP64 residual, dots and diagonal.  Stops at ||r|| < tol ||b||. */
 int nmf_solver_solve(nmf_solver s, int mixed, double tol, int maxit,
                      nmf_solve_info *info);
+/* The same solve without blocking, for callers that interleave many solves
+ * in their own loop: begin enqueues the set-up; each advance enqueues the
+ * next 8 iterations (or a reliable update) once the previous chunk's state
+ * has reached the host, and sets *done when the solve has finished
+ * (wait != 0: block for the chunk in flight instead of returning at once);
+ * finish returns the result.  While a solve is in progress the solver's
+ * other calls (forcing, solve, apply_global, L2 error, autotune) return
+ * NMF_ERR_STATE. */
+int nmf_solver_begin(nmf_solver s, int mixed, double tol, int maxit);
+int nmf_solver_advance(nmf_solver s, int wait, int *done);
+int nmf_solver_finish(nmf_solver s, nmf_solve_info *info);
 /* L2 norm of u_h - sin(k pi x) sin(k pi y) sin(k pi z). */
 int nmf_solver_l2_error_manufactured(nmf_solver s, int k, double *l2);
 /* Host copies of the global solution / assembled rhs / inverse diagonal. */

@@ -115,6 +115,9 @@ _SIGNATURES = {
     "nmf_solver_set_forcing_phys": (ctypes.c_int, [_P, _P]),
     "nmf_solver_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         ctypes.POINTER(SolveInfo)]),
+    "nmf_solver_begin": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "nmf_solver_advance": (ctypes.c_int, [_P, ctypes.c_int, _I]),
+    "nmf_solver_finish": (ctypes.c_int, [_P, ctypes.POINTER(SolveInfo)]),
     "nmf_solver_l2_error_manufactured": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "nmf_solver_get_solution": (ctypes.c_int, [_P, _D]),
     "nmf_solver_get_rhs": (ctypes.c_int, [_P, _D]),
@@ -500,6 +503,20 @@ class HelmholtzSolver:
     def solve(self, mixed: bool = False, tol: float = 1e-10, maxit: int = 5000) -> SolveInfo:
         info = SolveInfo()
         check(lib().nmf_solver_solve(self._h, int(mixed), tol, maxit, ctypes.byref(info)))
+        return info
+
+    # --- non-blocking solve ------------------------------------------------
+    def begin(self, mixed: bool = False, tol: float = 1e-10, maxit: int = 5000) -> None:
+        check(lib().nmf_solver_begin(self._h, int(mixed), tol, maxit))
+
+    def advance(self, wait: bool = False) -> bool:
+        d = ctypes.c_int()
+        check(lib().nmf_solver_advance(self._h, int(wait), ctypes.byref(d)))
+        return bool(d.value)
+
+    def finish(self) -> SolveInfo:
+        info = SolveInfo()
+        check(lib().nmf_solver_finish(self._h, ctypes.byref(info)))
         return info
 
     def autotune(self):

@@ -474,6 +474,40 @@ int nmf_solver_solve(nmf_solver s, int mixed, double tol, int maxit, nmf_solve_i
     });
 }
 
+int nmf_solver_begin(nmf_solver s, int mixed, double tol, int maxit)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        NMF_REQUIRE(tol > 0.0 && maxit > 0, NMF_ERR_INVALID, "tol and maxit must be positive");
+        s->impl->begin_solve(mixed != 0, tol, maxit);
+    });
+}
+
+int nmf_solver_advance(nmf_solver s, int wait, int *done)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        NMF_REQUIRE(done, NMF_ERR_INVALID, "null done flag");
+        *done = s->impl->advance(wait != 0) ? 1 : 0;
+    });
+}
+
+int nmf_solver_finish(nmf_solver s, nmf_solve_info *info)
+{
+    return guard([&] {
+        NMF_SOLVER_CHECK();
+        const SolveInfo r = s->impl->finish();
+        if (info) {
+            info->iterations = r.iterations;
+            info->outer_iterations = r.outer_iterations;
+            info->converged = r.converged;
+            info->final_error = r.final_error;
+            info->rhs_magnitude = r.rhs_magnitude;
+            info->solve_ms = r.solve_ms;
+        }
+    });
+}
+
 int nmf_solver_l2_error_manufactured(nmf_solver s, int k, double *l2)
 {
     return guard([&] {

@@ -230,6 +230,7 @@ void HexSolver::apply_local(bool fp32, const void *xg, void *sloc, const int *do
 
 void HexSolver::autotune(int *tile64, int *tile32, std::vector<double> *times64, std::vector<double> *times32)
 {
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "autotune during a solve");
     ensure_fp32();
     for (int prec = 0; prec < 2; ++prec) {
         const bool fp32 = prec == 1;
@@ -263,6 +264,12 @@ void HexSolver::autotune(int *tile64, int *tile32, std::vector<double> *times64,
 }
 
 void HexSolver::apply_global(const double *x, double *y)
+{
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "apply_global during a solve (it shares the solve's work buffers)");
+    apply_global_fp64(x, y);
+}
+
+void HexSolver::apply_global_fp64(const double *x, double *y)
 {
     apply_local(false, x, sloc64_.get(), nullptr);
     NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), y, nullptr, nullptr, nullptr));
@@ -405,7 +412,7 @@ void HexSolver::enqueue_reliable_update()
     float *y = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>();
     NMF_HIP(solver::launch_add<float>(stream_, NG_, x_.as<double>(), y));
     NMF_HIP(solver::launch_zero<float>(stream_, NG_, y));
-    apply_global(x_.as<double>(), tmp64_.as<double>());
+    apply_global_fp64(x_.as<double>(), tmp64_.as<double>());
     NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
                                     pmu_.as<double>()));
     NMF_HIP(solver::launch_ru_resume(stream_, N_, r64_.as<double>(), r, invd_.as<double>(), pa_.as<double>()));
@@ -512,6 +519,7 @@ void HexSolver::set_lambda(double lambda)
 
 double HexSolver::solution_norm()
 {
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "solution_norm during a solve");
     NMF_HIP(solver::launch_sumsq(stream_, NG_, x_.as<double>(), pmu_.as<double>()));
     NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, scalar_.as<double>()));
     NMF_HIP(hipMemcpyAsync(host_scalar_, scalar_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -521,6 +529,7 @@ double HexSolver::solution_norm()
 
 double HexSolver::l2_error_manufactured(int k)
 {
+    NMF_REQUIRE(!job_.running, NMF_ERR_STATE, "l2_error_manufactured during a solve");
     const size_t nq3 = (size_t)nq_ * nq_ * nq_;
     DeviceBuffer u((size_t)nelmt_ * nq3 * sizeof(double));
     SolverArgs a{SolverKernel::BwdGather, false, deformed_, n_, N_, tab64_.get(), nullptr, nullptr, lambda_,

@@ -158,6 +158,7 @@ private:
     void build_geometry();
     void build_diagonal();
     void apply_local(bool fp32, const void *xg, void *sloc, const int *done);
+    void apply_global_fp64(const double *x, double *y);  // no state check (used inside solves)
     void capture_iterations(bool fp32);
     void assemble_rhs(const double *f_dev);
     void enqueue_state_read();

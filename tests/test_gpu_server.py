@@ -115,3 +115,44 @@ def test_server_errors(nmf):
         srv2.run([bad])
     results, stats = srv2.run([])
     assert results == [] and stats["solvers_created"] == 0
+
+
+def test_non_blocking_solves_interleaved(nmf):
+    """Two solves driven chunk by chunk from one loop give the same results
+    as blocking solves; while a solve runs, the solver refuses the calls
+    that would share its work buffers."""
+    a = nmf.HelmholtzSolver(4, 5, 6, 1.3)
+    b = nmf.HelmholtzSolver(3, 6, 7, 0.7, perturb=0.1, seed=2)
+    a.set_forcing_manufactured(1)
+    b.set_forcing_manufactured(2)
+    ra = a.solve(False, 1e-10, 5000)
+    xa = a.solution()
+    rb = b.solve(True, 1e-10, 5000)
+    xb = b.solution()
+    a.begin(False, 1e-10, 5000)
+    b.begin(True, 1e-10, 5000)
+    with pytest.raises(nmf.NektarError):
+        a.finish()  # not finished yet
+    x = torch.zeros(a.nglobal, dtype=torch.float64, device="cuda")
+    for call in (lambda: a.solve(False, 1e-10, 10), lambda: a.set_forcing_manufactured(1),
+                 lambda: a.l2_error_manufactured(1), lambda: a.apply_global(x, x), lambda: a.autotune(),
+                 lambda: a.begin(False, 1e-10, 10)):
+        with pytest.raises(nmf.NektarError) as err:
+            call()
+        assert err.value.code == 3  # NMF_ERR_STATE
+    done_a = done_b = False
+    steps = 0
+    while not (done_a and done_b):
+        if not done_a:
+            done_a = a.advance(False)
+        if not done_b:
+            done_b = b.advance(False)
+        steps += 1
+        assert steps < 10_000_000
+    ia, ib = a.finish(), b.finish()
+    assert ia.iterations == ra.iterations and ia.final_error == ra.final_error
+    assert ib.iterations == rb.iterations and ib.final_error == rb.final_error
+    assert ib.outer_iterations == rb.outer_iterations
+    assert np.array_equal(a.solution(), xa) and np.array_equal(b.solution(), xb)
+    # usable again afterwards
+    assert a.solve(False, 1e-10, 5000).iterations == ra.iterations
