@@ -65,6 +65,17 @@ std::vector<double> MFOperator::build_tables(Shape shape, int nm, int nq0)
     for (double z : b1.pts.z) t.push_back(0.5 * (1.0 + z));
     for (double z : b1.pts.z) t.push_back(2.0 / (1.0 - z));
     for (double z : b2.pts.z) t.push_back(2.0 / (1.0 - z));
+    // zero-padded B1 and B2 rows that the kernels stage into LDS: from a
+    // multiple of 4, rows of pitch rp = nq1 rounded up to 4, each table
+    // followed by nm zero rows (tet::Dims::oB1P, RP, NB1R, NB2R)
+    const int nq1 = nq0 - 1, rp = (nq1 + 3) / 4 * 4;
+    t.resize((t.size() + 3) / 4 * 4, 0.0);
+    auto padded_rows = [&](const Basis1D &b, int nmodes) {
+        for (int m = 0; m < nmodes + nm; ++m)
+            for (int k = 0; k < rp; ++k) t.push_back(m < nmodes && k < nq1 ? b.bdata[m * nq1 + k] : 0.0);
+    };
+    padded_rows(b1, nm * (nm + 1) / 2);
+    padded_rows(b2, nm * (nm + 1) * (nm + 2) / 6);
     return t;
 }
 

@@ -15,6 +15,8 @@
 //   D2[NQ2^2] W0[NQ0] W1[NQ1] W2[NQ2] H0[NQ0] H1[NQ1] H2[NQ1] H3[NQ2]
 //   (W1, W2 carry the 0.5 / 0.25 Radau pre-scaling, Operator.hpp:600-616;
 //    H* are the half-space factors of HelmholtzKernels.hpp:1368-1389.)
+//   then, from a multiple of 4, zero-padded B1 and B2 rows of pitch RP that
+//   each kernel stages into LDS (Dims::oB1P, Dims::LTAB).
 // Geometry: IProduct jac[e] | jac[e*NQT+q]; PhysDeriv df[e*9+d] |
 //   df[(e*9+d)*NQT+q]; Helmholtz geo[e*10+c] | geo[(e*10+c)*NQT+q] with
 //   c = 0..8: df, c = 9: J.
@@ -40,8 +42,18 @@ struct Dims {
     static constexpr int oW0 = oD2 + NQ2 * NQ2, oW1 = oW0 + NQ0, oW2 = oW1 + NQ1;
     static constexpr int oH0 = oW2 + NQ2, oH1 = oH0 + NQ0, oH2 = oH1 + NQ1, oH3 = oH2 + NQ1;
     static constexpr int TABSIZE = oH3 + NQ2;
+    // Zero-padded copies of B1 and B2 after the tables above, staged into
+    // LDS by every kernel that runs the collapsed contractions: rows of
+    // pitch RP (a multiple of 4, so rows start on 16-byte boundaries in
+    // either precision), each table followed by NM zero rows so that the
+    // triangular loops can run to NM without predication.
+    static constexpr int RP = (NQ - 1 + 3) / 4 * 4;
+    static constexpr int NB1R = NMB + NM, NB2R = NC + NM;
+    static constexpr int oB1P = (TABSIZE + 3) / 4 * 4;
+    static constexpr int LTAB = (NB1R + NB2R) * RP;
     // slab sizes (per element): quadrature slab and pair/line scratch
     static constexpr int SLAB = NQT + 2 * NQ2;  // >= NMB*NQ2 and NQ1*NQ2*NM
+    static_assert((NMB + NM - 1) * NQ2 <= SLAB, "unpredicated pair reads must stay inside the slab");
 };
 
 template <int NM, int NQ>
@@ -101,13 +113,58 @@ struct Tab {
     __device__ __forceinline__ T H3(int k) const { return b[Dm::oH3 + k]; }
 };
 
+// The staged B1/B2 rows in LDS.  The collapsed contractions index these
+// tables by a lane-dependent mode (the (p,q) pair or the p of a line), so
+// they are read from LDS rather than as scattered global loads.
+template <typename T, int NM, int NQ>
+struct LTab {
+    using Dm = Dims<NM, NQ>;
+    const T *l;
+    __device__ __forceinline__ const T *B1(int m) const { return l + m * Dm::RP; }
+    __device__ __forceinline__ const T *B2(int m) const { return l + (Dm::NB1R + m) * Dm::RP; }
+};
+
+// N consecutive values of a staged row, read as 16-byte vectors (rows are
+// 16-byte aligned and padded to RP >= N rounded up to the vector width).
+template <typename T, int N>
+__device__ __forceinline__ void load_row(const T *row, T (&b)[N])
+{
+    constexpr int VW = 16 / sizeof(T);
+    using V = T __attribute__((ext_vector_type(VW)));
+#pragma unroll
+    for (int c = 0; c < (N + VW - 1) / VW; ++c) {
+        const V v = *reinterpret_cast<const V *>(row + c * VW);
+#pragma unroll
+        for (int w = 0; w < VW; ++w)
+            if (c * VW + w < N) b[c * VW + w] = v[w];
+    }
+}
+
+// Copy the padded B1/B2 rows of the table buffer into LDS (whole block).
+template <typename T, int NM, int NQ>
+__device__ __forceinline__ void stage_tables(const T *__restrict__ tabp, T *lt)
+{
+    using Dm = Dims<NM, NQ>;
+    constexpr int VW = 16 / sizeof(T);
+    using V = T __attribute__((ext_vector_type(VW)));
+    static_assert(Dm::LTAB % VW == 0 && Dm::oB1P % VW == 0, "staged rows must be whole vectors");
+    const V *src = reinterpret_cast<const V *>(tabp + Dm::oB1P);
+    V *dst = reinterpret_cast<V *>(lt);
+    for (int x = threadIdx.x; x < Dm::LTAB / VW; x += blockDim.x) dst[x] = src[x];
+    __syncthreads();
+}
+
 // ---- BwdTrans: coefficients (global) -> k-column u of item (i,j)
 // Stage 1 (pairs): fpq[pair][k]      -> S1[pair*NQ2 + k]
 // Stage 2 (p,k)  : fp[k][j][p]       -> S2[(k*NQ1 + j)*NM + p]
 // Stage 3 (j,k)  : u(i,j,k) + corr.  -> S3[(k*NQ1 + j)*NQ0 + i]
+// The triangular loops run over all NM rows: the coefficients (or partial
+// sums) beyond a pair's length are zero, and the rows they meet are real
+// modes of the next pair or the zero padding, so no lane needs a branch.
 template <typename T, int NM, int NQ>
-__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const T *__restrict__ xin, bool valid,
-                                              int t, T *S1, T *S2, T *S3, T (&u)[NQ - 1])
+__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
+                                              const T *__restrict__ xin, bool valid, int t, T *S1, T *S2, T *S3,
+                                              T (&u)[NQ - 1])
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
@@ -118,32 +175,51 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const T
         const int len = NM - p - q;
         T x[NM];
 #pragma unroll
-        for (int r = 0; r < NM; ++r) x[r] = (valid && r < len) ? xin[base + r] : T(0);
+        for (int r = 0; r < NM; ++r) x[r] = T(0);
+        if (valid) {
 #pragma unroll
-        for (int k = 0; k < NQ2; ++k) {
-            T s = T(0);
-#pragma unroll
-            for (int r = 0; r < NM; ++r)
-                if (r < len) s = fma(x[r], tab.B2(base + r, k), s);
-            S1[t * NQ2 + k] = s;
+            for (int r = 0; r < NM; ++r) {
+                const T xv = xin[min(base + r, Dm::NC - 1)];
+                x[r] = r < len ? xv : T(0);
+            }
         }
+        T s[NQ2];
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) s[k] = T(0);
+#pragma unroll
+        for (int r = 0; r < NM; ++r) {
+            T b[NQ2];
+            load_row(lt.B2(base + r), b);
+#pragma unroll
+            for (int k = 0; k < NQ2; ++k) s[k] = fma(x[r], b[k], s[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) S1[t * NQ2 + k] = s[k];
     }
     __syncthreads();
     if (t < NM * NQ2) {
         const int p = t % NM, k = t / NM;
         const int pb = modeb<NM>(p, 0);
+        // rows pb+q < NMB+NM-1 stay inside the slab, so the reads need no
+        // branch; what they find beyond the pair is discarded
         T a[NM];
 #pragma unroll
-        for (int q = 0; q < NM; ++q) a[q] = (q < NM - p) ? S1[(pb + q) * NQ2 + k] : T(0);
-        const int mb = modeb<NM>(p, 0);
-#pragma unroll
-        for (int j = 0; j < NQ1; ++j) {
-            T s = T(0);
-#pragma unroll
-            for (int q = 0; q < NM; ++q)
-                if (q < NM - p) s = fma(a[q], tab.B1(mb + q, j), s);
-            S2[(k * NQ1 + j) * NM + p] = s;
+        for (int q = 0; q < NM; ++q) {
+            const T av = S1[(pb + q) * NQ2 + k];
+            a[q] = q < NM - p ? av : T(0);
         }
+        T s[NQ1];
+#pragma unroll
+        for (int j = 0; j < NQ1; ++j) s[j] = T(0);
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            T b[NQ1];
+            load_row(lt.B1(pb + q), b);
+#pragma unroll
+            for (int j = 0; j < NQ1; ++j) s[j] = fma(a[q], b[j], s[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NQ1; ++j) S2[(k * NQ1 + j) * NM + p] = s[j];
     }
     __syncthreads();
     // singular-mode coefficients (broadcast reads)
@@ -161,11 +237,11 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const T
         for (int p = 0; p < NM; ++p) f[p] = S2[(k * NQ1 + j) * NM + p];
         // correction weight along (i): top uses B0_0 B1_1 + B0_1 (B1_0 + B1_1),
         // bottom and edge use B0_1 B1_1 (separable in i)
-        const T b10 = tab.B1(0, j), b11 = tab.B1(1, j);
-        T edge = c_bot * tab.B2(0, k);
+        const T b10 = lt.B1(0)[j], b11 = lt.B1(1)[j];
+        T edge = c_bot * lt.B2(0)[k];
 #pragma unroll
-        for (int r = 1; r < NM - 1; ++r) edge = fma(c_edge[r - 1], tab.B2(r + 1, k), edge);
-        const T b21 = tab.B2(1, k);
+        for (int r = 1; r < NM - 1; ++r) edge = fma(c_edge[r - 1], lt.B2(r + 1)[k], edge);
+        const T b21 = lt.B2(1)[k];
 #pragma unroll
         for (int i = 0; i < NQ0; ++i) {
             T s = T(0);
@@ -194,8 +270,9 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const T
 // Stage I2 (p,k): sum_j B1 s            -> S1[pair*NQ2 + k]
 // Stage I3 (pairs): sum_k B2            -> out
 template <typename T, int NM, int NQ>
-__device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const T (&y)[NQ - 1], bool valid,
-                                                  int t, T *S1, T *S2, T *S3, T *__restrict__ xout)
+__device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
+                                                  const T (&y)[NQ - 1], bool valid, int t, T *S1, T *S2, T *S3,
+                                                  T *__restrict__ xout)
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
@@ -239,15 +316,14 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
         T v[NQ1];
 #pragma unroll
         for (int j = 0; j < NQ1; ++j) v[j] = S2[(k * NQ1 + j) * NM + p];
-        const int mb = modeb<NM>(p, 0);
 #pragma unroll
         for (int q = 0; q < NM; ++q) {
-            if (q < NM - p) {
-                T s = v[0] * tab.B1(mb + q, 0);
+            T b[NQ1];
+            load_row(lt.B1(pb + q), b);
+            T s = v[0] * b[0];
 #pragma unroll
-                for (int j = 1; j < NQ1; ++j) s = fma(v[j], tab.B1(mb + q, j), s);
-                S1[(pb + q) * NQ2 + k] = s;
-            }
+            for (int j = 1; j < NQ1; ++j) s = fma(v[j], b[j], s);
+            if (q < NM - p) S1[(pb + q) * NQ2 + k] = s;
         }
     }
     __syncthreads();
@@ -262,10 +338,12 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
         const bool p00 = (p == 0 && q == 0), p01 = (p == 0 && q == 1);
 #pragma unroll
         for (int r = 0; r < NM; ++r) {
-            if (r < len) {
-                T s = v[0] * tab.B2(base + r, 0);
+            T b[NQ2];
+            load_row(lt.B2(base + r), b);
+            T s = v[0] * b[0];
 #pragma unroll
-                for (int k = 1; k < NQ2; ++k) s = fma(v[k], tab.B2(base + r, k), s);
+            for (int k = 1; k < NQ2; ++k) s = fma(v[k], b[k], s);
+            if (r < len) {
                 if (p00 && r == 1) {
                     // top vertex: sum_k TOP(k) B2_1(k)
 #pragma unroll
@@ -289,13 +367,16 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void bwdtrans_kernel(
     int nelmt, const T *__restrict__ tabp, const T *__restrict__ in, T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
+    __shared__ __align__(16) T ltab[Dm::LTAB];
+    stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
+    const LTab<T, NM, NQ> lt{ltab};
     const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
     const int e = blockIdx.x * Dm::EPB + le;
     const bool valid = e < nelmt;
     T u[NQ - 1];
-    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * Dm::NC, valid, t, lds[le][1], lds[le][2], lds[le][0], u);
+    bwd_to_column<T, NM, NQ>(tab, lt, in + (size_t)e * Dm::NC, valid, t, lds[le][0], lds[le][1], lds[le][0], u);
     if (valid) {
         const int i = t % Dm::NQ0, j = t / Dm::NQ0;
 #pragma unroll
@@ -309,8 +390,11 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_kernel(
     T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
+    __shared__ __align__(16) T ltab[Dm::LTAB];
+    stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
+    const LTab<T, NM, NQ> lt{ltab};
     const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
     const int e = blockIdx.x * Dm::EPB + le;
     const bool valid = e < nelmt;
@@ -326,7 +410,7 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_kernel(
         const T jq = DEF ? (valid ? jac[(size_t)e * Dm::NQT + c] : T(0)) : jv;
         y[k] = fv * jq * (wij * tab.W2(k));
     }
-    iprod_from_column<T, NM, NQ>(tab, y, valid, t, lds[le][1], lds[le][2], lds[le][0], out + (size_t)e * Dm::NC);
+    iprod_from_column<T, NM, NQ>(tab, lt, y, valid, t, lds[le][0], lds[le][1], lds[le][0], out + (size_t)e * Dm::NC);
 }
 
 // collocation derivatives along i (lines (j,k)) and j (lines (i,k)) from A
@@ -440,15 +524,18 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
 // Fused Helmholtz (Helmholtz.h:549-599 with DiffusionCoeffTetKernel,
 // HelmholtzKernels.hpp:700-913).
 // Second half shared by the fused Helmholtz apply and IProductWRTDerivBase:
-// the weighted integrands of the collapsed directions 0 (in Bs), 1 (in C)
+// the weighted integrands of the collapsed directions 0 (in Y), 1 (in X)
 // and 2 (register column v2) at the quadrature points; y (register column)
 // gets sum_d D_d^T of them, then the unweighted transpose transform (with
-// the collapsed-vertex corrections) gives the coefficients.  The caller's
-// writes of Bs and C must not be followed by a barrier yet.
+// the collapsed-vertex corrections) gives the coefficients.  D0^T runs in
+// place on the (j,k) lines of Y and D1^T in place on the (i,k) lines of X
+// (each line belongs to one work-item, which reads it whole before writing
+// it), so the element needs two slabs.  The caller's writes of X and Y
+// must not be followed by a barrier yet.
 template <typename T, int NM, int NQ>
-__device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &tab, bool valid, int t, int e,
-                                                         T (&y)[NQ - 1], const T (&v2)[NQ - 1], T *A, T *Bs,
-                                                         T *C, T *__restrict__ out)
+__device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
+                                                         bool valid, int t, int e, T (&y)[NQ - 1],
+                                                         const T (&v2)[NQ - 1], T *X, T *Y, T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
@@ -461,37 +548,40 @@ __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &t
         y[k] = s;
     }
     __syncthreads();
-    if (t < NQ1 * NQ2) {  // D0^T along i, lines (j,k): A <- D0^T Bs
+    if (t < NQ1 * NQ2) {  // D0^T along i, lines (j,k) of Y, in place
         const int jj = t % NQ1, k = t / NQ1;
+        T *line = Y + (k * NQ1 + jj) * NQ0;
         T v[NQ0];
 #pragma unroll
-        for (int l = 0; l < NQ0; ++l) v[l] = Bs[(k * NQ1 + jj) * NQ0 + l];
+        for (int l = 0; l < NQ0; ++l) v[l] = line[l];
 #pragma unroll
         for (int ii = 0; ii < NQ0; ++ii) {
             T s = v[0] * tab.D0(ii, 0);
 #pragma unroll
             for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(ii, l), s);
-            A[(k * NQ1 + jj) * NQ0 + ii] = s;
+            line[ii] = s;
         }
     }
-    __syncthreads();
-    if (t < NQ0 * NQ2) {  // D1^T along j, lines (i,k): A += D1^T C
+    if (t < NQ0 * NQ2) {  // D1^T along j, lines (i,k) of X, in place
         const int ii = t % NQ0, k = t / NQ0;
         T v[NQ1];
 #pragma unroll
-        for (int l = 0; l < NQ1; ++l) v[l] = C[(k * NQ1 + l) * NQ0 + ii];
+        for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * NQ0 + ii];
 #pragma unroll
         for (int jj = 0; jj < NQ1; ++jj) {
-            T s = A[(k * NQ1 + jj) * NQ0 + ii];
+            T s = v[0] * tab.D1(jj, 0);
 #pragma unroll
-            for (int l = 0; l < NQ1; ++l) s = fma(v[l], tab.D1(jj, l), s);
-            A[(k * NQ1 + jj) * NQ0 + ii] = s;
+            for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(jj, l), s);
+            X[(k * NQ1 + jj) * NQ0 + ii] = s;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NQ2; ++k) y[k] += A[(k * NQ1 + j) * NQ0 + i];
-    iprod_from_column<T, NM, NQ>(tab, y, valid, t, Bs, C, A, out + (size_t)e * Dm::NC);
+    for (int k = 0; k < NQ2; ++k) {
+        const int c = (k * NQ1 + j) * NQ0 + i;
+        y[k] += Y[c] + X[c];
+    }
+    iprod_from_column<T, NM, NQ>(tab, lt, y, valid, t, X, Y, X, out + (size_t)e * Dm::NC);
 }
 
 // One workgroup of the fused Helmholtz apply: elements blk*EPB ..
@@ -499,22 +589,51 @@ __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &t
 template <typename T, int NM, int NQ, bool DEF>
 __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__restrict__ tabp,
                                                 const T *__restrict__ geo, T lambda, const T *__restrict__ in,
-                                                T *__restrict__ out, T (*lds)[3][Dims<NM, NQ>::SLAB])
+                                                T *__restrict__ out, T (*lds)[2][Dims<NM, NQ>::SLAB], const T *ltab)
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
     const Tab<T, NM, NQ> tab{tabp};
+    const LTab<T, NM, NQ> lt{ltab};
     const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
     const int e = blk * Dm::EPB + le;
     const bool valid = e < nelmt;
     const int i = t % NQ0, j = t / NQ0;
-    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
+    T *X = lds[le][0], *Y = lds[le][1];
 
+    // X holds the quadrature values after the transform; Y <- D0 X along
+    // the (j,k) lines, then D1 in place on the (i,k) lines of X
     T u[NQ - 1], d2[NQ - 1];
-    bwd_to_column<T, NM, NQ>(tab, in + (size_t)e * Dm::NC, valid, t, Bs, C, A, u);
+    bwd_to_column<T, NM, NQ>(tab, lt, in + (size_t)e * Dm::NC, valid, t, X, Y, X, u);
     deriv2_column<T, NM, NQ>(tab, u, d2);
+    if (t < NQ1 * NQ2) {
+        const int jj = t % NQ1, k = t / NQ1;
+        const T *line = X + (k * NQ1 + jj) * NQ0;
+        T v[NQ0];
+#pragma unroll
+        for (int l = 0; l < NQ0; ++l) v[l] = line[l];
+#pragma unroll
+        for (int ii = 0; ii < NQ0; ++ii) {
+            T s = v[0] * tab.D0(0, ii);
+#pragma unroll
+            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(l, ii), s);
+            Y[(k * NQ1 + jj) * NQ0 + ii] = s;
+        }
+    }
     __syncthreads();
-    deriv01<T, NM, NQ>(tab, t, A, Bs, C);
+    if (t < NQ0 * NQ2) {
+        const int ii = t % NQ0, k = t / NQ0;
+        T v[NQ1];
+#pragma unroll
+        for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * NQ0 + ii];
+#pragma unroll
+        for (int jj = 0; jj < NQ1; ++jj) {
+            T s = v[0] * tab.D1(0, jj);
+#pragma unroll
+            for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(l, jj), s);
+            X[(k * NQ1 + jj) * NQ0 + ii] = s;
+        }
+    }
     __syncthreads();
 
     T y[NQ - 1], v2[NQ - 1];
@@ -565,7 +684,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             T g2 = g[2] * g[2];
             g2 = fma(g[5], g[5], g2);
             g2 = fma(g[8], g[8], g2);
-            const T a0 = Bs[c], a1 = C[c], a2 = d2[k];
+            const T a0 = Y[c], a1 = X[c], a2 = d2[k];
             const T wj = wij * tab.W2(k) * g[9];
             T o0 = g0 * a0;
             o0 = fma(g3, a1, o0);
@@ -576,13 +695,13 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             T o2 = g4 * a0;
             o2 = fma(g5, a1, o2);
             o2 = fma(g2, a2, o2);
-            Bs[c] = wj * o0;
-            C[c] = wj * o1;
+            Y[c] = wj * o0;
+            X[c] = wj * o1;
             v2[k] = wj * o2;
             y[k] = (lambda * wj) * u[k];
         }
     }
-    grad_transpose_to_coeffs<T, NM, NQ>(tab, valid, t, e, y, v2, A, Bs, C, out);
+    grad_transpose_to_coeffs<T, NM, NQ>(tab, lt, valid, t, e, y, v2, X, Y, out);
 }
 
 template <typename T, int NM, int NQ, bool DEF>
@@ -591,8 +710,10 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
     T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
-    helmholtz_block<T, NM, NQ, DEF>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds);
+    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
+    __shared__ __align__(16) T ltab[Dm::LTAB];
+    stage_tables<T, NM, NQ>(tabp, ltab);
+    helmholtz_block<T, NM, NQ, DEF>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab);
 }
 
 // Grouped form: several operators of the same kind in one launch; block b
@@ -602,12 +723,14 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_grouped_kern
     int count, const GroupItem<T> *__restrict__ items, const T *__restrict__ tabp)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
+    __shared__ __align__(16) T ltab[Dm::LTAB];
+    stage_tables<T, NM, NQ>(tabp, ltab);
     const int b = blockIdx.x;
     int it = 0;
     while (it + 1 < count && b >= items[it + 1].block0) ++it;
     const GroupItem<T> I = items[it];
-    helmholtz_block<T, NM, NQ, DEF>(b - I.block0, I.nelmt, tabp, I.geo, I.lambda, I.in, I.out, lds);
+    helmholtz_block<T, NM, NQ, DEF>(b - I.block0, I.nelmt, tabp, I.geo, I.lambda, I.in, I.out, lds, ltab);
 }
 
 // IProductWRTDerivBase for tets (IProductWRTDerivBase.h:751-788, pointwise
@@ -622,13 +745,16 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
-    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
+    __shared__ __align__(16) T ltab[Dm::LTAB];
+    stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
+    const LTab<T, NM, NQ> lt{ltab};
     const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
     const int e = blockIdx.x * Dm::EPB + le;
     const bool valid = e < nelmt;
     const int i = t % NQ0, j = t / NQ0;
-    T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
+    T *X = lds[le][0], *Y = lds[le][1];
     T g[10];
     if (!DEF) {
 #pragma unroll
@@ -659,12 +785,12 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
         t0 = fma(t1 + t2, h0, t0) * (h2 * h3);
         t1 = fma(t2, h1, t1) * h3;
         const T wj = wij * tab.W2(k) * g[9];
-        Bs[c] = wj * t0;
-        C[c] = wj * t1;
+        Y[c] = wj * t0;
+        X[c] = wj * t1;
         v2[k] = wj * t2;
         y[k] = T(0);
     }
-    grad_transpose_to_coeffs<T, NM, NQ>(tab, valid, t, e, y, v2, A, Bs, C, out);
+    grad_transpose_to_coeffs<T, NM, NQ>(tab, lt, valid, t, e, y, v2, X, Y, out);
 }
 
 }  // namespace tet
