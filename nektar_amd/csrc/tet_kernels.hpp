@@ -51,9 +51,21 @@ struct Dims {
     static constexpr int NB1R = NMB + NM, NB2R = NC + NM;
     static constexpr int oB1P = (TABSIZE + 3) / 4 * 4;
     static constexpr int LTAB = (NB1R + NB2R) * RP;
-    // slab sizes (per element): quadrature slab and pair/line scratch
-    static constexpr int SLAB = NQT + 2 * NQ2;  // >= NMB*NQ2 and NQ1*NQ2*NM
-    static_assert((NMB + NM - 1) * NQ2 <= SLAB, "unpredicated pair reads must stay inside the slab");
+};
+
+// LDS slab of an element: quadrature values at (k*NQ1 + j)*PI + i, then
+// 2*NQ2 values of vertex sums; the same slab holds the (pair, k) and
+// (k, j, p) partial sums of the transforms (SIZE >= NMB*NQ2, NQ1*NQ2*NM).
+// In FP64 the i-lines are padded to an odd pitch, so that work-items
+// walking their own lines fall in different banks; in FP32 the padding
+// costs a workgroup per CU and is measured slower, so it is left out.
+template <typename T, int NM, int NQ>
+struct Slab {
+    using Dm = Dims<NM, NQ>;
+    static constexpr int PI = (sizeof(T) == 8 && Dm::NQ0 % 2 == 0) ? Dm::NQ0 + 1 : Dm::NQ0;
+    static constexpr int NQTL = Dm::NQ1 * Dm::NQ2 * PI;
+    static constexpr int SIZE = NQTL + 2 * Dm::NQ2;
+    static_assert((Dm::NMB + NM - 1) * Dm::NQ2 <= SIZE, "unpredicated pair reads must stay inside the slab");
 };
 
 template <int NM, int NQ>
@@ -115,17 +127,26 @@ struct Tab {
 
 // The staged B1/B2 rows in LDS.  The collapsed contractions index these
 // tables by a lane-dependent mode (the (p,q) pair or the p of a line), so
-// they are read from LDS rather than as scattered global loads.
+// they are read from LDS rather than as scattered global loads.  In FP64
+// a row that does not fill whole 16-byte units is padded to an odd number
+// of units (LP values), so that the row reads of neighbouring work-items
+// start in different banks; rows that fill their units exactly, and FP32
+// rows, keep the plain pitch (both measured faster).
 template <typename T, int NM, int NQ>
 struct LTab {
     using Dm = Dims<NM, NQ>;
+    static constexpr int VW = 16 / sizeof(T);
+    static constexpr int UNITS = (Dm::NQ2 + VW - 1) / VW;
+    static constexpr bool PAD = sizeof(T) == 8 && Dm::NQ2 % VW != 0 && UNITS % 2 == 0;
+    static constexpr int LP = (PAD ? UNITS + 1 : UNITS) * VW;
+    static constexpr int SIZE = (Dm::NB1R + Dm::NB2R) * LP;
     const T *l;
-    __device__ __forceinline__ const T *B1(int m) const { return l + m * Dm::RP; }
-    __device__ __forceinline__ const T *B2(int m) const { return l + (Dm::NB1R + m) * Dm::RP; }
+    __device__ __forceinline__ const T *B1(int m) const { return l + m * LP; }
+    __device__ __forceinline__ const T *B2(int m) const { return l + (Dm::NB1R + m) * LP; }
 };
 
 // N consecutive values of a staged row, read as 16-byte vectors (rows are
-// 16-byte aligned and padded to RP >= N rounded up to the vector width).
+// 16-byte aligned and padded to LP >= N rounded up to the vector width).
 template <typename T, int N>
 __device__ __forceinline__ void load_row(const T *row, T (&b)[N])
 {
@@ -140,24 +161,32 @@ __device__ __forceinline__ void load_row(const T *row, T (&b)[N])
     }
 }
 
-// Copy the padded B1/B2 rows of the table buffer into LDS (whole block).
+// Copy the padded B1/B2 rows of the table buffer (pitch RP) into LDS
+// (pitch LP), whole block.  Units past RP are the zero padding.
 template <typename T, int NM, int NQ>
 __device__ __forceinline__ void stage_tables(const T *__restrict__ tabp, T *lt)
 {
     using Dm = Dims<NM, NQ>;
-    constexpr int VW = 16 / sizeof(T);
+    using L = LTab<T, NM, NQ>;
+    constexpr int VW = L::VW;
     using V = T __attribute__((ext_vector_type(VW)));
-    static_assert(Dm::LTAB % VW == 0 && Dm::oB1P % VW == 0, "staged rows must be whole vectors");
+    static_assert(Dm::RP % VW == 0 && Dm::oB1P % VW == 0, "table rows must be whole vectors");
+    constexpr int UG = Dm::RP / VW, UL = L::LP / VW;
     const V *src = reinterpret_cast<const V *>(tabp + Dm::oB1P);
     V *dst = reinterpret_cast<V *>(lt);
-    for (int x = threadIdx.x; x < Dm::LTAB / VW; x += blockDim.x) dst[x] = src[x];
+    for (int x = threadIdx.x; x < (Dm::NB1R + Dm::NB2R) * UL; x += blockDim.x) {
+        const int r = x / UL, c = x % UL;
+        V v = V(T(0));
+        if (c < UG) v = src[r * UG + c];
+        dst[x] = v;
+    }
     __syncthreads();
 }
 
 // ---- BwdTrans: coefficients (global) -> k-column u of item (i,j)
 // Stage 1 (pairs): fpq[pair][k]      -> S1[pair*NQ2 + k]
 // Stage 2 (p,k)  : fp[k][j][p]       -> S2[(k*NQ1 + j)*NM + p]
-// Stage 3 (j,k)  : u(i,j,k) + corr.  -> S3[(k*NQ1 + j)*NQ0 + i]
+// Stage 3 (j,k)  : u(i,j,k) + corr.  -> S3[(k*NQ1 + j)*PI + i]
 // The triangular loops run over all NM rows: the coefficients (or partial
 // sums) beyond a pair's length are zero, and the rows they meet are real
 // modes of the next pair or the zero padding, so no lane needs a branch.
@@ -252,14 +281,14 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const L
             top = fma(tab.B0(1, i), b11, top);
             s = fma(top * b21, c_top, s);
             s = fma(tab.B0(1, i) * b11, edge, s);
-            S3[(k * NQ1 + j) * NQ0 + i] = s;
+            S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = s;
         }
     }
     __syncthreads();
     {
         const int i = t % NQ0, j = t / NQ0;
 #pragma unroll
-        for (int k = 0; k < NQ2; ++k) u[k] = S3[(k * NQ1 + j) * NQ0 + i];
+        for (int k = 0; k < NQ2; ++k) u[k] = S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i];
     }
 }
 
@@ -279,14 +308,14 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
     {
         const int i = t % NQ0, j = t / NQ0;
 #pragma unroll
-        for (int k = 0; k < NQ2; ++k) S3[(k * NQ1 + j) * NQ0 + i] = y[k];
+        for (int k = 0; k < NQ2; ++k) S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = y[k];
     }
     __syncthreads();
     if (t < NQ1 * NQ2) {
         const int j = t % NQ1, k = t / NQ1;
         T v[NQ0];
 #pragma unroll
-        for (int i = 0; i < NQ0; ++i) v[i] = S3[(k * NQ1 + j) * NQ0 + i];
+        for (int i = 0; i < NQ0; ++i) v[i] = S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i];
 #pragma unroll
         for (int p = 0; p < NM; ++p) {
             T s = v[0] * tab.B0(p, 0);
@@ -307,8 +336,8 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
             top = fma(s1, b10 + b11, top);
             bot = fma(s1, b11, bot);
         }
-        S3[Dm::NQT + k] = top;
-        S3[Dm::NQT + NQ2 + k] = bot;
+        S3[Slab<T, NM, NQ>::NQTL + k] = top;
+        S3[Slab<T, NM, NQ>::NQTL + NQ2 + k] = bot;
     }
     if (t < NM * NQ2) {
         const int p = t % NM, k = t / NM;
@@ -347,13 +376,13 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
                 if (p00 && r == 1) {
                     // top vertex: sum_k TOP(k) B2_1(k)
 #pragma unroll
-                    for (int k = 0; k < NQ2; ++k) s = fma(S3[Dm::NQT + k], tab.B2(1, k), s);
+                    for (int k = 0; k < NQ2; ++k) s = fma(S3[Slab<T, NM, NQ>::NQTL + k], tab.B2(1, k), s);
                 }
                 if (p01 && r < NM - 1) {
                     // bottom vertex (r = 0) and singular edge (r >= 1): B2_{r+1}, B2_0
                     const int m2 = (r == 0) ? 0 : r + 1;
 #pragma unroll
-                    for (int k = 0; k < NQ2; ++k) s = fma(S3[Dm::NQT + NQ2 + k], tab.B2(m2, k), s);
+                    for (int k = 0; k < NQ2; ++k) s = fma(S3[Slab<T, NM, NQ>::NQTL + NQ2 + k], tab.B2(m2, k), s);
                 }
                 xout[base + r] = s;
             }
@@ -367,8 +396,8 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void bwdtrans_kernel(
     int nelmt, const T *__restrict__ tabp, const T *__restrict__ in, T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
-    __shared__ __align__(16) T ltab[Dm::LTAB];
+    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
     const LTab<T, NM, NQ> lt{ltab};
@@ -390,8 +419,8 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_kernel(
     T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
-    __shared__ __align__(16) T ltab[Dm::LTAB];
+    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
     const LTab<T, NM, NQ> lt{ltab};
@@ -423,26 +452,26 @@ __device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const 
         const int j = t % NQ1, k = t / NQ1;
         T v[NQ0];
 #pragma unroll
-        for (int l = 0; l < NQ0; ++l) v[l] = A[(k * NQ1 + j) * NQ0 + l];
+        for (int l = 0; l < NQ0; ++l) v[l] = A[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + l];
 #pragma unroll
         for (int i = 0; i < NQ0; ++i) {
             T s = v[0] * tab.D0(0, i);
 #pragma unroll
             for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(l, i), s);
-            Bs[(k * NQ1 + j) * NQ0 + i] = s;
+            Bs[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = s;
         }
     }
     if (t < NQ0 * NQ2) {
         const int i = t % NQ0, k = t / NQ0;
         T v[NQ1];
 #pragma unroll
-        for (int l = 0; l < NQ1; ++l) v[l] = A[(k * NQ1 + l) * NQ0 + i];
+        for (int l = 0; l < NQ1; ++l) v[l] = A[(k * NQ1 + l) * Slab<T, NM, NQ>::PI + i];
 #pragma unroll
         for (int j = 0; j < NQ1; ++j) {
             T s = v[0] * tab.D1(0, j);
 #pragma unroll
             for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(l, j), s);
-            C[(k * NQ1 + j) * NQ0 + i] = s;
+            C[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = s;
         }
     }
 }
@@ -468,7 +497,7 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
     T *__restrict__ out0, T *__restrict__ out1, T *__restrict__ out2)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][3][Dm::SLAB];
+    __shared__ T lds[Dm::EPB][3][Slab<T, NM, NQ>::SIZE];
     const Tab<T, NM, NQ> tab{tabp};
     const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
     const int e = blockIdx.x * Dm::EPB + le;
@@ -480,7 +509,7 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
     for (int k = 0; k < Dm::NQ2; ++k) {
         const int c = (k * Dm::NQ1 + j) * Dm::NQ0 + i;
         u[k] = valid ? in[(size_t)e * Dm::NQT + c] : T(0);
-        A[c] = u[k];
+        A[(k * Dm::NQ1 + j) * Slab<T, NM, NQ>::PI + i] = u[k];
     }
     deriv2_column<T, NM, NQ>(tab, u, d2);
     __syncthreads();
@@ -501,9 +530,10 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
             for (int d = 0; d < 9; ++d) g[d] = df[((size_t)e * 9 + d) * Dm::NQT + c];
         }
         const T h3 = tab.H3(k);
-        const T a0 = (h2 * h3) * Bs[c];
+        const int cl = (k * Dm::NQ1 + j) * Slab<T, NM, NQ>::PI + i;
+        const T a0 = (h2 * h3) * Bs[cl];
         const T w0 = h0 * a0;
-        const T w1 = h3 * C[c];
+        const T w1 = h3 * C[cl];
         const T a1 = w0 + w1;
         const T a2 = fma(w1, h1, w0) + d2[k];
         T o0 = a0 * g[0];
@@ -550,7 +580,7 @@ __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &t
     __syncthreads();
     if (t < NQ1 * NQ2) {  // D0^T along i, lines (j,k) of Y, in place
         const int jj = t % NQ1, k = t / NQ1;
-        T *line = Y + (k * NQ1 + jj) * NQ0;
+        T *line = Y + (k * NQ1 + jj) * Slab<T, NM, NQ>::PI;
         T v[NQ0];
 #pragma unroll
         for (int l = 0; l < NQ0; ++l) v[l] = line[l];
@@ -566,19 +596,19 @@ __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &t
         const int ii = t % NQ0, k = t / NQ0;
         T v[NQ1];
 #pragma unroll
-        for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * NQ0 + ii];
+        for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * Slab<T, NM, NQ>::PI + ii];
 #pragma unroll
         for (int jj = 0; jj < NQ1; ++jj) {
             T s = v[0] * tab.D1(jj, 0);
 #pragma unroll
             for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(jj, l), s);
-            X[(k * NQ1 + jj) * NQ0 + ii] = s;
+            X[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = s;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NQ2; ++k) {
-        const int c = (k * NQ1 + j) * NQ0 + i;
+        const int c = (k * NQ1 + j) * Slab<T, NM, NQ>::PI + i;
         y[k] += Y[c] + X[c];
     }
     iprod_from_column<T, NM, NQ>(tab, lt, y, valid, t, X, Y, X, out + (size_t)e * Dm::NC);
@@ -589,7 +619,7 @@ __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &t
 template <typename T, int NM, int NQ, bool DEF>
 __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__restrict__ tabp,
                                                 const T *__restrict__ geo, T lambda, const T *__restrict__ in,
-                                                T *__restrict__ out, T (*lds)[2][Dims<NM, NQ>::SLAB], const T *ltab)
+                                                T *__restrict__ out, T (*lds)[2][Slab<T, NM, NQ>::SIZE], const T *ltab)
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
@@ -608,7 +638,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
     deriv2_column<T, NM, NQ>(tab, u, d2);
     if (t < NQ1 * NQ2) {
         const int jj = t % NQ1, k = t / NQ1;
-        const T *line = X + (k * NQ1 + jj) * NQ0;
+        const T *line = X + (k * NQ1 + jj) * Slab<T, NM, NQ>::PI;
         T v[NQ0];
 #pragma unroll
         for (int l = 0; l < NQ0; ++l) v[l] = line[l];
@@ -617,7 +647,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             T s = v[0] * tab.D0(0, ii);
 #pragma unroll
             for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(l, ii), s);
-            Y[(k * NQ1 + jj) * NQ0 + ii] = s;
+            Y[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = s;
         }
     }
     __syncthreads();
@@ -625,13 +655,13 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
         const int ii = t % NQ0, k = t / NQ0;
         T v[NQ1];
 #pragma unroll
-        for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * NQ0 + ii];
+        for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * Slab<T, NM, NQ>::PI + ii];
 #pragma unroll
         for (int jj = 0; jj < NQ1; ++jj) {
             T s = v[0] * tab.D1(0, jj);
 #pragma unroll
             for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(l, jj), s);
-            X[(k * NQ1 + jj) * NQ0 + ii] = s;
+            X[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = s;
         }
     }
     __syncthreads();
@@ -647,7 +677,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
         const T wij = tab.W0(i) * tab.W1(j);
 #pragma unroll
         for (int k = 0; k < NQ2; ++k) {
-            const int c = (k * NQ1 + j) * NQ0 + i;
+            const int c = (k * NQ1 + j) * NQ0 + i, cl = (k * NQ1 + j) * Slab<T, NM, NQ>::PI + i;
             if (DEF) {
 #pragma unroll
                 for (int cc = 0; cc < 10; ++cc) g[cc] = valid ? geo[((size_t)e * 10 + cc) * NQT + c] : T(0);
@@ -684,7 +714,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             T g2 = g[2] * g[2];
             g2 = fma(g[5], g[5], g2);
             g2 = fma(g[8], g[8], g2);
-            const T a0 = Y[c], a1 = X[c], a2 = d2[k];
+            const T a0 = Y[cl], a1 = X[cl], a2 = d2[k];
             const T wj = wij * tab.W2(k) * g[9];
             T o0 = g0 * a0;
             o0 = fma(g3, a1, o0);
@@ -695,8 +725,8 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             T o2 = g4 * a0;
             o2 = fma(g5, a1, o2);
             o2 = fma(g2, a2, o2);
-            Y[c] = wj * o0;
-            X[c] = wj * o1;
+            Y[cl] = wj * o0;
+            X[cl] = wj * o1;
             v2[k] = wj * o2;
             y[k] = (lambda * wj) * u[k];
         }
@@ -710,8 +740,8 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
     T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
-    __shared__ __align__(16) T ltab[Dm::LTAB];
+    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     helmholtz_block<T, NM, NQ, DEF>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab);
 }
@@ -723,8 +753,8 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_grouped_kern
     int count, const GroupItem<T> *__restrict__ items, const T *__restrict__ tabp)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
-    __shared__ __align__(16) T ltab[Dm::LTAB];
+    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const int b = blockIdx.x;
     int it = 0;
@@ -745,8 +775,8 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
-    __shared__ T lds[Dm::EPB][2][Dm::SLAB];
-    __shared__ __align__(16) T ltab[Dm::LTAB];
+    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
     const LTab<T, NM, NQ> lt{ltab};
@@ -765,7 +795,7 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
     T y[NQ - 1], v2[NQ - 1];
 #pragma unroll
     for (int k = 0; k < NQ2; ++k) {
-        const int c = (k * NQ1 + j) * NQ0 + i;
+        const int c = (k * NQ1 + j) * NQ0 + i, cl = (k * NQ1 + j) * Slab<T, NM, NQ>::PI + i;
         if (DEF) {
 #pragma unroll
             for (int cc = 0; cc < 10; ++cc) g[cc] = valid ? geo[((size_t)e * 10 + cc) * NQT + c] : T(0);
@@ -785,8 +815,8 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
         t0 = fma(t1 + t2, h0, t0) * (h2 * h3);
         t1 = fma(t2, h1, t1) * h3;
         const T wj = wij * tab.W2(k) * g[9];
-        Y[c] = wj * t0;
-        X[c] = wj * t1;
+        Y[cl] = wj * t0;
+        X[cl] = wj * t1;
         v2[k] = wj * t2;
         y[k] = T(0);
     }
