@@ -78,9 +78,12 @@ public:
     bool NeedsDF() const
     {
         return m_key.rfind("PhysDeriv", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0 ||
-               m_key.rfind("IProductWRTDerivBase", 0) == 0;
+               m_key.rfind("IProductWRTDerivBase", 0) == 0 || IsAdvDiffReact();
     }
-    bool NeedsJac() const { return m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0; }
+    bool NeedsJac() const
+    {
+        return m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0 || IsAdvDiffReact();
+    }
 
     // SetJac / SetDF take per-element (not SIMD-interleaved) factors.
     void SetJac(const std::vector<double> &jac)
@@ -109,14 +112,16 @@ public:
 
 protected:
     size_t GeomPoints() const { return (size_t)m_nElmt * (m_deformed ? m_nqtot : 1); }
+    bool IsAdvDiffReact() const { return m_key.rfind("LinearAdvectionDiffusionReaction", 0) == 0; }
     size_t InputSize() const
     {
-        const bool coeffs = m_key.rfind("BwdTrans", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0;
+        const bool coeffs = m_key.rfind("BwdTrans", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0 || IsAdvDiffReact();
         return (size_t)m_nElmt * (coeffs ? m_ncoeff : m_nqtot);
     }
     size_t OutputSize() const
     {
-        const bool coeffs = m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0;  // incl. WRTDerivBase
+        const bool coeffs = m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0 ||  // incl. WRTDerivBase
+                            IsAdvDiffReact();
         return (size_t)m_nElmt * (coeffs ? m_ncoeff : m_nqtot);
     }
     static void CheckSize(size_t got, size_t want, const char *what)
@@ -172,6 +177,27 @@ public:
     }
 };
 
+// LinearAdvectionDiffusionReaction (Operator.hpp:368-497): out =
+// lambda IProduct(u + V . grad u) + L u.  The Collections wrapper passes
+// -lambda and V / (-lambda) (Collections/LinearAdvectionDiffusionReaction
+// .cpp:578-589, 660-669).
+class LinearAdvectionDiffusionReaction : public Operator {
+public:
+    using Operator::Operator;
+    void SetLambda(double lambda) { NmfCheck(nmf_operator_set_lambda(m_op, lambda)); }
+    void SetConstVarDiffusion(const std::vector<double> &diff)
+    {
+        CheckSize(diff.size(), 6, "SetConstVarDiffusion");
+        NmfCheck(nmf_operator_set_const_var_diffusion(m_op, diff.data()));
+    }
+    // vel[(e*nqtot + q)*3 + j]: velocity component j at point q of element e
+    void SetAdvectionVelocities(const std::vector<double> &vel)
+    {
+        CheckSize(vel.size(), (size_t)m_nElmt * m_nqtot * 3, "SetAdvectionVelocities");
+        NmfCheck(nmf_operator_set_advection(m_op, vel.data()));
+    }
+};
+
 // String-keyed factory (NekFactory<std::string, Operator, basis, nElmt>).
 class OperatorFactory {
 public:
@@ -182,6 +208,8 @@ public:
         if (key.rfind("IProductWRTDerivBase", 0) == 0)
             return std::make_shared<IProductWRTDerivBase>(key, basis, nElmt, precision);
         if (key.rfind("Helmholtz", 0) == 0) return std::make_shared<Helmholtz>(key, basis, nElmt, precision);
+        if (key.rfind("LinearAdvectionDiffusionReaction", 0) == 0)
+            return std::make_shared<LinearAdvectionDiffusionReaction>(key, basis, nElmt, precision);
         return std::make_shared<Operator>(key, basis, nElmt, precision);
     }
 };

@@ -91,7 +91,7 @@ int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0,
 /* MatrixFree::GetOperatorFactory().CreateInstance(key, basis, nElmt)
  * (Operator.cpp:40-90).  key = "<Op>_<Shape>_<Regular|Deformed>" as built
  * by GetOpstring, Op in {BwdTrans, IProduct, IProductWRTDerivBase,
- * PhysDeriv, Helmholtz},
+ * PhysDeriv, Helmholtz, LinearAdvectionDiffusionReaction},
  * Shape in {Hex, Tet}.  nummodes = P+1 per direction, numpoints0 = Q in
  * direction 0 (tets use Q-1 Gauss-Radau points in directions 1, 2). */
 int nmf_operator_create(const char *key, int nummodes, int numpoints0,
@@ -109,6 +109,15 @@ int nmf_operator_set_lambda(nmf_operator op, double lambda);
 /* Helmholtz::SetConstVarDiffusion (Operator.hpp:303-313): d6 = {d00, d01,
  * d11, d02, d12, d22}, symmetric positive definite; NULL resets to I. */
 int nmf_operator_set_const_var_diffusion(nmf_operator op, const double *d6);
+/* LinearAdvectionDiffusionReaction::SetAdvectionVelocities
+ * (Operator.hpp:476-481): vel[(e*nqtot + q)*3 + j], velocity component j
+ * at point q of element e (the de-interleaved form of the reference's
+ * [block][q][j][lane] array, LinearAdvectionDiffusionReaction.cpp:690-714).
+ * The operator computes lambda IProduct(u + V . grad u) + L u, as the
+ * MatrixFree operator does; the Collections wrapper passes -lambda and
+ * V / (-lambda) (LinearAdvectionDiffusionReaction.cpp:578-589, 660-669).
+ * SetLambda and SetConstVarDiffusion apply to this operator as well. */
+int nmf_operator_set_advection(nmf_operator op, const double *vel);
 /* operator()(input, output) for BwdTrans / IProduct / Helmholtz
  * (Operator.hpp:119-121, 181-183, 294-296) */
 int nmf_operator_apply(nmf_operator op, const void *in, void *out,

@@ -82,6 +82,7 @@ _SIGNATURES = {
     "nmf_operator_set_df": (ctypes.c_int, [_P, _D]),
     "nmf_operator_set_lambda": (ctypes.c_int, [_P, ctypes.c_double]),
     "nmf_operator_set_const_var_diffusion": (ctypes.c_int, [_P, _D]),
+    "nmf_operator_set_advection": (ctypes.c_int, [_P, _D]),
     "nmf_operator_apply": (ctypes.c_int, [_P, _P, _P, _P]),
     "nmf_operator_apply_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "nmf_operator_apply_host": (ctypes.c_int, [_P, _D, _D]),
@@ -241,12 +242,23 @@ class Operator:
         d = np.ascontiguousarray(d6, dtype=np.float64)
         check(lib().nmf_operator_set_const_var_diffusion(self._h, _dptr(d)))
 
+    def SetAdvectionVelocities(self, vel) -> None:
+        """LinearAdvectionDiffusionReaction: vel[(e*nqtot + q)*3 + j], the
+        velocity component j at point q of element e."""
+        v = np.ascontiguousarray(vel, dtype=np.float64)
+        if v.size != 3 * self.nelmt * self.nqtot:
+            raise NektarError(1, f"SetAdvectionVelocities: expected {3 * self.nelmt * self.nqtot} values, "
+                                 f"got {v.size}")
+        check(lib().nmf_operator_set_advection(self._h, _dptr(v)))
+
     # --- sizes (Collections::Operator::GetInputSize/GetOutputSize) --------
     def input_size(self) -> int:
-        return self.nelmt * (self.ncoeff if self.op in ("BwdTrans", "Helmholtz") else self.nqtot)
+        return self.nelmt * (self.ncoeff if self.op in ("BwdTrans", "Helmholtz", "LinearAdvectionDiffusionReaction")
+                             else self.nqtot)
 
     def output_size(self) -> int:
-        return self.nelmt * (self.ncoeff if self.op in ("IProduct", "Helmholtz", "IProductWRTDerivBase")
+        return self.nelmt * (self.ncoeff if self.op in ("IProduct", "Helmholtz", "IProductWRTDerivBase",
+                                                        "LinearAdvectionDiffusionReaction")
                              else self.nqtot)
 
     def torch_dtype(self):

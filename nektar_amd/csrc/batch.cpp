@@ -74,7 +74,7 @@ int64_t Batch::dofs() const
     int64_t d = 0;
     for (const auto &e : entries_) {
         const MFOperator &o = *e.op;
-        const bool coeffs = o.op == OpType::BwdTrans || o.op == OpType::Helmholtz;
+        const bool coeffs = o.op == OpType::BwdTrans || o.op == OpType::Helmholtz || o.op == OpType::AdvDiffReact;
         d += (int64_t)o.nelmt * (coeffs ? o.ncoeff : o.nqtot);
     }
     return d;

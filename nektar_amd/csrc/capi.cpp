@@ -151,7 +151,8 @@ int nmf_operator_set_lambda(nmf_operator op, double lambda)
 {
     return guard([&] {
         NMF_OP_CHECK();
-        NMF_REQUIRE(op->impl->op == OpType::Helmholtz, NMF_ERR_INVALID, "SetLambda: not a Helmholtz operator");
+        NMF_REQUIRE(op->impl->op == OpType::Helmholtz || op->impl->op == OpType::AdvDiffReact, NMF_ERR_INVALID,
+                    "SetLambda: not a Helmholtz or LinearAdvectionDiffusionReaction operator");
         op->impl->set_lambda(lambda);
     });
 }
@@ -160,8 +161,17 @@ int nmf_operator_set_const_var_diffusion(nmf_operator op, const double *d6)
 {
     return guard([&] {
         NMF_OP_CHECK();
-        NMF_REQUIRE(op->impl->op == OpType::Helmholtz, NMF_ERR_INVALID, "SetConstVarDiffusion: not a Helmholtz operator");
+        NMF_REQUIRE(op->impl->op == OpType::Helmholtz || op->impl->op == OpType::AdvDiffReact, NMF_ERR_INVALID,
+                    "SetConstVarDiffusion: not a Helmholtz or LinearAdvectionDiffusionReaction operator");
         op->impl->set_diffusion(d6);
+    });
+}
+
+int nmf_operator_set_advection(nmf_operator op, const double *vel)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        op->impl->set_advection(vel);
     });
 }
 
@@ -189,13 +199,14 @@ int nmf_operator_apply_physderiv(nmf_operator op, const void *in, void *out0, vo
 namespace {
 size_t in_count(const MFOperator &o)
 {
-    return (size_t)o.nelmt * (o.op == OpType::BwdTrans || o.op == OpType::Helmholtz ? o.ncoeff : o.nqtot);
+    const bool coeffs = o.op == OpType::BwdTrans || o.op == OpType::Helmholtz || o.op == OpType::AdvDiffReact;
+    return (size_t)o.nelmt * (coeffs ? o.ncoeff : o.nqtot);
 }
 size_t out_count(const MFOperator &o)
 {
-    return (size_t)o.nelmt *
-           (o.op == OpType::IProduct || o.op == OpType::Helmholtz || o.op == OpType::IProductDeriv ? o.ncoeff
-                                                                                                   : o.nqtot);
+    const bool coeffs = o.op == OpType::IProduct || o.op == OpType::Helmholtz || o.op == OpType::IProductDeriv ||
+                        o.op == OpType::AdvDiffReact;
+    return (size_t)o.nelmt * (coeffs ? o.ncoeff : o.nqtot);
 }
 void to_device(DeviceBuffer &b, const double *h, size_t n, bool fp32)
 {

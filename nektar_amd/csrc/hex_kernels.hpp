@@ -366,10 +366,14 @@ __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &t
 // per element: the direction-1 passes work in place on lines that belong
 // to one work-item each, so the third slab of a straightforward staging is
 // not needed (more elements resident per CU).
-template <typename T, int NM, int NQ, bool DEF, typename Fetch>
+// ADV: LinearAdvectionDiffusionReaction (LinearAdvectionDiffusionReaction.h
+// :600-658): the mass integrand u becomes u + V . grad_x u, with the
+// advection written as sum_d c_d du/dxi_d (c_d = sum_j V_j df[j*3+d], per
+// point, packed at set-up: adv[(e*3 + d)*Q^3 + q]).
+template <typename T, int NM, int NQ, bool DEF, typename Fetch, bool ADV = false>
 __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, const Fetch &fetch, bool valid, int e,
                                                   int t, const T *__restrict__ geo, T lambda, T *A, T *B,
-                                                  T *__restrict__ xout)
+                                                  T *__restrict__ xout, const T *__restrict__ adv = nullptr)
 {
     constexpr int NQ3 = NQ * NQ * NQ, P = pitch<NQ>();
     // column (i, j) of this work-item; as a line index: (l0, k0) = (i, j)
@@ -442,7 +446,15 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
             B[lc] = t0;
             A[lc] = t1;
             v2[k] = t2;
-            y[k] = (lambda * g[6]) * (DEF ? u[k] : w * u[k]);
+            T m = u[k];
+            if constexpr (ADV) {
+                const T *ac = adv + (size_t)e * 3 * NQ3 + c;
+                const T c0 = valid ? ac[0] : T(0), c1 = valid ? ac[NQ3] : T(0), c2 = valid ? ac[2 * NQ3] : T(0);
+                m = fma(c0, a0, m);
+                m = fma(c1, a1, m);
+                m = fma(c2, a2, m);
+            }
+            y[k] = (lambda * g[6]) * (DEF ? m : w * m);
         }
     }
     grad_transpose_to_coeffs<T, NM, NQ>(tab, valid, t, y, v2, A, B, xout);
@@ -517,6 +529,23 @@ template <typename T, int NQ, int EPB>
 constexpr bool tile_fits()
 {
     return EPB * NQ * NQ <= 1024 && EPB * 2 * slab<NQ>() * (int)sizeof(T) <= 65536;
+}
+
+// LinearAdvectionDiffusionReaction y = lambda*IProduct(u + V.grad u) + L x,
+// one element per workgroup
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__(NQ * NQ) void adv_diff_react_kernel(int nelmt, const T *__restrict__ tabp,
+                                                                 const T *__restrict__ geo,
+                                                                 const T *__restrict__ adv, T lambda,
+                                                                 const T *__restrict__ in, T *__restrict__ out)
+{
+    __shared__ T lds[2][slab<NQ>()];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int e = blockIdx.x, t = threadIdx.x;
+    const bool valid = e < nelmt;
+    using F = LocalFetch<T, NM>;
+    helmholtz_element<T, NM, NQ, DEF, F, true>(tab, F{in + (size_t)e * NM * NM * NM}, valid, e, t, geo, lambda,
+                                               lds[0], lds[1], out + (size_t)e * NM * NM * NM, adv);
 }
 
 // Fused Helmholtz y = lambda*M x + L x (Helmholtz.h:549-599)

@@ -7,7 +7,7 @@
 
 namespace nektar_amd {
 
-enum class OpType { BwdTrans = 0, IProduct = 1, PhysDeriv = 2, Helmholtz = 3, IProductDeriv = 4 };
+enum class OpType { BwdTrans = 0, IProduct = 1, PhysDeriv = 2, Helmholtz = 3, IProductDeriv = 4, AdvDiffReact = 5 };
 
 struct KernelArgs {
     OpType op;
@@ -22,6 +22,7 @@ struct KernelArgs {
     hipStream_t stream;
     int tile = 0;  // Helmholtz: elements per workgroup (0 = default)
     const void *in1 = nullptr, *in2 = nullptr;  // IProductDeriv: inputs 1, 2 (in = input 0)
+    const void *adv = nullptr;  // AdvDiffReact: advection coefficients per point
 };
 
 // Order-templated kernels of the structured-hex C0 solver.

@@ -22,10 +22,11 @@ void parse_operator_key(const std::string &key, OpType &op, Shape &shape, bool &
     else if (o == "PhysDeriv") op = OpType::PhysDeriv;
     else if (o == "Helmholtz") op = OpType::Helmholtz;
     else if (o == "IProductWRTDerivBase") op = OpType::IProductDeriv;
+    else if (o == "LinearAdvectionDiffusionReaction") op = OpType::AdvDiffReact;
     else
         throw Error(NMF_ERR_UNSUPPORTED, "operator '" + o +
                                              "' is not provided (BwdTrans, IProduct, IProductWRTDerivBase, PhysDeriv, "
-                                             "Helmholtz)");
+                                             "Helmholtz, LinearAdvectionDiffusionReaction)");
     if (s == "Hex") shape = Shape::Hex;
     else if (s == "Tet") shape = Shape::Tet;
     else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet)");
@@ -114,6 +115,18 @@ void MFOperator::set_df(const double *d)
     df.assign(d, d + 9 * geom_points());
     have_df = true;
     geo_dirty = true;
+    adv_dirty = true;
+    ++version;
+}
+
+void MFOperator::set_advection(const double *v)
+{
+    NMF_REQUIRE(op == OpType::AdvDiffReact, NMF_ERR_INVALID,
+                "advection velocities belong to LinearAdvectionDiffusionReaction operators");
+    NMF_REQUIRE(v || nelmt == 0, NMF_ERR_INVALID, "SetAdvectionVelocities: null pointer");
+    vel.assign(v, v + (size_t)nelmt * nqtot * 3);
+    have_vel = true;
+    adv_dirty = true;
     ++version;
 }
 
@@ -266,10 +279,54 @@ std::vector<double> MFOperator::pack_geometry() const
     return g;
 }
 
+// Advection in the reference directions, c_d = sum_j V_j df[j*3+d]
+// (LinearAdvectionDiffusionReactionKernels.hpp:193-277); for tets the
+// collapsed-coordinate factors of the derivatives are folded in
+// (:279-386): c_0 = h2 h3 (a_0 + h0 (a_1 + a_2)), c_1 = h3 (a_1 + h1 a_2),
+// c_2 = a_2.  Uses the geometric df (the diffusion tensor acts on the
+// Laplacian only).
+std::vector<double> MFOperator::pack_advection() const
+{
+    std::vector<double> c((size_t)nelmt * 3 * nqtot);
+    std::vector<double> z0, z1, z2;
+    if (shape == Shape::Tet) {
+        z0 = make_basis(BasisType::ModifiedA, nm, nq[0]).pts.z;
+        z1 = make_basis(BasisType::ModifiedB, nm, nq[1]).pts.z;
+        z2 = make_basis(BasisType::ModifiedC, nm, nq[2]).pts.z;
+    }
+    for (int e = 0; e < nelmt; ++e)
+        for (int k = 0, q = 0; k < nq[2]; ++k)
+            for (int j = 0; j < nq[1]; ++j)
+                for (int i = 0; i < nq[0]; ++i, ++q) {
+                    const size_t pt = (size_t)e * nqtot + q;
+                    const double *f = &df[(deformed ? pt : (size_t)e) * 9];
+                    const double *v = &vel[pt * 3];
+                    double a[3];
+                    for (int d = 0; d < 3; ++d) a[d] = v[0] * f[d] + v[1] * f[3 + d] + v[2] * f[6 + d];
+                    if (shape == Shape::Tet) {
+                        const double h0 = 0.5 * (1.0 + z0[i]), h1 = 0.5 * (1.0 + z1[j]);
+                        const double h2 = 2.0 / (1.0 - z1[j]), h3 = 2.0 / (1.0 - z2[k]);
+                        const double a0 = a[0], a1 = a[1], a2 = a[2];
+                        a[0] = h2 * h3 * (a0 + h0 * (a1 + a2));
+                        a[1] = h3 * (a1 + h1 * a2);
+                        a[2] = a2;
+                    }
+                    for (int d = 0; d < 3; ++d) c[((size_t)e * 3 + d) * nqtot + q] = a[d];
+                }
+    return c;
+}
+
 void MFOperator::ensure_geometry()
 {
     if (needs_jac()) NMF_REQUIRE(have_jac, NMF_ERR_STATE, "Jacobian not set (SetJac) before apply");
     if (needs_df()) NMF_REQUIRE(have_df, NMF_ERR_STATE, "derivative factors not set (SetDF) before apply");
+    if (op == OpType::AdvDiffReact) {
+        NMF_REQUIRE(have_vel, NMF_ERR_STATE, "advection velocities not set (SetAdvectionVelocities) before apply");
+        if (adv_dirty) {
+            adv.upload(pack_advection(), fp32);
+            adv_dirty = false;
+        }
+    }
     if (!geo_dirty || op == OpType::BwdTrans) return;
     geo.upload(pack_geometry(), fp32);
     geo_dirty = false;
@@ -292,6 +349,7 @@ void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipSt
     if (op == OpType::PhysDeriv) NMF_REQUIRE(nelmt == 0 || (out1 && out2), NMF_ERR_INVALID, "PhysDeriv needs three outputs");
     ensure_geometry();
     KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in, out0, out1, out2, s, tile};
+    a.adv = adv.get();
     hipError_t e = shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
 }

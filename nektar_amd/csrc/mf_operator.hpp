@@ -29,8 +29,11 @@ struct MFOperator {
 
     std::vector<double> jac, df;
     bool have_jac = false, have_df = false, geo_dirty = true;
+    // LinearAdvectionDiffusionReaction: velocity per point, vel[(e*nqtot + q)*3 + j]
+    std::vector<double> vel;
+    bool have_vel = false, adv_dirty = true;
 
-    DeviceBuffer tab, geo;
+    DeviceBuffer tab, geo, adv;
     int tile = 0;  // elements per workgroup of the Helmholtz kernel (0 = default)
     uint64_t version = 0;  // bumped by every change of geometry, factors or tile
 
@@ -38,11 +41,13 @@ struct MFOperator {
 
     bool needs_jac() const
     {
-        return op == OpType::IProduct || op == OpType::Helmholtz || op == OpType::IProductDeriv;
+        return op == OpType::IProduct || op == OpType::Helmholtz || op == OpType::IProductDeriv ||
+               op == OpType::AdvDiffReact;
     }
     bool needs_df() const
     {
-        return op == OpType::PhysDeriv || op == OpType::Helmholtz || op == OpType::IProductDeriv;
+        return op == OpType::PhysDeriv || op == OpType::Helmholtz || op == OpType::IProductDeriv ||
+               op == OpType::AdvDiffReact;
     }
     size_t geom_points() const { return deformed ? (size_t)nelmt * nqtot : (size_t)nelmt; }
 
@@ -54,6 +59,9 @@ struct MFOperator {
         ++version;
     }
     void set_diffusion(const double *d6);
+    // LinearAdvectionDiffusionReaction::SetAdvectionVelocities
+    // (Operator.hpp:476-481): three velocity components at every point
+    void set_advection(const double *v);
 
     void apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s);
     // IProductWRTDerivBase: three inputs at the quadrature points, one output
@@ -72,6 +80,9 @@ struct MFOperator {
     // Packed geometry exactly as the kernels read it (host FP64); exposed
     // so solvers can reuse it.
     std::vector<double> pack_geometry() const;
+    // Advection coefficients of the reference directions at every point,
+    // adv[(e*3 + d)*nqtot + q] (LinearAdvectionDiffusionReaction only).
+    std::vector<double> pack_advection() const;
     static std::vector<double> build_tables(Shape shape, int nm, int nq0);
 
     TuneKey tune_key() const;

@@ -616,10 +616,11 @@ __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &t
 
 // One workgroup of the fused Helmholtz apply: elements blk*EPB ..
 // blk*EPB+EPB-1 of an operator with nelmt elements.
-template <typename T, int NM, int NQ, bool DEF>
+template <typename T, int NM, int NQ, bool DEF, bool ADV = false>
 __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__restrict__ tabp,
                                                 const T *__restrict__ geo, T lambda, const T *__restrict__ in,
-                                                T *__restrict__ out, T (*lds)[2][Slab<T, NM, NQ>::SIZE], const T *ltab)
+                                                T *__restrict__ out, T (*lds)[2][Slab<T, NM, NQ>::SIZE], const T *ltab,
+                                                const T *__restrict__ adv = nullptr)
 {
     using Dm = Dims<NM, NQ>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
@@ -728,7 +729,15 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             Y[cl] = wj * o0;
             X[cl] = wj * o1;
             v2[k] = wj * o2;
-            y[k] = (lambda * wj) * u[k];
+            T m = u[k];
+            if constexpr (ADV) {
+                const T *ac = adv + (size_t)e * 3 * NQT + c;
+                const T c0 = valid ? ac[0] : T(0), c1 = valid ? ac[NQT] : T(0), c2 = valid ? ac[2 * NQT] : T(0);
+                m = fma(c0, a0, m);
+                m = fma(c1, a1, m);
+                m = fma(c2, a2, m);
+            }
+            y[k] = (lambda * wj) * m;
         }
     }
     grad_transpose_to_coeffs<T, NM, NQ>(tab, lt, valid, t, e, y, v2, X, Y, out);
@@ -744,6 +753,25 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     helmholtz_block<T, NM, NQ, DEF>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab);
+}
+
+// LinearAdvectionDiffusionReaction (LinearAdvectionDiffusionReaction.h
+// :600-658, tet advection LinearAdvectionDiffusionReactionKernels.hpp
+// :279-386): the mass integrand u becomes u + V . grad_x u, written as
+// sum_d c_d du/deta_d over the collapsed derivatives, with
+// c_0 = h2 h3 (a_0 + h0 (a_1 + a_2)), c_1 = h3 (a_1 + h1 a_2), c_2 = a_2,
+// a_d = sum_j V_j df[j*3+d], packed per point at set-up:
+// adv[(e*3 + d)*NQT + q].
+template <typename T, int NM, int NQ, bool DEF>
+__global__ __launch_bounds__((block_size<NM, NQ>())) void adv_diff_react_kernel(
+    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, const T *__restrict__ adv, T lambda,
+    const T *__restrict__ in, T *__restrict__ out)
+{
+    using Dm = Dims<NM, NQ>;
+    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
+    stage_tables<T, NM, NQ>(tabp, ltab);
+    helmholtz_block<T, NM, NQ, DEF, true>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab, adv);
 }
 
 // Grouped form: several operators of the same kind in one launch; block b

@@ -695,6 +695,98 @@ void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed,
     tables_free(&T);
 }
 
+/* LinearAdvectionDiffusionReactionKernels.hpp:193-277 (hex) and :279-386
+ * (tet): bwd += V . grad_x u at every point, with the velocity per point
+ * (vel[cnt*3 + j] of the element) and grad_x u = df^T (collapsed ->
+ * standard derivatives); the same operation order as the reference. */
+static void advection_pointwise(const tables_t *T, int DEF, const vd *df, const vd *vel,
+                                const vd *d0, const vd *d1, const vd *d2, vd *bwd)
+{
+    const int nq0 = T->nq[0], nq1 = T->nq[1], nq2 = T->nq[2];
+    vd f[9];
+    if (!DEF)
+        for (int c = 0; c < 9; ++c) f[c] = df[c];
+    const int tet = T->shape == OR_TET;  /* the h factors exist for tets only */
+    for (int k = 0, cnt = 0; k < nq2; ++k) {
+        const double h3 = tet ? 2.0 / (1.0 - T->Z[2][k]) : 0.0;
+        for (int j = 0; j < nq1; ++j) {
+            const double z1 = T->Z[1][j];
+            const double h1 = 0.5 * (1.0 + z1), h2 = tet ? 2.0 / (1.0 - z1) : 0.0;
+            const double h2h3 = h2 * h3, h1h3 = h1 * h3;
+            for (int i = 0; i < nq0; ++i, ++cnt) {
+                if (DEF)
+                    for (int c = 0; c < 9; ++c) f[c] = df[(size_t)cnt * 9 + c];
+                const vd vx = vel[cnt * 3], vy = vel[cnt * 3 + 1], vzz = vel[cnt * 3 + 2];
+                vd m0 = d0[cnt], m1 = d1[cnt], m2 = d2[cnt];
+                if (tet) {
+                    const double h0 = 0.5 * (1.0 + T->Z[0][i]);
+                    const vd metric0 = vmuls(d0[cnt], h2h3);
+                    vd metric1 = vmuls(metric0, h0);
+                    vd metric2 = vadd(metric1, d2[cnt]);
+                    metric2 = vfmas(metric2, d1[cnt], h1h3);
+                    metric1 = vfmas(metric1, d1[cnt], h3);
+                    m0 = metric0;
+                    m1 = metric1;
+                    m2 = metric2;
+                }
+                vd tmp = vmul(f[0], m0);
+                tmp = vfma(tmp, f[1], m1);
+                tmp = vfma(tmp, f[2], m2);
+                tmp = vmul(tmp, vx);
+                vd tmp2 = vmul(f[3], m0);
+                tmp2 = vfma(tmp2, f[4], m1);
+                tmp2 = vfma(tmp2, f[5], m2);
+                tmp = vfma(tmp, tmp2, vy);
+                tmp2 = vmul(f[6], m0);
+                tmp2 = vfma(tmp2, f[7], m1);
+                tmp2 = vfma(tmp2, f[8], m2);
+                tmp = vfma(tmp, tmp2, vzz);
+                bwd[cnt] = vadd(bwd[cnt], tmp);
+            }
+        }
+    }
+}
+
+/* LinearAdvectionDiffusionReaction.h:600-658 (fusedKernel3D): BwdTrans,
+ * collocation derivatives, bwd += V . grad u, out = lambda IProduct(bwd),
+ * diffusion metric, out += IProduct w.r.t. the three derivative bases. */
+void or_adv_diff_react(int shape, int nm, int nq0, int nelmt, int deformed,
+                       const double *jac, const double *df, const double *vel,
+                       double lambda, const double *diff6, const double *in,
+                       double *out)
+{
+    tables_t T;
+    tables_init(&T, shape, nm, nq0);
+    const int nblk = (nelmt + VW - 1) / VW;
+#pragma omp parallel num_threads(g_threads)
+    {
+        ws_t w;
+        ws_alloc(&w, &T, deformed);
+        vd *v = (vd *)calloc(3 * (size_t)T.nqtot, sizeof(vd));
+#pragma omp for schedule(static)
+        for (int b = 0; b < nblk; ++b) {
+            load_block(in, nelmt, b, T.ncoeff, w.in);
+            load_geom(&T, nelmt, b, deformed, jac, df, &w);
+            load_block(vel, nelmt, b, 3 * T.nqtot, v);
+            bwd_block(&T, w.in, &w, w.bwd);
+            deriv_tensor(T.nq[0], T.nq[1], T.nq[2], w.bwd, T.D[0], T.D[1], T.D[2], w.d0, w.d1, w.d2);
+            advection_pointwise(&T, deformed, w.df, v, w.d0, w.d1, w.d2, w.bwd);
+            iprod_block(&T, 1, 0, deformed, w.bwd, T.b[0], T.b[1], T.b[2], w.jac, &w, w.out, lambda);
+            if (T.shape == OR_HEX)
+                diffusion_hex(deformed, T.nqtot, diff6 != NULL, diff6, w.df, w.d0, w.d1, w.d2);
+            else
+                diffusion_tet(deformed, &T, diff6 != NULL, diff6, w.df, w.d0, w.d1, w.d2);
+            iprod_block(&T, 0, 1, deformed, w.d0, T.db[0], T.b[1], T.b[2], w.jac, &w, w.out, 1.0);
+            iprod_block(&T, 0, 1, deformed, w.d1, T.b[0], T.db[1], T.b[2], w.jac, &w, w.out, 1.0);
+            iprod_block(&T, 0, 1, deformed, w.d2, T.b[0], T.b[1], T.db[2], w.jac, &w, w.out, 1.0);
+            store_block(w.out, nelmt, b, T.ncoeff, out);
+        }
+        free(v);
+        ws_free(&w);
+    }
+    tables_free(&T);
+}
+
 /* IProductWRTDerivBaseKernels.hpp:194-247 (hex) and :249-335 (tet): the
  * integrand of direction d is t_d = sum_j df[j*3+d] in_j at every point;
  * for tets it is then mapped to the collapsed directions:

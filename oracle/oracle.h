@@ -81,6 +81,13 @@ void or_iproduct_deriv(int shape, int nm, int nq0, int nelmt, int deformed,
 void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed,
                   const double *jac, const double *df, double lambda,
                   const double *diff6, const double *in, double *out);
+/* LinearAdvectionDiffusionReaction (fused, LinearAdvectionDiffusionReaction.h
+ * :600-658): out = lambda IProduct(u + V.grad u) + L u, u = BwdTrans(in),
+ * vel[(e*nqtot + q)*3 + j] per point. */
+void or_adv_diff_react(int shape, int nm, int nq0, int nelmt, int deformed,
+                       const double *jac, const double *df, const double *vel,
+                       double lambda, const double *diff6, const double *in,
+                       double *out);
 /* Diagonal of each element's Helmholtz matrix (ncoeff per element),
  * obtained by applying the element operator to unit vectors (the same
  * numbers as the diagonal of the reference's dense local matrices). */

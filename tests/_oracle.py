@@ -114,6 +114,16 @@ def helmholtz(shape, nm, nq0, nelmt, deformed, jac, df, lam, x, diff6=None):
     return out
 
 
+def adv_diff_react(shape, nm, nq0, nelmt, deformed, jac, df, vel, lam, x, diff6=None):
+    """LinearAdvectionDiffusionReaction: vel[(e*nqtot + q)*3 + j] per point."""
+    out = np.zeros(nelmt * ncoeffs(shape, nm))
+    d = None if diff6 is None else np.ascontiguousarray(diff6, dtype=np.float64)
+    lib().or_adv_diff_react(shape, nm, nq0, nelmt, int(deformed), p(jac), p(df),
+                            p(np.ascontiguousarray(vel, dtype=np.float64)), ctypes.c_double(lam), p(d),
+                            p(np.ascontiguousarray(x)), p(out))
+    return out
+
+
 def helmholtz_diag(shape, nm, nq0, nelmt, deformed, jac, df, lam):
     out = np.zeros(nelmt * ncoeffs(shape, nm))
     lib().or_helmholtz_diag_bruteforce(shape, nm, nq0, nelmt, int(deformed), p(jac), p(df),
