@@ -115,14 +115,15 @@ protected:
     bool IsAdvDiffReact() const { return m_key.rfind("LinearAdvectionDiffusionReaction", 0) == 0; }
     size_t InputSize() const
     {
-        const bool coeffs = m_key.rfind("BwdTrans", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0 || IsAdvDiffReact();
-        return (size_t)m_nElmt * (coeffs ? m_ncoeff : m_nqtot);
+        int64_t in = 0;
+        NmfCheck(nmf_operator_io_size(m_op, &in, nullptr));
+        return (size_t)in;
     }
     size_t OutputSize() const
     {
-        const bool coeffs = m_key.rfind("IProduct", 0) == 0 || m_key.rfind("Helmholtz", 0) == 0 ||  // incl. WRTDerivBase
-                            IsAdvDiffReact();
-        return (size_t)m_nElmt * (coeffs ? m_ncoeff : m_nqtot);
+        int64_t out = 0;
+        NmfCheck(nmf_operator_io_size(m_op, nullptr, &out));
+        return (size_t)out;
     }
     static void CheckSize(size_t got, size_t want, const char *what)
     {
@@ -198,6 +199,15 @@ public:
     }
 };
 
+// PhysInterp1DScaled (Operator.hpp:128-163): values at the quadrature
+// points -> values at the scaled point sets; the Collections wrapper sets
+// the factor from eFactorConst (Collections/PhysInterp1DScaled.cpp:158-186).
+class PhysInterp1DScaled : public Operator {
+public:
+    using Operator::Operator;
+    void SetScalingFactor(double factor) { NmfCheck(nmf_operator_set_scaling_factor(m_op, factor)); }
+};
+
 // String-keyed factory (NekFactory<std::string, Operator, basis, nElmt>).
 class OperatorFactory {
 public:
@@ -208,6 +218,8 @@ public:
         if (key.rfind("IProductWRTDerivBase", 0) == 0)
             return std::make_shared<IProductWRTDerivBase>(key, basis, nElmt, precision);
         if (key.rfind("Helmholtz", 0) == 0) return std::make_shared<Helmholtz>(key, basis, nElmt, precision);
+        if (key.rfind("PhysInterp1DScaled", 0) == 0)
+            return std::make_shared<PhysInterp1DScaled>(key, basis, nElmt, precision);
         if (key.rfind("LinearAdvectionDiffusionReaction", 0) == 0)
             return std::make_shared<LinearAdvectionDiffusionReaction>(key, basis, nElmt, precision);
         return std::make_shared<Operator>(key, basis, nElmt, precision);

@@ -91,7 +91,8 @@ int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0,
 /* MatrixFree::GetOperatorFactory().CreateInstance(key, basis, nElmt)
  * (Operator.cpp:40-90).  key = "<Op>_<Shape>_<Regular|Deformed>" as built
  * by GetOpstring, Op in {BwdTrans, IProduct, IProductWRTDerivBase,
- * PhysDeriv, Helmholtz, LinearAdvectionDiffusionReaction},
+ * PhysDeriv, Helmholtz, LinearAdvectionDiffusionReaction,
+ * PhysInterp1DScaled},
  * Shape in {Hex, Tet}.  nummodes = P+1 per direction, numpoints0 = Q in
  * direction 0 (tets use Q-1 Gauss-Radau points in directions 1, 2). */
 int nmf_operator_create(const char *key, int nummodes, int numpoints0,
@@ -118,6 +119,16 @@ int nmf_operator_set_const_var_diffusion(nmf_operator op, const double *d6);
  * V / (-lambda) (LinearAdvectionDiffusionReaction.cpp:578-589, 660-669).
  * SetLambda and SetConstVarDiffusion apply to this operator as well. */
 int nmf_operator_set_advection(nmf_operator op, const double *vel);
+/* PhysInterp1DScaled::SetScalingFactor + SetUpInterp1D (Operator.hpp
+ * :151-155, 561-589; default 1.5): point values at the quadrature points
+ * -> values at (int)(nq*factor) points of the same family per direction
+ * ((int)(nq0*factor - 1) in the tet's collapsed directions 1 and 2).
+ * Output sizes beyond one workgroup's 64 KiB of staging are rejected with
+ * NMF_ERR_UNSUPPORTED. */
+int nmf_operator_set_scaling_factor(nmf_operator op, double factor);
+/* Collections::Operator::GetInputSize / GetOutputSize (Operator.h:178-230):
+ * values in the whole input and output arrays of operator(). */
+int nmf_operator_io_size(nmf_operator op, int64_t *in_size, int64_t *out_size);
 /* operator()(input, output) for BwdTrans / IProduct / Helmholtz
  * (Operator.hpp:119-121, 181-183, 294-296) */
 int nmf_operator_apply(nmf_operator op, const void *in, void *out,

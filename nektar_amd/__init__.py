@@ -83,6 +83,8 @@ _SIGNATURES = {
     "nmf_operator_set_lambda": (ctypes.c_int, [_P, ctypes.c_double]),
     "nmf_operator_set_const_var_diffusion": (ctypes.c_int, [_P, _D]),
     "nmf_operator_set_advection": (ctypes.c_int, [_P, _D]),
+    "nmf_operator_set_scaling_factor": (ctypes.c_int, [_P, ctypes.c_double]),
+    "nmf_operator_io_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "nmf_operator_apply": (ctypes.c_int, [_P, _P, _P, _P]),
     "nmf_operator_apply_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "nmf_operator_apply_host": (ctypes.c_int, [_P, _D, _D]),
@@ -252,14 +254,20 @@ class Operator:
         check(lib().nmf_operator_set_advection(self._h, _dptr(v)))
 
     # --- sizes (Collections::Operator::GetInputSize/GetOutputSize) --------
+    def _io_size(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().nmf_operator_io_size(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def input_size(self) -> int:
-        return self.nelmt * (self.ncoeff if self.op in ("BwdTrans", "Helmholtz", "LinearAdvectionDiffusionReaction")
-                             else self.nqtot)
+        return self._io_size()[0]
 
     def output_size(self) -> int:
-        return self.nelmt * (self.ncoeff if self.op in ("IProduct", "Helmholtz", "IProductWRTDerivBase",
-                                                        "LinearAdvectionDiffusionReaction")
-                             else self.nqtot)
+        return self._io_size()[1]
+
+    def SetScalingFactor(self, factor: float) -> None:
+        """PhysInterp1DScaled: points per direction scaled by factor."""
+        check(lib().nmf_operator_set_scaling_factor(self._h, float(factor)))
 
     def torch_dtype(self):
         return torch.float32 if self.precision == FP32 else torch.float64

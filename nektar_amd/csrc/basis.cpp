@@ -247,4 +247,34 @@ double weight_prescale(PointsType t)
     return t == PointsType::RadauM_A1B0 ? 0.5 : t == PointsType::RadauM_A2B0 ? 0.25 : 1.0;
 }
 
+std::vector<double> interp_matrix(PointsType t, int nin, int nout)
+{
+    const Points1D pi = make_points(t, nin), po = make_points(t, nout);
+    std::vector<double> I((size_t)nin * nout);
+    for (int p = 0; p < nin; ++p)
+        for (int i = 0; i < nout; ++i) {
+            // an output point that is an input point interpolates exactly
+            int hit = -1;
+            for (int k = 0; k < nin; ++k)
+                if (std::fabs(po.z[i] - pi.z[k]) < 1e-15) hit = k;
+            double h = 1.0;
+            if (hit >= 0) {
+                h = hit == p ? 1.0 : 0.0;
+            } else {
+                for (int k = 0; k < nin; ++k)
+                    if (k != p) h *= (po.z[i] - pi.z[k]) / (pi.z[p] - pi.z[k]);
+            }
+            I[(size_t)p * nout + i] = h;
+        }
+    return I;
+}
+
+void interp_points(Shape s, int nq0, double factor, int out[3])
+{
+    int nq[3];
+    default_points(s, nq0, nq);
+    for (int d = 0; d < 3; ++d)
+        out[d] = (nq0 - nq[d] == 1) ? (int)(nq0 * factor - 1) : (int)(nq[d] * factor);
+}
+
 }  // namespace nektar_amd

@@ -48,4 +48,12 @@ PointsType default_points_type(Shape s, int dir);
 // Radau weight pre-scaling used by the matrix-free operators.
 double weight_prescale(PointsType t);
 
+// Interpolation from nin points of family t to nout points of the same
+// family: I[p*nout + i] = h_p(zo_i), the Lagrange polynomial of input point
+// p at output point i (GaussPoints::CalculateInterpMatrix,
+// GaussPoints.cpp:283-380, via Polylib Imglj / Imgrjm).
+std::vector<double> interp_matrix(PointsType t, int nin, int nout);
+// Points per direction of PhysInterp1DScaled (Operator.hpp:568-575).
+void interp_points(Shape s, int nq0, double factor, int out[3]);
+
 }  // namespace nektar_amd

@@ -167,6 +167,24 @@ int nmf_operator_set_const_var_diffusion(nmf_operator op, const double *d6)
     });
 }
 
+int nmf_operator_set_scaling_factor(nmf_operator op, double factor)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        op->impl->set_scaling_factor(factor);
+    });
+}
+
+int nmf_operator_io_size(nmf_operator op, int64_t *in_size, int64_t *out_size)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        const MFOperator &o = *op->impl;
+        if (in_size) *in_size = (int64_t)o.nelmt * (int64_t)o.in_per_elmt();
+        if (out_size) *out_size = (int64_t)o.nelmt * (int64_t)o.out_per_elmt();
+    });
+}
+
 int nmf_operator_set_advection(nmf_operator op, const double *vel)
 {
     return guard([&] {
@@ -197,17 +215,8 @@ int nmf_operator_apply_physderiv(nmf_operator op, const void *in, void *out0, vo
 }
 
 namespace {
-size_t in_count(const MFOperator &o)
-{
-    const bool coeffs = o.op == OpType::BwdTrans || o.op == OpType::Helmholtz || o.op == OpType::AdvDiffReact;
-    return (size_t)o.nelmt * (coeffs ? o.ncoeff : o.nqtot);
-}
-size_t out_count(const MFOperator &o)
-{
-    const bool coeffs = o.op == OpType::IProduct || o.op == OpType::Helmholtz || o.op == OpType::IProductDeriv ||
-                        o.op == OpType::AdvDiffReact;
-    return (size_t)o.nelmt * (coeffs ? o.ncoeff : o.nqtot);
-}
+size_t in_count(const MFOperator &o) { return (size_t)o.nelmt * o.in_per_elmt(); }
+size_t out_count(const MFOperator &o) { return (size_t)o.nelmt * o.out_per_elmt(); }
 void to_device(DeviceBuffer &b, const double *h, size_t n, bool fp32)
 {
     std::vector<double> v(h, h + n);

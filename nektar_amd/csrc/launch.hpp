@@ -7,7 +7,15 @@
 
 namespace nektar_amd {
 
-enum class OpType { BwdTrans = 0, IProduct = 1, PhysDeriv = 2, Helmholtz = 3, IProductDeriv = 4, AdvDiffReact = 5 };
+enum class OpType {
+    BwdTrans = 0,
+    IProduct = 1,
+    PhysDeriv = 2,
+    Helmholtz = 3,
+    IProductDeriv = 4,
+    AdvDiffReact = 5,
+    PhysInterp = 6
+};
 
 struct KernelArgs {
     OpType op;
@@ -79,6 +87,21 @@ constexpr int tet_block_elems(int nq0) { return 256 / (nq0 * (nq0 - 1)) > 0 ? 25
 inline int hex_tile_elems(int nq, int tile) { return tile > 0 ? tile : hex_fill_elems(nq); }
 
 hipError_t launch_hex_grouped(int nm, int nq, const GroupArgs &a);
+
+// PhysInterp1DScaled: ni / no points per direction in and out, tables
+// I0 | I1 | I2 with Id[p*no[d] + i] (k_interp.hip).
+struct InterpArgs {
+    bool fp32;
+    int nelmt;
+    int ni[3], no[3];
+    const void *tab;
+    const void *in;
+    void *out;
+    hipStream_t stream;
+};
+constexpr size_t kInterpMaxLds = 65536;  // dynamic LDS of one workgroup
+size_t interp_lds_bytes(const int ni[3], const int no[3], bool fp32);
+hipError_t launch_phys_interp(const InterpArgs &a);
 hipError_t launch_tet_grouped(int nm, int nq0, const GroupArgs &a);
 
 constexpr int kMinModes = 2, kMaxModes = 10;

@@ -23,10 +23,11 @@ void parse_operator_key(const std::string &key, OpType &op, Shape &shape, bool &
     else if (o == "Helmholtz") op = OpType::Helmholtz;
     else if (o == "IProductWRTDerivBase") op = OpType::IProductDeriv;
     else if (o == "LinearAdvectionDiffusionReaction") op = OpType::AdvDiffReact;
+    else if (o == "PhysInterp1DScaled") op = OpType::PhysInterp;
     else
         throw Error(NMF_ERR_UNSUPPORTED, "operator '" + o +
                                              "' is not provided (BwdTrans, IProduct, IProductWRTDerivBase, PhysDeriv, "
-                                             "Helmholtz, LinearAdvectionDiffusionReaction)");
+                                             "Helmholtz, LinearAdvectionDiffusionReaction, PhysInterp1DScaled)");
     if (s == "Hex") shape = Shape::Hex;
     else if (s == "Tet") shape = Shape::Tet;
     else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet)");
@@ -96,8 +97,45 @@ MFOperator::MFOperator(const std::string &key, int nummodes, int numpoints0, int
     nqtot = nq[0] * nq[1] * nq[2];
     ncoeff = num_coeffs(shape, nummodes);
     tab.upload(build_tables(shape, nm, numpoints0), fp32);
-    if (op == OpType::BwdTrans) deformed = false;  // BwdTrans needs no geometry
+    if (op == OpType::BwdTrans || op == OpType::PhysInterp) deformed = false;  // no geometry
+    if (op == OpType::PhysInterp) set_scaling_factor(1.5);  // the reference's default (Operator.hpp:136-141)
     tile = tuned_tile(tune_key());
+}
+
+void MFOperator::set_scaling_factor(double f)
+{
+    NMF_REQUIRE(op == OpType::PhysInterp, NMF_ERR_INVALID, "SetScalingFactor: not a PhysInterp1DScaled operator");
+    NMF_REQUIRE(std::isfinite(f) && f > 0.0, NMF_ERR_INVALID, "SetScalingFactor: factor must be positive");
+    int no[3];
+    interp_points(shape, nq[0], f, no);
+    for (int d = 0; d < 3; ++d)
+        NMF_REQUIRE(no[d] >= 1, NMF_ERR_INVALID, "SetScalingFactor: factor leaves no points in a direction");
+    NMF_REQUIRE(interp_lds_bytes(nq, no, fp32) <= kInterpMaxLds, NMF_ERR_UNSUPPORTED,
+                "SetScalingFactor: " + std::to_string(no[0]) + "x" + std::to_string(no[1]) + "x" +
+                    std::to_string(no[2]) + " output points exceed the 64 KiB workgroup staging of the kernel");
+    std::vector<double> t;
+    for (int d = 0; d < 3; ++d) {
+        const auto I = interp_matrix(default_points_type(shape, d), nq[d], no[d]);
+        t.insert(t.end(), I.begin(), I.end());
+    }
+    itab.upload(t, fp32);
+    factor = f;
+    for (int d = 0; d < 3; ++d) nqo[d] = no[d];
+    ++version;
+}
+
+size_t MFOperator::in_per_elmt() const
+{
+    const bool coeffs = op == OpType::BwdTrans || op == OpType::Helmholtz || op == OpType::AdvDiffReact;
+    return coeffs ? (size_t)ncoeff : (size_t)nqtot;
+}
+
+size_t MFOperator::out_per_elmt() const
+{
+    if (op == OpType::PhysInterp) return (size_t)nqo[0] * nqo[1] * nqo[2];
+    const bool coeffs = op == OpType::IProduct || op == OpType::Helmholtz || op == OpType::IProductDeriv ||
+                        op == OpType::AdvDiffReact;
+    return coeffs ? (size_t)ncoeff : (size_t)nqtot;
 }
 
 void MFOperator::set_jac(const double *j)
@@ -327,7 +365,7 @@ void MFOperator::ensure_geometry()
             adv_dirty = false;
         }
     }
-    if (!geo_dirty || op == OpType::BwdTrans) return;
+    if (!geo_dirty || op == OpType::BwdTrans || op == OpType::PhysInterp) return;
     geo.upload(pack_geometry(), fp32);
     geo_dirty = false;
 }
@@ -348,6 +386,12 @@ void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipSt
     NMF_REQUIRE(nelmt == 0 || (in && out0), NMF_ERR_INVALID, "null input/output pointer");
     if (op == OpType::PhysDeriv) NMF_REQUIRE(nelmt == 0 || (out1 && out2), NMF_ERR_INVALID, "PhysDeriv needs three outputs");
     ensure_geometry();
+    if (op == OpType::PhysInterp) {
+        const InterpArgs ia{fp32, nelmt, {nq[0], nq[1], nq[2]}, {nqo[0], nqo[1], nqo[2]}, itab.get(), in, out0, s};
+        const hipError_t e = launch_phys_interp(ia);
+        NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        return;
+    }
     KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in, out0, out1, out2, s, tile};
     a.adv = adv.get();
     hipError_t e = shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);

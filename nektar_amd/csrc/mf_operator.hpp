@@ -33,7 +33,12 @@ struct MFOperator {
     std::vector<double> vel;
     bool have_vel = false, adv_dirty = true;
 
-    DeviceBuffer tab, geo, adv;
+    // PhysInterp1DScaled: scaling factor, output points per direction and
+    // the interpolation matrices I0 | I1 | I2 on the device
+    double factor = 1.5;
+    int nqo[3] = {0, 0, 0};
+
+    DeviceBuffer tab, geo, adv, itab;
     int tile = 0;  // elements per workgroup of the Helmholtz kernel (0 = default)
     uint64_t version = 0;  // bumped by every change of geometry, factors or tile
 
@@ -59,6 +64,12 @@ struct MFOperator {
         ++version;
     }
     void set_diffusion(const double *d6);
+    // PhysInterp1DScaled::SetScalingFactor (Operator.hpp:151-155) and
+    // SetUpInterp1D (:561-589): output points per direction and tables
+    void set_scaling_factor(double f);
+    // Collections::Operator::GetInputSize / GetOutputSize per element
+    size_t in_per_elmt() const;
+    size_t out_per_elmt() const;
     // LinearAdvectionDiffusionReaction::SetAdvectionVelocities
     // (Operator.hpp:476-481): three velocity components at every point
     void set_advection(const double *v);

@@ -889,3 +889,100 @@ void or_helmholtz_diag_bruteforce(int shape, int nm, int nq0, int nelmt,
     }
     tables_free(&T);
 }
+
+/* ---- PhysInterp1DScaled -------------------------------------------- */
+
+/* Points per direction after scaling (Operator.hpp:568-575): a direction
+ * with one point fewer than direction 0 (the tet's collapsed directions)
+ * gets (int)(nq0*factor - 1), every other (int)(nq*factor). */
+void or_interp_nq(int shape, int nq0, double factor, int out[3])
+{
+    int nq[3];
+    or_default_nq(shape, nq0, nq);
+    for (int d = 0; d < 3; ++d)
+        out[d] = (nq0 - nq[d] == 1) ? (int)(nq0 * factor - 1) : (int)(nq[d] * factor);
+}
+
+/* Interpolation matrix from nq_in points of one family to nq_out points of
+ * the same family, I[p*nq_out + i] = h_p(zo_i), the Lagrange polynomial of
+ * input point p at output point i (GaussPoints.cpp:283-380 via Polylib
+ * Imglj / Imgrjm, Polylib.cpp:1085-1160).  Evaluated here as the product
+ * formula of the same polynomial, h_p(x) = prod_{k != p} (x - z_k)/(z_p -
+ * z_k), and exactly 1 / 0 where an output point coincides with an input
+ * point. */
+void or_interp_matrix(int ptype, int nq_in, int nq_out, double *I)
+{
+    double *zi = (double *)malloc(sizeof(double) * nq_in), *wi = (double *)malloc(sizeof(double) * nq_in);
+    double *Di = (double *)malloc(sizeof(double) * nq_in * nq_in);
+    double *zo = (double *)malloc(sizeof(double) * nq_out), *wo = (double *)malloc(sizeof(double) * nq_out);
+    double *Do = (double *)malloc(sizeof(double) * nq_out * nq_out);
+    or_points(ptype, nq_in, zi, wi, Di);
+    or_points(ptype, nq_out, zo, wo, Do);
+    for (int p = 0; p < nq_in; ++p)
+        for (int i = 0; i < nq_out; ++i) {
+            double h = 1.0;
+            int hit = -1;
+            for (int k = 0; k < nq_in; ++k)
+                if (fabs(zo[i] - zi[k]) < 1e-15) hit = k;
+            if (hit >= 0) {
+                h = hit == p ? 1.0 : 0.0;
+            } else {
+                for (int k = 0; k < nq_in; ++k)
+                    if (k != p) h *= (zo[i] - zi[k]) / (zi[p] - zi[k]);
+            }
+            I[p * nq_out + i] = h;
+        }
+    free(zi); free(wi); free(Di); free(zo); free(wo); free(Do);
+}
+
+/* PhysInterp1DScaled3DKernel (PhysInterp1DScaledKernels.hpp:151-217): the
+ * three 1-D interpolations in turn, directions 0, 1, 2, in the reference's
+ * loop and summation order. */
+void or_phys_interp1d_scaled(int shape, int nq0, int nelmt, double factor,
+                             const double *in, double *out)
+{
+    int ni[3], no[3];
+    or_default_nq(shape, nq0, ni);
+    or_interp_nq(shape, nq0, factor, no);
+    const int pt[3] = {OR_GLL, shape == OR_TET ? OR_RADAU_A1 : OR_GLL, shape == OR_TET ? OR_RADAU_A2 : OR_GLL};
+    double *I[3];
+    for (int d = 0; d < 3; ++d) {
+        I[d] = (double *)malloc(sizeof(double) * ni[d] * no[d]);
+        or_interp_matrix(pt[d], ni[d], no[d], I[d]);
+    }
+    const int nin = ni[0] * ni[1] * ni[2], nout = no[0] * no[1] * no[2];
+#pragma omp parallel num_threads(g_threads)
+    {
+        double *s_irq = (double *)malloc(sizeof(double) * no[0] * ni[1] * ni[2]);
+        double *s_jir = (double *)malloc(sizeof(double) * no[1] * no[0] * ni[2]);
+#pragma omp for schedule(static)
+        for (int e = 0; e < nelmt; ++e) {
+            const double *x = in + (size_t)e * nin;
+            double *y = out + (size_t)e * nout;
+            for (int i = 0, c = 0; i < no[0]; ++i)
+                for (int r = 0, rqp = 0; r < ni[2]; ++r)
+                    for (int q = 0; q < ni[1]; ++q, ++c) {
+                        double t = x[rqp++] * I[0][i];
+                        for (int p = 1; p < ni[0]; ++p, ++rqp) t = fma(x[rqp], I[0][p * no[0] + i], t);
+                        s_irq[c] = t;
+                    }
+            for (int j = 0, c = 0; j < no[1]; ++j)
+                for (int i = 0, irq = 0; i < no[0]; ++i)
+                    for (int r = 0; r < ni[2]; ++r, ++c) {
+                        double t = s_irq[irq++] * I[1][j];
+                        for (int q = 1; q < ni[1]; ++q) t = fma(s_irq[irq++], I[1][q * no[1] + j], t);
+                        s_jir[c] = t;
+                    }
+            for (int k = 0, c = 0; k < no[2]; ++k)
+                for (int j = 0, jir = 0; j < no[1]; ++j)
+                    for (int i = 0; i < no[0]; ++i, ++c) {
+                        double t = s_jir[jir++] * I[2][k];
+                        for (int r = 1; r < ni[2]; ++r) t = fma(s_jir[jir++], I[2][r * no[2] + k], t);
+                        y[c] = t;
+                    }
+        }
+        free(s_irq);
+        free(s_jir);
+    }
+    for (int d = 0; d < 3; ++d) free(I[d]);
+}

@@ -88,6 +88,13 @@ void or_adv_diff_react(int shape, int nm, int nq0, int nelmt, int deformed,
                        const double *jac, const double *df, const double *vel,
                        double lambda, const double *diff6, const double *in,
                        double *out);
+
+/* PhysInterp1DScaled (PhysInterp1DScaled.h:360-430): point values of
+ * nq_in points per direction -> or_interp_nq points per direction. */
+void or_interp_nq(int shape, int nq0, double factor, int out[3]);
+void or_interp_matrix(int ptype, int nq_in, int nq_out, double *I);
+void or_phys_interp1d_scaled(int shape, int nq0, int nelmt, double factor,
+                             const double *in, double *out);
 /* Diagonal of each element's Helmholtz matrix (ncoeff per element),
  * obtained by applying the element operator to unit vectors (the same
  * numbers as the diagonal of the reference's dense local matrices). */

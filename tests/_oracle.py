@@ -124,6 +124,27 @@ def adv_diff_react(shape, nm, nq0, nelmt, deformed, jac, df, vel, lam, x, diff6=
     return out
 
 
+def interp_nq(shape, nq0, factor):
+    out = (ctypes.c_int * 3)()
+    lib().or_interp_nq(shape, nq0, ctypes.c_double(factor), out)
+    return list(out)
+
+
+def interp_matrix(ptype, nq_in, nq_out):
+    """I[p*nq_out + i] = h_p(zo_i)."""
+    out = np.zeros(nq_in * nq_out)
+    lib().or_interp_matrix(ptype, nq_in, nq_out, p(out))
+    return out
+
+
+def phys_interp1d_scaled(shape, nq0, nelmt, factor, u):
+    no = interp_nq(shape, nq0, factor)
+    out = np.zeros(nelmt * no[0] * no[1] * no[2])
+    lib().or_phys_interp1d_scaled(shape, nq0, nelmt, ctypes.c_double(factor),
+                                  p(np.ascontiguousarray(u, dtype=np.float64)), p(out))
+    return out
+
+
 def helmholtz_diag(shape, nm, nq0, nelmt, deformed, jac, df, lam):
     out = np.zeros(nelmt * ncoeffs(shape, nm))
     lib().or_helmholtz_diag_bruteforce(shape, nm, nq0, nelmt, int(deformed), p(jac), p(df),
