@@ -89,7 +89,8 @@ inline int hex_tile_elems(int nq, int tile) { return tile > 0 ? tile : hex_fill_
 hipError_t launch_hex_grouped(int nm, int nq, const GroupArgs &a);
 
 // PhysInterp1DScaled: ni / no points per direction in and out, tables
-// I0 | I1 | I2 with Id[p*no[d] + i] (k_interp.hip).
+// IT0 | IT1 | IT2 with ITd[i*ni[d] + p] = weight of input point p for
+// output point i (k_interp.hip).
 struct InterpArgs {
     bool fp32;
     int nelmt;

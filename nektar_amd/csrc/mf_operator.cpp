@@ -113,11 +113,15 @@ void MFOperator::set_scaling_factor(double f)
     NMF_REQUIRE(interp_lds_bytes(nq, no, fp32) <= kInterpMaxLds, NMF_ERR_UNSUPPORTED,
                 "SetScalingFactor: " + std::to_string(no[0]) + "x" + std::to_string(no[1]) + "x" +
                     std::to_string(no[2]) + " output points exceed the 64 KiB workgroup staging of the kernel");
+    // transposed for the kernel: IT[i*nin + p] = I[p*nout + i], the weights
+    // of one output contiguous
     std::vector<double> t;
     for (int d = 0; d < 3; ++d) {
         const auto I = interp_matrix(default_points_type(shape, d), nq[d], no[d]);
-        t.insert(t.end(), I.begin(), I.end());
+        for (int i = 0; i < no[d]; ++i)
+            for (int p = 0; p < nq[d]; ++p) t.push_back(I[(size_t)p * no[d] + i]);
     }
+    t.resize(t.size() + 16, 0.0);  // the kernel reads whole rows of up to 12 values
     itab.upload(t, fp32);
     factor = f;
     for (int d = 0; d < 3; ++d) nqo[d] = no[d];
