@@ -394,6 +394,18 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
         for (int ii = 0; ii < NQ; ++ii) B[(k0 * NQ + l0) * P + ii] = o[ii];
     }
     __syncthreads();
+    // advection coefficients of this column: loads issued here, so that
+    // their latency hides behind the direction-1 and direction-2 passes
+    T cadv[3][ADV ? NQ : 1];
+    if constexpr (ADV) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const T *ac = adv + (size_t)e * 3 * NQ3 + (k * NQ + j) * NQ + i;
+            cadv[0][k] = valid ? ac[0] : T(0);
+            cadv[1][k] = valid ? ac[NQ3] : T(0);
+            cadv[2][k] = valid ? ac[2 * NQ3] : T(0);
+        }
+    }
     {   // d1 along j on line (i = l0, k = k0), in place in A
         T v[NQ], o[NQ];
 #pragma unroll
@@ -448,11 +460,9 @@ __device__ __forceinline__ void helmholtz_element(const Tab<T, NM, NQ> &tab, con
             v2[k] = t2;
             T m = u[k];
             if constexpr (ADV) {
-                const T *ac = adv + (size_t)e * 3 * NQ3 + c;
-                const T c0 = valid ? ac[0] : T(0), c1 = valid ? ac[NQ3] : T(0), c2 = valid ? ac[2 * NQ3] : T(0);
-                m = fma(c0, a0, m);
-                m = fma(c1, a1, m);
-                m = fma(c2, a2, m);
+                m = fma(cadv[0][k], a0, m);
+                m = fma(cadv[1][k], a1, m);
+                m = fma(cadv[2][k], a2, m);
             }
             y[k] = (lambda * g[6]) * (DEF ? m : w * m);
         }

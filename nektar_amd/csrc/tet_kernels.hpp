@@ -652,6 +652,18 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
         }
     }
     __syncthreads();
+    // advection coefficients of this column: loads issued here, so that
+    // their latency hides behind the direction-1 pass
+    T cadv[3][ADV ? NQ2 : 1];
+    if constexpr (ADV) {
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) {
+            const T *ac = adv + (size_t)e * 3 * NQT + (k * NQ1 + j) * NQ0 + i;
+            cadv[0][k] = valid ? ac[0] : T(0);
+            cadv[1][k] = valid ? ac[NQT] : T(0);
+            cadv[2][k] = valid ? ac[2 * NQT] : T(0);
+        }
+    }
     if (t < NQ0 * NQ2) {
         const int ii = t % NQ0, k = t / NQ0;
         T v[NQ1];
@@ -731,11 +743,9 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             v2[k] = wj * o2;
             T m = u[k];
             if constexpr (ADV) {
-                const T *ac = adv + (size_t)e * 3 * NQT + c;
-                const T c0 = valid ? ac[0] : T(0), c1 = valid ? ac[NQT] : T(0), c2 = valid ? ac[2 * NQT] : T(0);
-                m = fma(c0, a0, m);
-                m = fma(c1, a1, m);
-                m = fma(c2, a2, m);
+                m = fma(cadv[0][k], a0, m);
+                m = fma(cadv[1][k], a1, m);
+                m = fma(cadv[2][k], a2, m);
             }
             y[k] = (lambda * wj) * m;
         }
