@@ -85,6 +85,79 @@ def time_apply(op, x, y, steps, warmup, torch):
     return s.elapsed_time(e) / steps, wall
 
 
+def tet_flops_per_elem(nm, nq0):
+    """Algorithmic flops of the fused tet Helmholtz apply (reference loop
+    counts, BwdTransKernels.hpp:242-328, IProductKernels.hpp:369-454):
+    F = 2 T + G + 30 nqt, T = 2(nq2 nc + nq1 nq2 nmb + nqt nm),
+    G = 4 nqt (nq0 + nq1 + nq2)."""
+    nq1 = nq2 = nq0 - 1
+    nc = nm * (nm + 1) * (nm + 2) // 6
+    nqt = nq0 * nq1 * nq2
+    T = 2 * (nq2 * nc + nq1 * nq2 * nm * (nm + 1) // 2 + nqt * nm)
+    G = 4 * nqt * (nq0 + nq1 + nq2)
+    return 2 * T + G + 30 * nqt
+
+
+def bench_ops(N, torch, steps=20, warmup=3):
+    """Every operator of the Collections interface on 32,768 affine
+    elements, P=6 (7 modes), Q=8, hex and tet, FP64 and FP32: average
+    launch time over back-to-back launches, algorithmic HBM bytes (inputs,
+    outputs, geometry, velocities) and, for the fused Helmholtz-type
+    operators, algorithmic flops."""
+    import numpy as np
+
+    E, nm, nq = 32768, 7, 8
+    rng = np.random.default_rng(6)
+    s = rng.uniform(0.5, 2.0, E)[:, None, None]
+    M = s * (np.eye(3)[None] + 0.25 * rng.uniform(-1, 1, (E, 3, 3))) * 0.05
+    jac = np.abs(np.linalg.det(M))
+    df = np.ascontiguousarray(np.transpose(np.linalg.inv(M), (0, 2, 1))).reshape(-1)
+    out = {}
+    for shape in ("Hex", "Tet"):
+        nc = nm ** 3 if shape == "Hex" else nm * (nm + 1) * (nm + 2) // 6
+        nqt = nq ** 3 if shape == "Hex" else nq * (nq - 1) ** 2
+        fh = hex_flops_per_elem(nm, nq) if shape == "Hex" else tet_flops_per_elem(nm, nq)
+        g = 7 if shape == "Hex" else 10
+        vel = rng.uniform(-1, 1, E * nqt * 3)
+        for prec, tag in ((N.FP64, "fp64"), (N.FP32, "fp32")):
+            sv = 8 if prec == N.FP64 else 4
+            for key in ("Helmholtz", "LinearAdvectionDiffusionReaction", "PhysInterp1DScaled", "BwdTrans",
+                        "IProduct"):
+                op = N.GetOperatorFactory().CreateInstance(f"{key}_{shape}_Regular", nm, nq, E, prec)
+                if key not in ("BwdTrans", "PhysInterp1DScaled"):
+                    op.SetJac(jac)
+                if key in ("Helmholtz", "LinearAdvectionDiffusionReaction"):
+                    op.SetDF(df)
+                    op.SetLambda(1.0)
+                if key == "LinearAdvectionDiffusionReaction":
+                    op.SetAdvectionVelocities(vel)
+                if key == "Helmholtz":
+                    op.autotune()
+                dt = torch.float64 if prec == N.FP64 else torch.float32
+                x = torch.rand(op.input_size(), dtype=dt, device="cuda") * 2 - 1
+                y = torch.zeros(op.output_size(), dtype=dt, device="cuda")
+                ms, _ = time_apply(op, x, y, steps, warmup, torch)
+                us = ms * 1e3
+                if key == "Helmholtz":
+                    byts, flops = E * (2 * nc + g) * sv, E * fh
+                elif key == "LinearAdvectionDiffusionReaction":
+                    byts, flops = E * (2 * nc + g + 3 * nqt) * sv, E * (fh + 6 * nqt)
+                elif key == "PhysInterp1DScaled":
+                    byts, flops = (op.input_size() + op.output_size()) * sv, None
+                elif key == "BwdTrans":
+                    byts, flops = E * (nc + nqt) * sv, None
+                else:
+                    byts, flops = E * (nqt + nc + 1) * sv, None
+                r = {"us": round(us, 2), "alg_gbs": round(byts / us / 1e3, 1),
+                     "hbm_frac": round(byts / us / 1e3 / 8000.0, 3)}
+                if flops:
+                    r["tflops"] = round(flops / us / 1e6, 2)
+                    r["fp_frac"] = round(flops / us / 1e6 / (PEAK_FP64_TFLOPS if sv == 8 else 157.3), 3)
+                out[f"{key}_{shape}_{tag}"] = r
+                del op, x, y
+    return out
+
+
 def batch_workload(N, seed=4, count=64):
     """64 Helmholtz applies: hex/tet 50/50, P ~ U{3..9}, Q = P+2, E
     log-uniform in [1e3, 1e5], affine elements with per-element scale
@@ -278,6 +351,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-curved", action="store_true", help="skip the curved 48^3 P=7 solves")
+    ap.add_argument("--no-ops", action="store_true",
+                    help="skip the per-operator timings (hex and tet, 32,768 elements, P=6)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 512 MiB ten times (PMC byte-counter calibration)")
@@ -368,6 +443,8 @@ def main():
         extra["unknowns"] = nunk
         del s
 
+    if not args.no_ops:
+        extra["ops_p6_32768"] = bench_ops(N, torch)
     if not args.no_curved:
         extra["curved_48_p7"] = bench_curved(N)
     if not args.no_batch:
