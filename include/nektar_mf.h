@@ -58,6 +58,8 @@ typedef struct nmf_operator_s *nmf_operator;
 typedef struct nmf_solver_s *nmf_solver;
 typedef struct nmf_batch_s *nmf_batch;
 typedef struct nmf_server_s *nmf_server;
+typedef struct nmf_assembly_s *nmf_assembly;
+typedef struct nmf_linsys_s *nmf_linsys;
 
 /* Last error message of the calling thread ("" if none). */
 const char *nmf_last_error(void);
@@ -330,6 +332,45 @@ int nmf_server_run(nmf_server s, const nmf_request *reqs, int count,
 /* The mesh generator used by nmf_solver_create_hex_box: (n+1)^3*3
  * vertex coordinates, vertex (ix,iy,iz) at ((iz*(n+1)+iy)*(n+1)+ix)*3. */
 int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz);
+
+/* ---------------------------------------------------------------------
+ * General C0 assembly and iterative Helmholtz solve on any conforming mesh
+ * (hex and tet collections sharing one map).
+ * AssemblyMapCG (AssemblyMapCG.cpp:2804-2897): map[i] and sign[i] (+1/-1;
+ * NULL: all +1) of every entry i of the concatenated element-local
+ * coefficient vector (the operators' local vectors in the order they are
+ * passed to nmf_linsys_create), as m_localToGlobalMap /
+ * m_localToGlobalSign hold them; the ndir Dirichlet DOFs are numbered
+ * first (m_numGlobalDirBndCoeffs).  Vectors are device FP64.  Assembly is
+ * deterministic: every global value is summed in increasing local index.
+ * ------------------------------------------------------------------- */
+int nmf_assembly_create(const int *map, const double *sign, int64_t nloc,
+                        int nglobal, int ndir, nmf_assembly *out);
+int nmf_assembly_destroy(nmf_assembly a);
+/* GlobalToLocal: loc[i] = sign[i] glob[map[i]] (:2804-2827) */
+int nmf_assembly_global_to_local(nmf_assembly a, const double *glob,
+                                 double *loc, void *stream);
+/* Assemble: glob[g] = sum_{map[i]=g} sign[i] loc[i] (:2860-2886) */
+int nmf_assembly_assemble(nmf_assembly a, const double *loc, double *glob,
+                          void *stream);
+/* GlobalLinSysIterativeFull + NekLinSysIterCG with the Diagonal
+ * preconditioner (GlobalLinSysIterativeFull.cpp:96-383,
+ * NekLinSysIterCG.cpp:118-372, PreconditionerDiagonal.cpp:93-189) over FP64
+ * Helmholtz operators (hex and/or tet collections) and homogeneous
+ * Dirichlet data.  The operators and the map must outlive the handle; the
+ * diagonal is built at creation from the operators' current lambda and
+ * geometry (exact element-matrix diagonals, one apply per local mode). */
+int nmf_linsys_create(nmf_assembly a, const nmf_operator *ops, int nops,
+                      void *stream, nmf_linsys *out);
+int nmf_linsys_destroy(nmf_linsys l);
+/* b = Assemble(bloc) (device, local layout; e.g. IProductWRTBase of f) */
+int nmf_linsys_set_rhs_local(nmf_linsys l, const double *bloc);
+/* Stops at ||r|| < tol ||b|| (unpreconditioned, Dirichlet rows excluded) */
+int nmf_linsys_solve(nmf_linsys l, double tol, int maxit, nmf_solve_info *info);
+int nmf_linsys_get_solution(nmf_linsys l, double *x_host);
+int nmf_linsys_get_inv_diag(nmf_linsys l, double *d_host);
+/* y = A x on global device vectors, Dirichlet rows zeroed */
+int nmf_linsys_apply_global(nmf_linsys l, const double *x_dev, double *y_dev);
 
 #ifdef __cplusplus
 }

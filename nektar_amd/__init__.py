@@ -127,6 +127,17 @@ _SIGNATURES = {
     "nmf_solver_get_inv_diag": (ctypes.c_int, [_P, _D]),
     "nmf_solver_apply_global": (ctypes.c_int, [_P, _P, _P]),
     "nmf_hex_box_vertices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
+    "nmf_assembly_create": (ctypes.c_int, [_I, _D, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
+    "nmf_assembly_destroy": (ctypes.c_int, [_P]),
+    "nmf_assembly_global_to_local": (ctypes.c_int, [_P, _P, _P, _P]),
+    "nmf_assembly_assemble": (ctypes.c_int, [_P, _P, _P, _P]),
+    "nmf_linsys_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, _P, ctypes.POINTER(_P)]),
+    "nmf_linsys_destroy": (ctypes.c_int, [_P]),
+    "nmf_linsys_set_rhs_local": (ctypes.c_int, [_P, _P]),
+    "nmf_linsys_solve": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(SolveInfo)]),
+    "nmf_linsys_get_solution": (ctypes.c_int, [_P, _D]),
+    "nmf_linsys_get_inv_diag": (ctypes.c_int, [_P, _D]),
+    "nmf_linsys_apply_global": (ctypes.c_int, [_P, _P, _P]),
 }
 
 
@@ -571,3 +582,75 @@ class HelmholtzSolver:
         if torch is not None:
             torch.cuda.current_stream().synchronize()  # the solver runs on its own stream
         check(lib().nmf_solver_apply_global(self._h, px, py))
+
+
+class AssemblyMap:
+    """``AssemblyMapCG`` on the device for any conforming mesh: map[i] and
+    sign[i] (+1/-1, or None for all +1) of every entry of the concatenated
+    element-local coefficient vector; the ndir Dirichlet DOFs first."""
+
+    def __init__(self, map_, sign, nglobal: int, ndir: int):
+        m = np.ascontiguousarray(map_, dtype=np.int32)
+        sg = None if sign is None else np.ascontiguousarray(sign, dtype=np.float64)
+        if sg is not None and sg.size != m.size:
+            raise NektarError(1, "map and sign differ in length")
+        self.nloc, self.nglobal, self.ndir = int(m.size), int(nglobal), int(ndir)
+        self._h = ctypes.c_void_p()
+        check(lib().nmf_assembly_create(m.ctypes.data_as(_I), None if sg is None else _dptr(sg), self.nloc,
+                                        self.nglobal, self.ndir, ctypes.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nmf_assembly_destroy(self._h)
+            self._h = None
+
+    def global_to_local(self, glob, loc, stream=None) -> None:
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().nmf_assembly_global_to_local(self._h, _device_ptr(glob, torch.float64, self.nglobal),
+                                                 _device_ptr(loc, torch.float64, self.nloc), s))
+
+    def assemble(self, loc, glob, stream=None) -> None:
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().nmf_assembly_assemble(self._h, _device_ptr(loc, torch.float64, self.nloc),
+                                          _device_ptr(glob, torch.float64, self.nglobal), s))
+
+
+class LinearSystem:
+    """``GlobalLinSysIterativeFull`` + ``NekLinSysIterCG`` (Diagonal
+    preconditioner) over FP64 Helmholtz operators sharing an AssemblyMap;
+    the operators' local vectors, concatenated in the order given, are the
+    map's local vector."""
+
+    def __init__(self, amap: AssemblyMap, ops, stream=None):
+        self._amap, self._ops = amap, list(ops)  # keep both alive
+        arr = (ctypes.c_void_p * len(self._ops))(*[o._h.value for o in self._ops])
+        self._h = ctypes.c_void_p()
+        check(lib().nmf_linsys_create(amap._h, arr, len(self._ops), stream, ctypes.byref(self._h)))
+        self.nglobal = amap.nglobal
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nmf_linsys_destroy(self._h)
+            self._h = None
+
+    def set_rhs_local(self, bloc) -> None:
+        check(lib().nmf_linsys_set_rhs_local(self._h, _device_ptr(bloc, torch.float64, self._amap.nloc)))
+
+    def solve(self, tol: float = 1e-10, maxit: int = 5000) -> SolveInfo:
+        info = SolveInfo()
+        check(lib().nmf_linsys_solve(self._h, tol, maxit, ctypes.byref(info)))
+        return info
+
+    def solution(self) -> np.ndarray:
+        x = np.zeros(self.nglobal)
+        check(lib().nmf_linsys_get_solution(self._h, _dptr(x)))
+        return x
+
+    def inv_diag(self) -> np.ndarray:
+        d = np.zeros(self.nglobal)
+        check(lib().nmf_linsys_get_inv_diag(self._h, _dptr(d)))
+        return d
+
+    def apply_global(self, x, y) -> None:
+        check(lib().nmf_linsys_apply_global(self._h, _device_ptr(x, torch.float64, self.nglobal),
+                                            _device_ptr(y, torch.float64, self.nglobal)))

@@ -7,6 +7,7 @@
 #include "basis.hpp"
 #include "batch.hpp"
 #include "common.hpp"
+#include "assembly.hpp"
 #include "mf_operator.hpp"
 #include "server.hpp"
 #include "solver.hpp"
@@ -25,6 +26,14 @@ struct nmf_server_s {
 
 struct nmf_solver_s {
     std::unique_ptr<nektar_amd::HexSolver> impl;
+};
+
+struct nmf_assembly_s {
+    std::unique_ptr<nektar_amd::AssemblyMap> impl;
+};
+
+struct nmf_linsys_s {
+    std::unique_ptr<nektar_amd::IterativeSolver> impl;
 };
 
 namespace nektar_amd {
@@ -662,3 +671,123 @@ int nmf_hex_box_vertices(int n, double perturb, uint64_t seed, double *xyz)
 }
 
 }  // extern "C"
+
+/* ---- general C0 assembly and iterative solve ---------------------- */
+
+int nmf_assembly_create(const int *map, const double *sign, int64_t nloc, int nglobal, int ndir, nmf_assembly *out)
+{
+    return guard([&] {
+        NMF_REQUIRE(out, NMF_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        auto h = std::make_unique<nmf_assembly_s>();
+        h->impl = std::make_unique<AssemblyMap>(map, sign, nloc, nglobal, ndir);
+        *out = h.release();
+    });
+}
+
+int nmf_assembly_destroy(nmf_assembly a)
+{
+    return guard([&] { delete a; });
+}
+
+#define NMF_ASM_CHECK() NMF_REQUIRE(a && a->impl, NMF_ERR_INVALID, "null assembly handle")
+
+int nmf_assembly_global_to_local(nmf_assembly a, const double *glob, double *loc, void *stream)
+{
+    return guard([&] {
+        NMF_ASM_CHECK();
+        NMF_REQUIRE(a->impl->nloc() == 0 || (glob && loc), NMF_ERR_INVALID, "null vector");
+        a->impl->global_to_local(glob, loc, static_cast<hipStream_t>(stream));
+    });
+}
+
+int nmf_assembly_assemble(nmf_assembly a, const double *loc, double *glob, void *stream)
+{
+    return guard([&] {
+        NMF_ASM_CHECK();
+        NMF_REQUIRE(a->impl->nglobal() == 0 || (glob && (loc || a->impl->nloc() == 0)), NMF_ERR_INVALID,
+                    "null vector");
+        a->impl->assemble(loc, glob, static_cast<hipStream_t>(stream));
+    });
+}
+
+int nmf_linsys_create(nmf_assembly a, const nmf_operator *ops, int nops, void *stream, nmf_linsys *out)
+{
+    return guard([&] {
+        NMF_ASM_CHECK();
+        NMF_REQUIRE(out, NMF_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        NMF_REQUIRE(nops >= 1 && ops, NMF_ERR_INVALID, "at least one operator");
+        std::vector<MFOperator *> v;
+        for (int i = 0; i < nops; ++i) {
+            NMF_REQUIRE(ops[i] && ops[i]->impl, NMF_ERR_INVALID, "null operator handle");
+            v.push_back(ops[i]->impl.get());
+        }
+        auto h = std::make_unique<nmf_linsys_s>();
+        h->impl = std::make_unique<IterativeSolver>(*a->impl, v, static_cast<hipStream_t>(stream));
+        *out = h.release();
+    });
+}
+
+int nmf_linsys_destroy(nmf_linsys l)
+{
+    return guard([&] { delete l; });
+}
+
+#define NMF_LS_CHECK() NMF_REQUIRE(l && l->impl, NMF_ERR_INVALID, "null linear-system handle")
+
+int nmf_linsys_set_rhs_local(nmf_linsys l, const double *bloc)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        l->impl->set_rhs_local(bloc);
+    });
+}
+
+int nmf_linsys_solve(nmf_linsys l, double tol, int maxit, nmf_solve_info *info)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        NMF_REQUIRE(tol > 0.0 && maxit > 0, NMF_ERR_INVALID, "tol and maxit must be positive");
+        SolveInfo r = l->impl->solve(tol, maxit);
+        if (info) {
+            info->iterations = r.iterations;
+            info->outer_iterations = r.outer_iterations;
+            info->converged = r.converged;
+            info->final_error = r.final_error;
+            info->rhs_magnitude = r.rhs_magnitude;
+            info->solve_ms = r.solve_ms;
+        }
+    });
+}
+
+int nmf_linsys_get_solution(nmf_linsys l, double *x_host)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        NMF_REQUIRE(x_host, NMF_ERR_INVALID, "null output");
+        NMF_HIP(hipDeviceSynchronize());
+        NMF_HIP(hipMemcpy(x_host, l->impl->solution(), (size_t)l->impl->nglobal() * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    });
+}
+
+int nmf_linsys_get_inv_diag(nmf_linsys l, double *d_host)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        NMF_REQUIRE(d_host, NMF_ERR_INVALID, "null output");
+        NMF_HIP(hipDeviceSynchronize());
+        NMF_HIP(hipMemcpy(d_host, l->impl->inv_diag(), (size_t)l->impl->nglobal() * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    });
+}
+
+int nmf_linsys_apply_global(nmf_linsys l, const double *x_dev, double *y_dev)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        NMF_REQUIRE(x_dev && y_dev, NMF_ERR_INVALID, "null vector");
+        l->impl->apply_global(x_dev, y_dev);
+    });
+}

@@ -123,6 +123,90 @@ __device__ __forceinline__ void metric_from(const double M[3][3], double &J, dou
     J = fabs(det);
 }
 
+// ---- general C0 assembly (AssemblyMapCG.cpp:2804-2897) for any mesh:
+// the caller's local-to-global map with signs; Dirichlet DOFs first.
+
+// loc[i] = sign[i] glob[map[i]]   (GlobalToLocal, Vmath::Gathr with signs)
+__global__ void gather_signed_kernel(size_t nloc, const int *__restrict__ map, const double *__restrict__ sign,
+                                     const double *__restrict__ glob, double *__restrict__ loc)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nloc; i += (size_t)gridDim.x * blockDim.x) {
+        const double g = glob[map[i]];
+        loc[i] = sign ? sign[i] * g : g;
+    }
+}
+
+// glob[g] = sum over the local entries of g, in increasing local index, of
+// sign loc (Assemble, Vmath::Assmb): the transpose of the map as CSR rows,
+// so every global value is one work-item's fixed-order sum.  Rows g < ndir
+// are set to zero when zero_dir; with wdot, the partial sums of glob.wdot
+// (the CG's s.w) per block.  sq: sum sign^2 loc instead (diagonals).
+__global__ void assemble_csr_kernel(int nglob, int ndir, const int *__restrict__ row, const int *__restrict__ col,
+                                    const double *__restrict__ csign, const double *__restrict__ loc,
+                                    double *__restrict__ glob, const double *__restrict__ wdot,
+                                    double *__restrict__ partials, int zero_dir, int sq)
+{
+    double acc[1] = {0.0};
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        if (!(zero_dir && g < ndir)) {
+            for (int k = row[g]; k < row[g + 1]; ++k) {
+                const double sg = csign ? csign[k] : 1.0;
+                v = fma(sq ? sg * sg : sg, loc[col[k]], v);
+            }
+        }
+        glob[g] = v;
+        if (wdot) acc[0] += v * wdot[g];
+    }
+    if (partials) block_sum<1>(acc, partials + blockIdx.x);
+}
+
+// x_e = unit vector m of every element of a collection (ncoeff nc)
+__global__ void unit_mode_kernel(size_t n, int nc, int m, double *__restrict__ x)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] = (int)(i % nc) == m ? 1.0 : 0.0;
+}
+
+// d[e*nc + m] = y[e*nc + m] (the diagonal entry m of every element matrix)
+__global__ void take_mode_kernel(size_t nelmt, int nc, int m, const double *__restrict__ y, double *__restrict__ d)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nelmt; e += (size_t)gridDim.x * blockDim.x)
+        d[e * nc + m] = y[e * nc + m];
+}
+
+// 1/d on rows g >= ndir, 0 on the Dirichlet rows
+__global__ void invert_diag_dir_kernel(int nglob, int ndir, double *__restrict__ d)
+{
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x)
+        d[g] = g < ndir ? 0.0 : 1.0 / d[g];
+}
+
+// NekLinSysIterCG.cpp:118-213 with the Dirichlet DOFs first: r = b on rows
+// g >= ndir (0 on the Dirichlet rows), w = M^-1 r, x = p = q = 0; partial
+// sums r.r, r.w and b.b (over all rows, Set_Rhs_Magnitude).
+__global__ void cg_init_dirfirst_kernel(int nglob, int ndir, const double *__restrict__ b,
+                                        const double *__restrict__ inv_diag, double *__restrict__ x,
+                                        double *__restrict__ r, double *__restrict__ p, double *__restrict__ q,
+                                        double *__restrict__ w, double *__restrict__ partials)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x) {
+        const double bv = b[g];
+        acc[2] += bv * bv;
+        const double rv = g < ndir ? 0.0 : bv;
+        const double wv = inv_diag[g] * rv;
+        x[g] = 0.0;
+        p[g] = 0.0;
+        q[g] = 0.0;
+        r[g] = rv;
+        w[g] = wv;
+        acc[0] += rv * rv;  // the order cg_scalar_init reads: r.r, r.w, b.b
+        acc[1] += rv * wv;
+    }
+    block_sum<3>(acc, partials + 3 * blockIdx.x);
+}
+
 }  // namespace
 
 // Packed Helmholtz geometry of the structured mesh from its vertices
@@ -843,6 +927,54 @@ template hipError_t launch_cg_init<double>(hipStream_t, int, const double *, con
                                            double *, double *, double *, double *);
 template hipError_t launch_cg_init<float>(hipStream_t, int, const double *, const double *, float *, float *, float *,
                                           float *, float *, double *);
+hipError_t launch_gather_signed(hipStream_t stream, size_t nloc, const int *map, const double *sign,
+                                const double *glob, double *loc)
+{
+    NMF_LAUNCH(gather_signed_kernel, grid_for(nloc), nloc, map, sign, glob, loc);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble_csr(hipStream_t stream, int nglob, int ndir, const int *row, const int *col,
+                               const double *csign, const double *loc, double *glob, const double *wdot,
+                               double *partials, bool zero_dir, bool sq)
+{
+    NMF_LAUNCH(assemble_csr_kernel, grid_for((size_t)nglob), nglob, ndir, row, col, csign, loc, glob, wdot, partials,
+               zero_dir ? 1 : 0, sq ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_unit_mode(hipStream_t stream, size_t n, int nc, int m, double *x)
+{
+    NMF_LAUNCH(unit_mode_kernel, grid_for(n), n, nc, m, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_take_mode(hipStream_t stream, size_t nelmt, int nc, int m, const double *y, double *d)
+{
+    NMF_LAUNCH(take_mode_kernel, grid_for(nelmt), nelmt, nc, m, y, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_invert_diag_dir(hipStream_t stream, int nglob, int ndir, double *d)
+{
+    NMF_LAUNCH(invert_diag_dir_kernel, grid_for((size_t)nglob), nglob, ndir, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_init_dirfirst(hipStream_t stream, int nglob, int ndir, const double *b, const double *inv_diag,
+                                   double *x, double *r, double *p, double *q, double *w, double *partials)
+{
+    NMF_LAUNCH(cg_init_dirfirst_kernel, grid_for((size_t)nglob), nglob, ndir, b, inv_diag, x, r, p, q, w, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_update_n(hipStream_t stream, size_t NG, const CGState *st, double *p, double *q, double *x,
+                              double *r, double *w, const double *s, const double *inv_diag, double *partials)
+{
+    NMF_LAUNCH((cg_update_kernel<double>), grid_for(NG), NG, st, p, q, x, r, w, s, inv_diag, partials);
+    return hipGetLastError();
+}
+
 template hipError_t launch_cg_update<double>(hipStream_t, int, const CGState *, double *, double *, double *, double *,
                                              double *, const double *, const double *, double *);
 template hipError_t launch_cg_update<float>(hipStream_t, int, const CGState *, float *, float *, float *, float *,

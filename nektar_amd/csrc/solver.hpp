@@ -79,6 +79,21 @@ hipError_t launch_mixed_state_init(hipStream_t, CGState *st, const double *dsc, 
 hipError_t launch_sumsq(hipStream_t, size_t n, const double *a, double *partials);
 hipError_t launch_cast(hipStream_t, size_t n, const double *a, float *b);
 
+// General C0 assembly with the caller's map and signs, Dirichlet DOFs
+// first (assembly.hpp); reductions use reduction_blocks(nglob) blocks.
+hipError_t launch_gather_signed(hipStream_t, size_t nloc, const int *map, const double *sign, const double *glob,
+                                double *loc);
+hipError_t launch_assemble_csr(hipStream_t, int nglob, int ndir, const int *row, const int *col, const double *csign,
+                               const double *loc, double *glob, const double *wdot, double *partials, bool zero_dir,
+                               bool sq);
+hipError_t launch_unit_mode(hipStream_t, size_t n, int nc, int m, double *x);
+hipError_t launch_take_mode(hipStream_t, size_t nelmt, int nc, int m, const double *y, double *d);
+hipError_t launch_invert_diag_dir(hipStream_t, int nglob, int ndir, double *d);
+hipError_t launch_cg_init_dirfirst(hipStream_t, int nglob, int ndir, const double *b, const double *inv_diag,
+                                   double *x, double *r, double *p, double *q, double *w, double *partials);
+hipError_t launch_cg_update_n(hipStream_t, size_t NG, const CGState *st, double *p, double *q, double *x, double *r,
+                              double *w, const double *s, const double *inv_diag, double *partials);
+
 }  // namespace solver
 
 struct SolveInfo {
