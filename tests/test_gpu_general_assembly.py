@@ -134,3 +134,98 @@ def test_bad_maps_are_rejected(nmf):
         nmf.AssemblyMap(np.array([0, 1]), np.array([1.0, 0.5]), 3, 0)  # sign not +-1
     with pytest.raises(nmf.NektarError):
         nmf.AssemblyMap(np.array([0, 1]), None, 3, 4)        # ndir > nglobal
+
+
+# ---- unstructured tet meshes (tests/tet_mesh.py, pinned on the CPU) -------
+import brute as B  # noqa: E402
+import tet_mesh as TM  # noqa: E402
+from test_oracle_tet_assembly import _dense_system  # noqa: E402
+
+
+def _tet_setup(nmf, n, nm, lam, split):
+    coords, tets, mp, sg, dirichlet, jac, A = _dense_system(n, nm, lam)
+    nel, nc = len(tets), len(B.tet_modes(nm))
+    _, df = O.tet_geom(TM.tet_vertices(coords, tets), nel)
+    ops = []
+    for a, b in ((0, split), (split, nel)):
+        op = nmf.GetOperatorFactory().CreateInstance("Helmholtz_Tet_Regular", nm, nm + 1, b - a, nmf.FP64)
+        op.SetJac(jac[a:b])
+        op.SetDF(df[9 * a:9 * b])
+        op.SetLambda(lam)
+        ops.append(op)
+    return dict(coords=coords, tets=tets, mp=mp, sg=sg, dirichlet=dirichlet, jac=jac, A=A, ops=ops, nel=nel, nc=nc)
+
+
+@pytest.mark.parametrize("n,nm,lam,split", [(2, 3, 1.0, 17), (2, 5, 0.5, 30), (3, 4, 2.0, 100)])
+def test_tet_mesh_against_dense_oracle(nmf, n, nm, lam, split):
+    S = _tet_setup(nmf, n, nm, lam, split)
+    new, ndir = G.dirichlet_first(S["dirichlet"])
+    ng = len(S["dirichlet"])
+    amap = nmf.AssemblyMap(new[S["mp"]], S["sg"], ng, ndir)
+    ls = nmf.LinearSystem(amap, S["ops"])
+    free = ~S["dirichlet"]
+    A = S["A"]
+    # inverse diagonal on the free rows
+    d = ls.inv_diag()[new]
+    assert _rel(d[free], 1.0 / np.diag(A)[free]) <= 1e-12
+    assert np.all(d[S["dirichlet"]] == 0.0)
+    # assembled operator
+    x = np.random.default_rng(n + nm).uniform(-1, 1, ng)
+    x[S["dirichlet"]] = 0.0
+    xn = np.empty(ng)
+    xn[new] = x
+    y = torch.zeros(ng, dtype=torch.float64, device="cuda")
+    ls.apply_global(torch.from_numpy(xn).cuda(), y)
+    torch.cuda.synchronize()
+    ref = A @ x
+    ref[S["dirichlet"]] = 0.0
+    assert _rel(y.cpu().numpy()[new], ref) <= 1e-12
+    # manufactured solve against the dense solve of the same system
+    X = TM.tet_quadrature_coords(S["coords"], S["tets"], nm + 1)
+    f = (3 * np.pi ** 2 + lam) * np.prod(np.sin(np.pi * X), axis=2).ravel()
+    bl = O.iproduct(O.TET, nm, nm + 1, S["nel"], False, S["jac"], f)
+    ls.set_rhs_local(torch.from_numpy(bl).cuda())
+    info = ls.solve(tol=1e-12, maxit=5000)
+    assert info.converged
+    b = np.zeros(ng)
+    np.add.at(b, S["mp"], S["sg"] * bl)
+    xd = np.zeros(ng)
+    xd[free] = np.linalg.solve(A[np.ix_(free, free)], b[free])
+    assert _rel(ls.solution()[new], xd) <= 1e-9
+
+
+def test_hex_and_tet_collections_in_one_system(nmf):
+    # a hex region and a tet region in one system (not connected: a
+    # conforming hex-tet interface needs prisms or pyramids); each part must
+    # match its own independent solution
+    lam = 1.0
+    H = _setup(nmf, 2, 4, 5, lam, 0.0, 3, 4)
+    Tt = _tet_setup(nmf, 2, 3, lam, 20)
+    ngh, ngt = len(H["dirichlet"]), len(Tt["dirichlet"])
+    mp = np.concatenate([H["mr"], Tt["mp"] + ngh])
+    sg = np.concatenate([H["sr"], Tt["sg"]])
+    dirichlet = np.concatenate([H["dirichlet"], Tt["dirichlet"]])
+    new, ndir = G.dirichlet_first(dirichlet)
+    amap = nmf.AssemblyMap(new[mp], sg, ngh + ngt, ndir)
+    ops = H["ops"] + Tt["ops"]  # local vectors concatenated in this order
+    ls = nmf.LinearSystem(amap, ops)
+    xyz = O.hex_coords(H["Vr"], H["nel"], 5)
+    fh = (3 * np.pi ** 2 + lam) * np.prod(np.sin(np.pi * xyz), axis=2).ravel()
+    bh = O.iproduct(O.HEX, 4, 5, H["nel"], False, H["jac"], fh)
+    X = TM.tet_quadrature_coords(Tt["coords"], Tt["tets"], 4)
+    ft = (3 * np.pi ** 2 + lam) * np.prod(np.sin(np.pi * X), axis=2).ravel()
+    bt = O.iproduct(O.TET, 3, 4, Tt["nel"], False, Tt["jac"], ft)
+    ls.set_rhs_local(torch.from_numpy(np.concatenate([bh, bt])).cuda())
+    info = ls.solve(tol=1e-12, maxit=5000)
+    assert info.converged
+    sol = ls.solution()[new]
+    s = nmf.HelmholtzSolver(2, 4, 5, lam, 0.0, 3)
+    s.set_forcing_manufactured(1)
+    s.solve(tol=1e-12, maxit=2000)
+    assert _rel(sol[:ngh], s.solution()) <= 1e-9
+    free = ~Tt["dirichlet"]
+    b = np.zeros(ngt)
+    np.add.at(b, Tt["mp"], Tt["sg"] * bt)
+    xd = np.zeros(ngt)
+    xd[free] = np.linalg.solve(Tt["A"][np.ix_(free, free)], b[free])
+    assert _rel(sol[ngh:], xd) <= 1e-9
