@@ -158,6 +158,97 @@ def bench_ops(N, torch, steps=20, warmup=3):
     return out
 
 
+def rotated_box_map(n, nm, seed=9):
+    """The 32^3-type structured box with every element in a random local
+    frame (axis permutation and reflections): map and sign of every local
+    coefficient into the structured C0 numbering (N = n(nm-1)+1 nodes per
+    direction, node of mode p in element e: e(nm-1) + {0, nm-1, p-1}), the
+    Dirichlet mask, and the frames (perm, flip).  A reflected axis swaps the
+    two vertex modes and gives bubble mode p the sign (-1)^p."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    E, N = n ** 3, n * (nm - 1) + 1
+    perm = np.argsort(rng.random((E, 3)), axis=1)
+    flip = rng.integers(0, 2, (E, 3))
+    ez, ey, ex = np.unravel_index(np.arange(E), (n, n, n))
+    ecoord = np.stack([ex, ey, ez], axis=1)
+    p = np.arange(nm)
+    off = np.where(p == 0, 0, np.where(p == 1, nm - 1, p - 1))
+    stride = np.array([1, N, N * N])
+    terms, signs = [], []
+    for a in range(3):
+        fa = flip[:, a][:, None]
+        gm = np.where(fa == 1, np.where(p < 2, 1 - p, p), p)          # (E, nm) global-frame mode
+        gs = np.where((fa == 1) & (p >= 2), (-1.0) ** p, 1.0)          # (E, nm)
+        d = perm[:, a]
+        terms.append((ecoord[np.arange(E), d][:, None] * (nm - 1) + off[gm]) * stride[d][:, None])
+        signs.append(gs)
+    gid = terms[2][:, :, None, None] + terms[1][:, None, :, None] + terms[0][:, None, None, :]
+    sg = signs[2][:, :, None, None] * signs[1][:, None, :, None] * signs[0][:, None, None, :]
+    g = np.arange(N ** 3)
+    gx, gy, gz = g % N, (g // N) % N, g // (N * N)
+    dirichlet = (gx == 0) | (gx == N - 1) | (gy == 0) | (gy == N - 1) | (gz == 0) | (gz == N - 1)
+    return gid.reshape(-1), sg.reshape(-1), dirichlet, perm, flip
+
+
+def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
+    """The general assembly and CG (nmf_assembly / nmf_linsys) on the 32^3,
+    P=6 box with every element in a random local frame (orientation signs
+    on odd bubble modes), against the structured solver on the same box:
+    solve time, iterations and the largest difference of the solutions."""
+    import numpy as np
+
+    gid, sg, dirichlet, perm, flip = rotated_box_map(n, nm)
+    order = np.concatenate([np.flatnonzero(dirichlet), np.flatnonzero(~dirichlet)])
+    new = np.empty_like(order)
+    new[order] = np.arange(len(order))
+    E, h = n ** 3, 1.0 / n
+    # geometry of an axis-aligned cube in a rotated frame: d x_{perm[a]} / d xi_a = s_a h/2
+    s_ax = np.where(flip == 1, -1.0, 1.0)
+    df = np.zeros((E, 9))
+    for k in range(3):
+        df[np.arange(E), perm[:, k] * 3 + k] = s_ax[:, k] * 2.0 / h
+    jac = np.full(E, (h / 2) ** 3)
+    op = N.GetOperatorFactory().CreateInstance("Helmholtz_Hex_Regular", nm, nq, E, N.FP64)
+    op.SetJac(jac)
+    op.SetDF(df.reshape(-1))
+    op.SetLambda(lam)
+    # forcing f = (3 pi^2 + lam) prod sin(pi x) at the local quadrature points
+    z = N.basis_tables(0, 0, nm, nq)["z"]
+    ez, ey, ex = np.unravel_index(np.arange(E), (n, n, n))
+    centre = (np.stack([ex, ey, ez], axis=1) + 0.5) * h
+    K, J, I = np.meshgrid(np.arange(nq), np.arange(nq), np.arange(nq), indexing="ij")
+    loc_idx = [I.ravel(), J.ravel(), K.ravel()]
+    X = np.repeat(centre[:, None, :], nq ** 3, axis=1)
+    eye = np.eye(3)
+    for a in range(3):
+        # local axis a of element e runs along global axis perm[e, a]
+        X += ((s_ax[:, a][:, None] * (h / 2)) * z[loc_idx[a]][None, :])[:, :, None] * eye[perm[:, a]][:, None, :]
+    f = (3 * np.pi ** 2 + lam) * np.prod(np.sin(np.pi * X), axis=2).reshape(-1)
+    ip = N.GetOperatorFactory().CreateInstance("IProduct_Hex_Regular", nm, nq, E, N.FP64)
+    ip.SetJac(jac)
+    fd = torch.from_numpy(f).cuda()
+    bl = torch.zeros(ip.output_size(), dtype=torch.float64, device="cuda")
+    ip(fd, bl)
+    amap = N.AssemblyMap(new[gid], sg, len(dirichlet), int(dirichlet.sum()))
+    ls = N.LinearSystem(amap, [op])
+    ls.set_rhs_local(bl)
+    ls.solve(tol=1e-10)  # warm
+    info = ls.solve(tol=1e-10)
+    s = N.HelmholtzSolver(n, nm, nq, lam)
+    s.set_forcing_manufactured(1)
+    s.solve(tol=1e-10)
+    ref = s.solve(tol=1e-10)
+    diff = float(np.max(np.abs(ls.solution()[new] - s.solution())) / np.max(np.abs(s.solution())))
+    nunk = int((~dirichlet).sum())
+    return {"unknowns": nunk, "general_solve_ms": round(info.solve_ms, 3), "general_iterations": info.iterations,
+            "general_converged": bool(info.converged),
+            "general_solve_gdofs": round(nunk / (info.solve_ms * 1e-3) / 1e9, 4),
+            "structured_solve_ms": round(ref.solve_ms, 3), "structured_iterations": ref.iterations,
+            "solution_rel_diff": diff, "negative_signs": int((sg < 0).sum())}
+
+
 def batch_workload(N, seed=4, count=64):
     """64 Helmholtz applies: hex/tet 50/50, P ~ U{3..9}, Q = P+2, E
     log-uniform in [1e3, 1e5], affine elements with per-element scale
@@ -351,6 +442,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-curved", action="store_true", help="skip the curved 48^3 P=7 solves")
+    ap.add_argument("--no-general", action="store_true",
+                    help="skip the general assembly solve (32^3 P=6 box in random element frames)")
     ap.add_argument("--no-ops", action="store_true",
                     help="skip the per-operator timings (hex and tet, 32,768 elements, P=6)")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -445,6 +538,8 @@ def main():
 
     if not args.no_ops:
         extra["ops_p6_32768"] = bench_ops(N, torch)
+    if not args.no_general:
+        extra["general_32_p6"] = bench_general(N, torch)
     if not args.no_curved:
         extra["curved_48_p7"] = bench_curved(N)
     if not args.no_batch:
