@@ -196,7 +196,8 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
     """The general assembly and CG (nmf_assembly / nmf_linsys) on the 32^3,
     P=6 box with every element in a random local frame (orientation signs
     on odd bubble modes), against the structured solver on the same box:
-    solve time, iterations and the largest difference of the solutions."""
+    solve time, iterations and the largest difference of the solutions, for
+    the assembled-vector CG and the element-local CG."""
     import numpy as np
 
     gid, sg, dirichlet, perm, flip = rotated_box_map(n, nm)
@@ -241,10 +242,19 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
     s.solve(tol=1e-10)
     ref = s.solve(tol=1e-10)
     diff = float(np.max(np.abs(ls.solution()[new] - s.solution())) / np.max(np.abs(s.solution())))
+    del ls
+    # the element-local CG (LinSysIterSolver ConjugateGradientLoc) on the same system
+    ll = N.LinearSystem(amap, [op], solver="ConjugateGradientLoc")
+    ll.set_rhs_local(bl)
+    ll.solve(tol=1e-10)  # warm
+    linfo = ll.solve(tol=1e-10)
+    ldiff = float(np.max(np.abs(ll.solution()[new] - s.solution())) / np.max(np.abs(s.solution())))
     nunk = int((~dirichlet).sum())
     return {"unknowns": nunk, "general_solve_ms": round(info.solve_ms, 3), "general_iterations": info.iterations,
             "general_converged": bool(info.converged),
             "general_solve_gdofs": round(nunk / (info.solve_ms * 1e-3) / 1e9, 4),
+            "local_cg_solve_ms": round(linfo.solve_ms, 3), "local_cg_iterations": linfo.iterations,
+            "local_cg_converged": bool(linfo.converged), "local_cg_rel_diff": ldiff,
             "structured_solve_ms": round(ref.solve_ms, 3), "structured_iterations": ref.iterations,
             "solution_rel_diff": diff, "negative_signs": int((sg < 0).sum())}
 

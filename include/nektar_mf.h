@@ -362,12 +362,28 @@ int nmf_assembly_assemble(nmf_assembly a, const double *loc, double *glob,
  * geometry (exact element-matrix diagonals, one apply per local mode). */
 int nmf_linsys_create(nmf_assembly a, const nmf_operator *ops, int nops,
                       void *stream, nmf_linsys *out);
+/* The same with the reference's LinSysIterSolver name
+ * (GlobalLinSysIterative.cpp:86-88): "ConjugateGradient" (NekLinSysIterCG,
+ * assembled vectors; what nmf_linsys_create builds) or
+ * "ConjugateGradientLoc" (NekLinSysIterCGLoc.cpp:114-308, element-local
+ * vectors: s = A w without assembly, the Diagonal preconditioner assembles
+ * the residual, PreconditionerDiagonal.cpp:174-184).  Other names:
+ * NMF_ERR_UNSUPPORTED. */
+int nmf_linsys_create_solver(nmf_assembly a, const nmf_operator *ops,
+                             int nops, const char *solver, void *stream,
+                             nmf_linsys *out);
 int nmf_linsys_destroy(nmf_linsys l);
-/* b = Assemble(bloc) (device, local layout; e.g. IProductWRTBase of f) */
+/* Right-hand side in the local layout (device; e.g. IProductWRTBase of f):
+ * the global CG solves with b = Assemble(bloc), the local CG starts from
+ * r = bloc */
 int nmf_linsys_set_rhs_local(nmf_linsys l, const double *bloc);
 /* Stops at ||r|| < tol ||b|| (unpreconditioned, Dirichlet rows excluded) */
 int nmf_linsys_solve(nmf_linsys l, double tol, int maxit, nmf_solve_info *info);
 int nmf_linsys_get_solution(nmf_linsys l, double *x_host);
+/* The solution in the local layout (device, nloc entries), enqueued on the
+ * solver's stream: GlobalToLocal of the global solution, or the local CG's
+ * own vector (GlobalLinSysIterativeFull.cpp:367-371) */
+int nmf_linsys_get_solution_local(nmf_linsys l, double *xloc_dev);
 int nmf_linsys_get_inv_diag(nmf_linsys l, double *d_host);
 /* y = A x on global device vectors, Dirichlet rows zeroed */
 int nmf_linsys_apply_global(nmf_linsys l, const double *x_dev, double *y_dev);

@@ -132,6 +132,9 @@ _SIGNATURES = {
     "nmf_assembly_global_to_local": (ctypes.c_int, [_P, _P, _P, _P]),
     "nmf_assembly_assemble": (ctypes.c_int, [_P, _P, _P, _P]),
     "nmf_linsys_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, _P, ctypes.POINTER(_P)]),
+    "nmf_linsys_create_solver": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, ctypes.c_char_p, _P,
+                                                ctypes.POINTER(_P)]),
+    "nmf_linsys_get_solution_local": (ctypes.c_int, [_P, _P]),
     "nmf_linsys_destroy": (ctypes.c_int, [_P]),
     "nmf_linsys_set_rhs_local": (ctypes.c_int, [_P, _P]),
     "nmf_linsys_solve": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(SolveInfo)]),
@@ -616,17 +619,22 @@ class AssemblyMap:
 
 
 class LinearSystem:
-    """``GlobalLinSysIterativeFull`` + ``NekLinSysIterCG`` (Diagonal
-    preconditioner) over FP64 Helmholtz operators sharing an AssemblyMap;
-    the operators' local vectors, concatenated in the order given, are the
-    map's local vector."""
+    """``GlobalLinSysIterativeFull`` with the Diagonal preconditioner over
+    FP64 Helmholtz operators sharing an AssemblyMap; the operators' local
+    vectors, concatenated in the order given, are the map's local vector.
+    ``solver`` is the reference's LinSysIterSolver
+    (GlobalLinSysIterative.cpp:86-88): "ConjugateGradient"
+    (NekLinSysIterCG, assembled vectors) or "ConjugateGradientLoc"
+    (NekLinSysIterCGLoc, element-local vectors)."""
 
-    def __init__(self, amap: AssemblyMap, ops, stream=None):
+    def __init__(self, amap: AssemblyMap, ops, stream=None, solver: str = "ConjugateGradient"):
         self._amap, self._ops = amap, list(ops)  # keep both alive
         arr = (ctypes.c_void_p * len(self._ops))(*[o._h.value for o in self._ops])
         self._h = ctypes.c_void_p()
-        check(lib().nmf_linsys_create(amap._h, arr, len(self._ops), stream, ctypes.byref(self._h)))
+        check(lib().nmf_linsys_create_solver(amap._h, arr, len(self._ops), solver.encode(), stream,
+                                             ctypes.byref(self._h)))
         self.nglobal = amap.nglobal
+        self.solver = solver
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _LIB is not None:
@@ -645,6 +653,11 @@ class LinearSystem:
         x = np.zeros(self.nglobal)
         check(lib().nmf_linsys_get_solution(self._h, _dptr(x)))
         return x
+
+    def solution_local(self, xloc) -> None:
+        """The solution in the local layout into the device tensor xloc
+        (enqueued on the solver's stream)."""
+        check(lib().nmf_linsys_get_solution_local(self._h, _device_ptr(xloc, torch.float64, self._amap.nloc)))
 
     def inv_diag(self) -> np.ndarray:
         d = np.zeros(self.nglobal)

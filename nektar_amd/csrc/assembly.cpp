@@ -77,8 +77,23 @@ void AssemblyMap::assemble(const double *loc, double *glob, hipStream_t s, bool 
                                         sq));
 }
 
-IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOperator *> &ops, hipStream_t stream)
-    : map_(map), ops_(ops), stream_(stream)
+void AssemblyMap::precon_local(bool init, const double *r, const double *inv_diag, double *wg, double *partials,
+                               hipStream_t s) const
+{
+    NMF_HIP(solver::launch_cgloc_precon(s, init, nglobal_, ndir_, row_.as<int>(), col_.as<int>(),
+                                        has_sign_ ? csign_.as<double>() : nullptr, r, inv_diag, wg, partials));
+}
+
+void AssemblyMap::global_values(const double *loc, double *glob, hipStream_t s) const
+{
+    if (nglobal_ == 0) return;
+    NMF_HIP(solver::launch_csr_first(s, nglobal_, row_.as<int>(), col_.as<int>(),
+                                     has_sign_ ? csign_.as<double>() : nullptr, loc, glob));
+}
+
+IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOperator *> &ops, hipStream_t stream,
+                                 CGSpace space)
+    : map_(map), ops_(ops), stream_(stream), space_(space)
 {
     int64_t n = 0;
     for (MFOperator *o : ops_) {
@@ -93,17 +108,20 @@ IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOpe
                 "operators hold " + std::to_string(n) + " local coefficients, the map " + std::to_string(map_.nloc()));
     const int NG = map_.nglobal();
     nb_ = solver::reduction_blocks((size_t)std::max(NG, 1));
+    // Krylov vectors and the right-hand side: global (NG) or local (n)
+    const size_t nv = space_ == CGSpace::Local ? (size_t)std::max<int64_t>(n, 1) : (size_t)std::max(NG, 1);
     xloc_.allocate(std::max<int64_t>(n, 1) * sizeof(double));
     yloc_.allocate(std::max<int64_t>(n, 1) * sizeof(double));
-    b_.allocate(std::max(NG, 1) * sizeof(double));
+    b_.allocate(nv * sizeof(double));
     invd_.allocate(std::max(NG, 1) * sizeof(double));
-    for (auto &v : cg_) v.allocate(std::max(NG, 1) * sizeof(double));
+    xg_.allocate(std::max(NG, 1) * sizeof(double));
+    for (auto &v : cg_) v.allocate(nv * sizeof(double));
     state_.allocate(sizeof(solver::CGState));
     pa_.allocate((size_t)nb_ * 3 * sizeof(double));
     pmu_.allocate((size_t)nb_ * sizeof(double));
     NMF_HIP(hipHostMalloc((void **)&host_state_, sizeof(solver::CGState), hipHostMallocDefault));
-    NMF_HIP(solver::launch_zero<double>(stream_, (size_t)std::max(NG, 1), b_.as<double>()));
-    NMF_HIP(solver::launch_zero<double>(stream_, (size_t)std::max(NG, 1), cg_[0].as<double>()));
+    NMF_HIP(solver::launch_zero<double>(stream_, nv, b_.as<double>()));
+    NMF_HIP(solver::launch_zero<double>(stream_, nv, cg_[0].as<double>()));
     for (MFOperator *o : ops_) o->prepare();
     build_diagonal();
 }
@@ -147,8 +165,32 @@ void IterativeSolver::build_diagonal()
 void IterativeSolver::set_rhs_local(const double *bloc)
 {
     NMF_REQUIRE(bloc || map_.nloc() == 0, NMF_ERR_INVALID, "null right-hand side");
-    map_.assemble(bloc, b_.as<double>(), stream_);
+    if (space_ == CGSpace::Local) {
+        if (map_.nloc())
+            NMF_HIP(hipMemcpyAsync(b_.get(), bloc, (size_t)map_.nloc() * sizeof(double), hipMemcpyDeviceToDevice,
+                                   stream_));
+    } else {
+        map_.assemble(bloc, b_.as<double>(), stream_);
+    }
     have_rhs_ = true;
+}
+
+const double *IterativeSolver::solution()
+{
+    if (space_ == CGSpace::Global) return cg_[0].as<double>();
+    map_.global_values(cg_[0].as<double>(), xg_.as<double>(), stream_);
+    return xg_.as<double>();
+}
+
+void IterativeSolver::solution_local(double *xloc)
+{
+    const size_t n = (size_t)map_.nloc();
+    if (n == 0) return;
+    NMF_REQUIRE(xloc, NMF_ERR_INVALID, "null output");
+    if (space_ == CGSpace::Global)
+        map_.global_to_local(cg_[0].as<double>(), xloc, stream_);
+    else
+        NMF_HIP(hipMemcpyAsync(xloc, cg_[0].get(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
 }
 
 void IterativeSolver::apply_global(const double *x, double *y)
@@ -163,14 +205,22 @@ void IterativeSolver::apply_global(const double *x, double *y)
 // assembly: per iteration one update of the vectors, s = A w (gather,
 // element operators, assembly with the s.w partial sums) and one scalar
 // step on the device; the host reads the state every few iterations.
+// CGSpace::Local runs NekLinSysIterCGLoc::DoConjugateGradient
+// (NekLinSysIterCGLoc.cpp:114-308) instead: the same recurrence and scalar
+// steps on element-local vectors, s = A w without assembly, and the
+// residual assembled once per iteration by the preconditioner, whose
+// assembled values also give r.w and the stopping norm.
 SolveInfo IterativeSolver::solve(double tol, int maxit)
 {
     NMF_REQUIRE(have_rhs_, NMF_ERR_STATE, "right-hand side not set");
     NMF_REQUIRE(maxit >= 0, NMF_ERR_INVALID, "maxit must be >= 0");
     tol = std::max(tol, 1.0e-16);  // NekLinSysIter.cpp:57
     const int NG = map_.nglobal(), ND = map_.ndir();
+    const bool local = space_ == CGSpace::Local;
+    const size_t NL = (size_t)map_.nloc();
     double *x = cg_[0].as<double>(), *r = cg_[1].as<double>(), *p = cg_[2].as<double>(), *q = cg_[3].as<double>(),
            *w = cg_[4].as<double>(), *s = cg_[5].as<double>();
+    double *wg = xg_.as<double>();  // local CG: the assembled, scaled residual
     solver::CGState *st = state_.as<solver::CGState>();
     hipEvent_t e0, e1;
     NMF_HIP(hipEventCreate(&e0));
@@ -183,11 +233,25 @@ SolveInfo IterativeSolver::solve(double tol, int maxit)
         *host_state_ = init;
         NMF_HIP(hipEventRecord(e0, stream_));
         NMF_HIP(hipMemcpyAsync(st, host_state_, sizeof(solver::CGState), hipMemcpyHostToDevice, stream_));
-        NMF_HIP(solver::launch_cg_init_dirfirst(stream_, NG, ND, b_.as<double>(), invd_.as<double>(), x, r, p, q, w,
-                                                pa_.as<double>()));
-        map_.global_to_local(w, xloc_.as<double>(), stream_);
-        apply_local(xloc_.as<double>(), yloc_.as<double>());
-        map_.assemble(yloc_.as<double>(), s, stream_, true, w, pmu_.as<double>());
+        if (local) {
+            // NekLinSysIterCGLoc.cpp:145-196: r = b, x = 0, w = M^-1 r in
+            // local form, s = A w element by element; eps, r.w, b.b from
+            // the assembled residual, s.w as a local dot
+            if (NL) {
+                NMF_HIP(hipMemcpyAsync(r, b_.get(), NL * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+                for (double *v : {x, p, q}) NMF_HIP(solver::launch_zero<double>(stream_, NL, v));
+            }
+            map_.precon_local(true, r, invd_.as<double>(), wg, pa_.as<double>(), stream_);
+            map_.global_to_local(wg, w, stream_);
+            apply_local(w, s);
+            NMF_HIP(solver::launch_dot(stream_, NL, nb_, s, w, pmu_.as<double>()));
+        } else {
+            NMF_HIP(solver::launch_cg_init_dirfirst(stream_, NG, ND, b_.as<double>(), invd_.as<double>(), x, r, p, q,
+                                                    w, pa_.as<double>()));
+            map_.global_to_local(w, xloc_.as<double>(), stream_);
+            apply_local(xloc_.as<double>(), yloc_.as<double>());
+            map_.assemble(yloc_.as<double>(), s, stream_, true, w, pmu_.as<double>());
+        }
         NMF_HIP(solver::launch_cg_scalar_init(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
         const int max_rounds = maxit / kItersPerChunk + 3;
         for (int round = 0;; ++round) {
@@ -195,11 +259,20 @@ SolveInfo IterativeSolver::solve(double tol, int maxit)
             NMF_HIP(hipStreamSynchronize(stream_));
             if (host_state_->done || round >= max_rounds) break;
             for (int it = 0; it < kItersPerChunk; ++it) {
-                NMF_HIP(solver::launch_cg_update_n(stream_, (size_t)NG, st, p, q, x, r, w, s, invd_.as<double>(),
-                                                   pa_.as<double>()));
-                map_.global_to_local(w, xloc_.as<double>(), stream_);
-                apply_local(xloc_.as<double>(), yloc_.as<double>());
-                map_.assemble(yloc_.as<double>(), s, stream_, true, w, pmu_.as<double>());
+                if (local) {
+                    // NekLinSysIterCGLoc.cpp:210-252
+                    NMF_HIP(solver::launch_cgloc_update(stream_, NL, st, p, q, x, r, w, s));
+                    map_.precon_local(false, r, invd_.as<double>(), wg, pa_.as<double>(), stream_);
+                    map_.global_to_local(wg, w, stream_);
+                    apply_local(w, s);
+                    NMF_HIP(solver::launch_dot(stream_, NL, nb_, s, w, pmu_.as<double>()));
+                } else {
+                    NMF_HIP(solver::launch_cg_update_n(stream_, (size_t)NG, st, p, q, x, r, w, s, invd_.as<double>(),
+                                                       pa_.as<double>()));
+                    map_.global_to_local(w, xloc_.as<double>(), stream_);
+                    apply_local(xloc_.as<double>(), yloc_.as<double>());
+                    map_.assemble(yloc_.as<double>(), s, stream_, true, w, pmu_.as<double>());
+                }
                 NMF_HIP(solver::launch_cg_scalar(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
             }
         }

@@ -2,8 +2,8 @@
 // mesh: the GPU counterpart of AssemblyMapCG (GlobalToLocal / Assemble,
 // AssemblyMapCG.cpp:2804-2897) and of GlobalLinSysIterativeFull with
 // NekLinSysIterCG and the Diagonal preconditioner
-// (GlobalLinSysIterativeFull.cpp:96-383, NekLinSysIterCG.cpp:118-372,
-// PreconditionerDiagonal.cpp:93-189), for collections of hex and tet
+// (GlobalLinSysIterativeFull.cpp:96-383, NekLinSysIterCG.cpp:118-372 or
+// NekLinSysIterCGLoc.cpp:114-308, PreconditionerDiagonal.cpp:93-189), for collections of hex and tet
 // Helmholtz operators that share one local-to-global map.
 //
 // The caller supplies the map as AssemblyMapCG holds it: for every entry i
@@ -37,6 +37,15 @@ public:
     void assemble(const double *loc, double *glob, hipStream_t s, bool zero_dir = false, const double *wdot = nullptr,
                   double *partials = nullptr, bool sq = false) const;
 
+    // the element-local CG's preconditioner step (NekLinSysIterCGLoc with
+    // PreconditionerDiagonal in local form): wg = inv_diag Assemble(r) off
+    // the Dirichlet rows, with the partial sums of its dots
+    // (solver::launch_cgloc_precon); scatter wg back with global_to_local
+    void precon_local(bool init, const double *r, const double *inv_diag, double *wg, double *partials,
+                      hipStream_t s) const;
+    // glob[g] = sign loc of g's first local entry (loc continuous)
+    void global_values(const double *loc, double *glob, hipStream_t s) const;
+
     int64_t nloc() const { return nloc_; }
     int nglobal() const { return nglobal_; }
     int ndir() const { return ndir_; }
@@ -48,26 +57,37 @@ private:
     DeviceBuffer map_, sign_, row_, col_, csign_;
 };
 
+// The reference's LinSysIterSolver choices for this system
+// (GlobalLinSysIterative.cpp:86-88): NekLinSysIterCG on the assembled
+// vectors, or NekLinSysIterCGLoc on the element-local ones.
+enum class CGSpace { Global, Local };
+
 class IterativeSolver {
 public:
     // ops: Helmholtz operators (FP64) whose element-local vectors,
     // concatenated in this order, are the map's local vector.  The operators
     // must outlive the solver and keep their geometry and lambda while it
     // is used (the diagonal is built here).
-    IterativeSolver(const AssemblyMap &map, const std::vector<MFOperator *> &ops, hipStream_t stream);
+    IterativeSolver(const AssemblyMap &map, const std::vector<MFOperator *> &ops, hipStream_t stream,
+                    CGSpace space = CGSpace::Global);
     ~IterativeSolver();
     IterativeSolver(const IterativeSolver &) = delete;
     IterativeSolver &operator=(const IterativeSolver &) = delete;
 
-    // b = Assemble(bloc) (bloc: device, local layout, e.g. IProductWRTBase
-    // of the forcing), ContField::v_HelmSolve's right-hand side
+    // the right-hand side in the local layout (bloc: device, e.g.
+    // IProductWRTBase of the forcing), ContField::v_HelmSolve's; the global
+    // CG solves with b = Assemble(bloc), the local CG starts from r = bloc
     void set_rhs_local(const double *bloc);
     SolveInfo solve(double tol, int maxit);
     // y = A x on global device vectors, Dirichlet rows zeroed
     void apply_global(const double *x, double *y);
-    const double *solution() const { return cg_[0].as<double>(); }
+    // the solution as global values (nglobal) / element-local (nloc), on
+    // the device, enqueued on the solver's stream
+    const double *solution();
+    void solution_local(double *xloc);
     const double *inv_diag() const { return invd_.as<double>(); }
     int nglobal() const { return map_.nglobal(); }
+    CGSpace space() const { return space_; }
 
 private:
     void apply_local(const double *xloc, double *yloc);
@@ -77,9 +97,10 @@ private:
     std::vector<MFOperator *> ops_;
     std::vector<int64_t> off_;
     hipStream_t stream_;
+    CGSpace space_;
     int nb_ = 1;
-    DeviceBuffer xloc_, yloc_, b_, invd_, state_, pa_, pmu_;
-    DeviceBuffer cg_[6];  // x r p q w s
+    DeviceBuffer xloc_, yloc_, b_, invd_, state_, pa_, pmu_, xg_;
+    DeviceBuffer cg_[6];  // x r p q w s: global (Global) or local (Local) vectors
     solver::CGState *host_state_ = nullptr;
     bool have_rhs_ = false;
 };

@@ -713,8 +713,19 @@ int nmf_assembly_assemble(nmf_assembly a, const double *loc, double *glob, void 
 
 int nmf_linsys_create(nmf_assembly a, const nmf_operator *ops, int nops, void *stream, nmf_linsys *out)
 {
+    return nmf_linsys_create_solver(a, ops, nops, "ConjugateGradient", stream, out);
+}
+
+int nmf_linsys_create_solver(nmf_assembly a, const nmf_operator *ops, int nops, const char *solver, void *stream,
+                             nmf_linsys *out)
+{
     return guard([&] {
         NMF_ASM_CHECK();
+        NMF_REQUIRE(solver, NMF_ERR_INVALID, "null solver name");
+        const std::string name(solver);
+        NMF_REQUIRE(name == "ConjugateGradient" || name == "ConjugateGradientLoc", NMF_ERR_UNSUPPORTED,
+                    "LinSysIterSolver '" + name + "': this library has ConjugateGradient and ConjugateGradientLoc");
+        const CGSpace space = name == "ConjugateGradientLoc" ? CGSpace::Local : CGSpace::Global;
         NMF_REQUIRE(out, NMF_ERR_INVALID, "null output handle");
         *out = nullptr;
         NMF_REQUIRE(nops >= 1 && ops, NMF_ERR_INVALID, "at least one operator");
@@ -724,7 +735,7 @@ int nmf_linsys_create(nmf_assembly a, const nmf_operator *ops, int nops, void *s
             v.push_back(ops[i]->impl.get());
         }
         auto h = std::make_unique<nmf_linsys_s>();
-        h->impl = std::make_unique<IterativeSolver>(*a->impl, v, static_cast<hipStream_t>(stream));
+        h->impl = std::make_unique<IterativeSolver>(*a->impl, v, static_cast<hipStream_t>(stream), space);
         *out = h.release();
     });
 }
@@ -766,9 +777,17 @@ int nmf_linsys_get_solution(nmf_linsys l, double *x_host)
     return guard([&] {
         NMF_LS_CHECK();
         NMF_REQUIRE(x_host, NMF_ERR_INVALID, "null output");
+        const double *x = l->impl->solution();
         NMF_HIP(hipDeviceSynchronize());
-        NMF_HIP(hipMemcpy(x_host, l->impl->solution(), (size_t)l->impl->nglobal() * sizeof(double),
-                          hipMemcpyDeviceToHost));
+        NMF_HIP(hipMemcpy(x_host, x, (size_t)l->impl->nglobal() * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int nmf_linsys_get_solution_local(nmf_linsys l, double *xloc_dev)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        l->impl->solution_local(xloc_dev);
     });
 }
 

@@ -207,6 +207,84 @@ __global__ void cg_init_dirfirst_kernel(int nglob, int ndir, const double *__res
     block_sum<3>(acc, partials + 3 * blockIdx.x);
 }
 
+// ---- element-local CG (NekLinSysIterCGLoc.cpp:114-308): the Krylov
+// vectors are element-local; only the preconditioner assembles.
+
+// PreconditionerDiagonal.cpp:174-184 with Preconditioner::DoAssembleLoc
+// (Preconditioner.cpp:89-102) folded in: v_g = Assemble(r)_g (CSR rows,
+// fixed order), wg_g = inv_diag_g v_g off the Dirichlet rows (0 on them),
+// to be scattered back by the signed gather.  The local dots follow from
+// the assembled values: <r, GlobalToLocal(wg)> = sum_g v_g wg_g (r.w) and
+// <r, AssembleLoc(r, zero Dirichlet)> = sum_{g >= ndir} v_g^2 (eps).
+// INIT: partials {eps, r.w, b.b (all rows: Set_Rhs_Magnitude,
+// NekLinSysIter.cpp:104-109)} in cg_scalar_init's order; else {r.w, eps}
+// in cg_scalar's.
+template <int INIT>
+__global__ void cgloc_precon_kernel(int nglob, int ndir, const int *__restrict__ row, const int *__restrict__ col,
+                                    const double *__restrict__ csign, const double *__restrict__ r,
+                                    const double *__restrict__ inv_diag, double *__restrict__ wg,
+                                    double *__restrict__ partials)
+{
+    double acc[INIT ? 3 : 2] = {};
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int k = row[g]; k < row[g + 1]; ++k) v = fma(csign ? csign[k] : 1.0, r[col[k]], v);
+        const bool dir = g < ndir;
+        const double wv = dir ? 0.0 : inv_diag[g] * v;
+        const double e = dir ? 0.0 : v * v;
+        wg[g] = wv;
+        if constexpr (INIT) {
+            acc[0] += e;
+            acc[1] += v * wv;
+            acc[2] += v * v;
+        } else {
+            acc[0] += v * wv;
+            acc[1] += e;
+        }
+    }
+    block_sum<INIT ? 3 : 2>(acc, partials + (INIT ? 3 : 2) * blockIdx.x);
+}
+
+// NekLinSysIterCGLoc.cpp:210-221 on the local vectors: search directions,
+// solution and residual
+__global__ void cgloc_update_kernel(size_t n, const CGState *__restrict__ st, double *__restrict__ p,
+                                    double *__restrict__ q, double *__restrict__ x, double *__restrict__ r,
+                                    const double *__restrict__ w, const double *__restrict__ s)
+{
+    if (st->done) return;
+    const double alpha = st->alpha, beta = st->beta;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double pv = beta * p[i] + w[i];
+        const double qv = beta * q[i] + s[i];
+        x[i] = alpha * pv + x[i];
+        r[i] = -alpha * qv + r[i];
+        p[i] = pv;
+        q[i] = qv;
+    }
+}
+
+// partial sums of a.b over a grid-stride loop (one per block)
+__global__ void dot_kernel(size_t n, const double *__restrict__ a, const double *__restrict__ b,
+                           double *__restrict__ partials)
+{
+    double acc[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc[0] += a[i] * b[i];
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+// glob[g] = sign loc of the first local entry of g (0 for a DOF no element
+// holds): the global values of a continuous local vector
+__global__ void csr_first_kernel(int nglob, const int *__restrict__ row, const int *__restrict__ col,
+                                 const double *__restrict__ csign, const double *__restrict__ loc,
+                                 double *__restrict__ glob)
+{
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x) {
+        const int k = row[g];
+        glob[g] = k < row[g + 1] ? (csign ? csign[k] : 1.0) * loc[col[k]] : 0.0;
+    }
+}
+
 }  // namespace
 
 // Packed Helmholtz geometry of the structured mesh from its vertices
@@ -972,6 +1050,39 @@ hipError_t launch_cg_update_n(hipStream_t stream, size_t NG, const CGState *st, 
                               double *r, double *w, const double *s, const double *inv_diag, double *partials)
 {
     NMF_LAUNCH((cg_update_kernel<double>), grid_for(NG), NG, st, p, q, x, r, w, s, inv_diag, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_cgloc_precon(hipStream_t stream, bool init, int nglob, int ndir, const int *row, const int *col,
+                               const double *csign, const double *r, const double *inv_diag, double *wg,
+                               double *partials)
+{
+    const int grid = grid_for((size_t)nglob);
+    if (init)
+        NMF_LAUNCH(cgloc_precon_kernel<1>, grid, nglob, ndir, row, col, csign, r, inv_diag, wg, partials);
+    else
+        NMF_LAUNCH(cgloc_precon_kernel<0>, grid, nglob, ndir, row, col, csign, r, inv_diag, wg, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_cgloc_update(hipStream_t stream, size_t n, const CGState *st, double *p, double *q, double *x,
+                               double *r, const double *w, const double *s)
+{
+    NMF_LAUNCH(cgloc_update_kernel, grid_for(n), n, st, p, q, x, r, w, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot(hipStream_t stream, size_t n, int nb, const double *a, const double *b, double *partials)
+{
+    if (nb < 1) return hipErrorInvalidValue;
+    NMF_LAUNCH(dot_kernel, nb, n, a, b, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_csr_first(hipStream_t stream, int nglob, const int *row, const int *col, const double *csign,
+                            const double *loc, double *glob)
+{
+    NMF_LAUNCH(csr_first_kernel, grid_for((size_t)nglob), nglob, row, col, csign, loc, glob);
     return hipGetLastError();
 }
 

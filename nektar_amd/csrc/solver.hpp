@@ -93,6 +93,18 @@ hipError_t launch_cg_init_dirfirst(hipStream_t, int nglob, int ndir, const doubl
                                    double *x, double *r, double *p, double *q, double *w, double *partials);
 hipError_t launch_cg_update_n(hipStream_t, size_t NG, const CGState *st, double *p, double *q, double *x, double *r,
                               double *w, const double *s, const double *inv_diag, double *partials);
+// element-local CG (NekLinSysIterCGLoc): preconditioner with the assembled
+// dots (reduction_blocks(nglob) blocks; init: {eps, r.w, b.b}, else
+// {r.w, eps}), local vector update, local dot over nb blocks, and the
+// global values of a continuous local vector
+hipError_t launch_cgloc_precon(hipStream_t, bool init, int nglob, int ndir, const int *row, const int *col,
+                               const double *csign, const double *r, const double *inv_diag, double *wg,
+                               double *partials);
+hipError_t launch_cgloc_update(hipStream_t, size_t n, const CGState *st, double *p, double *q, double *x, double *r,
+                               const double *w, const double *s);
+hipError_t launch_dot(hipStream_t, size_t n, int nb, const double *a, const double *b, double *partials);
+hipError_t launch_csr_first(hipStream_t, int nglob, const int *row, const int *col, const double *csign,
+                            const double *loc, double *glob);
 
 }  // namespace solver
 

@@ -159,3 +159,118 @@ int or_cg_helmholtz_hex(int n, int nm, int nq, int deformed,
     free(map); free(dir); free(h.loc_in); free(h.loc_out); free(nd);
     return it;
 }
+
+/* ---- element-local CG (NekLinSysIterCGLoc.cpp:114-308) ---------------
+ * Every Krylov vector lives in the element-local layout (nloc entries, the
+ * concatenated element coefficients).  The operator is the block-diagonal
+ * element matrix (GlobalLinSysIterativeFull.cpp:192-198: GeneralMatrixOp
+ * without assembly); the preconditioner assembles, scales the
+ * non-Dirichlet rows by the inverse diagonal, zeroes the Dirichlet rows and
+ * scatters back (PreconditionerDiagonal.cpp:174-184); the residual norm is
+ * <AssembleLoc(r), r> with the Dirichlet rows zeroed
+ * (Preconditioner.cpp:89-102, NekLinSysIterCGLoc.cpp:154-156,:248-252) and
+ * the rhs magnitude <AssembleLoc(b), b> over all rows
+ * (NekLinSysIter.cpp:104-109).  Blocks: nblk dense row-major matrices of
+ * sizes nk = boff[k+1]-boff[k], stored one after the other in Ablk. */
+typedef struct {
+    int nloc, nglob, ndir;
+    const int *map;
+    const double *sign;
+    double *wk;
+} loc_map;
+
+static void assemble_loc(const loc_map *m, const double *loc, double *glob)
+{
+    memset(glob, 0, (size_t)m->nglob * sizeof(double));
+    for (int i = 0; i < m->nloc; ++i) glob[m->map[i]] += (m->sign ? m->sign[i] : 1.0) * loc[i];
+}
+
+static void to_local(const loc_map *m, const double *glob, double *loc)
+{
+    for (int i = 0; i < m->nloc; ++i) loc[i] = (m->sign ? m->sign[i] : 1.0) * glob[m->map[i]];
+}
+
+/* DoAssembleLoc(in, out, zero_dir) */
+static void assemble_scatter(const loc_map *m, const double *in, double *out, int zero_dir)
+{
+    assemble_loc(m, in, m->wk);
+    if (zero_dir) memset(m->wk, 0, (size_t)m->ndir * sizeof(double));
+    to_local(m, m->wk, out);
+}
+
+static void precon_loc(const loc_map *m, const double *inv_diag, const double *in, double *out)
+{
+    assemble_loc(m, in, m->wk);
+    for (int g = m->ndir; g < m->nglob; ++g) m->wk[g] *= inv_diag ? inv_diag[g] : 1.0;
+    memset(m->wk, 0, (size_t)m->ndir * sizeof(double));
+    to_local(m, m->wk, out);
+}
+
+static void block_apply(int nblk, const int *boff, const double *Ablk, const double *x, double *y)
+{
+    size_t off = 0;
+    for (int k = 0; k < nblk; ++k) {
+        const int b0 = boff[k], nk = boff[k + 1] - boff[k];
+        for (int i = 0; i < nk; ++i) {
+            double t = 0.0;
+            for (int j = 0; j < nk; ++j) t += Ablk[off + (size_t)i * nk + j] * x[b0 + j];
+            y[b0 + i] = t;
+        }
+        off += (size_t)nk * nk;
+    }
+}
+
+int or_cg_loc(int nblk, const int *boff, const double *Ablk, const int *map, const double *sign, int nglob,
+              int ndir, const double *inv_diag, const double *b_loc, double tol, int maxit, double *x_loc,
+              or_cg_info *info)
+{
+    const int n = boff[nblk];
+    loc_map m = {n, nglob, ndir, map, sign, (double *)calloc(nglob > 0 ? nglob : 1, sizeof(double))};
+    double *w = (double *)calloc(n, sizeof(double));
+    double *s = (double *)calloc(n, sizeof(double));
+    double *p = (double *)calloc(n, sizeof(double));
+    double *r = (double *)calloc(n, sizeof(double));
+    double *q = (double *)calloc(n, sizeof(double));
+    double *wk = (double *)calloc(n, sizeof(double));
+    tol = fmax(tol, 1.0e-16); /* NekLinSysIter.cpp:57 */
+
+    memcpy(r, b_loc, (size_t)n * sizeof(double));
+    memset(x_loc, 0, (size_t)n * sizeof(double));
+    assemble_scatter(&m, r, wk, 1);
+    double eps = dot(n, wk, r);
+    assemble_scatter(&m, b_loc, wk, 0);
+    double rhs_mag = dot(n, wk, b_loc);
+    rhs_mag = rhs_mag > 1.0e-6 ? rhs_mag : 1.0;
+    int iters = 0;
+    if (eps >= tol * tol * rhs_mag) {
+        precon_loc(&m, inv_diag, r, w);
+        block_apply(nblk, boff, Ablk, w, s);
+        double rho = dot(n, r, w), mu = dot(n, s, w);
+        double beta = 0.0, alpha = rho / mu, rho_new;
+        iters = 1;
+        while (iters <= maxit) {
+            for (int i = 0; i < n; ++i) p[i] = beta * p[i] + w[i];
+            for (int i = 0; i < n; ++i) q[i] = beta * q[i] + s[i];
+            for (int i = 0; i < n; ++i) x_loc[i] = alpha * p[i] + x_loc[i];
+            for (int i = 0; i < n; ++i) r[i] = -alpha * q[i] + r[i];
+            precon_loc(&m, inv_diag, r, w);
+            block_apply(nblk, boff, Ablk, w, s);
+            rho_new = dot(n, r, w);
+            mu = dot(n, s, w);
+            assemble_scatter(&m, r, wk, 1);
+            eps = dot(n, wk, r);
+            iters++;
+            if (eps < tol * tol * rhs_mag) break;
+            beta = rho_new / rho;
+            alpha = rho_new / (mu - rho_new * beta / alpha);
+            rho = rho_new;
+        }
+    }
+    if (info) {
+        info->iterations = iters;
+        info->final_error = sqrt(eps / rhs_mag);
+        info->rhs_magnitude = rhs_mag;
+    }
+    free(m.wk); free(w); free(s); free(p); free(r); free(q); free(wk);
+    return iters;
+}

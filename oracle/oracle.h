@@ -145,6 +145,15 @@ int or_cg_helmholtz_hex(int n, int nm, int nq, int deformed,
 int or_cg_dense(int n, int ndir, const double *A, const double *b,
                 const double *inv_diag, double tol, int maxit, double *x,
                 or_cg_info *info);
+/* Element-local CG (NekLinSysIterCGLoc.cpp:114-308): Krylov vectors in
+ * the element-local layout, operator = nblk dense element blocks (sizes
+ * boff[k+1]-boff[k], row-major, consecutive in Ablk), signed map with the
+ * ndir Dirichlet DOFs first, Jacobi preconditioner in local form
+ * (inv_diag: nglob entries, 0 on Dirichlet rows).  x_loc: boff[nblk]. */
+int or_cg_loc(int nblk, const int *boff, const double *Ablk, const int *map,
+              const double *sign, int nglob, int ndir, const double *inv_diag,
+              const double *b_loc, double tol, int maxit, double *x_loc,
+              or_cg_info *info);
 
 #ifdef __cplusplus
 }

@@ -209,6 +209,23 @@ def cg_dense(A, b, ndir=0, inv_diag=None, tol=1e-10, maxit=5000):
     return x, info
 
 
+def cg_loc(blocks, mp, sign, nglob, ndir, b_loc, inv_diag=None, tol=1e-10, maxit=5000):
+    """Element-local CG (or_cg_loc): blocks = list of dense element
+    matrices, whose local vectors concatenated in order are mp's layout."""
+    boff = np.zeros(len(blocks) + 1, dtype=np.int32)
+    boff[1:] = np.cumsum([len(a) for a in blocks])
+    A = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in blocks]))
+    mp = np.ascontiguousarray(mp, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sign = None if sign is None else np.ascontiguousarray(sign, dtype=np.float64)
+    x = np.zeros(int(boff[-1]))
+    info = CGInfo()
+    lib().or_cg_loc(len(blocks), boff.ctypes.data_as(ip), p(A), mp.ctypes.data_as(ip), p(sign), nglob, ndir,
+                    p(inv_diag), p(np.ascontiguousarray(b_loc, dtype=np.float64)), ctypes.c_double(tol), maxit,
+                    p(x), ctypes.byref(info))
+    return x, info
+
+
 def splitmix_uniform(seed: int, n: int) -> np.ndarray:
     """SplitMix64 stream mapped to [-1, 1) as 2*(x>>11)*2^-53 - 1."""
     out = np.empty(n)
