@@ -242,6 +242,10 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
     s.solve(tol=1e-10)
     ref = s.solve(tol=1e-10)
     diff = float(np.max(np.abs(ls.solution()[new] - s.solution())) / np.max(np.abs(s.solution())))
+    # static-condensation diagonal of the same system (host checks of the map included)
+    t0 = time.perf_counter()
+    scd = ls.sc_diagonal()
+    sc_ms = (time.perf_counter() - t0) * 1e3
     del ls
     # the element-local CG (LinSysIterSolver ConjugateGradientLoc) on the same system
     ll = N.LinearSystem(amap, [op], solver="ConjugateGradientLoc")
@@ -255,6 +259,7 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
             "general_solve_gdofs": round(nunk / (info.solve_ms * 1e-3) / 1e9, 4),
             "local_cg_solve_ms": round(linfo.solve_ms, 3), "local_cg_iterations": linfo.iterations,
             "local_cg_converged": bool(linfo.converged), "local_cg_rel_diff": ldiff,
+            "sc_diagonal_ms": round(sc_ms, 1), "sc_diagonal_rows": int(np.count_nonzero(scd)),
             "structured_solve_ms": round(ref.solve_ms, 3), "structured_iterations": ref.iterations,
             "solution_rel_diff": diff, "negative_signs": int((sg < 0).sum())}
 

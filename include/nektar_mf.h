@@ -385,6 +385,17 @@ int nmf_linsys_get_solution(nmf_linsys l, double *x_host);
  * own vector (GlobalLinSysIterativeFull.cpp:367-371) */
 int nmf_linsys_get_solution_local(nmf_linsys l, double *xloc_dev);
 int nmf_linsys_get_inv_diag(nmf_linsys l, double *d_host);
+/* The static-condensation diagonal into d_host (nglobal): every element's
+ * Schur complement A_bb - A_bi A_ii^-1 A_ib of its interior modes (hex
+ * p,q,r >= 2; tet p >= 2, q,r >= 1), its diagonal summed over the global
+ * boundary DOFs -- PreconditionerDiagonal::StaticCondDiagonalPreconditionerSum
+ * before the inversion (PreconditionerDiagonal.cpp:132-154,
+ * Preconditioner.cpp:185-236).  Dirichlet rows and interior DOFs are 0.
+ * FP64 on the device: element columns from operator applies to unit modes,
+ * a Cholesky factorisation of A_ii per element.  At most 512 interior modes
+ * per element; interior modes must not be shared; an element's boundary
+ * modes must map to distinct global DOFs (NMF_ERR_UNSUPPORTED otherwise). */
+int nmf_linsys_sc_diagonal(nmf_linsys l, double *d_host);
 /* y = A x on global device vectors, Dirichlet rows zeroed */
 int nmf_linsys_apply_global(nmf_linsys l, const double *x_dev, double *y_dev);
 

@@ -135,6 +135,7 @@ _SIGNATURES = {
     "nmf_linsys_create_solver": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, ctypes.c_char_p, _P,
                                                 ctypes.POINTER(_P)]),
     "nmf_linsys_get_solution_local": (ctypes.c_int, [_P, _P]),
+    "nmf_linsys_sc_diagonal": (ctypes.c_int, [_P, _D]),
     "nmf_linsys_destroy": (ctypes.c_int, [_P]),
     "nmf_linsys_set_rhs_local": (ctypes.c_int, [_P, _P]),
     "nmf_linsys_solve": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(SolveInfo)]),
@@ -662,6 +663,13 @@ class LinearSystem:
     def inv_diag(self) -> np.ndarray:
         d = np.zeros(self.nglobal)
         check(lib().nmf_linsys_get_inv_diag(self._h, _dptr(d)))
+        return d
+
+    def sc_diagonal(self) -> np.ndarray:
+        """Static-condensation diagonal (Schur complement of the interior
+        modes, assembled on the boundary DOFs; Dirichlet and interior rows 0)."""
+        d = np.zeros(self.nglobal)
+        check(lib().nmf_linsys_sc_diagonal(self._h, _dptr(d)))
         return d
 
     def apply_global(self, x, y) -> None:

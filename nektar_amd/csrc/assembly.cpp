@@ -2,6 +2,9 @@
 #include "assembly.hpp"
 
 #include <algorithm>
+#include <string>
+
+#include "schur.hpp"
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -91,6 +94,13 @@ void AssemblyMap::global_values(const double *loc, double *glob, hipStream_t s) 
                                      has_sign_ ? csign_.as<double>() : nullptr, loc, glob));
 }
 
+std::vector<int> AssemblyMap::host_map() const
+{
+    std::vector<int> h((size_t)nloc_);
+    if (nloc_) NMF_HIP(hipMemcpy(h.data(), map_.get(), (size_t)nloc_ * sizeof(int), hipMemcpyDeviceToHost));
+    return h;
+}
+
 IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOperator *> &ops, hipStream_t stream,
                                  CGSpace space)
     : map_(map), ops_(ops), stream_(stream), space_(space)
@@ -159,6 +169,99 @@ void IterativeSolver::build_diagonal()
     }
     map_.assemble(dloc, invd_.as<double>(), stream_, false, nullptr, nullptr, true);
     NMF_HIP(solver::launch_invert_diag_dir(stream_, map_.nglobal(), map_.ndir(), invd_.as<double>()));
+    NMF_HIP(hipStreamSynchronize(stream_));
+}
+
+// The element matrices' columns come from operator applies to unit modes
+// (every element of a collection at once); per element chunk: A_ii from the
+// interior columns, its Cholesky factor, then the boundary columns in
+// batches of schur::kBatch, each giving S_kk = A_kk - |L^-1 a_k|^2 in the
+// element-local layout; assembled with the Dirichlet rows zeroed.
+void IterativeSolver::sc_diagonal(double *dglob)
+{
+    NMF_REQUIRE(dglob || map_.nglobal() == 0, NMF_ERR_INVALID, "null output");
+    // bytes of element blocks per chunk (NMF_SC_CHUNK_BYTES overrides the 8 GiB default)
+    size_t kBudget = (size_t)8 << 30;
+    if (const char *v = std::getenv("NMF_SC_CHUNK_BYTES"))
+        if (std::strtoull(v, nullptr, 10) > 0) kBudget = std::strtoull(v, nullptr, 10);
+    const int KB = schur::kBatch;
+    const std::vector<int> hmap = map_.host_map();
+    std::vector<int> cnt(std::max(map_.nglobal(), 1), 0);
+    for (int g : hmap) ++cnt[g];
+    DeviceBuffer sloc(std::max<int64_t>(map_.nloc(), 1) * sizeof(double));
+    NMF_HIP(solver::launch_zero<double>(stream_, (size_t)std::max<int64_t>(map_.nloc(), 1), sloc.as<double>()));
+    for (size_t k = 0; k < ops_.size(); ++k) {
+        MFOperator &o = *ops_[k];
+        const std::vector<int> I = schur::interior_modes(o.shape, o.nm), Bd = schur::boundary_modes(o.shape, o.nm);
+        const int E = o.nelmt, nc = o.ncoeff, ni = (int)I.size(), nb = (int)Bd.size();
+        NMF_REQUIRE(ni <= schur::kMaxInterior, NMF_ERR_UNSUPPORTED,
+                    "static-condensation diagonal: more than " + std::to_string(schur::kMaxInterior) +
+                        " interior modes per element");
+        std::vector<int> gb(nb);
+        for (int e = 0; e < E; ++e) {
+            const int *me = hmap.data() + off_[k] + (size_t)e * nc;
+            for (int m : I)
+                NMF_REQUIRE(cnt[me[m]] == 1, NMF_ERR_INVALID,
+                            "static-condensation diagonal: interior mode " + std::to_string(m) + " of element " +
+                                std::to_string(e) + " of collection " + std::to_string(k) +
+                                " shares its global DOF");
+            for (int j = 0; j < nb; ++j) gb[j] = me[Bd[j]];
+            std::sort(gb.begin(), gb.end());
+            NMF_REQUIRE(std::adjacent_find(gb.begin(), gb.end()) == gb.end(), NMF_ERR_UNSUPPORTED,
+                        "static-condensation diagonal: two boundary modes of element " + std::to_string(e) +
+                            " map to one global DOF (periodic single-element meshes)");
+        }
+        if (E == 0 || nb == 0) continue;
+        DeviceBuffer dI(std::max(ni, 1) * sizeof(int)), dB(nb * sizeof(int));
+        if (ni) NMF_HIP(hipMemcpy(dI.get(), I.data(), ni * sizeof(int), hipMemcpyHostToDevice));
+        NMF_HIP(hipMemcpy(dB.get(), Bd.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+        const size_t per = ((size_t)ni * ni + (size_t)ni * KB + KB) * sizeof(double);
+        const int Ec = (int)std::max<size_t>(1, std::min<size_t>((size_t)E, kBudget / per));
+        DeviceBuffer Aii(std::max<size_t>((size_t)Ec * ni * ni, 1) * sizeof(double));
+        DeviceBuffer Ab(std::max<size_t>((size_t)Ec * ni * KB, 1) * sizeof(double));
+        DeviceBuffer dkk((size_t)Ec * KB * sizeof(double)), info((size_t)Ec * sizeof(int));
+        std::vector<int> hinfo(Ec);
+        const size_t n = (size_t)E * nc;
+        double *u = xloc_.as<double>() + off_[k], *y = yloc_.as<double>() + off_[k];
+        auto column = [&](int m) {
+            NMF_HIP(solver::launch_unit_mode(stream_, n, nc, m, u));
+            o.apply(u, y, nullptr, nullptr, stream_);
+        };
+        for (int e0 = 0; e0 < E; e0 += Ec) {
+            const int ne = std::min(Ec, E - e0);
+            for (int c = 0; c < ni; ++c) {
+                column(I[c]);
+                NMF_HIP(schur::launch_take_rows(stream_, ne, e0, nc, ni, dI.as<int>(), 0, y, Aii.as<double>(),
+                                                (size_t)ni * ni, c, ni));
+            }
+            if (ni) {
+                NMF_HIP(schur::launch_cholesky(stream_, ne, ni, Aii.as<double>(), info.as<int>()));
+                NMF_HIP(hipMemcpyAsync(hinfo.data(), info.get(), ne * sizeof(int), hipMemcpyDeviceToHost, stream_));
+                NMF_HIP(hipStreamSynchronize(stream_));
+                for (int e = 0; e < ne; ++e)
+                    NMF_REQUIRE(hinfo[e] == 0, NMF_ERR_INVALID,
+                                "static-condensation diagonal: the interior block of element " +
+                                    std::to_string(e0 + e) + " of collection " + std::to_string(k) +
+                                    " is not positive definite");
+            }
+            for (int b0 = 0; b0 < nb; b0 += KB) {
+                const int kb = std::min(KB, nb - b0);
+                for (int c = 0; c < kb; ++c) {
+                    const int m = Bd[b0 + c];
+                    column(m);
+                    NMF_HIP(schur::launch_take_rows(stream_, ne, e0, nc, ni, dI.as<int>(), 0, y, Ab.as<double>(),
+                                                    (size_t)ni * KB, c, ni));
+                    NMF_HIP(schur::launch_take_rows(stream_, ne, e0, nc, 1, nullptr, m, y, dkk.as<double>(),
+                                                    (size_t)KB, c, 1));
+                }
+                NMF_HIP(schur::launch_schur_diag(stream_, ne, ni, kb, KB, Aii.as<double>(), Ab.as<double>(),
+                                                 dkk.as<double>(), dB.as<int>() + b0, nc, e0,
+                                                 sloc.as<double>() + off_[k]));
+            }
+        }
+        NMF_HIP(hipStreamSynchronize(stream_));  // the chunk buffers are freed here
+    }
+    map_.assemble(sloc.as<double>(), dglob, stream_, true, nullptr, nullptr, true);
     NMF_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -45,6 +45,8 @@ public:
                       hipStream_t s) const;
     // glob[g] = sign loc of g's first local entry (loc continuous)
     void global_values(const double *loc, double *glob, hipStream_t s) const;
+    // the map back on the host (a synchronous copy)
+    std::vector<int> host_map() const;
 
     int64_t nloc() const { return nloc_; }
     int nglobal() const { return nglobal_; }
@@ -88,6 +90,14 @@ public:
     const double *inv_diag() const { return invd_.as<double>(); }
     int nglobal() const { return map_.nglobal(); }
     CGSpace space() const { return space_; }
+    // the static-condensation diagonal (PreconditionerDiagonal.cpp:132-154
+    // before its inversion; Preconditioner.cpp:185-236): each element's
+    // Schur complement A_bb - A_bi A_ii^-1 A_ib of its interior modes,
+    // diagonal assembled over the boundary DOFs into dglob (device,
+    // nglobal; Dirichlet rows and interior DOFs 0).  Needs interior modes
+    // that no other local entry shares and distinct global DOFs for the
+    // boundary modes of one element.
+    void sc_diagonal(double *dglob);
 
 private:
     void apply_local(const double *xloc, double *yloc);

@@ -791,6 +791,19 @@ int nmf_linsys_get_solution_local(nmf_linsys l, double *xloc_dev)
     });
 }
 
+int nmf_linsys_sc_diagonal(nmf_linsys l, double *d_host)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        NMF_REQUIRE(d_host, NMF_ERR_INVALID, "null output");
+        const size_t n = (size_t)std::max(l->impl->nglobal(), 1);
+        DeviceBuffer d(n * sizeof(double));
+        l->impl->sc_diagonal(d.as<double>());
+        if (l->impl->nglobal())
+            NMF_HIP(hipMemcpy(d_host, d.get(), (size_t)l->impl->nglobal() * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
 int nmf_linsys_get_inv_diag(nmf_linsys l, double *d_host)
 {
     return guard([&] {

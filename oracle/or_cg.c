@@ -274,3 +274,69 @@ int or_cg_loc(int nblk, const int *boff, const double *Ablk, const int *map, con
     free(m.wk); free(w); free(s); free(p); free(r); free(q); free(wk);
     return iters;
 }
+
+/* ---- static-condensation diagonal ------------------------------------
+ * Preconditioner::AssembleStaticCondGlobalDiagonals (Preconditioner.cpp:
+ * 185-236) with PreconditionerDiagonal::StaticCondDiagonalPreconditionerSum
+ * (PreconditionerDiagonal.cpp:132-154) before the inversion: for every
+ * element the Schur complement S = A_bb - A_bi A_ii^-1 A_ib of its matrix
+ * (the boundary block of the statically condensed matrix), and
+ * d[g] = sum over the element's boundary pairs (i, j) with map[i] = map[j]
+ * = g of sign_i sign_j S_ij, for the non-Dirichlet g (Dirichlet and
+ * interior-only rows stay 0).  interior[i] flags the interior local
+ * entries.  A_ii^-1 by a dense Cholesky factorisation.  Returns 0, or
+ * 1 + the block whose interior block is not positive definite. */
+int or_sc_diagonal(int nblk, const int *boff, const double *Ablk, const uint8_t *interior, const int *map,
+                   const double *sign, int nglob, int ndir, double *d)
+{
+    memset(d, 0, (size_t)nglob * sizeof(double));
+    size_t off = 0;
+    int ret = 0;
+    for (int k = 0; k < nblk && !ret; ++k) {
+        const int b0 = boff[k], n = boff[k + 1] - boff[k];
+        const double *A = Ablk + off;
+        off += (size_t)n * n;
+        int *ii = (int *)malloc((size_t)n * sizeof(int)), *bb = (int *)malloc((size_t)n * sizeof(int));
+        int ni = 0, nb = 0;
+        for (int i = 0; i < n; ++i) {
+            if (interior[b0 + i]) ii[ni++] = i;
+            else bb[nb++] = i;
+        }
+        /* L L^T = A_ii (row-major lower triangle) */
+        double *L = (double *)calloc((size_t)(ni > 0 ? ni : 1) * (ni > 0 ? ni : 1), sizeof(double));
+        for (int c = 0; c < ni && !ret; ++c) {
+            double s = A[(size_t)ii[c] * n + ii[c]];
+            for (int j = 0; j < c; ++j) s -= L[(size_t)c * ni + j] * L[(size_t)c * ni + j];
+            if (!(s > 0.0)) { ret = k + 1; break; }
+            const double dcc = sqrt(s);
+            L[(size_t)c * ni + c] = dcc;
+            for (int r = c + 1; r < ni; ++r) {
+                double t = A[(size_t)ii[r] * n + ii[c]];
+                for (int j = 0; j < c; ++j) t -= L[(size_t)r * ni + j] * L[(size_t)c * ni + j];
+                L[(size_t)r * ni + c] = t / dcc;
+            }
+        }
+        /* Y[:, j] = L^-1 a_j for every boundary j; S_ij = A_ij - Y_i . Y_j */
+        double *Y = (double *)calloc((size_t)(ni > 0 ? ni : 1) * (nb > 0 ? nb : 1), sizeof(double));
+        for (int j = 0; j < nb && !ret; ++j) {
+            for (int r = 0; r < ni; ++r) {
+                double t = A[(size_t)ii[r] * n + bb[j]];
+                for (int c = 0; c < r; ++c) t -= L[(size_t)r * ni + c] * Y[(size_t)c * nb + j];
+                Y[(size_t)r * nb + j] = t / L[(size_t)r * ni + r];
+            }
+        }
+        for (int i = 0; i < nb && !ret; ++i) {
+            const int gi = map[b0 + bb[i]];
+            if (gi < ndir) continue;
+            for (int j = 0; j < nb; ++j) {
+                if (map[b0 + bb[j]] != gi) continue;
+                double sij = A[(size_t)bb[i] * n + bb[j]];
+                for (int r = 0; r < ni; ++r) sij -= Y[(size_t)r * nb + i] * Y[(size_t)r * nb + j];
+                const double si = sign ? sign[b0 + bb[i]] : 1.0, sj = sign ? sign[b0 + bb[j]] : 1.0;
+                d[gi] += si * sj * sij;
+            }
+        }
+        free(ii); free(bb); free(L); free(Y);
+    }
+    return ret;
+}

@@ -155,6 +155,15 @@ int or_cg_loc(int nblk, const int *boff, const double *Ablk, const int *map,
               const double *b_loc, double tol, int maxit, double *x_loc,
               or_cg_info *info);
 
+/* Static-condensation diagonal (Preconditioner.cpp:185-236): per block the
+ * Schur complement of its interior entries (interior[i] != 0), summed over
+ * boundary pairs with equal global id into d (nglob; rows < ndir and
+ * interior-only rows 0).  Returns 0, or 1 + a block whose interior block is
+ * not positive definite. */
+int or_sc_diagonal(int nblk, const int *boff, const double *Ablk,
+                   const uint8_t *interior, const int *map,
+                   const double *sign, int nglob, int ndir, double *d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -226,6 +226,34 @@ def cg_loc(blocks, mp, sign, nglob, ndir, b_loc, inv_diag=None, tol=1e-10, maxit
     return x, info
 
 
+def interior_modes(shape, nm):
+    """Flags of the interior modes in the coefficient order: hex p, q, r >= 2
+    (p fastest); tet p >= 2, q >= 1, r >= 1 (r fastest,
+    StdTetExp.cpp:1156-1190)."""
+    if shape == HEX:
+        r, q, p = np.meshgrid(np.arange(nm), np.arange(nm), np.arange(nm), indexing="ij")
+        return ((p >= 2) & (q >= 2) & (r >= 2)).ravel()
+    return np.array([p >= 2 and q >= 1 and r >= 1 for p in range(nm) for q in range(nm - p) for r in range(nm - p - q)])
+
+
+def sc_diagonal(blocks, interior, mp, sign, nglob, ndir):
+    """Static-condensation diagonal (or_sc_diagonal); interior: flags over
+    the concatenated local entries."""
+    boff = np.zeros(len(blocks) + 1, dtype=np.int32)
+    boff[1:] = np.cumsum([len(a) for a in blocks])
+    A = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in blocks]))
+    fl = np.ascontiguousarray(interior, dtype=np.uint8)
+    mp = np.ascontiguousarray(mp, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sign = None if sign is None else np.ascontiguousarray(sign, dtype=np.float64)
+    d = np.zeros(nglob)
+    rc = lib().or_sc_diagonal(len(blocks), boff.ctypes.data_as(ip), p(A), fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                              mp.ctypes.data_as(ip), p(sign), nglob, ndir, p(d))
+    if rc:
+        raise ValueError(f"interior block of element {rc - 1} is not positive definite")
+    return d
+
+
 def splitmix_uniform(seed: int, n: int) -> np.ndarray:
     """SplitMix64 stream mapped to [-1, 1) as 2*(x>>11)*2^-53 - 1."""
     out = np.empty(n)

@@ -19,6 +19,7 @@ opposite trace, with sign -1.  A mode with no such partner raises, so the
 builder cannot return a map that is not conforming mode by mode; the tests
 then check continuity of assembled fields across every shared face and the
 spectral convergence of a manufactured solve."""
+import functools
 import itertools
 
 import numpy as np
@@ -72,6 +73,7 @@ def _monomials(xi, P):
     return np.stack([xi[:, 0] ** i * xi[:, 1] ** j * xi[:, 2] ** k for i, j, k in ex], axis=1)
 
 
+@functools.lru_cache(maxsize=None)
 def mode_polynomials(nm):
     """Every tet mode as a polynomial of total degree <= P = nm-1 in the
     reference coordinates xi (the modes span exactly that space): monomial
