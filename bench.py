@@ -197,7 +197,8 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
     P=6 box with every element in a random local frame (orientation signs
     on odd bubble modes), against the structured solver on the same box:
     solve time, iterations and the largest difference of the solutions, for
-    the assembled-vector CG and the element-local CG."""
+    the assembled-vector CG, the element-local CG and the iterative static
+    condensation; and the static-condensation diagonal."""
     import numpy as np
 
     gid, sg, dirichlet, perm, flip = rotated_box_map(n, nm)
@@ -253,6 +254,18 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
     ll.solve(tol=1e-10)  # warm
     linfo = ll.solve(tol=1e-10)
     ldiff = float(np.max(np.abs(ll.solution()[new] - s.solution())) / np.max(np.abs(s.solution())))
+    del ll
+    # iterative static condensation (GlobalSysSoln IterativeStaticCond) of the same system:
+    # setup (element Schur complements), then solves with the same right-hand side (the
+    # smoothed rhs magnitude then stays the first solve's)
+    t0 = time.perf_counter()
+    sc = N.LinearSystem(amap, [op], solver="IterativeStaticCond")
+    sc_setup_ms = (time.perf_counter() - t0) * 1e3
+    sc.set_rhs_local(bl)
+    sc.solve(tol=1e-10)  # warm
+    sinfo = sc.solve(tol=1e-10)
+    sdiff = float(np.max(np.abs(sc.solution()[new] - s.solution())) / np.max(np.abs(s.solution())))
+    del sc
     nunk = int((~dirichlet).sum())
     return {"unknowns": nunk, "general_solve_ms": round(info.solve_ms, 3), "general_iterations": info.iterations,
             "general_converged": bool(info.converged),
@@ -260,6 +273,9 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
             "local_cg_solve_ms": round(linfo.solve_ms, 3), "local_cg_iterations": linfo.iterations,
             "local_cg_converged": bool(linfo.converged), "local_cg_rel_diff": ldiff,
             "sc_diagonal_ms": round(sc_ms, 1), "sc_diagonal_rows": int(np.count_nonzero(scd)),
+            "static_cond_setup_ms": round(sc_setup_ms, 1), "static_cond_solve_ms": round(sinfo.solve_ms, 3),
+            "static_cond_iterations": sinfo.iterations, "static_cond_converged": bool(sinfo.converged),
+            "static_cond_rel_diff": sdiff,
             "structured_solve_ms": round(ref.solve_ms, 3), "structured_iterations": ref.iterations,
             "solution_rel_diff": diff, "negative_signs": int((sg < 0).sum())}
 

@@ -362,13 +362,17 @@ int nmf_assembly_assemble(nmf_assembly a, const double *loc, double *glob,
  * geometry (exact element-matrix diagonals, one apply per local mode). */
 int nmf_linsys_create(nmf_assembly a, const nmf_operator *ops, int nops,
                       void *stream, nmf_linsys *out);
-/* The same with the reference's LinSysIterSolver name
- * (GlobalLinSysIterative.cpp:86-88): "ConjugateGradient" (NekLinSysIterCG,
+/* The same with the reference's solver name: LinSysIterSolver
+ * (GlobalLinSysIterative.cpp:86-88) "ConjugateGradient" (NekLinSysIterCG,
  * assembled vectors; what nmf_linsys_create builds) or
  * "ConjugateGradientLoc" (NekLinSysIterCGLoc.cpp:114-308, element-local
  * vectors: s = A w without assembly, the Diagonal preconditioner assembles
- * the residual, PreconditionerDiagonal.cpp:174-184).  Other names:
- * NMF_ERR_UNSUPPORTED. */
+ * the residual, PreconditionerDiagonal.cpp:174-184); or GlobalSysSoln
+ * "IterativeStaticCond" (GlobalLinSysStaticCond.cpp:98-230,
+ * GlobalLinSysIterativeStaticCond.cpp:378-492: single-level condensation
+ * of the interior modes, CG on the assembled Schur complement with its
+ * diagonal as preconditioner; get_inv_diag then returns the inverse Schur
+ * diagonal on the boundary DOFs).  Other names: NMF_ERR_UNSUPPORTED. */
 int nmf_linsys_create_solver(nmf_assembly a, const nmf_operator *ops,
                              int nops, const char *solver, void *stream,
                              nmf_linsys *out);

@@ -626,7 +626,9 @@ class LinearSystem:
     ``solver`` is the reference's LinSysIterSolver
     (GlobalLinSysIterative.cpp:86-88): "ConjugateGradient"
     (NekLinSysIterCG, assembled vectors) or "ConjugateGradientLoc"
-    (NekLinSysIterCGLoc, element-local vectors)."""
+    (NekLinSysIterCGLoc, element-local vectors); or its GlobalSysSoln
+    "IterativeStaticCond" (single-level static condensation, CG on the
+    Schur complement)."""
 
     def __init__(self, amap: AssemblyMap, ops, stream=None, solver: str = "ConjugateGradient"):
         self._amap, self._ops = amap, list(ops)  # keep both alive

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "schur.hpp"
+#include "static_cond.hpp"
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -101,6 +102,15 @@ std::vector<int> AssemblyMap::host_map() const
     return h;
 }
 
+std::vector<double> AssemblyMap::host_sign() const
+{
+    std::vector<double> h;
+    if (!has_sign_ || nloc_ == 0) return h;
+    h.resize((size_t)nloc_);
+    NMF_HIP(hipMemcpy(h.data(), sign_.get(), (size_t)nloc_ * sizeof(double), hipMemcpyDeviceToHost));
+    return h;
+}
+
 IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOperator *> &ops, hipStream_t stream,
                                  CGSpace space)
     : map_(map), ops_(ops), stream_(stream), space_(space)
@@ -119,7 +129,7 @@ IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOpe
     const int NG = map_.nglobal();
     nb_ = solver::reduction_blocks((size_t)std::max(NG, 1));
     // Krylov vectors and the right-hand side: global (NG) or local (n)
-    const size_t nv = space_ == CGSpace::Local ? (size_t)std::max<int64_t>(n, 1) : (size_t)std::max(NG, 1);
+    const size_t nv = space_ != CGSpace::Global ? (size_t)std::max<int64_t>(n, 1) : (size_t)std::max(NG, 1);
     xloc_.allocate(std::max<int64_t>(n, 1) * sizeof(double));
     yloc_.allocate(std::max<int64_t>(n, 1) * sizeof(double));
     b_.allocate(nv * sizeof(double));
@@ -134,6 +144,7 @@ IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOpe
     NMF_HIP(solver::launch_zero<double>(stream_, nv, cg_[0].as<double>()));
     for (MFOperator *o : ops_) o->prepare();
     build_diagonal();
+    if (space_ == CGSpace::StaticCond) setup_static_cond();
 }
 
 IterativeSolver::~IterativeSolver()
@@ -268,7 +279,7 @@ void IterativeSolver::sc_diagonal(double *dglob)
 void IterativeSolver::set_rhs_local(const double *bloc)
 {
     NMF_REQUIRE(bloc || map_.nloc() == 0, NMF_ERR_INVALID, "null right-hand side");
-    if (space_ == CGSpace::Local) {
+    if (space_ != CGSpace::Global) {
         if (map_.nloc())
             NMF_HIP(hipMemcpyAsync(b_.get(), bloc, (size_t)map_.nloc() * sizeof(double), hipMemcpyDeviceToDevice,
                                    stream_));
@@ -281,6 +292,7 @@ void IterativeSolver::set_rhs_local(const double *bloc)
 const double *IterativeSolver::solution()
 {
     if (space_ == CGSpace::Global) return cg_[0].as<double>();
+    if (space_ == CGSpace::StaticCond) return sc_->xg.as<double>();
     map_.global_values(cg_[0].as<double>(), xg_.as<double>(), stream_);
     return xg_.as<double>();
 }
@@ -293,7 +305,8 @@ void IterativeSolver::solution_local(double *xloc)
     if (space_ == CGSpace::Global)
         map_.global_to_local(cg_[0].as<double>(), xloc, stream_);
     else
-        NMF_HIP(hipMemcpyAsync(xloc, cg_[0].get(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        NMF_HIP(hipMemcpyAsync(xloc, space_ == CGSpace::StaticCond ? sc_->xloc.get() : cg_[0].get(),
+                               n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
 }
 
 void IterativeSolver::apply_global(const double *x, double *y)
@@ -317,6 +330,7 @@ SolveInfo IterativeSolver::solve(double tol, int maxit)
 {
     NMF_REQUIRE(have_rhs_, NMF_ERR_STATE, "right-hand side not set");
     NMF_REQUIRE(maxit >= 0, NMF_ERR_INVALID, "maxit must be >= 0");
+    if (space_ == CGSpace::StaticCond) return solve_static_cond(tol, maxit);
     tol = std::max(tol, 1.0e-16);  // NekLinSysIter.cpp:57
     const int NG = map_.nglobal(), ND = map_.ndir();
     const bool local = space_ == CGSpace::Local;

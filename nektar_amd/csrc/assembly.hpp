@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
@@ -45,8 +46,10 @@ public:
                       hipStream_t s) const;
     // glob[g] = sign loc of g's first local entry (loc continuous)
     void global_values(const double *loc, double *glob, hipStream_t s) const;
-    // the map back on the host (a synchronous copy)
+    // the map and signs back on the host (synchronous copies; signs empty
+    // when the map has none)
     std::vector<int> host_map() const;
+    std::vector<double> host_sign() const;
 
     int64_t nloc() const { return nloc_; }
     int nglobal() const { return nglobal_; }
@@ -59,10 +62,14 @@ private:
     DeviceBuffer map_, sign_, row_, col_, csign_;
 };
 
-// The reference's LinSysIterSolver choices for this system
-// (GlobalLinSysIterative.cpp:86-88): NekLinSysIterCG on the assembled
-// vectors, or NekLinSysIterCGLoc on the element-local ones.
-enum class CGSpace { Global, Local };
+// The reference's solver choices for this system: GlobalSysSoln
+// IterativeFull with LinSysIterSolver NekLinSysIterCG on the assembled
+// vectors or NekLinSysIterCGLoc on the element-local ones
+// (GlobalLinSysIterative.cpp:86-88), or GlobalSysSoln IterativeStaticCond
+// (CG on the assembled Schur complement of the interior modes).
+enum class CGSpace { Global, Local, StaticCond };
+
+struct StaticCondData;  // static_cond.cpp
 
 class IterativeSolver {
 public:
@@ -87,7 +94,10 @@ public:
     // the device, enqueued on the solver's stream
     const double *solution();
     void solution_local(double *xloc);
-    const double *inv_diag() const { return invd_.as<double>(); }
+    // the preconditioner's inverse diagonal in the caller's numbering: the
+    // Jacobi one, or (StaticCond) the inverse Schur diagonal on the
+    // boundary DOFs, 0 on interior DOFs
+    const double *inv_diag() const;
     int nglobal() const { return map_.nglobal(); }
     CGSpace space() const { return space_; }
     // the static-condensation diagonal (PreconditionerDiagonal.cpp:132-154
@@ -98,10 +108,15 @@ public:
     // that no other local entry shares and distinct global DOFs for the
     // boundary modes of one element.
     void sc_diagonal(double *dglob);
+    // StaticCond: CG iterations of the last solve ran on this many boundary
+    // unknowns (global boundary DOFs, Dirichlet included)
+    int nboundary() const;
 
 private:
     void apply_local(const double *xloc, double *yloc);
     void build_diagonal();
+    void setup_static_cond();
+    SolveInfo solve_static_cond(double tol, int maxit);
 
     const AssemblyMap &map_;
     std::vector<MFOperator *> ops_;
@@ -113,6 +128,7 @@ private:
     DeviceBuffer cg_[6];  // x r p q w s: global (Global) or local (Local) vectors
     solver::CGState *host_state_ = nullptr;
     bool have_rhs_ = false;
+    std::unique_ptr<StaticCondData> sc_;
 };
 
 }  // namespace nektar_amd

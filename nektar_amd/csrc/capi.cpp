@@ -723,9 +723,13 @@ int nmf_linsys_create_solver(nmf_assembly a, const nmf_operator *ops, int nops, 
         NMF_ASM_CHECK();
         NMF_REQUIRE(solver, NMF_ERR_INVALID, "null solver name");
         const std::string name(solver);
-        NMF_REQUIRE(name == "ConjugateGradient" || name == "ConjugateGradientLoc", NMF_ERR_UNSUPPORTED,
-                    "LinSysIterSolver '" + name + "': this library has ConjugateGradient and ConjugateGradientLoc");
-        const CGSpace space = name == "ConjugateGradientLoc" ? CGSpace::Local : CGSpace::Global;
+        NMF_REQUIRE(name == "ConjugateGradient" || name == "ConjugateGradientLoc" || name == "IterativeStaticCond",
+                    NMF_ERR_UNSUPPORTED,
+                    "solver '" + name +
+                        "': this library has ConjugateGradient, ConjugateGradientLoc and IterativeStaticCond");
+        const CGSpace space = name == "ConjugateGradientLoc"  ? CGSpace::Local
+                              : name == "IterativeStaticCond" ? CGSpace::StaticCond
+                                                              : CGSpace::Global;
         NMF_REQUIRE(out, NMF_ERR_INVALID, "null output handle");
         *out = nullptr;
         NMF_REQUIRE(nops >= 1 && ops, NMF_ERR_INVALID, "at least one operator");

@@ -20,17 +20,98 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// dst[(e - e0) * stride + col * ld + r] = y[e * nc + rows[r]] for the
-// elements e0 .. e0 + ne - 1 (rows == nullptr: the single row `row`)
+// dst[(e - e0) * stride + col * ld + r] = (acc ? dst : 0) + alpha y[e * nc + rows[r]]
+// for the elements e0 .. e0 + ne - 1 (rows == nullptr: the single row `row`)
 __global__ void take_rows_kernel(int ne, int e0, int nc, int nrow, const int *__restrict__ rows, int row,
                                  const double *__restrict__ y, double *__restrict__ dst, size_t stride, int col,
-                                 int ld)
+                                 int ld, double alpha, int acc)
 {
     const size_t n = (size_t)ne * nrow;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int e = (int)(i / nrow), r = (int)(i % nrow);
         const int src = rows ? rows[r] : row;
-        dst[(size_t)e * stride + (size_t)col * ld + r] = y[(size_t)(e0 + e) * nc + src];
+        double *d = dst + (size_t)e * stride + (size_t)col * ld + r;
+        const double v = alpha * y[(size_t)(e0 + e) * nc + src];
+        *d = acc ? *d + v : v;
+    }
+}
+
+// y[(e0 + e) * nc + rows[r]] = alpha src[e * stride + col * ld + r] (the
+// inverse of take_rows; the other entries of y are left as they are)
+__global__ void put_rows_kernel(int ne, int e0, int nc, int nrow, const int *__restrict__ rows,
+                                const double *__restrict__ src, size_t stride, int col, int ld, double alpha,
+                                double *__restrict__ y)
+{
+    const size_t n = (size_t)ne * nrow;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int e = (int)(i / nrow), r = (int)(i % nrow);
+        y[(size_t)(e0 + e) * nc + rows[r]] = alpha * src[(size_t)e * stride + (size_t)col * ld + r];
+    }
+}
+
+// B <- A_ii^-1 B = L^-T L^-1 B for kb columns (leading dimension ni, kbld
+// columns per element) of one element per workgroup, in LDS
+__global__ __launch_bounds__(kThreads) void chol_solve_kernel(int ni, int kb, int kbld, const double *__restrict__ L,
+                                                              double *__restrict__ B)
+{
+    extern __shared__ double Z[];  // kb * ni
+    const int tid = threadIdx.x, e = blockIdx.x;
+    const double *Le = L + (size_t)e * ni * ni;
+    double *Be = B + (size_t)e * ni * kbld;
+    for (int idx = tid; idx < kb * ni; idx += kThreads) Z[idx] = Be[idx];
+    __syncthreads();
+    for (int k = 0; k < ni; ++k) {  // L z = b, column-oriented
+        const double lkk = Le[(size_t)k * ni + k];
+        if (tid < kb) Z[tid * ni + k] /= lkk;
+        __syncthreads();
+        const int rows = ni - k - 1;
+        for (int idx = tid; idx < rows * kb; idx += kThreads) {
+            const int r = k + 1 + idx % rows, c = idx / rows;
+            Z[c * ni + r] -= Le[(size_t)k * ni + r] * Z[c * ni + k];
+        }
+        __syncthreads();
+    }
+    for (int k = ni - 1; k >= 0; --k) {  // L^T x = z: x_k, then z_r -= L(k, r) x_k for r < k
+        if (tid < kb) Z[tid * ni + k] /= Le[(size_t)k * ni + k];
+        __syncthreads();
+        for (int idx = tid; idx < k * kb; idx += kThreads) {
+            const int r = idx % k, c = idx / k;
+            Z[c * ni + r] -= Le[(size_t)r * ni + k] * Z[c * ni + k];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < kb * ni; idx += kThreads) Be[idx] = Z[idx];
+}
+
+// y_e = M_e x_e for dense column-major n x n blocks (the interior
+// inverses A_ii^-1, m_invD of GlobalLinSysStaticCond), `per` elements per
+// workgroup (n <= 256 / per), one row per work-item; HBM-bound on M
+__global__ __launch_bounds__(kThreads) void block_gemv_kernel(int ne, int n, int per, const double *__restrict__ M,
+                                                              const double *__restrict__ x, double *__restrict__ y)
+{
+    extern __shared__ double xs[];  // per * n
+    const int tpe = kThreads / per, slot = threadIdx.x / tpe, lt = threadIdx.x % tpe;
+    const int e = blockIdx.x * per + slot;
+    double *xe = xs + slot * n;
+    if (e < ne)
+        for (int c = lt; c < n; c += tpe) xe[c] = x[(size_t)e * n + c];
+    __syncthreads();
+    if (e >= ne) return;
+    const double *Me = M + (size_t)e * n * n;
+    for (int r = lt; r < n; r += tpe) {
+        double acc = 0.0;
+        for (int c = 0; c < n; ++c) acc = fma(Me[(size_t)c * n + r], xe[c], acc);
+        y[(size_t)e * n + r] = acc;
+    }
+}
+
+// A_e = I (n x n) for ne elements
+__global__ void identity_blocks_kernel(int ne, int n, double *__restrict__ A)
+{
+    const size_t tot = (size_t)ne * n * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = i % ((size_t)n * n);
+        A[i] = (k / n) == (k % n) ? 1.0 : 0.0;
     }
 }
 
@@ -122,11 +203,47 @@ int grid_for(size_t n)
 }  // namespace
 
 hipError_t launch_take_rows(hipStream_t stream, int ne, int e0, int nc, int nrow, const int *rows, int row,
-                            const double *y, double *dst, size_t stride, int col, int ld)
+                            const double *y, double *dst, size_t stride, int col, int ld, double alpha, bool acc)
 {
     if (ne <= 0 || nrow <= 0) return hipSuccess;
     hipLaunchKernelGGL(take_rows_kernel, dim3(grid_for((size_t)ne * nrow)), dim3(kThreads), 0, stream, ne, e0, nc,
-                       nrow, rows, row, y, dst, stride, col, ld);
+                       nrow, rows, row, y, dst, stride, col, ld, alpha, acc ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_put_rows(hipStream_t stream, int ne, int e0, int nc, int nrow, const int *rows, const double *src,
+                           size_t stride, int col, int ld, double alpha, double *y)
+{
+    if (ne <= 0 || nrow <= 0) return hipSuccess;
+    hipLaunchKernelGGL(put_rows_kernel, dim3(grid_for((size_t)ne * nrow)), dim3(kThreads), 0, stream, ne, e0, nc,
+                       nrow, rows, src, stride, col, ld, alpha, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_chol_solve(hipStream_t stream, int ne, int ni, int kb, int kbld, const double *L, double *B)
+{
+    if (ne <= 0 || ni <= 0 || kb <= 0) return hipSuccess;
+    if (ni > kMaxInterior || kb > kbld || kb > kBatch) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(chol_solve_kernel, dim3(ne), dim3(kThreads), (size_t)kb * ni * sizeof(double), stream, ni, kb,
+                       kbld, L, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_gemv(hipStream_t stream, int ne, int n, const double *M, const double *x, double *y)
+{
+    if (ne <= 0 || n <= 0) return hipSuccess;
+    if (n > kMaxInterior) return hipErrorInvalidValue;
+    const int per = n <= 64 ? 4 : n <= 128 ? 2 : 1;
+    hipLaunchKernelGGL(block_gemv_kernel, dim3((ne + per - 1) / per), dim3(kThreads), (size_t)per * n * sizeof(double),
+                       stream, ne, n, per, M, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_identity_blocks(hipStream_t stream, int ne, int n, double *A)
+{
+    if (ne <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(identity_blocks_kernel, dim3(grid_for((size_t)ne * n * n)), dim3(kThreads), 0, stream, ne, n,
+                       A);
     return hipGetLastError();
 }
 

@@ -273,6 +273,13 @@ __global__ void dot_kernel(size_t n, const double *__restrict__ a, const double 
     block_sum<1>(acc, partials + blockIdx.x);
 }
 
+// dst[idx[i]] = src[i]
+__global__ void scatter_index_kernel(int n, const int *__restrict__ idx, const double *__restrict__ src,
+                                     double *__restrict__ dst)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[idx[i]] = src[i];
+}
+
 // glob[g] = sign loc of the first local entry of g (0 for a DOF no element
 // holds): the global values of a continuous local vector
 __global__ void csr_first_kernel(int nglob, const int *__restrict__ row, const int *__restrict__ col,
@@ -1076,6 +1083,13 @@ hipError_t launch_dot(hipStream_t stream, size_t n, int nb, const double *a, con
 {
     if (nb < 1) return hipErrorInvalidValue;
     NMF_LAUNCH(dot_kernel, nb, n, a, b, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_index(hipStream_t stream, int n, const int *idx, const double *src, double *dst)
+{
+    if (n <= 0) return hipSuccess;
+    NMF_LAUNCH(scatter_index_kernel, grid_for((size_t)n), n, idx, src, dst);
     return hipGetLastError();
 }
 

@@ -23,10 +23,19 @@ constexpr int kMaxInterior = 512;   // interior modes per element (hex nm = 10):
 std::vector<int> interior_modes(Shape shape, int nm);
 std::vector<int> boundary_modes(Shape shape, int nm);
 
-// dst[(e - e0) * stride + col * ld + r] = y[e * nc + rows[r]] (rows null:
-// the single row `row`) for ne elements from e0
+// dst[(e - e0) * stride + col * ld + r] = (acc ? dst : 0) + alpha y[e * nc + rows[r]]
+// (rows null: the single row `row`) for ne elements from e0
 hipError_t launch_take_rows(hipStream_t, int ne, int e0, int nc, int nrow, const int *rows, int row, const double *y,
-                            double *dst, size_t stride, int col, int ld);
+                            double *dst, size_t stride, int col, int ld, double alpha = 1.0, bool acc = false);
+// y[(e0 + e) * nc + rows[r]] = alpha src[e * stride + col * ld + r]
+hipError_t launch_put_rows(hipStream_t, int ne, int e0, int nc, int nrow, const int *rows, const double *src,
+                           size_t stride, int col, int ld, double alpha, double *y);
+// B_e <- A_ii,e^-1 B_e with the Cholesky factors L (kb columns of kbld)
+hipError_t launch_chol_solve(hipStream_t, int ne, int ni, int kb, int kbld, const double *L, double *B);
+// y_e = M_e x_e for ne dense column-major n x n blocks (n <= kMaxInterior)
+hipError_t launch_block_gemv(hipStream_t, int ne, int n, const double *M, const double *x, double *y);
+// A_e = I for ne n x n blocks
+hipError_t launch_identity_blocks(hipStream_t, int ne, int n, double *A);
 // in-place Cholesky of ne column-major ni x ni blocks; info[e] = 1 if not
 // positive definite
 hipError_t launch_cholesky(hipStream_t, int ne, int ni, double *A, int *info);

@@ -105,6 +105,8 @@ hipError_t launch_cgloc_update(hipStream_t, size_t n, const CGState *st, double 
 hipError_t launch_dot(hipStream_t, size_t n, int nb, const double *a, const double *b, double *partials);
 hipError_t launch_csr_first(hipStream_t, int nglob, const int *row, const int *col, const double *csign,
                             const double *loc, double *glob);
+// dst[idx[i]] = src[i], i < n
+hipError_t launch_scatter_index(hipStream_t, int n, const int *idx, const double *src, double *dst);
 
 }  // namespace solver
 

@@ -26,9 +26,22 @@ static double dot(int n, const double *a, const double *b)
 }
 
 /* nd[]: the nNonDir non-Dirichlet global ids in ascending order. */
+static int cg_core_mag(int nglob, int nnd, const int *nd, matvec_fn A, void *ctx,
+                       const double *b, const double *inv_diag, double tol,
+                       int maxit, double rhs_mag_set, double *x, or_cg_info *info);
+
 static int cg_core(int nglob, int nnd, const int *nd, matvec_fn A, void *ctx,
                    const double *b, const double *inv_diag, double tol,
                    int maxit, double *x, or_cg_info *info)
+{
+    return cg_core_mag(nglob, nnd, nd, A, ctx, b, inv_diag, tol, maxit, -1.0, x, info);
+}
+
+/* rhs_mag_set > 0: the rhs magnitude the caller set (SetRhsMagnitude,
+ * e.g. GlobalLinSysIterativeStaticCond::v_PreSolve); else Set_Rhs_Magnitude */
+static int cg_core_mag(int nglob, int nnd, const int *nd, matvec_fn A, void *ctx,
+                       const double *b, const double *inv_diag, double tol,
+                       int maxit, double rhs_mag_set, double *x, or_cg_info *info)
 {
     double *w = (double *)calloc(nglob, sizeof(double));
     double *s = (double *)calloc(nglob, sizeof(double));
@@ -43,6 +56,7 @@ static int cg_core(int nglob, int nnd, const int *nd, matvec_fn A, void *ctx,
     double eps = dot(nnd, r, r);
     double rhs_mag = dot(nglob, b, b);
     rhs_mag = rhs_mag > 1.0e-6 ? rhs_mag : 1.0;
+    if (rhs_mag_set > 0.0) rhs_mag = rhs_mag_set;
     int iters = 0;
     if (eps < tol * tol * rhs_mag) goto done;
 
@@ -110,6 +124,18 @@ int or_cg_dense(int n, int ndir, const double *A, const double *b,
     for (int i = ndir; i < n; ++i) nd[i - ndir] = i;
     dense_ctx c = {n, A};
     int it = cg_core(n, n - ndir, nd, dense_mv, &c, b, inv_diag, tol, maxit, x, info);
+    free(nd);
+    return it;
+}
+
+int or_cg_dense_mag(int n, int ndir, const double *A, const double *b,
+                    const double *inv_diag, double tol, int maxit,
+                    double rhs_mag, double *x, or_cg_info *info)
+{
+    int *nd = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    for (int i = ndir; i < n; ++i) nd[i - ndir] = i;
+    dense_ctx c = {n, A};
+    int it = cg_core_mag(n, n - ndir, nd, dense_mv, &c, b, inv_diag, tol, maxit, rhs_mag, x, info);
     free(nd);
     return it;
 }

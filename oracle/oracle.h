@@ -145,6 +145,12 @@ int or_cg_helmholtz_hex(int n, int nm, int nq, int deformed,
 int or_cg_dense(int n, int ndir, const double *A, const double *b,
                 const double *inv_diag, double tol, int maxit, double *x,
                 or_cg_info *info);
+/* The same with the rhs magnitude set by the caller (rhs_mag > 0), as
+ * GlobalLinSysIterativeStaticCond::v_PreSolve sets it from the
+ * uncondensed boundary forcing (GlobalLinSysIterativeStaticCond.cpp:436-492). */
+int or_cg_dense_mag(int n, int ndir, const double *A, const double *b,
+                    const double *inv_diag, double tol, int maxit,
+                    double rhs_mag, double *x, or_cg_info *info);
 /* Element-local CG (NekLinSysIterCGLoc.cpp:114-308): Krylov vectors in
  * the element-local layout, operator = nblk dense element blocks (sizes
  * boff[k+1]-boff[k], row-major, consecutive in Ablk), signed map with the

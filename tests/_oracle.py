@@ -209,6 +209,15 @@ def cg_dense(A, b, ndir=0, inv_diag=None, tol=1e-10, maxit=5000):
     return x, info
 
 
+def cg_dense_mag(A, b, ndir, inv_diag, rhs_mag, tol=1e-10, maxit=5000):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    x = np.zeros(len(b))
+    info = CGInfo()
+    lib().or_cg_dense_mag(len(b), ndir, p(A), p(np.ascontiguousarray(b, dtype=np.float64)), p(inv_diag),
+                          ctypes.c_double(tol), maxit, ctypes.c_double(rhs_mag), p(x), ctypes.byref(info))
+    return x, info
+
+
 def cg_loc(blocks, mp, sign, nglob, ndir, b_loc, inv_diag=None, tol=1e-10, maxit=5000):
     """Element-local CG (or_cg_loc): blocks = list of dense element
     matrices, whose local vectors concatenated in order are mp's layout."""
