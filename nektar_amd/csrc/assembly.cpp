@@ -42,63 +42,56 @@ AssemblyMap::AssemblyMap(const int *map, const double *sign, int64_t nloc, int n
         ++row[map[i] + 1];
     }
     std::partial_sum(row.begin(), row.end(), row.begin());
-    std::vector<int> fill(row.begin(), row.end() - 1);
-    std::vector<double> csign;
-    if (sign) csign.resize(nloc);
+    // signed indices: the sign of an entry folded into its index (~i for -1)
+    std::vector<int> fill(row.begin(), row.end() - 1), menc(std::max<int64_t>(nloc, 1));
     for (int64_t i = 0; i < nloc; ++i) {
+        const bool neg = sign && sign[i] < 0.0;
         const int k = fill[map[i]]++;
-        col[k] = (int)i;
-        if (sign) csign[k] = sign[i];
+        col[k] = neg ? ~(int)i : (int)i;
+        menc[i] = neg ? ~map[i] : map[i];
     }
-    map_.allocate(nloc * sizeof(int));
+    map_.allocate(std::max<int64_t>(nloc, 1) * sizeof(int));
     row_.allocate(row.size() * sizeof(int));
-    col_.allocate(nloc * sizeof(int));
+    col_.allocate(std::max<int64_t>(nloc, 1) * sizeof(int));
     if (nloc) {
-        NMF_HIP(hipMemcpy(map_.get(), map, nloc * sizeof(int), hipMemcpyHostToDevice));
+        NMF_HIP(hipMemcpy(map_.get(), menc.data(), nloc * sizeof(int), hipMemcpyHostToDevice));
         NMF_HIP(hipMemcpy(col_.get(), col.data(), nloc * sizeof(int), hipMemcpyHostToDevice));
     }
     NMF_HIP(hipMemcpy(row_.get(), row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (sign) {
-        std::vector<double> s(sign, sign + nloc);
-        sign_.upload(s, false);
-        csign_.upload(csign, false);
-    }
 }
 
 void AssemblyMap::global_to_local(const double *glob, double *loc, hipStream_t s) const
 {
     if (nloc_ == 0) return;
-    NMF_HIP(solver::launch_gather_signed(s, (size_t)nloc_, map_.as<int>(), has_sign_ ? sign_.as<double>() : nullptr,
-                                         glob, loc));
+    NMF_HIP(solver::launch_gather_signed(s, (size_t)nloc_, map_.as<int>(), glob, loc));
 }
 
 void AssemblyMap::assemble(const double *loc, double *glob, hipStream_t s, bool zero_dir, const double *wdot,
                            double *partials, bool sq) const
 {
     if (nglobal_ == 0) return;
-    NMF_HIP(solver::launch_assemble_csr(s, nglobal_, ndir_, row_.as<int>(), col_.as<int>(),
-                                        has_sign_ ? csign_.as<double>() : nullptr, loc, glob, wdot, partials, zero_dir,
-                                        sq));
+    NMF_HIP(solver::launch_assemble_csr(s, nglobal_, ndir_, row_.as<int>(), col_.as<int>(), loc, glob, wdot, partials,
+                                        zero_dir, sq));
 }
 
 void AssemblyMap::precon_local(bool init, const double *r, const double *inv_diag, double *wg, double *partials,
                                hipStream_t s) const
 {
-    NMF_HIP(solver::launch_cgloc_precon(s, init, nglobal_, ndir_, row_.as<int>(), col_.as<int>(),
-                                        has_sign_ ? csign_.as<double>() : nullptr, r, inv_diag, wg, partials));
+    NMF_HIP(solver::launch_cgloc_precon(s, init, nglobal_, ndir_, row_.as<int>(), col_.as<int>(), r, inv_diag, wg,
+                                        partials));
 }
 
 void AssemblyMap::global_values(const double *loc, double *glob, hipStream_t s) const
 {
     if (nglobal_ == 0) return;
-    NMF_HIP(solver::launch_csr_first(s, nglobal_, row_.as<int>(), col_.as<int>(),
-                                     has_sign_ ? csign_.as<double>() : nullptr, loc, glob));
+    NMF_HIP(solver::launch_csr_first(s, nglobal_, row_.as<int>(), col_.as<int>(), loc, glob));
 }
 
 std::vector<int> AssemblyMap::host_map() const
 {
     std::vector<int> h((size_t)nloc_);
     if (nloc_) NMF_HIP(hipMemcpy(h.data(), map_.get(), (size_t)nloc_ * sizeof(int), hipMemcpyDeviceToHost));
+    for (int &m : h) m = m < 0 ? ~m : m;
     return h;
 }
 
@@ -106,8 +99,10 @@ std::vector<double> AssemblyMap::host_sign() const
 {
     std::vector<double> h;
     if (!has_sign_ || nloc_ == 0) return h;
+    std::vector<int> m((size_t)nloc_);
+    NMF_HIP(hipMemcpy(m.data(), map_.get(), (size_t)nloc_ * sizeof(int), hipMemcpyDeviceToHost));
     h.resize((size_t)nloc_);
-    NMF_HIP(hipMemcpy(h.data(), sign_.get(), (size_t)nloc_ * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < m.size(); ++i) h[i] = m[i] < 0 ? -1.0 : 1.0;
     return h;
 }
 

@@ -59,7 +59,9 @@ private:
     int64_t nloc_;
     int nglobal_, ndir_;
     bool has_sign_;
-    DeviceBuffer map_, sign_, row_, col_, csign_;
+    // signed indices (entry m >= 0: index m, sign +1; m < 0: index ~m, sign -1):
+    // map_ local -> global, col_ the CSR columns of the transpose
+    DeviceBuffer map_, row_, col_;
 };
 
 // The reference's solver choices for this system: GlobalSysSoln

@@ -126,13 +126,19 @@ __device__ __forceinline__ void metric_from(const double M[3][3], double &J, dou
 // ---- general C0 assembly (AssemblyMapCG.cpp:2804-2897) for any mesh:
 // the caller's local-to-global map with signs; Dirichlet DOFs first.
 
+// Signed indices: an entry m >= 0 is index m with sign +1, m < 0 is index
+// ~m with sign -1 (the orientation sign of AssemblyMapCG folded into the
+// 32-bit index, so the gathers read no separate sign array).
+__device__ __forceinline__ int sidx(int m) { return m < 0 ? ~m : m; }
+__device__ __forceinline__ double sapply(int m, double v) { return m < 0 ? -v : v; }
+
 // loc[i] = sign[i] glob[map[i]]   (GlobalToLocal, Vmath::Gathr with signs)
-__global__ void gather_signed_kernel(size_t nloc, const int *__restrict__ map, const double *__restrict__ sign,
-                                     const double *__restrict__ glob, double *__restrict__ loc)
+__global__ void gather_signed_kernel(size_t nloc, const int *__restrict__ menc, const double *__restrict__ glob,
+                                     double *__restrict__ loc)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nloc; i += (size_t)gridDim.x * blockDim.x) {
-        const double g = glob[map[i]];
-        loc[i] = sign ? sign[i] * g : g;
+        const int m = menc[i];
+        loc[i] = sapply(m, glob[sidx(m)]);
     }
 }
 
@@ -141,18 +147,19 @@ __global__ void gather_signed_kernel(size_t nloc, const int *__restrict__ map, c
 // so every global value is one work-item's fixed-order sum.  Rows g < ndir
 // are set to zero when zero_dir; with wdot, the partial sums of glob.wdot
 // (the CG's s.w) per block.  sq: sum sign^2 loc instead (diagonals).
-__global__ void assemble_csr_kernel(int nglob, int ndir, const int *__restrict__ row, const int *__restrict__ col,
-                                    const double *__restrict__ csign, const double *__restrict__ loc,
-                                    double *__restrict__ glob, const double *__restrict__ wdot,
-                                    double *__restrict__ partials, int zero_dir, int sq)
+__global__ void assemble_csr_kernel(int nglob, int ndir, const int *__restrict__ row, const int *__restrict__ cenc,
+                                    const double *__restrict__ loc, double *__restrict__ glob,
+                                    const double *__restrict__ wdot, double *__restrict__ partials, int zero_dir,
+                                    int sq)
 {
     double acc[1] = {0.0};
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x) {
         double v = 0.0;
         if (!(zero_dir && g < ndir)) {
             for (int k = row[g]; k < row[g + 1]; ++k) {
-                const double sg = csign ? csign[k] : 1.0;
-                v = fma(sq ? sg * sg : sg, loc[col[k]], v);
+                const int c = cenc[k];
+                const double lv = loc[sidx(c)];
+                v += sq ? lv : sapply(c, lv);
             }
         }
         glob[g] = v;
@@ -220,15 +227,17 @@ __global__ void cg_init_dirfirst_kernel(int nglob, int ndir, const double *__res
 // NekLinSysIter.cpp:104-109)} in cg_scalar_init's order; else {r.w, eps}
 // in cg_scalar's.
 template <int INIT>
-__global__ void cgloc_precon_kernel(int nglob, int ndir, const int *__restrict__ row, const int *__restrict__ col,
-                                    const double *__restrict__ csign, const double *__restrict__ r,
-                                    const double *__restrict__ inv_diag, double *__restrict__ wg,
-                                    double *__restrict__ partials)
+__global__ void cgloc_precon_kernel(int nglob, int ndir, const int *__restrict__ row, const int *__restrict__ cenc,
+                                    const double *__restrict__ r, const double *__restrict__ inv_diag,
+                                    double *__restrict__ wg, double *__restrict__ partials)
 {
     double acc[INIT ? 3 : 2] = {};
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x) {
         double v = 0.0;
-        for (int k = row[g]; k < row[g + 1]; ++k) v = fma(csign ? csign[k] : 1.0, r[col[k]], v);
+        for (int k = row[g]; k < row[g + 1]; ++k) {
+            const int c = cenc[k];
+            v += sapply(c, r[sidx(c)]);
+        }
         const bool dir = g < ndir;
         const double wv = dir ? 0.0 : inv_diag[g] * v;
         const double e = dir ? 0.0 : v * v;
@@ -282,13 +291,12 @@ __global__ void scatter_index_kernel(int n, const int *__restrict__ idx, const d
 
 // glob[g] = sign loc of the first local entry of g (0 for a DOF no element
 // holds): the global values of a continuous local vector
-__global__ void csr_first_kernel(int nglob, const int *__restrict__ row, const int *__restrict__ col,
-                                 const double *__restrict__ csign, const double *__restrict__ loc,
-                                 double *__restrict__ glob)
+__global__ void csr_first_kernel(int nglob, const int *__restrict__ row, const int *__restrict__ cenc,
+                                 const double *__restrict__ loc, double *__restrict__ glob)
 {
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nglob; g += gridDim.x * blockDim.x) {
         const int k = row[g];
-        glob[g] = k < row[g + 1] ? (csign ? csign[k] : 1.0) * loc[col[k]] : 0.0;
+        glob[g] = k < row[g + 1] ? sapply(cenc[k], loc[sidx(cenc[k])]) : 0.0;
     }
 }
 
@@ -1012,18 +1020,17 @@ template hipError_t launch_cg_init<double>(hipStream_t, int, const double *, con
                                            double *, double *, double *, double *);
 template hipError_t launch_cg_init<float>(hipStream_t, int, const double *, const double *, float *, float *, float *,
                                           float *, float *, double *);
-hipError_t launch_gather_signed(hipStream_t stream, size_t nloc, const int *map, const double *sign,
-                                const double *glob, double *loc)
+hipError_t launch_gather_signed(hipStream_t stream, size_t nloc, const int *menc, const double *glob, double *loc)
 {
-    NMF_LAUNCH(gather_signed_kernel, grid_for(nloc), nloc, map, sign, glob, loc);
+    NMF_LAUNCH(gather_signed_kernel, grid_for(nloc), nloc, menc, glob, loc);
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_csr(hipStream_t stream, int nglob, int ndir, const int *row, const int *col,
-                               const double *csign, const double *loc, double *glob, const double *wdot,
-                               double *partials, bool zero_dir, bool sq)
+hipError_t launch_assemble_csr(hipStream_t stream, int nglob, int ndir, const int *row, const int *cenc,
+                               const double *loc, double *glob, const double *wdot, double *partials, bool zero_dir,
+                               bool sq)
 {
-    NMF_LAUNCH(assemble_csr_kernel, grid_for((size_t)nglob), nglob, ndir, row, col, csign, loc, glob, wdot, partials,
+    NMF_LAUNCH(assemble_csr_kernel, grid_for((size_t)nglob), nglob, ndir, row, cenc, loc, glob, wdot, partials,
                zero_dir ? 1 : 0, sq ? 1 : 0);
     return hipGetLastError();
 }
@@ -1060,15 +1067,14 @@ hipError_t launch_cg_update_n(hipStream_t stream, size_t NG, const CGState *st, 
     return hipGetLastError();
 }
 
-hipError_t launch_cgloc_precon(hipStream_t stream, bool init, int nglob, int ndir, const int *row, const int *col,
-                               const double *csign, const double *r, const double *inv_diag, double *wg,
-                               double *partials)
+hipError_t launch_cgloc_precon(hipStream_t stream, bool init, int nglob, int ndir, const int *row, const int *cenc,
+                               const double *r, const double *inv_diag, double *wg, double *partials)
 {
     const int grid = grid_for((size_t)nglob);
     if (init)
-        NMF_LAUNCH(cgloc_precon_kernel<1>, grid, nglob, ndir, row, col, csign, r, inv_diag, wg, partials);
+        NMF_LAUNCH(cgloc_precon_kernel<1>, grid, nglob, ndir, row, cenc, r, inv_diag, wg, partials);
     else
-        NMF_LAUNCH(cgloc_precon_kernel<0>, grid, nglob, ndir, row, col, csign, r, inv_diag, wg, partials);
+        NMF_LAUNCH(cgloc_precon_kernel<0>, grid, nglob, ndir, row, cenc, r, inv_diag, wg, partials);
     return hipGetLastError();
 }
 
@@ -1093,10 +1099,10 @@ hipError_t launch_scatter_index(hipStream_t stream, int n, const int *idx, const
     return hipGetLastError();
 }
 
-hipError_t launch_csr_first(hipStream_t stream, int nglob, const int *row, const int *col, const double *csign,
-                            const double *loc, double *glob)
+hipError_t launch_csr_first(hipStream_t stream, int nglob, const int *row, const int *cenc, const double *loc,
+                            double *glob)
 {
-    NMF_LAUNCH(csr_first_kernel, grid_for((size_t)nglob), nglob, row, col, csign, loc, glob);
+    NMF_LAUNCH(csr_first_kernel, grid_for((size_t)nglob), nglob, row, cenc, loc, glob);
     return hipGetLastError();
 }
 

@@ -81,11 +81,11 @@ hipError_t launch_cast(hipStream_t, size_t n, const double *a, float *b);
 
 // General C0 assembly with the caller's map and signs, Dirichlet DOFs
 // first (assembly.hpp); reductions use reduction_blocks(nglob) blocks.
-hipError_t launch_gather_signed(hipStream_t, size_t nloc, const int *map, const double *sign, const double *glob,
-                                double *loc);
-hipError_t launch_assemble_csr(hipStream_t, int nglob, int ndir, const int *row, const int *col, const double *csign,
-                               const double *loc, double *glob, const double *wdot, double *partials, bool zero_dir,
-                               bool sq);
+// Indices are signed: m >= 0 is index m with sign +1, m < 0 index ~m with
+// sign -1 (menc: local -> global, cenc: the CSR columns, global -> local).
+hipError_t launch_gather_signed(hipStream_t, size_t nloc, const int *menc, const double *glob, double *loc);
+hipError_t launch_assemble_csr(hipStream_t, int nglob, int ndir, const int *row, const int *cenc, const double *loc,
+                               double *glob, const double *wdot, double *partials, bool zero_dir, bool sq);
 hipError_t launch_unit_mode(hipStream_t, size_t n, int nc, int m, double *x);
 hipError_t launch_take_mode(hipStream_t, size_t nelmt, int nc, int m, const double *y, double *d);
 hipError_t launch_invert_diag_dir(hipStream_t, int nglob, int ndir, double *d);
@@ -97,14 +97,13 @@ hipError_t launch_cg_update_n(hipStream_t, size_t NG, const CGState *st, double 
 // dots (reduction_blocks(nglob) blocks; init: {eps, r.w, b.b}, else
 // {r.w, eps}), local vector update, local dot over nb blocks, and the
 // global values of a continuous local vector
-hipError_t launch_cgloc_precon(hipStream_t, bool init, int nglob, int ndir, const int *row, const int *col,
-                               const double *csign, const double *r, const double *inv_diag, double *wg,
-                               double *partials);
+hipError_t launch_cgloc_precon(hipStream_t, bool init, int nglob, int ndir, const int *row, const int *cenc,
+                               const double *r, const double *inv_diag, double *wg, double *partials);
 hipError_t launch_cgloc_update(hipStream_t, size_t n, const CGState *st, double *p, double *q, double *x, double *r,
                                const double *w, const double *s);
 hipError_t launch_dot(hipStream_t, size_t n, int nb, const double *a, const double *b, double *partials);
-hipError_t launch_csr_first(hipStream_t, int nglob, const int *row, const int *col, const double *csign,
-                            const double *loc, double *glob);
+hipError_t launch_csr_first(hipStream_t, int nglob, const int *row, const int *cenc, const double *loc,
+                            double *glob);
 // dst[idx[i]] = src[i], i < n
 hipError_t launch_scatter_index(hipStream_t, int n, const int *idx, const double *src, double *dst);
 
