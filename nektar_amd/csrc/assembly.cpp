@@ -66,6 +66,13 @@ void AssemblyMap::global_to_local(const double *glob, double *loc, hipStream_t s
     NMF_HIP(solver::launch_gather_signed(s, (size_t)nloc_, map_.as<int>(), glob, loc));
 }
 
+void AssemblyMap::global_to_local_range(int64_t off, int64_t n, const double *glob, double *loc, hipStream_t s) const
+{
+    NMF_REQUIRE(off >= 0 && n >= 0 && off + n <= nloc_, NMF_ERR_INVALID, "gather range outside the map");
+    if (n == 0) return;
+    NMF_HIP(solver::launch_gather_signed(s, (size_t)n, map_.as<int>() + off, glob, loc));
+}
+
 void AssemblyMap::assemble(const double *loc, double *glob, hipStream_t s, bool zero_dir, const double *wdot,
                            double *partials, bool sq) const
 {
@@ -152,6 +159,20 @@ void IterativeSolver::apply_local(const double *xloc, double *yloc)
 {
     for (size_t k = 0; k < ops_.size(); ++k)
         ops_[k]->apply(xloc + off_[k], yloc + off_[k], nullptr, nullptr, stream_);
+}
+
+void IterativeSolver::apply_from_global(const double *xg, double *yloc)
+{
+    for (size_t k = 0; k < ops_.size(); ++k) {
+        MFOperator &o = *ops_[k];
+        if (o.can_apply_mapped()) {
+            o.apply_mapped(xg, map_.signed_map() + off_[k], yloc + off_[k], stream_);
+        } else {
+            const int64_t n = (int64_t)o.nelmt * o.ncoeff;
+            map_.global_to_local_range(off_[k], n, xg, xloc_.as<double>() + off_[k], stream_);
+            o.apply(xloc_.as<double>() + off_[k], yloc + off_[k], nullptr, nullptr, stream_);
+        }
+    }
 }
 
 // PreconditionerDiagonal.cpp:93-125: the diagonal of every element matrix,
@@ -306,8 +327,7 @@ void IterativeSolver::solution_local(double *xloc)
 
 void IterativeSolver::apply_global(const double *x, double *y)
 {
-    map_.global_to_local(x, xloc_.as<double>(), stream_);
-    apply_local(xloc_.as<double>(), yloc_.as<double>());
+    apply_from_global(x, yloc_.as<double>());
     map_.assemble(yloc_.as<double>(), y, stream_, true);
 }
 
@@ -360,8 +380,7 @@ SolveInfo IterativeSolver::solve(double tol, int maxit)
         } else {
             NMF_HIP(solver::launch_cg_init_dirfirst(stream_, NG, ND, b_.as<double>(), invd_.as<double>(), x, r, p, q,
                                                     w, pa_.as<double>()));
-            map_.global_to_local(w, xloc_.as<double>(), stream_);
-            apply_local(xloc_.as<double>(), yloc_.as<double>());
+            apply_from_global(w, yloc_.as<double>());
             map_.assemble(yloc_.as<double>(), s, stream_, true, w, pmu_.as<double>());
         }
         NMF_HIP(solver::launch_cg_scalar_init(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
@@ -381,8 +400,7 @@ SolveInfo IterativeSolver::solve(double tol, int maxit)
                 } else {
                     NMF_HIP(solver::launch_cg_update_n(stream_, (size_t)NG, st, p, q, x, r, w, s, invd_.as<double>(),
                                                        pa_.as<double>()));
-                    map_.global_to_local(w, xloc_.as<double>(), stream_);
-                    apply_local(xloc_.as<double>(), yloc_.as<double>());
+                    apply_from_global(w, yloc_.as<double>());
                     map_.assemble(yloc_.as<double>(), s, stream_, true, w, pmu_.as<double>());
                 }
                 NMF_HIP(solver::launch_cg_scalar(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));

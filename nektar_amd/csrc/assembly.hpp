@@ -31,6 +31,10 @@ public:
 
     // loc[i] = sign[i] glob[map[i]]
     void global_to_local(const double *glob, double *loc, hipStream_t s) const;
+    // the same for the local entries [off, off + n)
+    void global_to_local_range(int64_t off, int64_t n, const double *glob, double *loc, hipStream_t s) const;
+    // the signed map on the device (entry m >= 0: global m, sign +1; m < 0: ~m, sign -1)
+    const int *signed_map() const { return map_.as<int>(); }
     // glob[g] = sum_{map[i] = g} sign[i] loc[i], in increasing i; Dirichlet
     // rows zeroed when zero_dir; partial sums of glob.wdot into partials
     // (reduction_blocks(nglobal) blocks) when wdot is given; sq: sum of
@@ -116,6 +120,9 @@ public:
 
 private:
     void apply_local(const double *xloc, double *yloc);
+    // yloc = A GlobalToLocal(xg): hex collections gather inside the operator
+    // kernel, the others through the signed gather into xloc_
+    void apply_from_global(const double *xg, double *yloc);
     void build_diagonal();
     void setup_static_cond();
     SolveInfo solve_static_cond(double tol, int maxit);

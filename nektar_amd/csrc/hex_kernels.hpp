@@ -100,6 +100,22 @@ struct StructuredFetch {
     }
 };
 
+// Coefficient fetch through a signed local-to-global map (GlobalToLocal
+// with orientation signs fused into the operator, AssemblyMapCG.cpp:
+// 2804-2827): the element's entries me[(r*NM + q)*NM + p] hold the global
+// index m (sign +1) or ~m (sign -1).
+template <typename T, int NM>
+struct MappedFetch {
+    const T *__restrict__ g;
+    const int *__restrict__ me;
+    __device__ __forceinline__ T operator()(int p, int q, int r) const
+    {
+        const int m = me[(r * NM + q) * NM + p];
+        const T v = g[m < 0 ? ~m : m];
+        return m < 0 ? -v : v;
+    }
+};
+
 // ---- forward transform: coefficients -> u column (k) of item (i,j) in
 //      registers.
 template <typename T, int NM, int NQ, typename Fetch>
@@ -572,6 +588,26 @@ __global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_kernel(int nelmt, con
     const bool valid = e < nelmt;
     helmholtz_element<T, NM, NQ, DEF>(tab, LocalFetch<T, NM>{in + (size_t)e * NM * NM * NM}, valid, e, t, geo,
                                       lambda, lds[le][0], lds[le][1], out + (size_t)e * NM * NM * NM);
+}
+
+// Solver form on a general mesh: x gathered through the signed map from
+// the global vector, element results written element-major
+template <typename T, int NM, int NQ, bool DEF, int EPB>
+__global__ __launch_bounds__(EPB * NQ * NQ) void helmholtz_mapped_kernel(int nelmt, const T *__restrict__ tabp,
+                                                                         const T *__restrict__ geo, T lambda,
+                                                                         const T *__restrict__ xg,
+                                                                         const int *__restrict__ menc,
+                                                                         T *__restrict__ out)
+{
+    constexpr int TPE = NQ * NQ;
+    __shared__ T lds[EPB][2][slab<NQ>()];
+    const Tab<T, NM, NQ> tab{tabp};
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * EPB + le;
+    const bool valid = e < nelmt;
+    const MappedFetch<T, NM> f{xg, menc + (size_t)(valid ? e : 0) * NM * NM * NM};
+    helmholtz_element<T, NM, NQ, DEF>(tab, f, valid, e, t, geo, lambda, lds[le][0], lds[le][1],
+                                      out + (size_t)e * NM * NM * NM);
 }
 
 // Grouped form: several operators of the same kind in one launch; block b

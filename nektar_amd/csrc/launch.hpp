@@ -31,6 +31,7 @@ struct KernelArgs {
     int tile = 0;  // Helmholtz: elements per workgroup (0 = default)
     const void *in1 = nullptr, *in2 = nullptr;  // IProductDeriv: inputs 1, 2 (in = input 0)
     const void *adv = nullptr;  // AdvDiffReact: advection coefficients per point
+    const int *menc = nullptr;  // hex Helmholtz: signed local-to-global map; `in` is then the global vector
 };
 
 // Order-templated kernels of the structured-hex C0 solver.

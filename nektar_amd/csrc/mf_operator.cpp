@@ -402,6 +402,17 @@ void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipSt
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
 }
 
+void MFOperator::apply_mapped(const void *xg, const int *menc, void *out, hipStream_t s)
+{
+    NMF_REQUIRE(can_apply_mapped(), NMF_ERR_INVALID, "mapped apply: FP64 hex Helmholtz operators only");
+    NMF_REQUIRE(nelmt == 0 || (xg && menc && out), NMF_ERR_INVALID, "null input/output pointer");
+    ensure_geometry();
+    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, xg, out, nullptr, nullptr, s, tile};
+    a.menc = menc;
+    const hipError_t e = launch_hex(nm, nq[0], a);
+    NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+}
+
 TuneKey MFOperator::tune_key() const
 {
     return TuneKey{(int)shape, (int)op, nm, nq[0], fp32 ? 1 : 0, deformed ? 1 : 0, 0};

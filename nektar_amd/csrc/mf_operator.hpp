@@ -75,6 +75,10 @@ struct MFOperator {
     void set_advection(const double *v);
 
     void apply(const void *in, void *out0, void *out1, void *out2, hipStream_t s);
+    // FP64 hex Helmholtz only: out = A (GlobalToLocal of the global vector xg
+    // through the signed map menc, this collection's nelmt * ncoeff entries)
+    void apply_mapped(const void *xg, const int *menc, void *out, hipStream_t s);
+    bool can_apply_mapped() const { return shape == Shape::Hex && op == OpType::Helmholtz && !fp32; }
     // IProductWRTDerivBase: three inputs at the quadrature points, one output
     void apply_iproduct_deriv(const void *in0, const void *in1, const void *in2, void *out, hipStream_t s);
 
