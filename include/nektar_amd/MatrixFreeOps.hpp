@@ -109,6 +109,8 @@ public:
 
     int GetInputSize() const { return (int)InputSize(); }
     int GetOutputSize() const { return (int)OutputSize(); }
+    // the C-ABI handle (for nmf_linsys_* and the MultiRegions mirror)
+    nmf_operator GetHandle() const { return m_op; }
 
 protected:
     size_t GeomPoints() const { return (size_t)m_nElmt * (m_deformed ? m_nqtot : 1); }
