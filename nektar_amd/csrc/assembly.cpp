@@ -202,7 +202,7 @@ void IterativeSolver::build_diagonal()
 // The element matrices' columns come from operator applies to unit modes
 // (every element of a collection at once); per element chunk: A_ii from the
 // interior columns, its Cholesky factor, then the boundary columns in
-// batches of schur::kBatch, each giving S_kk = A_kk - |L^-1 a_k|^2 in the
+// batches of schur::batch_columns(ni), each giving S_kk = A_kk - |L^-1 a_k|^2 in the
 // element-local layout; assembled with the Dirichlet rows zeroed.
 void IterativeSolver::sc_diagonal(double *dglob)
 {
@@ -211,20 +211,21 @@ void IterativeSolver::sc_diagonal(double *dglob)
     size_t kBudget = (size_t)8 << 30;
     if (const char *v = std::getenv("NMF_SC_CHUNK_BYTES"))
         if (std::strtoull(v, nullptr, 10) > 0) kBudget = std::strtoull(v, nullptr, 10);
-    const int KB = schur::kBatch;
     const std::vector<int> hmap = map_.host_map();
     std::vector<int> cnt(std::max(map_.nglobal(), 1), 0);
     for (int g : hmap) ++cnt[g];
+    std::vector<int> stamp(std::max(map_.nglobal(), 1), 0);
+    int stamp_id = 0;
     DeviceBuffer sloc(std::max<int64_t>(map_.nloc(), 1) * sizeof(double));
     NMF_HIP(solver::launch_zero<double>(stream_, (size_t)std::max<int64_t>(map_.nloc(), 1), sloc.as<double>()));
     for (size_t k = 0; k < ops_.size(); ++k) {
         MFOperator &o = *ops_[k];
         const std::vector<int> I = schur::interior_modes(o.shape, o.nm), Bd = schur::boundary_modes(o.shape, o.nm);
         const int E = o.nelmt, nc = o.ncoeff, ni = (int)I.size(), nb = (int)Bd.size();
+        const int KB = schur::batch_columns(ni);
         NMF_REQUIRE(ni <= schur::kMaxInterior, NMF_ERR_UNSUPPORTED,
                     "static-condensation diagonal: more than " + std::to_string(schur::kMaxInterior) +
                         " interior modes per element");
-        std::vector<int> gb(nb);
         for (int e = 0; e < E; ++e) {
             const int *me = hmap.data() + off_[k] + (size_t)e * nc;
             for (int m : I)
@@ -232,11 +233,15 @@ void IterativeSolver::sc_diagonal(double *dglob)
                             "static-condensation diagonal: interior mode " + std::to_string(m) + " of element " +
                                 std::to_string(e) + " of collection " + std::to_string(k) +
                                 " shares its global DOF");
-            for (int j = 0; j < nb; ++j) gb[j] = me[Bd[j]];
-            std::sort(gb.begin(), gb.end());
-            NMF_REQUIRE(std::adjacent_find(gb.begin(), gb.end()) == gb.end(), NMF_ERR_UNSUPPORTED,
-                        "static-condensation diagonal: two boundary modes of element " + std::to_string(e) +
-                            " map to one global DOF (periodic single-element meshes)");
+            // distinct boundary DOFs: stamp each with the element's running number
+            ++stamp_id;
+            for (int j = 0; j < nb; ++j) {
+                int &st = stamp[me[Bd[j]]];
+                NMF_REQUIRE(st != stamp_id, NMF_ERR_UNSUPPORTED,
+                            "static-condensation diagonal: two boundary modes of element " + std::to_string(e) +
+                                " map to one global DOF (periodic single-element meshes)");
+                st = stamp_id;
+            }
         }
         if (E == 0 || nb == 0) continue;
         DeviceBuffer dI(std::max(ni, 1) * sizeof(int)), dB(nb * sizeof(int));

@@ -223,7 +223,8 @@ hipError_t launch_put_rows(hipStream_t stream, int ne, int e0, int nc, int nrow,
 hipError_t launch_chol_solve(hipStream_t stream, int ne, int ni, int kb, int kbld, const double *L, double *B)
 {
     if (ne <= 0 || ni <= 0 || kb <= 0) return hipSuccess;
-    if (ni > kMaxInterior || kb > kbld || kb > kBatch) return hipErrorInvalidValue;
+    if (ni > kMaxInterior || kb > kbld || kb > kMaxBatch || (size_t)kb * ni * sizeof(double) > kLdsBytes)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(chol_solve_kernel, dim3(ne), dim3(kThreads), (size_t)kb * ni * sizeof(double), stream, ni, kb,
                        kbld, L, B);
     return hipGetLastError();
@@ -260,7 +261,8 @@ hipError_t launch_schur_diag(hipStream_t stream, int ne, int ni, int kb, int kbl
                              const double *dkk, const int *bmode, int nc, int e0, double *sloc)
 {
     if (ne <= 0 || kb <= 0) return hipSuccess;
-    if (ni > kMaxInterior || kb > kbld || kbld > kBatch) return hipErrorInvalidValue;
+    if (ni > kMaxInterior || kb > kbld || kbld > kMaxBatch || (size_t)kb * ni * sizeof(double) > kLdsBytes)
+        return hipErrorInvalidValue;
     const size_t lds = (size_t)kb * ni * sizeof(double);
     hipLaunchKernelGGL(schur_diag_kernel, dim3(ne), dim3(kThreads), lds, stream, ni, kb, kbld, L, Ab, dkk, bmode, nc,
                        e0, sloc);

@@ -14,7 +14,19 @@ namespace nektar_amd {
 namespace schur {
 
 constexpr int kPanel = 16;          // columns per LDS panel of the Cholesky factorisation
-constexpr int kBatch = 16;          // boundary columns per forward-substitution pass
+constexpr int kMaxBatch = 16;       // columns per substitution pass, at most
+constexpr int kLdsBytes = 65536;    // LDS of one substitution pass: columns * ni doubles
+
+// columns per substitution pass for ni interior modes.  The passes are
+// latency-bound (one barrier per row); 16 columns (16 ni doubles of LDS)
+// keep more elements resident per CU than 64 columns, which measured
+// slower (hex P=6: 167 against 127 ms for the Schur diagonal, 92 against
+// 77 ms for the interior inverses)
+inline int batch_columns(int ni)
+{
+    const int b = ni > 0 ? kLdsBytes / (8 * ni) : kMaxBatch;
+    return b < 1 ? 1 : (b > kMaxBatch ? kMaxBatch : b);
+}
 constexpr int kMaxInterior = 512;   // interior modes per element (hex nm = 10): 64 KiB of LDS per pass
 
 // the interior modes of a shape's coefficient order: hex p, q, r >= 2 (p

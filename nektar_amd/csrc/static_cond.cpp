@@ -42,7 +42,7 @@ void IterativeSolver::setup_static_cond()
 {
     sc_ = std::make_unique<StaticCondData>();
     StaticCondData &d = *sc_;
-    const int NG = map_.nglobal(), ND = map_.ndir(), KB = schur::kBatch;
+    const int NG = map_.nglobal(), ND = map_.ndir();
     const std::vector<int> hmap = map_.host_map();
     const std::vector<double> hsign = map_.host_sign();
     std::vector<int> cnt(std::max(NG, 1), 0);
@@ -144,7 +144,7 @@ void IterativeSolver::setup_static_cond()
     for (size_t k = 0; k < d.blk.size(); ++k) {
         StaticCondBlock &b = d.blk[k];
         MFOperator &o = *ops_[k];
-        const int E = b.E, nc = b.nc, ni = b.ni, nb = b.nb;
+        const int E = b.E, nc = b.nc, ni = b.ni, nb = b.nb, KB = schur::batch_columns(ni);
         dbl(b.invD, (size_t)E * ni * ni);
         if (E == 0) continue;
         const size_t n = (size_t)E * nc;

@@ -89,9 +89,10 @@ def test_chunks_and_mixed_collections(nmf, monkeypatch):
     ls = nmf.LinearSystem(amap, ops_h + ops_t)
     d_one = ls.sc_diagonal()
     assert _rel(d_one, ref) <= 1e-12
-    # element chunks: an element's blocks hold ni^2 + 16 ni + 16 doubles
-    # (hex nm=6: ni = 64; tet nm=5: ni = 1); chunks of 3 hexes, then of 3
-    # tets (and single hexes): the same values bit for bit
+    # element chunks: an element's blocks hold ni^2 + kb ni + kb doubles,
+    # kb = min(16, 8192 // ni) columns per pass (hex nm=6: ni = 64; tet
+    # nm=5: ni = 1); chunks of 3 hexes, then of 3 tets (and single hexes):
+    # the same values bit for bit
     for per in (64 * 64 + 64 * 16 + 16, 1 + 16 + 16):
         monkeypatch.setenv("NMF_SC_CHUNK_BYTES", str(3 * per * 8))
         assert np.array_equal(ls.sc_diagonal(), d_one)
