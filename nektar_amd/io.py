@@ -1,0 +1,367 @@
+"""Nektar++ file formats for the solve path (host side).
+
+* ``read_mesh_xml``: the session/mesh XML geometry (MeshGraphIOXml.cpp,
+  MeshGraphIOXmlCompressed.cpp): VERTEX, EDGE, FACE (Q, T), ELEMENT (H, A),
+  COMPOSITE, DOMAIN and EXPANSIONS, in plain text or in the
+  ``B64Z-LittleEndian`` form (zlib-compressed, base64-encoded arrays of the
+  int64/double records of SpatialDomains/MeshEntities.hpp:51-98).
+* ``hex_local_vertices``: a hexahedron's local vertex order from its six
+  faces, as HexGeom derives it (HexGeom.cpp:200-430); this order defines the
+  element's reference map and so the meaning of its coefficients.
+* ``hex_geometry``: Jacobian and derivative factors of straight-sided
+  (parallelepiped) hexahedra, one set per element (GeomFactors.cpp:150-477
+  for the regular case).
+* ``hex_c0_map``: the C0 local-to-global map with orientation signs of a
+  conforming hex mesh (AssemblyMapCG's numbering role), from the elements'
+  global vertex ids: vertex, edge, face and interior modes; edge and face
+  modes are matched through a canonical frame per edge and face.
+* ``read_fld`` / ``write_fld``: field files (FieldIOXml.cpp): ELEMENTS blocks
+  with their FIELDS, SHAPE, BASIS, NUMMODESPERDIR and ID attributes and the
+  coefficients field by field, element by element, in the ID order.
+"""
+from __future__ import annotations
+
+import base64
+import re
+import xml.etree.ElementTree as ET
+import zlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_B64Z = "B64Z-LittleEndian"
+
+
+def _b64z_decode(text: str, dtype) -> np.ndarray:
+    return np.frombuffer(zlib.decompress(base64.b64decode(text.strip())), dtype=dtype)
+
+
+def _b64z_encode(a: np.ndarray) -> str:
+    return base64.b64encode(zlib.compress(np.ascontiguousarray(a).tobytes())).decode("ascii")
+
+
+def parse_ranges(s: str) -> list[int]:
+    """'0-3,7' -> [0, 1, 2, 3, 7] (ParseUtils::GenerateSeqVector)."""
+    out: list[int] = []
+    for part in s.replace(" ", "").split(","):
+        if not part:
+            continue
+        if "-" in part:
+            a, b = part.split("-")
+            out.extend(range(int(a), int(b) + 1))
+        else:
+            out.append(int(part))
+    return out
+
+
+def _ranges_string(ids) -> str:
+    ids = list(ids)
+    parts, i = [], 0
+    while i < len(ids):
+        j = i
+        while j + 1 < len(ids) and ids[j + 1] == ids[j] + 1:
+            j += 1
+        parts.append(str(ids[i]) if i == j else f"{ids[i]}-{ids[j]}")
+        i = j + 1
+    return ",".join(parts)
+
+
+@dataclass
+class MeshGraph:
+    vertices: dict = field(default_factory=dict)    # id -> xyz (3,)
+    edges: dict = field(default_factory=dict)       # id -> (v0, v1)
+    faces: dict = field(default_factory=dict)       # id -> ('Q' | 'T', [edge ids])
+    elements: dict = field(default_factory=dict)    # id -> ('H' | 'A', [face ids])
+    composites: dict = field(default_factory=dict)  # id -> [(letter, [ids])]
+    domain: list = field(default_factory=list)      # composite ids
+    expansions: list = field(default_factory=list)  # [{composites, nummodes, fields, type}]
+
+
+def _composite_refs(text: str):
+    """' H[0-7] F[1,2] ' -> [('H', [0..7]), ('F', [1, 2])]; 'C[0]' likewise."""
+    return [(m.group(1), parse_ranges(m.group(2))) for m in re.finditer(r"([A-Z])\[([^\]]*)\]", text)]
+
+
+def read_mesh_xml(path: str) -> MeshGraph:
+    root = ET.parse(path).getroot()
+    geo = root.find("GEOMETRY")
+    if geo is None:
+        raise ValueError(f"{path}: no GEOMETRY element")
+    g = MeshGraph()
+
+    def records(node, tag, width, dtype="<i8"):
+        """(id, values...) rows of a plain or compressed entity list."""
+        if node.get("COMPRESSED"):
+            if node.get("COMPRESSED") != _B64Z or node.get("BITSIZE", "64") != "64":
+                raise ValueError(f"unsupported compression {node.get('COMPRESSED')}/{node.get('BITSIZE')}")
+            return _b64z_decode(node.text, dtype).reshape(-1, width)
+        rows = []
+        for child in node.findall(tag):
+            rows.append([int(child.get("ID"))] + [float(x) for x in child.text.split()])
+        return np.array(rows).reshape(-1, width)
+
+    vnode = geo.find("VERTEX")
+    if vnode.get("COMPRESSED"):
+        v = _b64z_decode(vnode.text, np.dtype([("id", "<i8"), ("x", "<f8"), ("y", "<f8"), ("z", "<f8")]))
+        for r in v:
+            g.vertices[int(r["id"])] = np.array([r["x"], r["y"], r["z"]])
+    else:
+        for child in vnode.findall("V"):
+            g.vertices[int(child.get("ID"))] = np.array([float(x) for x in child.text.split()])
+    for r in records(geo.find("EDGE"), "E", 3):
+        g.edges[int(r[0])] = (int(r[1]), int(r[2]))
+    fnode = geo.find("FACE")
+    for tag, n in (("Q", 4), ("T", 3)):
+        for node in fnode.findall(tag) if fnode is not None else []:
+            rows = records(node, tag, n + 1) if node.get("COMPRESSED") else [
+                [int(node.get("ID"))] + [int(x) for x in node.text.split()]]
+            for r in rows:
+                g.faces[int(r[0])] = (tag, [int(x) for x in r[1:]])
+    enode = geo.find("ELEMENT")
+    for tag, n in (("H", 6), ("A", 4)):
+        for node in enode.findall(tag):
+            rows = records(node, tag, n + 1) if node.get("COMPRESSED") else [
+                [int(node.get("ID"))] + [int(x) for x in node.text.split()]]
+            for r in rows:
+                g.elements[int(r[0])] = (tag, [int(x) for x in r[1:]])
+    for tag in ("P", "R", "S"):  # prisms, pyramids, ... are not part of this path
+        if enode.find(tag) is not None:
+            raise NotImplementedError(f"element type {tag} is not supported")
+    for c in geo.find("COMPOSITE").findall("C"):
+        g.composites[int(c.get("ID"))] = _composite_refs(c.text)
+    dom = geo.find("DOMAIN")
+    if dom is not None:
+        g.domain = [i for _, ids in _composite_refs(dom.text) for i in ids]
+    exps = root.find("EXPANSIONS")
+    for e in exps.findall("E") if exps is not None else []:
+        g.expansions.append({"composites": [i for _, ids in _composite_refs(e.get("COMPOSITE")) for i in ids],
+                             "nummodes": int(e.get("NUMMODES")), "type": e.get("TYPE", "MODIFIED"),
+                             "fields": [f.strip() for f in e.get("FIELDS", "u").split(",")]})
+    return g
+
+
+def _shared_edge(mesh: MeshGraph, fa: int, fb: int) -> int:
+    common = set(mesh.faces[fa][1]) & set(mesh.faces[fb][1])
+    if len(common) != 1:
+        raise ValueError(f"faces {fa} and {fb} share {len(common)} edges, expected 1")
+    # the edge as face fa lists it (HexGeom takes m_faces[a]->GetEdge(i))
+    return next(e for e in mesh.faces[fa][1] if e in common)
+
+
+def hex_local_vertices(mesh: MeshGraph, eid: int) -> list[int]:
+    """Global ids of the 8 local vertices of hexahedron eid (HexGeom.cpp:
+    200-339 local edges from the faces, :341-430 vertices from the edges)."""
+    kind, f = mesh.elements[eid]
+    if kind != "H":
+        raise ValueError(f"element {eid} is not a hexahedron")
+    e = [None] * 12
+    for k in range(1, 5):
+        e[k - 1] = _shared_edge(mesh, f[0], f[k])
+    e[4] = _shared_edge(mesh, f[1], f[4])
+    for k in range(1, 4):
+        e[k + 4] = _shared_edge(mesh, f[k], f[k + 1])
+    for k in range(1, 5):
+        e[k + 7] = _shared_edge(mesh, f[5], f[k])
+    ev = [mesh.edges[x] for x in e]
+    v = [None] * 8
+    if ev[0][0] in ev[1]:
+        v[0], v[1] = ev[0][1], ev[0][0]
+    elif ev[0][1] in ev[1]:
+        v[0], v[1] = ev[0][0], ev[0][1]
+    else:
+        raise ValueError(f"element {eid}: edges {e[0]} and {e[1]} share no vertex")
+    for i in (1, 2):
+        if ev[i][0] == v[i]:
+            v[i + 1] = ev[i][1]
+        elif ev[i][1] == v[i]:
+            v[i + 1] = ev[i][0]
+        else:
+            raise ValueError(f"element {eid}: edge {e[i]} does not continue the bottom face")
+    if ev[8][0] in ev[9]:
+        v[4], v[5] = ev[8][1], ev[8][0]
+    elif ev[8][1] in ev[9]:
+        v[4], v[5] = ev[8][0], ev[8][1]
+    else:
+        raise ValueError(f"element {eid}: edges {e[8]} and {e[9]} share no vertex")
+    for i in (9, 10):
+        if ev[i][0] == v[i - 4]:
+            v[i - 3] = ev[i][1]
+        elif ev[i][1] == v[i - 4]:
+            v[i - 3] = ev[i][0]
+        else:
+            raise ValueError(f"element {eid}: edge {e[i]} does not continue the top face")
+    return v
+
+
+def hex_elements(mesh: MeshGraph):
+    """(element ids, local vertex ids [E, 8], local vertex coordinates
+    [E, 8, 3]) of the mesh's hexahedra in increasing id."""
+    eids = sorted(e for e, (k, _) in mesh.elements.items() if k == "H")
+    lv = np.array([hex_local_vertices(mesh, e) for e in eids], dtype=np.int64).reshape(-1, 8)
+    verts = np.array([[mesh.vertices[v] for v in row] for row in lv]).reshape(-1, 8, 3)
+    return eids, lv, verts
+
+
+# reference coordinates of the hex's local vertices (StdHexExp)
+HEX_REF = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                    [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+
+
+def hex_geometry(verts: np.ndarray, tol: float = 1e-12):
+    """(jac, df) of straight-sided parallelepiped hexahedra verts [E, 8, 3]
+    (local vertex order): J = |det(dx/dxi)|, df[j*3 + k] = dxi_k/dx_j, one
+    set per element.  Raises for elements that are not parallelepipeds (the
+    deformed per-point factors are not built here)."""
+    verts = np.asarray(verts, dtype=float)
+    A = np.stack([verts[:, 1] - verts[:, 0], verts[:, 3] - verts[:, 0], verts[:, 4] - verts[:, 0]], axis=2) / 2
+    # parallelepiped: every vertex is v0 + A (xi + 1)
+    pred = verts[:, :1, :] + np.einsum("eki,vi->evk", A, HEX_REF + 1)
+    scale = np.max(np.abs(verts)) + 1.0
+    if np.max(np.abs(pred - verts)) > tol * scale:
+        raise NotImplementedError("curved or non-parallelepiped hexahedra need per-point factors")
+    det = np.linalg.det(A)
+    inv = np.linalg.inv(A)  # inv[e, k, j] = dxi_k / dx_j
+    df = np.transpose(inv, (0, 2, 1)).reshape(-1, 9)  # df[j*3 + k]
+    return np.abs(det), np.ascontiguousarray(df.reshape(-1))
+
+
+def hex_c0_map(local_vertices: np.ndarray, nm: int, dirichlet_vertex_sets=()):
+    """C0 map of a conforming hex mesh from the elements' global vertex ids
+    local_vertices [E, 8] (local order).  Returns (map, sign, nglobal, ndir)
+    over the element-local coefficients (p fastest), numbered Dirichlet DOFs
+    first, then vertices, edges, faces and interiors.
+
+    Edge modes p >= 2 of a local edge running against its canonical
+    direction (low to high global vertex id) take the sign (-1)^p
+    (phi_p(-z) = (-1)^p phi_p(z)).  A face's canonical frame starts at its
+    lowest-id corner, axis A towards the lower-id neighbour, axis B towards
+    the other; a local face mode (p1, p2) maps to (p1, p2) or (p2, p1) on
+    (A, B) and takes (-1)^p for each reversed axis.  dirichlet_vertex_sets:
+    global vertex id sets of boundary faces whose DOFs are Dirichlet."""
+    lv = np.asarray(local_vertices)
+    E = len(lv)
+    nb = nm - 2
+    corner = {(a, b, c): i for i, (a, b, c) in enumerate(((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                                                          (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)))}
+    dset = [frozenset(s) for s in dirichlet_vertex_sets]
+    keys: dict = {}  # entity key -> (first id, kind)
+    nxt = [0]
+
+    def ident(key, count):
+        if key not in keys:
+            keys[key] = nxt[0]
+            nxt[0] += count
+        return keys[key]
+
+    mp = np.zeros((E, nm, nm, nm), dtype=np.int64)  # [e, r, q, p]
+    sg = np.ones((E, nm, nm, nm))
+    ent = np.empty((E, nm, nm, nm), dtype=object)  # entity vertex sets, for Dirichlet marking
+    for e in range(E):
+        g = lv[e]
+        for r in range(nm):
+            for q in range(nm):
+                for p in range(nm):
+                    idx = (p, q, r)
+                    bub = [d for d in range(3) if idx[d] >= 2]
+                    if len(bub) == 0:
+                        v = int(g[corner[idx]])
+                        mp[e, r, q, p] = ident(("v", v), 1)
+                        ent[e, r, q, p] = frozenset([v])
+                    elif len(bub) == 1:
+                        d = bub[0]
+                        lo, hi = list(idx), list(idx)
+                        lo[d], hi[d] = 0, 1
+                        va, vb = int(g[corner[tuple(lo)]]), int(g[corner[tuple(hi)]])
+                        base = ident(("e", min(va, vb), max(va, vb)), nb)
+                        mp[e, r, q, p] = base + idx[d] - 2
+                        sg[e, r, q, p] = 1.0 if va < vb else (-1.0) ** idx[d]
+                        ent[e, r, q, p] = frozenset([va, vb])
+                    elif len(bub) == 2:
+                        d1, d2 = bub
+                        fixed = [d for d in range(3) if d not in bub][0]
+                        cg = {}
+                        for s1 in (0, 1):
+                            for s2 in (0, 1):
+                                c = list(idx)
+                                c[d1], c[d2] = s1, s2
+                                cg[(s1, s2)] = int(g[corner[tuple(c)]])
+                        c0 = min(cg, key=cg.get)
+                        n1 = (1 - c0[0], c0[1])  # neighbour along d1
+                        n2 = (c0[0], 1 - c0[1])  # neighbour along d2
+                        d1_is_A = cg[n1] < cg[n2]
+                        rev1, rev2 = c0[0] == 1, c0[1] == 1
+                        p1, p2 = idx[d1], idx[d2]
+                        a, b = (p1, p2) if d1_is_A else (p2, p1)
+                        base = ident(("f",) + tuple(sorted(cg.values())), nb * nb)
+                        mp[e, r, q, p] = base + (a - 2) * nb + (b - 2)
+                        sg[e, r, q, p] = ((-1.0) ** p1 if rev1 else 1.0) * ((-1.0) ** p2 if rev2 else 1.0)
+                        ent[e, r, q, p] = frozenset(cg.values())
+                        del fixed
+                    else:
+                        mp[e, r, q, p] = ident(("i", e), nb ** 3) + ((r - 2) * nb + (q - 2)) * nb + (p - 2)
+                        ent[e, r, q, p] = None
+    nglob = nxt[0]
+    dirichlet = np.zeros(nglob, dtype=bool)
+    if dset:
+        flat_m, flat_e = mp.reshape(-1), ent.reshape(-1)
+        for m, s in zip(flat_m, flat_e):
+            if s is not None and any(s <= f for f in dset):
+                dirichlet[m] = True
+    order = np.concatenate([np.flatnonzero(dirichlet), np.flatnonzero(~dirichlet)])
+    new = np.empty(nglob, dtype=np.int64)
+    new[order] = np.arange(nglob)
+    return new[mp.reshape(-1)], sg.reshape(-1), nglob, int(dirichlet.sum())
+
+
+@dataclass
+class FieldBlock:
+    fields: list
+    shape: str
+    basis: list
+    nummodes: list
+    ids: list
+    data: dict  # field -> array [len(ids), ncoeff]
+
+
+def read_fld(path: str) -> list[FieldBlock]:
+    """ELEMENTS blocks of an XML field file (FieldIOXml.cpp): coefficients
+    stored field by field, each field element by element in the ID order."""
+    root = ET.parse(path).getroot()
+    blocks = []
+    for el in root.iter("ELEMENTS"):
+        fields = [f.strip() for f in el.get("FIELDS").split(",")]
+        nm = el.get("NUMMODESPERDIR")
+        if not nm.startswith("UNIORDER:"):
+            raise NotImplementedError("only UNIORDER expansions are read")
+        nummodes = [int(x) for x in nm.split(":", 1)[1].split(",")]
+        ids = parse_ranges(el.get("ID"))
+        if el.get("COMPRESSED"):
+            if el.get("COMPRESSED") != _B64Z or el.get("BITSIZE", "64") != "64":
+                raise ValueError(f"unsupported compression {el.get('COMPRESSED')}")
+            vals = _b64z_decode(el.text, "<f8")
+        else:
+            vals = np.array([float(x) for x in el.text.split()])
+        total = len(vals) // len(fields)
+        if total * len(fields) != len(vals) or total % len(ids):
+            raise ValueError("ELEMENTS data size does not match its fields and elements")
+        nc = total // len(ids)
+        data = {f: vals[i * total:(i + 1) * total].reshape(len(ids), nc).copy() for i, f in enumerate(fields)}
+        blocks.append(FieldBlock(fields, el.get("SHAPE"), el.get("BASIS").split(","), nummodes, ids, data))
+    return blocks
+
+
+def write_fld(path: str, blocks: list[FieldBlock], metadata: dict | None = None) -> None:
+    """An XML field file with the blocks' coefficients, B64Z-LittleEndian
+    compressed (FieldIOXml.cpp's default)."""
+    root = ET.Element("NEKTAR")
+    md = ET.SubElement(root, "Metadata")
+    for k, v in (metadata or {}).items():
+        ET.SubElement(md, k).text = str(v)
+    for b in blocks:
+        vals = np.concatenate([np.asarray(b.data[f], dtype="<f8").reshape(-1) for f in b.fields])
+        el = ET.SubElement(root, "ELEMENTS", {
+            "FIELDS": ",".join(b.fields), "SHAPE": b.shape, "BASIS": ",".join(b.basis),
+            "NUMMODESPERDIR": "UNIORDER:" + ",".join(str(n) for n in b.nummodes),
+            "ID": _ranges_string(b.ids), "COMPRESSED": _B64Z, "BITSIZE": "64"})
+        el.text = _b64z_encode(vals)
+    ET.ElementTree(root).write(path, encoding="utf-8", xml_declaration=True)
