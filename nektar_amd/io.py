@@ -11,6 +11,9 @@
 * ``hex_geometry``: Jacobian and derivative factors of straight-sided
   (parallelepiped) hexahedra, one set per element (GeomFactors.cpp:150-477
   for the regular case).
+* ``tet_local_vertices`` / ``tet_geometry``: the same for tetrahedra
+  (TetGeom.cpp:113-300); meshes of tetrahedra are read and their fields
+  evaluated, but no C0 map is built for them here.
 * ``hex_c0_map``: the C0 local-to-global map with orientation signs of a
   conforming hex mesh (AssemblyMapCG's numbering role), from the elements'
   global vertex ids: vertex, edge, face and interior modes; edge and face
@@ -200,6 +203,65 @@ def hex_elements(mesh: MeshGraph):
     lv = np.array([hex_local_vertices(mesh, e) for e in eids], dtype=np.int64).reshape(-1, 8)
     verts = np.array([[mesh.vertices[v] for v in row] for row in lv]).reshape(-1, 8, 3)
     return eids, lv, verts
+
+
+def tet_local_vertices(mesh: MeshGraph, eid: int) -> list[int]:
+    """Global ids of the 4 local vertices of tetrahedron eid (TetGeom.cpp:
+    113-230 local edges from the faces, :232-300 vertices from the edges):
+    edges 0-2 are the edges of face 0 that are the first edge of faces 1-3,
+    edge 3 is shared by faces 1 and 3; v0, v1 from edge 0 (v1 on edge 1),
+    v2 the far end of edge 1, v3 the far end of edge 3 from v0."""
+    kind, f = mesh.elements[eid]
+    if kind != "A":
+        raise ValueError(f"element {eid} is not a tetrahedron")
+    fe = [mesh.faces[x][1] for x in f]
+    if any(mesh.faces[x][0] != "T" for x in f):
+        raise ValueError(f"element {eid}: tetrahedron faces must be triangles")
+    if fe[0][0] != fe[1][0]:
+        raise ValueError(f"element {eid}: the first edges of faces 0 and 1 differ")
+    e = [None] * 4
+    for k in range(1, 4):
+        hits = [x for x in fe[0] if x == fe[k][0]]
+        if len(hits) != 1:
+            raise ValueError(f"element {eid}: face {f[k]}'s first edge is not on face {f[0]}")
+        e[k - 1] = hits[0]
+    e[3] = _shared_edge(mesh, f[1], f[3])
+    ev = [mesh.edges[x] for x in e]
+    if ev[0][0] in ev[1]:
+        v0, v1 = ev[0][1], ev[0][0]
+    elif ev[0][1] in ev[1]:
+        v0, v1 = ev[0][0], ev[0][1]
+    else:
+        raise ValueError(f"element {eid}: edges {e[0]} and {e[1]} share no vertex")
+    if ev[1][0] == v1:
+        v2 = ev[1][1]
+    elif ev[1][1] == v1:
+        v2 = ev[1][0]
+    else:
+        raise ValueError(f"element {eid}: edge {e[1]} does not continue edge {e[0]}")
+    v3 = ev[3][1] if ev[3][0] == v0 else ev[3][0]
+    return [v0, v1, v2, v3]
+
+
+def tet_elements(mesh: MeshGraph):
+    """(element ids, local vertex ids [E, 4], local vertex coordinates
+    [E, 4, 3]) of the mesh's tetrahedra in increasing id."""
+    eids = sorted(e for e, (k, _) in mesh.elements.items() if k == "A")
+    lv = np.array([tet_local_vertices(mesh, e) for e in eids], dtype=np.int64).reshape(-1, 4)
+    verts = np.array([[mesh.vertices[v] for v in row] for row in lv]).reshape(-1, 4, 3)
+    return eids, lv, verts
+
+
+def tet_geometry(verts: np.ndarray):
+    """(jac, df) of straight-sided tetrahedra verts [E, 4, 3] (local vertex
+    order; reference vertices (-1,-1,-1), (1,-1,-1), (-1,1,-1), (-1,-1,1)):
+    x = v0 + sum_d (xi_d + 1)/2 (v_{d+1} - v0), J = |det(dx/dxi)|,
+    df[j*3 + k] = dxi_k/dx_j, one set per element."""
+    verts = np.asarray(verts, dtype=float)
+    A = np.stack([verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0], verts[:, 3] - verts[:, 0]], axis=2) / 2
+    inv = np.linalg.inv(A)
+    df = np.transpose(inv, (0, 2, 1)).reshape(-1, 9)
+    return np.abs(np.linalg.det(A)), np.ascontiguousarray(df.reshape(-1))
 
 
 # reference coordinates of the hex's local vertices (StdHexExp)

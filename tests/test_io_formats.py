@@ -14,7 +14,11 @@ the unit cube, NUMMODES 4, B64Z-LittleEndian compressed):
 * write_fld round-trips the reference's field file;
 * the C0 map: (P n + 1)^3 global DOFs, random global coefficients give
   fields continuous across every shared point, and the boundary faces'
-  DOFs are the Dirichlet ones."""
+  DOFs are the Dirichlet ones;
+* the tetrahedral Tet_channel_m3 mesh and fields: TetGeom frames (geometry
+  equal to the oracle's, the tets fill the unit cube), and the reference's
+  fields agree across every shared face in those frames (a swapped vertex
+  pair breaks it)."""
 import os
 from collections import defaultdict
 
@@ -172,3 +176,52 @@ def test_plain_and_compressed_meshes_agree(mesh):
     ea, la, va = nio.hex_elements(plain)
     eb, lb, vb = nio.hex_elements(mesh)
     assert ea == eb and np.array_equal(la, lb) and np.array_equal(va, vb)
+
+
+def test_tet_mesh_and_fields():
+    # Tet_channel_m3 (utilities/FieldConvert/Tests): 48 tetrahedra on the unit
+    # cube in plain text, NUMMODES 3, and its fields u, v, w, p
+    import itertools
+
+    import tet_mesh as TM
+    d = os.path.dirname(G)
+    tm = nio.read_mesh_xml(os.path.join(d, "Tet_channel_m3.xml"))
+    assert len(tm.vertices) == 27 and len(tm.elements) == 48
+    eids, lv, verts = nio.tet_elements(tm)
+    jac, df = nio.tet_geometry(verts)
+    jo, dfo = O.tet_geom(np.ascontiguousarray(verts.reshape(-1)), len(eids))
+    assert np.max(np.abs(jac - jo)) <= 1e-15 and np.max(np.abs(df - dfo)) <= 1e-13
+    assert abs(np.sum(jac) * 4 / 3 - 1.0) <= 1e-13  # the tets fill the unit cube
+    A = np.stack([verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0], verts[:, 3] - verts[:, 0]], axis=2)
+    assert np.all(np.linalg.det(A) > 0)
+    (blk,) = nio.read_fld(os.path.join(d, "Tet_channel_m3.fld"))
+    assert blk.shape == "Tetrahedron" and blk.nummodes == [3, 3, 3] and blk.data["p"].shape == (48, 10)
+    # every field, evaluated in our frames on both sides of every shared face
+    # (the tet modes as polynomials, tests/tet_mesh.py), agrees
+    nm = 3
+    C = TM.mode_polynomials(nm)
+    pos = {e: i for i, e in enumerate(blk.ids)}
+    faces = {}
+    for k, e in enumerate(eids):
+        for c in itertools.combinations(range(4), 3):
+            faces.setdefault(frozenset(int(lv[k][i]) for i in c), []).append(k)
+    shared = [(key, ks) for key, ks in faces.items() if len(ks) == 2]
+    assert len(shared) == (4 * 48 - 6 * 2 * 2 * 2) // 2
+    W = TM._W_FACE[:6]
+
+    def worst_jump(vv):
+        worst = 0.0
+        for key, ks in shared:
+            pts = W @ np.array([tm.vertices[v] for v in sorted(key)])
+            vals = [TM._eval_modes(C, nm - 1, vv[k], pts).T for k in ks]
+            for f in blk.fields:
+                a = vals[0] @ blk.data[f][pos[eids[ks[0]]]]
+                b = vals[1] @ blk.data[f][pos[eids[ks[1]]]]
+                worst = max(worst, np.max(np.abs(a - b)))
+        return worst
+
+    assert worst_jump(verts) <= 1e-10
+    # a wrong frame breaks it: vertices 1 and 2 of the first element swapped
+    bad = verts.copy()
+    bad[0] = verts[0][[0, 2, 1, 3]]
+    assert worst_jump(bad) > 1e-2
