@@ -21,6 +21,8 @@
 * ``read_fld`` / ``write_fld``: field files (FieldIOXml.cpp): ELEMENTS blocks
   with their FIELDS, SHAPE, BASIS, NUMMODESPERDIR and ID attributes and the
   coefficients field by field, element by element, in the ID order.
+* ``c0_helmholtz_smoothing``: FieldConvert's C0Projection:helmsmoothing on
+  hexahedral meshes, from the files to the smoothed fields, on the GPU.
 """
 from __future__ import annotations
 
@@ -427,3 +429,55 @@ def write_fld(path: str, blocks: list[FieldBlock], metadata: dict | None = None)
             "ID": _ranges_string(b.ids), "COMPRESSED": _B64Z, "BITSIZE": "64"})
         el.text = _b64z_encode(vals)
     ET.ElementTree(root).write(path, encoding="utf-8", xml_declaration=True)
+
+
+def c0_helmholtz_smoothing(session_path: str, fld_path: str, length: float, tol: float = 1e-10,
+                           maxit: int = 5000):
+    """FieldConvert's ``-m C0Projection:helmsmoothing=L`` on a hexahedral
+    mesh (ProcessC0Projection.cpp:186-231): every field u of the field file
+    is replaced by the C0 solution of (L + lam M) u_s = lam M u with
+    lam = (2 pi / L)^2 and no Dirichlet DOFs (the module builds its C0
+    expansion without boundary conditions), through the MI355X operators
+    (BwdTrans, IProductWRTBase, Helmholtz) and the general CG.  Returns
+    (FieldBlock of the smoothed coefficients, {field: SolveInfo})."""
+    import torch
+
+    from . import AssemblyMap, FP64, GetOperatorFactory, LinearSystem
+
+    mesh = read_mesh_xml(session_path)
+    eids, lv, verts = hex_elements(mesh)
+    if len(eids) != len(mesh.elements):
+        raise NotImplementedError("c0_helmholtz_smoothing takes hexahedral meshes")
+    nm = mesh.expansions[0]["nummodes"]
+    nq, nc, E = nm + 1, nm ** 3, len(eids)
+    jac, df = hex_geometry(verts)
+    mp, sg, ng, nd = hex_c0_map(lv, nm)
+    blocks = [b for b in read_fld(fld_path) if b.shape == "Hexahedron"]
+    if len(blocks) != 1 or blocks[0].nummodes != [nm] * 3:
+        raise NotImplementedError("one hexahedral ELEMENTS block with the session's NUMMODES is expected")
+    blk = blocks[0]
+    pos = {e: i for i, e in enumerate(blk.ids)}
+    lam = (2.0 * np.pi / length) ** 2
+    F = GetOperatorFactory()
+    bwd = F.CreateInstance("BwdTrans_Hex_Regular", nm, nq, E, FP64)
+    ipr = F.CreateInstance("IProduct_Hex_Regular", nm, nq, E, FP64)
+    ipr.SetJac(jac)
+    helm = F.CreateInstance("Helmholtz_Hex_Regular", nm, nq, E, FP64)
+    helm.SetJac(jac)
+    helm.SetDF(df)
+    helm.SetLambda(lam)
+    ls = LinearSystem(AssemblyMap(mp, sg, ng, nd), [helm])
+    out, infos = {}, {}
+    uq = torch.zeros(E * nq ** 3, dtype=torch.float64, device="cuda")
+    bl = torch.zeros(E * nc, dtype=torch.float64, device="cuda")
+    xs = torch.zeros(E * nc, dtype=torch.float64, device="cuda")
+    for f in blk.fields:
+        c = np.ascontiguousarray(np.array([blk.data[f][pos[e]] for e in eids]).reshape(-1))
+        bwd(torch.from_numpy(c).cuda(), uq)
+        ipr(uq, bl)
+        ls.set_rhs_local(lam * bl)  # HelmSolve's rhs: -IProductWRTBase(-lam u)
+        infos[f] = ls.solve(tol=tol, maxit=maxit)
+        ls.solution_local(xs)
+        torch.cuda.synchronize()
+        out[f] = xs.cpu().numpy().reshape(E, nc).copy()
+    return FieldBlock(list(blk.fields), blk.shape, list(blk.basis), list(blk.nummodes), list(eids), out), infos
