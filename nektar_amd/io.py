@@ -12,8 +12,7 @@
   (parallelepiped) hexahedra, one set per element (GeomFactors.cpp:150-477
   for the regular case).
 * ``tet_local_vertices`` / ``tet_geometry``: the same for tetrahedra
-  (TetGeom.cpp:113-300); meshes of tetrahedra are read and their fields
-  evaluated, but no C0 map is built for them here.
+  (TetGeom.cpp:113-300).
 * ``hex_c0_map``: the C0 local-to-global map with orientation signs of a
   conforming hex mesh (AssemblyMapCG's numbering role), from the elements'
   global vertex ids: vertex, edge, face and interior modes; edge and face
@@ -21,8 +20,11 @@
 * ``read_fld`` / ``write_fld``: field files (FieldIOXml.cpp): ELEMENTS blocks
   with their FIELDS, SHAPE, BASIS, NUMMODESPERDIR and ID attributes and the
   coefficients field by field, element by element, in the ID order.
+* ``tet_c0_map``: the same for tetrahedra, following the reference's edge
+  and triangular-face trace maps (StdTetExp.cpp:1459-1660).
 * ``c0_helmholtz_smoothing``: FieldConvert's C0Projection:helmsmoothing on
-  hexahedral meshes, from the files to the smoothed fields, on the GPU.
+  hexahedral or tetrahedral meshes, from the files to the smoothed fields,
+  on the GPU.
 """
 from __future__ import annotations
 
@@ -431,12 +433,108 @@ def write_fld(path: str, blocks: list[FieldBlock], metadata: dict | None = None)
     ET.ElementTree(root).write(path, encoding="utf-8", xml_declaration=True)
 
 
+def _tet_index(nm):
+    """(p, q, r) -> coefficient index of the tet order (p slowest, r fastest)."""
+    idx, k = {}, 0
+    for p in range(nm):
+        for q in range(nm - p):
+            for r in range(nm - p - q):
+                idx[(p, q, r)] = k
+                k += 1
+    return idx
+
+
+def tet_c0_map(local_vertices: np.ndarray, nm: int, dirichlet_vertex_sets=()):
+    """C0 map of a conforming tet mesh from the elements' global vertex ids
+    local_vertices [E, 4] (TetGeom order).  Returns (map, sign, nglobal,
+    ndir) over the element-local coefficients (tet order), Dirichlet DOFs
+    first, then vertices, edges, faces and interiors.
+
+    Follows the reference's trace maps (StdTetExp.cpp:1459-1660): the
+    interior modes of local edge k are listed along the edge and a backward
+    edge (here: running from the higher to the lower global vertex id)
+    flips every odd one; a triangular face keeps its apex (the collapsed
+    direction) and may reverse its first direction, which flips the modes
+    with an odd first index (for face 2 an even j); face modes correspond
+    index by index between the two elements.  A face whose apex differs
+    between its elements is rejected, as the reference does."""
+    lv = np.asarray(local_vertices)
+    E = len(lv)
+    ix = _tet_index(nm)
+    nc = len(ix)
+    # local edges: (start vertex, end vertex, modes along it)
+    edges = [(0, 1, [(i + 2, 0, 0) for i in range(nm - 2)]), (1, 2, [(1, i + 1, 0) for i in range(nm - 2)]),
+             (0, 2, [(0, i + 2, 0) for i in range(nm - 2)]), (0, 3, [(0, 0, i + 2) for i in range(nm - 2)]),
+             (1, 3, [(1, 0, i + 1) for i in range(nm - 2)]), (2, 3, [(0, 1, i + 1) for i in range(nm - 2)])]
+    # local faces: (dir1 start, dir1 end, apex, [(mode, flips when dir1 is reversed)])
+    f0 = [((i, j, 0), i % 2 == 1) for i in range(2, nm) for j in range(1, nm - i)]
+    f1 = [((i, 0, k), i % 2 == 1) for i in range(2, nm) for k in range(1, nm - i)]
+    f2 = [((1, j, k), (j + 1) % 2 == 1) for j in range(1, nm - 1) for k in range(1, nm - 1 - j)]
+    f3 = [((0, j, k), j % 2 == 1) for j in range(2, nm) for k in range(1, nm - j)]
+    faces = [(0, 1, 2, f0), (0, 1, 3, f1), (1, 2, 3, f2), (0, 2, 3, f3)]
+    vmodes = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    keys: dict = {}
+    apex_of: dict = {}
+    nxt = [0]
+
+    def ident(key, count):
+        if key not in keys:
+            keys[key] = nxt[0]
+            nxt[0] += count
+        return keys[key]
+
+    mp = np.full((E, nc), -1, dtype=np.int64)
+    sg = np.ones((E, nc))
+    ent = [[None] * nc for _ in range(E)]
+    for e in range(E):
+        g = [int(x) for x in lv[e]]
+        for v, m in enumerate(vmodes):
+            mp[e, ix[m]] = ident(("v", g[v]), 1)
+            ent[e][ix[m]] = frozenset([g[v]])
+        for a, b, modes in edges:
+            base = ident(("e", min(g[a], g[b]), max(g[a], g[b])), nm - 2)
+            back = g[a] > g[b]
+            for i, m in enumerate(modes):
+                mp[e, ix[m]] = base + i
+                sg[e, ix[m]] = -1.0 if back and i % 2 == 1 else 1.0
+                ent[e][ix[m]] = frozenset([g[a], g[b]])
+        for a, b, c, modes in faces:
+            key = ("f", frozenset([g[a], g[b], g[c]]))
+            if key in apex_of and apex_of[key] != g[c]:
+                raise ValueError(f"element {e}: a face is met with a different apex by its neighbour "
+                                 "(tet vertex order not consistent)")
+            apex_of[key] = g[c]
+            base = ident(key, len(modes))
+            back = g[a] > g[b]
+            for i, (m, odd) in enumerate(modes):
+                mp[e, ix[m]] = base + i
+                sg[e, ix[m]] = -1.0 if back and odd else 1.0
+                ent[e][ix[m]] = frozenset([g[a], g[b], g[c]])
+        rest = [k for k in range(nc) if mp[e, k] < 0]
+        base = ident(("i", e), len(rest))
+        for i, k in enumerate(rest):
+            mp[e, k] = base + i
+    nglob = nxt[0]
+    dirichlet = np.zeros(nglob, dtype=bool)
+    dset = [frozenset(s) for s in dirichlet_vertex_sets]
+    if dset:
+        for e in range(E):
+            for k in range(nc):
+                s = ent[e][k]
+                if s is not None and any(s <= f for f in dset):
+                    dirichlet[mp[e, k]] = True
+    order = np.concatenate([np.flatnonzero(dirichlet), np.flatnonzero(~dirichlet)])
+    new = np.empty(nglob, dtype=np.int64)
+    new[order] = np.arange(nglob)
+    return new[mp.reshape(-1)], sg.reshape(-1), nglob, int(dirichlet.sum())
+
+
 def c0_helmholtz_smoothing(session_path: str, fld_path: str, length: float, tol: float = 1e-10,
                            maxit: int = 5000):
-    """FieldConvert's ``-m C0Projection:helmsmoothing=L`` on a hexahedral
-    mesh (ProcessC0Projection.cpp:186-231): every field u of the field file
-    is replaced by the C0 solution of (L + lam M) u_s = lam M u with
-    lam = (2 pi / L)^2 and no Dirichlet DOFs (the module builds its C0
+    """FieldConvert's ``-m C0Projection:helmsmoothing=L`` on a hexahedral or
+    tetrahedral mesh (ProcessC0Projection.cpp:186-231): every field u of the
+    field file is replaced by the C0 solution of (L + lam M) u_s = lam M u
+    with lam = (2 pi / L)^2 and no Dirichlet DOFs (the module builds its C0
     expansion without boundary conditions), through the MI355X operators
     (BwdTrans, IProductWRTBase, Helmholtz) and the general CG.  Returns
     (FieldBlock of the smoothed coefficients, {field: SolveInfo})."""
@@ -445,30 +543,38 @@ def c0_helmholtz_smoothing(session_path: str, fld_path: str, length: float, tol:
     from . import AssemblyMap, FP64, GetOperatorFactory, LinearSystem
 
     mesh = read_mesh_xml(session_path)
-    eids, lv, verts = hex_elements(mesh)
-    if len(eids) != len(mesh.elements):
-        raise NotImplementedError("c0_helmholtz_smoothing takes hexahedral meshes")
+    kinds = {k for k, _ in mesh.elements.values()}
+    if kinds == {"H"}:
+        eids, lv, verts = hex_elements(mesh)
+        tag, shape_name = "Hex", "Hexahedron"
+        jac, df = hex_geometry(verts)
+    elif kinds == {"A"}:
+        eids, lv, verts = tet_elements(mesh)
+        tag, shape_name = "Tet", "Tetrahedron"
+        jac, df = tet_geometry(verts)
+    else:
+        raise NotImplementedError("c0_helmholtz_smoothing takes meshes of hexahedra or of tetrahedra")
     nm = mesh.expansions[0]["nummodes"]
-    nq, nc, E = nm + 1, nm ** 3, len(eids)
-    jac, df = hex_geometry(verts)
-    mp, sg, ng, nd = hex_c0_map(lv, nm)
-    blocks = [b for b in read_fld(fld_path) if b.shape == "Hexahedron"]
+    nq, E = nm + 1, len(eids)
+    mp, sg, ng, nd = (hex_c0_map if tag == "Hex" else tet_c0_map)(lv, nm)
+    blocks = [b for b in read_fld(fld_path) if b.shape == shape_name]
     if len(blocks) != 1 or blocks[0].nummodes != [nm] * 3:
-        raise NotImplementedError("one hexahedral ELEMENTS block with the session's NUMMODES is expected")
+        raise NotImplementedError("one ELEMENTS block of the mesh's shape with the session's NUMMODES is expected")
     blk = blocks[0]
     pos = {e: i for i, e in enumerate(blk.ids)}
     lam = (2.0 * np.pi / length) ** 2
     F = GetOperatorFactory()
-    bwd = F.CreateInstance("BwdTrans_Hex_Regular", nm, nq, E, FP64)
-    ipr = F.CreateInstance("IProduct_Hex_Regular", nm, nq, E, FP64)
+    bwd = F.CreateInstance(f"BwdTrans_{tag}_Regular", nm, nq, E, FP64)
+    ipr = F.CreateInstance(f"IProduct_{tag}_Regular", nm, nq, E, FP64)
     ipr.SetJac(jac)
-    helm = F.CreateInstance("Helmholtz_Hex_Regular", nm, nq, E, FP64)
+    helm = F.CreateInstance(f"Helmholtz_{tag}_Regular", nm, nq, E, FP64)
     helm.SetJac(jac)
     helm.SetDF(df)
     helm.SetLambda(lam)
     ls = LinearSystem(AssemblyMap(mp, sg, ng, nd), [helm])
+    nc = helm.input_size() // max(E, 1)
     out, infos = {}, {}
-    uq = torch.zeros(E * nq ** 3, dtype=torch.float64, device="cuda")
+    uq = torch.zeros(bwd.output_size(), dtype=torch.float64, device="cuda")
     bl = torch.zeros(E * nc, dtype=torch.float64, device="cuda")
     xs = torch.zeros(E * nc, dtype=torch.float64, device="cuda")
     for f in blk.fields:

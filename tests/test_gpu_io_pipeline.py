@@ -100,3 +100,36 @@ def test_c0_smoothing_pipeline(nmf, tmp_path):
     (back,) = nio.read_fld(str(tmp_path / "smooth.fld"))
     for f in blk.fields:
         assert np.array_equal(back.data[f], res.data[f])
+
+
+def test_tet_c0_smoothing_against_oracle(nmf):
+    # the 48-tet Tet_channel_m3 mesh and fields (NUMMODES 3)
+    d = os.path.dirname(G)
+    res, infos = nio.c0_helmholtz_smoothing(os.path.join(d, "Tet_channel_m3.xml"),
+                                            os.path.join(d, "Tet_channel_m3.fld"), 0.6, tol=1e-13)
+    assert all(i.converged for i in infos.values())
+    mesh = nio.read_mesh_xml(os.path.join(d, "Tet_channel_m3.xml"))
+    eids, lv, verts = nio.tet_elements(mesh)
+    jac, df = nio.tet_geometry(verts)
+    nm, E = 3, len(eids)
+    nq, nc = nm + 1, 10
+    mp, sg, ng, nd = nio.tet_c0_map(lv, nm)
+    (blk,) = nio.read_fld(os.path.join(d, "Tet_channel_m3.fld"))
+    pos = {e: i for i, e in enumerate(blk.ids)}
+    cols = []
+    for m in range(nc):
+        u = np.zeros(E * nc)
+        u[m::nc] = 1.0
+        cols.append(O.helmholtz(O.TET, nm, nq, E, False, jac, df, LAM, u).reshape(E, nc))
+    Ae = np.stack(cols, axis=2)
+    A = np.zeros((ng, ng))
+    M, S = mp.reshape(E, nc), sg.reshape(E, nc)
+    for e in range(E):
+        A[np.ix_(M[e], M[e])] += S[e][:, None] * Ae[e] * S[e][None, :]
+    for f in blk.fields:
+        c = np.array([blk.data[f][pos[e]] for e in eids]).reshape(-1)
+        b = np.zeros(ng)
+        np.add.at(b, mp, sg * LAM * O.iproduct(O.TET, nm, nq, E, False, jac, O.bwdtrans(O.TET, nm, nq, E, c)))
+        xo = (sg * np.linalg.solve(A, b)[mp]).reshape(E, nc)
+        # (u is zero in this file: the oracle's dense solve leaves 1e-16 noise)
+        assert np.max(np.abs(res.data[f] - xo)) <= 1e-9 * np.max(np.abs(xo)) + 1e-13, f

@@ -225,3 +225,54 @@ def test_tet_mesh_and_fields():
     bad = verts.copy()
     bad[0] = verts[0][[0, 2, 1, 3]]
     assert worst_jump(bad) > 1e-2
+
+
+@pytest.mark.parametrize("nm", [3, 4, 5])
+def test_tet_c0_map(nm):
+    # on the reference's tet mesh: (2 (nm-1) + 1)^3 DOFs, random global
+    # coefficients continuous across every shared face, and a sign error on
+    # one face mode breaks it; on the Kuhn meshes of tests/tet_mesh.py the
+    # map corresponds one-to-one, signs included, with the trace-matched one
+    import itertools
+
+    import tet_mesh as TM
+    tm = nio.read_mesh_xml(os.path.join(os.path.dirname(G), "Tet_channel_m3.xml"))
+    eids, lv, verts = nio.tet_elements(tm)
+    mp, sg, ng, nd = nio.tet_c0_map(lv, nm)
+    assert ng == (2 * (nm - 1) + 1) ** 3 and nd == 0
+    C = TM.mode_polynomials(nm)
+    nc = C.shape[1]
+    faces = {}
+    for k in range(len(eids)):
+        for c in itertools.combinations(range(4), 3):
+            faces.setdefault(frozenset(int(lv[k][i]) for i in c), []).append(k)
+    shared = [(key, ks) for key, ks in faces.items() if len(ks) == 2]
+
+    def worst(s):
+        loc = (s * x[mp]).reshape(len(eids), nc)
+        w = 0.0
+        for key, ks in shared:
+            pts = TM._W_FACE[:8] @ np.array([tm.vertices[v] for v in sorted(key)])
+            a, b = (TM._eval_modes(C, nm - 1, verts[k], pts).T @ loc[k] for k in ks)
+            w = max(w, np.max(np.abs(a - b)))
+        return w
+
+    x = np.random.default_rng(nm).uniform(-1, 1, ng)
+    assert worst(sg) <= 1e-12
+    if nm >= 4:
+        # flip, in one element only, the first interior mode of a shared face
+        key, (k0, _) = shared[0]
+        where = sorted(i for i in range(4) if int(lv[k0][i]) in key)
+        first = {(0, 1, 2): (2, 1, 0), (0, 1, 3): (2, 0, 1), (1, 2, 3): (1, 1, 1), (0, 2, 3): (0, 2, 1)}[tuple(where)]
+        order = [(p, q, r) for p in range(nm) for q in range(nm - p) for r in range(nm - p - q)]
+        bad = sg.copy()
+        bad[k0 * nc + order.index(first)] *= -1
+        assert worst(bad) > 1e-3
+    # Kuhn tets (vertices ascending): the same space as the trace-matched map
+    coords, tets, mr, sr, dirichlet = TM.tet_c0_map(2, nm)
+    m2, s2, ng2, _ = nio.tet_c0_map(tets, nm)
+    assert ng2 == len(dirichlet)
+    pair = {}
+    for a, b, u, v in zip(m2, mr, s2, sr):
+        pair.setdefault(a, set()).add((b, u * v))
+    assert all(len(v) == 1 for v in pair.values())
