@@ -42,6 +42,8 @@ void Batch::add(MFOperator *op, const void *in, void *out0, void *out1, void *ou
     NMF_REQUIRE(op->nelmt == 0 || (in && out0), NMF_ERR_INVALID, "null input/output pointer");
     if (op->op == OpType::PhysDeriv)
         NMF_REQUIRE(op->nelmt == 0 || (out1 && out2), NMF_ERR_INVALID, "PhysDeriv needs three outputs");
+    NMF_REQUIRE(hex_supported(op->nm, op->nq[0]) || op->shape != Shape::Hex, NMF_ERR_UNSUPPORTED,
+                "batches take hex operators with Q = NUMMODES + 1 or + 2");
     entries_.push_back(Entry{op, in, out0, out1, out2, op->version});
     built_ = false;
 }

@@ -13,7 +13,10 @@ namespace nektar_amd {
 NMF_DECL(2) NMF_DECL(3) NMF_DECL(4) NMF_DECL(5) NMF_DECL(6) NMF_DECL(7) NMF_DECL(8) NMF_DECL(9) NMF_DECL(10)
 #undef NMF_DECL
 
+// hex: Q = NUMMODES + 1 (the MODIFIED default) and + 2 everywhere; the Collections operators
+// also take Q = NUMMODES for nm >= 3 (the batched, grouped and solver kernels do not)
 bool hex_supported(int nm, int nq) { return nm >= kMinModes && nm <= kMaxModes && (nq == nm + 1 || nq == nm + 2); }
+bool hex_op_supported(int nm, int nq) { return hex_supported(nm, nq) || (nq == nm && nm >= 3 && nm <= kMaxModes); }
 bool tet_supported(int nm, int nq0) { return nm >= kMinModes && nm <= kMaxModes && (nq0 == nm + 1 || nq0 == nm + 2); }
 
 #define NMF_CASE(shape, n) \

@@ -109,6 +109,7 @@ hipError_t launch_tet_grouped(int nm, int nq0, const GroupArgs &a);
 constexpr int kMinModes = 2, kMaxModes = 10;
 
 bool hex_supported(int nm, int nq);
+bool hex_op_supported(int nm, int nq);  // Collections operators: also Q = NUMMODES
 bool tet_supported(int nm, int nq0);
 hipError_t launch_hex(int nm, int nq, const KernelArgs &a);
 hipError_t launch_tet(int nm, int nq0, const KernelArgs &a);

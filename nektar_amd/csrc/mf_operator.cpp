@@ -89,7 +89,7 @@ MFOperator::MFOperator(const std::string &key, int nummodes, int numpoints0, int
     fp32 = precision == NMF_FP32;
     nm = nummodes;
     nelmt = n;
-    const bool ok = shape == Shape::Hex ? hex_supported(nummodes, numpoints0) : tet_supported(nummodes, numpoints0);
+    const bool ok = shape == Shape::Hex ? hex_op_supported(nummodes, numpoints0) : tet_supported(nummodes, numpoints0);
     NMF_REQUIRE(ok, NMF_ERR_UNSUPPORTED,
                 "(nummodes=" + std::to_string(nummodes) + ", numpoints=" + std::to_string(numpoints0) +
                     ") not instantiated: need 2 <= nummodes <= 10 and numpoints0 in {nummodes+1, nummodes+2}");

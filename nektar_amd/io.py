@@ -530,14 +530,16 @@ def tet_c0_map(local_vertices: np.ndarray, nm: int, dirichlet_vertex_sets=()):
 
 
 def c0_helmholtz_smoothing(session_path: str, fld_path: str, length: float, tol: float = 1e-10,
-                           maxit: int = 5000):
+                           maxit: int = 5000, numpoints: int | None = None):
     """FieldConvert's ``-m C0Projection:helmsmoothing=L`` on a hexahedral or
     tetrahedral mesh (ProcessC0Projection.cpp:186-231): every field u of the
     field file is replaced by the C0 solution of (L + lam M) u_s = lam M u
     with lam = (2 pi / L)^2 and no Dirichlet DOFs (the module builds its C0
     expansion without boundary conditions), through the MI355X operators
-    (BwdTrans, IProductWRTBase, Helmholtz) and the general CG.  Returns
-    (FieldBlock of the smoothed coefficients, {field: SolveInfo})."""
+    (BwdTrans, IProductWRTBase, Helmholtz) and the general CG.  numpoints:
+    GLL points per direction (default NUMMODES + 1, the MODIFIED default;
+    hexahedra also take NUMMODES).  Returns (FieldBlock of the smoothed
+    coefficients, {field: SolveInfo})."""
     import torch
 
     from . import AssemblyMap, FP64, GetOperatorFactory, LinearSystem
@@ -555,7 +557,7 @@ def c0_helmholtz_smoothing(session_path: str, fld_path: str, length: float, tol:
     else:
         raise NotImplementedError("c0_helmholtz_smoothing takes meshes of hexahedra or of tetrahedra")
     nm = mesh.expansions[0]["nummodes"]
-    nq, E = nm + 1, len(eids)
+    nq, E = (numpoints or nm + 1), len(eids)
     mp, sg, ng, nd = (hex_c0_map if tag == "Hex" else tet_c0_map)(lv, nm)
     blocks = [b for b in read_fld(fld_path) if b.shape == shape_name]
     if len(blocks) != 1 or blocks[0].nummodes != [nm] * 3:

@@ -133,3 +133,44 @@ def test_tet_c0_smoothing_against_oracle(nmf):
         xo = (sg * np.linalg.solve(A, b)[mp]).reshape(E, nc)
         # (u is zero in this file: the oracle's dense solve leaves 1e-16 noise)
         assert np.max(np.abs(res.data[f] - xo)) <= 1e-9 * np.max(np.abs(xo)) + 1e-13, f
+
+
+def test_reproduces_the_reference_fieldconvert_norms(nmf):
+    """utilities/FieldConvert/Tests/Hex_channel_C0helmsmoothing.tst runs
+    ``FieldConvert -m C0Projection:helmsmoothing=0.6 <xml> <fld> C0Hel.plt -e``
+    and records the printed L2 norms u, v, w 0.635792 and p 1.10303
+    (tolerance 1e-6).  That run solves with NUMMODES (4) GLL points per
+    direction, and because Tecplot output asks for evenly spaced points the
+    norms are taken on NUMMODES evenly spaced points per direction
+    (OutputFileBase.cpp ConvertExpToEquispaced, MeshGraph.cpp:1497-1531)
+    with PolyEPoints' weights, the integrals of their Lagrange polynomials
+    (PolyEPoints.cpp).  The smoothing on the GPU, normed that way and printed
+    with 6 significant digits as cout does, gives the recorded values."""
+    from scipy.special import eval_jacobi
+    res, infos = nio.c0_helmholtz_smoothing(G + ".xml", G + ".fld", 0.6, tol=1e-12, numpoints=4)
+    assert all(i.converged for i in infos.values())
+    mesh = nio.read_mesh_xml(G + ".xml")
+    eids, lv, verts = nio.hex_elements(mesh)
+    jac, _ = nio.hex_geometry(verts)
+    nm = 4
+    ze = -1 + 2 * np.arange(nm) / (nm - 1)
+    zg, wg = O.points(O.GLL, nm)[:2]
+
+    def lag(i, x):
+        return np.prod([(x - ze[j]) / (ze[i] - ze[j]) for j in range(nm) if j != i])
+
+    we = np.array([sum(wg[j] * lag(i, zg[j]) for j in range(nm)) for i in range(nm)])
+
+    def phi(p, z):
+        if p < 2:
+            return (1 - z) / 2 if p == 0 else (1 + z) / 2
+        return (1 - z) / 2 * (1 + z) / 2 * eval_jacobi(p - 2, 1, 1, z)
+
+    B = np.array([[phi(p, z) for p in range(nm)] for z in ze])
+    Phi = np.kron(B, np.kron(B, B))
+    W3 = np.kron(we, np.kron(we, we))
+    expected = {"u": 0.635792, "v": 0.635792, "w": 0.635792, "p": 1.10303}
+    for f, want in expected.items():
+        norm = np.sqrt(np.sum((res.data[f] @ Phi.T) ** 2 * W3[None, :] * jac[:, None]))
+        printed = float(f"{norm:.6g}")
+        assert abs(printed - want) <= 1e-6, (f, norm, want)
