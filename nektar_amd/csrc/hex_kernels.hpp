@@ -116,8 +116,26 @@ struct MappedFetch {
     }
 };
 
+// LDS row pitch of an element's coefficients staged as NM*NM rows (r,q)
+// of NM values: odd, so that lanes reading one row each (lane stride one
+// row) fall on different banks.
+template <int NM>
+constexpr int cpitch()
+{
+    return NM % 2 == 0 ? NM + 1 : NM;
+}
+// the staged coefficients fit one slab for every NQ >= NM
+template <int NM, int NQ>
+constexpr bool coeffs_fit_slab()
+{
+    return NM * NM * cpitch<NM>() <= slab<NQ>();
+}
+
 // ---- forward transform: coefficients -> u column (k) of item (i,j) in
-//      registers.
+//      registers.  (Copying element-local coefficients to LDS first, for
+//      contiguous loads, measured slower: BwdTrans P=6 56 against 43 us,
+//      Helmholtz 107 against 100 us; the barrier it needs keeps the
+//      coefficient loads from overlapping the first table rows.)
 template <typename T, int NM, int NQ, typename Fetch>
 __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const Fetch &xin, bool valid, int t,
                                               T *A, T *Bs, T (&u)[NQ])
@@ -156,12 +174,17 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
 }
 
 // ---- transpose transform: y column (registers, at quadrature points,
-//      already weighted) -> coefficients (global)
+//      already weighted) -> coefficients (global).  In FP64 the
+//      coefficients are staged in LDS and written by consecutive work-items
+//      (contiguous stores per wave): IProduct P=6 44 against 54 us, the
+//      fused Helmholtz unchanged.  FP32 stores them from the registers
+//      (staged: Helmholtz 61 against 57 us, IProduct 23.5 against 22 us).
 template <typename T, int NM, int NQ>
 __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const T (&y)[NQ], bool valid,
                                                   int t, T *Bs, T *C, T *__restrict__ xout)
 {
-    constexpr int P = pitch<NQ>();
+    constexpr int P = pitch<NQ>(), PM = cpitch<NM>();
+    static_assert(coeffs_fit_slab<NM, NQ>(), "staged coefficients exceed the slab");
     {
         const int i = t % NQ, j = t / NQ;
         T o[NM];
@@ -186,9 +209,18 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
 #pragma unroll
         for (int i = 0; i < NQ; ++i) z[i] = C[(r * NM + q) * P + i];
         contract<false>(z, [&](int p) { return tab.Brow(p); }, o);
-        if (valid) {
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll
+            for (int p = 0; p < NM; ++p) Bs[(r * NM + q) * PM + p] = o[p];
+        } else if (valid) {
 #pragma unroll
             for (int p = 0; p < NM; ++p) xout[(r * NM + q) * NM + p] = o[p];
+        }
+    }
+    if constexpr (sizeof(T) == 8) {
+        __syncthreads();
+        if (valid) {
+            for (int m = t; m < NM * NM * NM; m += NQ * NQ) xout[m] = Bs[(m / NM) * PM + m % NM];
         }
     }
 }
