@@ -25,6 +25,15 @@
 // 250 us for the run-time kernel (5.6 against 2.3 TB/s); two or four
 // elements per workgroup, stage 1 writing over stage 0's buffer, or stages
 // 1 and 2 fused per output column all measured 2-12% slower.
+//
+// Where an element's output is a whole number of 128-byte lines (hex
+// Q = 8 -> 12 in FP64 and FP32, among others) the compile-time kernel
+// writes it with non-temporal stores: 88 against 105 us at hex FP64
+// (6.7 TB/s), 49 against 58 us in FP32.  Where it is not (the tet sizes:
+// 11,616 bytes per element at Q = 8), partial lines reach memory unmerged
+// and the same stores measured slower (tet FP64 124 against 97 us), so
+// those keep ordinary stores, as does the run-time kernel.  Non-temporal
+// loads of the input measured slower as well (119-125 against 105 us).
 #include <hip/hip_runtime.h>
 
 #include "launch.hpp"
@@ -145,6 +154,8 @@ __global__ __launch_bounds__(kFixedThreads) void phys_interp_fixed(const T *__re
 {
     constexpr int NIN = I0 * I1 * I2, NOUT = O0 * O1 * O2;
     constexpr int L0 = I1 * I2, L1 = O0 * I2, L2 = O0 * O1;
+    // non-temporal output stores where an element's output is whole 128-byte lines
+    constexpr bool kStream = (NOUT * sizeof(T)) % 128 == 0;
     __shared__ T B[O0 * I1 * I2];
     __shared__ T A[O0 * O1 * I2];
     const T *__restrict__ IT0 = tab;
@@ -189,7 +200,12 @@ __global__ __launch_bounds__(kFixedThreads) void phys_interp_fixed(const T *__re
 #pragma unroll
             for (int r = 0; r < I2; ++r) v[r] = A[r * L2 + l];
 #pragma unroll
-            for (int k = 0; k < O2; ++k) y[k * L2 + l] = dot_fixed(v, IT2 + k * I2);
+            for (int k = 0; k < O2; ++k) {
+                if constexpr (kStream)
+                    __builtin_nontemporal_store(dot_fixed(v, IT2 + k * I2), y + k * L2 + l);
+                else
+                    y[k * L2 + l] = dot_fixed(v, IT2 + k * I2);
+            }
         }
     }
 }
