@@ -3,8 +3,10 @@ error <= 1e-14 (FP64, the same fused products in the same order) and
 <= 1e-6 (FP32), hex and tet, Q = P+2 and P+3, scaling factors 1, 1.5
 (default) and 2; every size of the default factor (Q = 3..12, which run
 on the compile-time-size kernel; the other factors run on the run-time-size
-kernel); the factor setter changes the output size; factors whose output
-would not fit the kernel's staging are rejected on the host."""
+kernel); polynomial reproduction at the benchmark's size (32,768
+elements, hex and tet, Q = 8); the factor setter changes the output size;
+factors whose output would not fit the kernel's staging are rejected on the
+host."""
 import numpy as np
 import pytest
 
@@ -77,6 +79,30 @@ def test_every_default_factor_size(nmf, shape, fp32):
         # their own point sets, which differ in the last bit; through three
         # 1-D sums of up to 12 terms that reaches 1.04e-14 at tet Q = 12
         assert G.rel_err(y, ref) <= (1e-6 if fp32 else 2e-14), nq0
+
+
+@pytest.mark.parametrize("shape", [O.HEX, O.TET], ids=["hex", "tet"])
+def test_full_size_reproduces_polynomials(nmf, shape):
+    """32,768 elements at Q = 8 and the default factor (the sizes of the
+    per-operator benchmark): a product of polynomials of degree n_d - 1 in
+    each (collapsed) direction is reproduced exactly at the output points
+    of the same families, with a different scale in every element."""
+    nel, nq0 = 32768, 8
+    ni = [nq0] * 3 if shape == O.HEX else [nq0, nq0 - 1, nq0 - 1]
+    no = O.interp_nq(shape, nq0, 1.5)
+    fam = [O.GLL] * 3 if shape == O.HEX else [O.GLL, O.RADAU_A1, O.RADAU_A2]
+    rng = np.random.default_rng(5)
+    coef = [rng.uniform(-1.0, 1.0, n) for n in ni]
+
+    def line(d, n):
+        return np.polynomial.polynomial.polyval(O.points(fam[d], n)[0], coef[d])
+
+    fin = np.einsum("k,j,i->kji", line(2, ni[2]), line(1, ni[1]), line(0, ni[0])).ravel()
+    fout = np.einsum("k,j,i->kji", line(2, no[2]), line(1, no[1]), line(0, no[0])).ravel()
+    scale = 1.0 + np.arange(nel) / nel
+    op = make(nmf, shape, nq0 - 2, nq0, nel)
+    y = apply(op, (scale[:, None] * fin[None, :]).ravel(), False)
+    assert G.rel_err(y, (scale[:, None] * fout[None, :]).ravel()) <= 1e-12
 
 
 def test_default_factor_and_host_apply(nmf):
