@@ -16,8 +16,9 @@
 // then FMAs in increasing input index.  The operation is HBM-bound (it
 // writes factor^3 times what it reads).
 //
-// The sizes of the default factor 1.5 (hex and tet, Q = 3..12) also have a
-// compile-time-size kernel, phys_interp_fixed: one element per 128-thread
+// The sizes of the default factor 1.5 (hex and tet, Q = 3..12) and of
+// factor 2 (Q = 3..11) also have a compile-time-size kernel,
+// phys_interp_fixed: one element per 128-thread
 // workgroup, stage 0 reading its input line straight from global memory
 // into registers, every loop unrolled, the same layouts and the same
 // summation order (so its results are bitwise those of the run-time
@@ -234,6 +235,17 @@ bool launch_fixed(const InterpArgs &a, hipError_t &e)
            NMF_INTERP_TET(5, 7) || NMF_INTERP_TET(6, 9) || NMF_INTERP_TET(7, 10) || NMF_INTERP_TET(8, 12) ||
            NMF_INTERP_TET(9, 13) || NMF_INTERP_TET(10, 15) || NMF_INTERP_TET(11, 16) || NMF_INTERP_TET(12, 18);
 }
+// factor 2: hex Q^3 -> (2Q)^3, tet (Q, Q-1, Q-1) -> (2Q, 2Q-1, 2Q-1), for
+// Q <= 11 (Q = 12 exceeds the 64 KiB staging and is rejected on the host)
+template <typename T>
+bool launch_fixed_f2(const InterpArgs &a, hipError_t &e)
+{
+    return NMF_INTERP_HEX(3, 6) || NMF_INTERP_HEX(4, 8) || NMF_INTERP_HEX(5, 10) || NMF_INTERP_HEX(6, 12) ||
+           NMF_INTERP_HEX(7, 14) || NMF_INTERP_HEX(8, 16) || NMF_INTERP_HEX(9, 18) || NMF_INTERP_HEX(10, 20) ||
+           NMF_INTERP_HEX(11, 22) || NMF_INTERP_TET(3, 6) || NMF_INTERP_TET(4, 8) || NMF_INTERP_TET(5, 10) ||
+           NMF_INTERP_TET(6, 12) || NMF_INTERP_TET(7, 14) || NMF_INTERP_TET(8, 16) || NMF_INTERP_TET(9, 18) ||
+           NMF_INTERP_TET(10, 20) || NMF_INTERP_TET(11, 22);
+}
 #undef NMF_INTERP_HEX
 #undef NMF_INTERP_TET
 
@@ -268,7 +280,7 @@ template <typename T>
 hipError_t launch_t(const InterpArgs &a, size_t lds)
 {
     hipError_t e = hipSuccess;
-    if (launch_fixed<T>(a, e)) return e;
+    if (launch_fixed<T>(a, e) || launch_fixed_f2<T>(a, e)) return e;
     const int m = a.ni[0] > a.ni[1] ? (a.ni[0] > a.ni[2] ? a.ni[0] : a.ni[2]) : (a.ni[1] > a.ni[2] ? a.ni[1] : a.ni[2]);
     if (m <= 4) return launch_mi<T, 4>(a, lds);
     if (m <= 8) return launch_mi<T, 8>(a, lds);

@@ -1,9 +1,9 @@
 """GPU parity of PhysInterp1DScaled against the FP64 oracle: relative max
 error <= 1e-14 (FP64, the same fused products in the same order) and
 <= 1e-6 (FP32), hex and tet, Q = P+2 and P+3, scaling factors 1, 1.5
-(default) and 2; every size of the default factor (Q = 3..12, which run
-on the compile-time-size kernel; the other factors run on the run-time-size
-kernel); polynomial reproduction at the benchmark's size (32,768
+(default) and 2; every size of factors 1.5 (Q = 3..12) and 2 (Q = 3..11),
+which run on the compile-time-size kernel (other factors run on the
+run-time-size kernel); polynomial reproduction at the benchmark's size (32,768
 elements, hex and tet, Q = 8); the factor setter changes the output size;
 factors whose output would not fit the kernel's staging are rejected on the
 host."""
@@ -66,15 +66,17 @@ def test_against_oracle(nmf, shape, nm, fp32):
 
 @pytest.mark.parametrize("shape", [O.HEX, O.TET], ids=["hex", "tet"])
 @pytest.mark.parametrize("fp32", [False, True], ids=["fp64", "fp32"])
-def test_every_default_factor_size(nmf, shape, fp32):
+@pytest.mark.parametrize("factor", [1.5, 2.0], ids=["f1.5", "f2"])
+def test_every_compiled_size(nmf, shape, fp32, factor):
     nel = 29
-    for nq0 in range(3, 13):
+    for nq0 in range(3, 13 if factor == 1.5 else 12):
         op = make(nmf, shape, max(nq0 - 2, 2), nq0, nel, fp32)
-        no = O.interp_nq(shape, nq0, 1.5)
+        op.SetScalingFactor(factor)
+        no = O.interp_nq(shape, nq0, factor)
         assert op.output_size() == nel * no[0] * no[1] * no[2]
         u = G.coeffs(op.input_size(), 300 + nq0)
         y = apply(op, u, fp32)
-        ref = O.phys_interp1d_scaled(shape, nq0, nel, 1.5, u)
+        ref = O.phys_interp1d_scaled(shape, nq0, nel, factor, u)
         # the library and the oracle evaluate the interpolation tables from
         # their own point sets, which differ in the last bit; through three
         # 1-D sums of up to 12 terms that reaches 1.04e-14 at tet Q = 12
