@@ -31,6 +31,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "contract.hpp"
 #include "launch.hpp"
 
@@ -82,6 +84,22 @@ template <typename T, int NM>
 struct LocalFetch {
     const T *__restrict__ xin;
     __device__ __forceinline__ T operator()(int p, int q, int r) const { return xin[(r * NM + q) * NM + p]; }
+    // the NM contiguous coefficients of line (q, r) with 16-byte loads (the
+    // line starts on a value boundary only, which global loads allow)
+    __device__ __forceinline__ void line(int q, int r, T (&x)[NM]) const
+    {
+        constexpr int VW = 16 / sizeof(T);
+        using V = T __attribute__((ext_vector_type(VW), aligned(sizeof(T))));
+        const T *l = xin + (r * NM + q) * NM;
+#pragma unroll
+        for (int c = 0; c < NM / VW; ++c) {
+            const V v = *reinterpret_cast<const V *>(l + c * VW);
+#pragma unroll
+            for (int w = 0; w < VW; ++w) x[c * VW + w] = v[w];
+        }
+#pragma unroll
+        for (int p = (NM / VW) * VW; p < NM; ++p) x[p] = l[p];
+    }
 };
 
 // Coefficient fetch straight from a global C0 vector of the structured
@@ -145,8 +163,17 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
     if (t < NM * NM) {
         const int q = t % NM, r = t / NM;
         T x[NM], o[NQ];
+        if constexpr (std::is_same<Fetch, LocalFetch<T, NM>>::value) {
+            if (valid) {
+                xin.line(q, r, x);
+            } else {
 #pragma unroll
-        for (int p = 0; p < NM; ++p) x[p] = valid ? xin(p, q, r) : T(0);
+                for (int p = 0; p < NM; ++p) x[p] = T(0);
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < NM; ++p) x[p] = valid ? xin(p, q, r) : T(0);
+        }
         contract<false>(x, [&](int i) { return tab.BTrow(i); }, o);
 #pragma unroll
         for (int i = 0; i < NQ; ++i) A[(r * NM + q) * P + i] = o[i];
