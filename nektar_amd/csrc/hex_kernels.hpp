@@ -206,6 +206,9 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
 //      (contiguous stores per wave): IProduct P=6 44 against 54 us, the
 //      fused Helmholtz unchanged.  FP32 stores them from the registers
 //      (staged: Helmholtz 61 against 57 us, IProduct 23.5 against 22 us).
+//      Per-work-item 16-byte vector stores instead of the staging measured
+//      equal for Helmholtz and slower for FP64 IProduct (52-60 against
+//      43 us).
 template <typename T, int NM, int NQ>
 __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const T (&y)[NQ], bool valid,
                                                   int t, T *Bs, T *C, T *__restrict__ xout)
