@@ -132,7 +132,36 @@ struct MappedFetch {
         const T v = g[m < 0 ? ~m : m];
         return m < 0 ? -v : v;
     }
+    // line (q, r): its NM contiguous map entries with 16-byte loads, then
+    // the NM gathers
+    __device__ __forceinline__ void line(int q, int r, T (&x)[NM]) const
+    {
+        using V = int __attribute__((ext_vector_type(4), aligned(4)));
+        const int *l = me + (r * NM + q) * NM;
+        int m[NM];
+#pragma unroll
+        for (int c = 0; c < NM / 4; ++c) {
+            const V v = *reinterpret_cast<const V *>(l + c * 4);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) m[c * 4 + w] = v[w];
+        }
+#pragma unroll
+        for (int p = (NM / 4) * 4; p < NM; ++p) m[p] = l[p];
+#pragma unroll
+        for (int p = 0; p < NM; ++p) {
+            const T v = g[m[p] < 0 ? ~m[p] : m[p]];
+            x[p] = m[p] < 0 ? -v : v;
+        }
+    }
 };
+
+// fetches that read a whole line (q, r) at once
+template <typename F>
+struct HasLine : std::false_type {};
+template <typename T, int NM>
+struct HasLine<LocalFetch<T, NM>> : std::true_type {};
+template <typename T, int NM>
+struct HasLine<MappedFetch<T, NM>> : std::true_type {};
 
 // LDS row pitch of an element's coefficients staged as NM*NM rows (r,q)
 // of NM values: odd, so that lanes reading one row each (lane stride one
@@ -163,7 +192,7 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const F
     if (t < NM * NM) {
         const int q = t % NM, r = t / NM;
         T x[NM], o[NQ];
-        if constexpr (std::is_same<Fetch, LocalFetch<T, NM>>::value) {
+        if constexpr (HasLine<Fetch>::value) {
             if (valid) {
                 xin.line(q, r, x);
             } else {
