@@ -280,34 +280,13 @@ def bench_general(N, torch, n=32, nm=7, nq=8, lam=1.0):
             "solution_rel_diff": diff, "negative_signs": int((sg < 0).sum())}
 
 
-def batch_workload(N, seed=4, count=64):
-    """64 Helmholtz applies: hex/tet 50/50, P ~ U{3..9}, Q = P+2, E
-    log-uniform in [1e3, 1e5], affine elements with per-element scale
-    U[0.5, 2], coefficients U[-1, 1].  Returns [(shape, nm, nq, E, jac, df,
-    lam)]."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        shape = "Hex" if i % 2 == 0 else "Tet"
-        nm = int(rng.integers(4, 11))
-        E = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
-        s = rng.uniform(0.5, 2.0, E)[:, None, None]
-        A = np.eye(3)[None] + 0.25 * rng.uniform(-1, 1, (E, 3, 3))
-        M = s * A * 0.05  # element size ~ 0.1
-        jac = np.abs(np.linalg.det(M)) / (1.0 if shape == "Hex" else 8.0)
-        Minv = np.linalg.inv(M if shape == "Hex" else M / 2)
-        df = np.ascontiguousarray(np.transpose(Minv, (0, 2, 1))).reshape(-1)  # df[j*3+k] = dxi_k/dx_j
-        out.append((shape, nm, nm + 1, E, jac, df, float(rng.uniform(0.1, 10))))
-    return out
-
-
 def bench_batch(N, torch, steps, warmup):
     """Batched uneven workload: Sum of element-local DOFs / time of one
     batch submission (graph of grouped launches), FP64 and FP32, against
     the entry-by-entry loop."""
-    work = batch_workload(N)
+    from nektar_amd.workloads import batch_workload
+
+    work = batch_workload()
     res = {"items": len(work)}
     for prec, tag in ((N.FP64, "fp64"), (N.FP32, "fp32")):
         dt = torch.float64 if prec == N.FP64 else torch.float32
@@ -374,27 +353,12 @@ def bench_curved(N):
     return out
 
 
-def serve_requests(count, seed=5):
-    """Helmholtz solve requests: hex box with E log-uniform in [2^10, 2^17]
-    (n = round(E^(1/3))), P ~ U{3..8}, Q = P+2, lambda ~ U[0.1, 10], k ~
-    U{1,2,3}, manufactured sine forcing, tol 1e-10."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        E = float(np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 17))))
-        P = int(rng.integers(3, 9))
-        out.append(dict(n=max(1, int(round(E ** (1 / 3)))), nummodes=P + 1, numpoints=P + 2,
-                        lam=float(rng.uniform(0.1, 10.0)), k=int(rng.integers(1, 4)), tol=1e-10, maxit=5000))
-    return out
-
-
 def bench_serve(N, count, budget_gib, streams, max_active):
     """Serving stream: capacity of plain FP64 one-at-a-time solves, Poisson
     arrivals at 80 % of that capacity served one at a time and by the
     serving loop (FP64 and mixed), and the serving loop's own capacity."""
     import numpy as np
+    from nektar_amd.workloads import poisson_arrivals, serve_requests
 
     reqs = serve_requests(count)
     budget = int(budget_gib * (1 << 30))
@@ -422,8 +386,7 @@ def bench_serve(N, count, budget_gib, streams, max_active):
            "unknowns_total": None}
     cap1 = run(zero, 0, 1, 1)
     rate = 0.8 * cap1["solves_per_s"]
-    arr = np.cumsum(np.random.default_rng(55).exponential(1.0 / rate, count))
-    arr -= arr[0]
+    arr = poisson_arrivals(count, rate)
     out["poisson_rate_per_s"] = round(rate, 3)
     out["fp64_one_at_a_time_capacity"] = cap1
     out["fp64_one_at_a_time_poisson"] = run(arr, 0, 1, 1)

@@ -138,6 +138,19 @@ int nmf_operator_apply(nmf_operator op, const void *in, void *out,
 /* PhysDeriv operator()(input, output[3]) (Operator.hpp:238-240) */
 int nmf_operator_apply_physderiv(nmf_operator op, const void *in, void *out0,
                                  void *out1, void *out2, void *stream);
+/* Collections::Operator::GetWspSize (Collections/Operator.h:178-230):
+ * workspace values (of the operator's precision) that the single-direction
+ * PhysDeriv form needs, 2*nelmt*nqtot; 0 for the other operators. */
+int nmf_operator_wsp_size(nmf_operator op, int64_t *wsp_size);
+/* PhysDeriv operator()(dir, input, output, wsp) (Collections/PhysDeriv.cpp
+ * :312-317): the derivative d/dx_dir (dir 0..2) into out; the other two
+ * directions are computed into wsp (device, nmf_operator_wsp_size values),
+ * as the reference computes all three and copies one.  wsp NULL: a
+ * stream-ordered workspace is allocated and freed for this call (the
+ * reference's Collection::ApplyOperator allocates one per call,
+ * Collection.h:148-155). */
+int nmf_operator_apply_physderiv_dir(nmf_operator op, int dir, const void *in,
+                                     void *out, void *wsp, void *stream);
 /* IProductWRTDerivBase operator()(input[3], output) (Operator.hpp:192-217,
  * IProductWRTDerivBase.h:592-825): out = sum_j IProduct w.r.t. dphi/dx_j of
  * in_j; inputs [e][nqtot] at the quadrature points, output [e][ncoeff].

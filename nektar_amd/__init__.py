@@ -87,6 +87,8 @@ _SIGNATURES = {
     "nmf_operator_io_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "nmf_operator_apply": (ctypes.c_int, [_P, _P, _P, _P]),
     "nmf_operator_apply_physderiv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "nmf_operator_wsp_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "nmf_operator_apply_physderiv_dir": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P]),
     "nmf_operator_apply_host": (ctypes.c_int, [_P, _D, _D]),
     "nmf_operator_apply_physderiv_host": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "nmf_operator_apply_iproduct_deriv": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
@@ -308,6 +310,27 @@ class Operator:
             check(lib().nmf_operator_apply_physderiv(self._h, pin, *ptrs, s))
         else:
             check(lib().nmf_operator_apply(self._h, pin, _device_ptr(out, dt, self.output_size()), s))
+
+    def GetWspSize(self) -> int:
+        """Collections::Operator::GetWspSize: workspace values of the
+        single-direction PhysDeriv form (0 for the other operators)."""
+        w = ctypes.c_int64()
+        check(lib().nmf_operator_wsp_size(self._h, ctypes.byref(w)))
+        return w.value
+
+    def apply_dir(self, direction: int, inp, out, wsp=None, stream=None):
+        """PhysDeriv ``operator()(dir, input, output, wsp)``
+        (Collections/PhysDeriv.cpp:312-317): the derivative in one
+        direction; wsp (GetWspSize() values) or None for a per-call one."""
+        if self.op != "PhysDeriv":
+            raise NektarError(1, "apply_dir: not a PhysDeriv operator")
+        dt = self.torch_dtype() if torch is not None else None
+        s = stream
+        if s is None and torch is not None:
+            s = torch.cuda.current_stream().cuda_stream
+        w = None if wsp is None else _device_ptr(wsp, dt, self.GetWspSize())
+        check(lib().nmf_operator_apply_physderiv_dir(self._h, int(direction), _device_ptr(inp, dt, self.input_size()),
+                                                     _device_ptr(out, dt, self.output_size()), w, s))
 
     # --- workgroup tiles ---------------------------------------------------
     def tile_candidates(self) -> list:

@@ -147,6 +147,19 @@ IterativeSolver::IterativeSolver(const AssemblyMap &map, const std::vector<MFOpe
     for (MFOperator *o : ops_) o->prepare();
     build_diagonal();
     if (space_ == CGSpace::StaticCond) setup_static_cond();
+    for (MFOperator *o : ops_) versions_.push_back(o->version);
+}
+
+void IterativeSolver::refresh()
+{
+    bool changed = false;
+    for (size_t k = 0; k < ops_.size(); ++k) changed |= ops_[k]->version != versions_[k];
+    if (!changed) return;
+    NMF_HIP(hipStreamSynchronize(stream_));
+    for (MFOperator *o : ops_) o->prepare();
+    build_diagonal();
+    if (space_ == CGSpace::StaticCond) setup_static_cond();
+    for (size_t k = 0; k < ops_.size(); ++k) versions_[k] = ops_[k]->version;
 }
 
 IterativeSolver::~IterativeSolver()
@@ -207,6 +220,7 @@ void IterativeSolver::build_diagonal()
 void IterativeSolver::sc_diagonal(double *dglob)
 {
     NMF_REQUIRE(dglob || map_.nglobal() == 0, NMF_ERR_INVALID, "null output");
+    refresh();
     // bytes of element blocks per chunk (NMF_SC_CHUNK_BYTES overrides the 8 GiB default)
     size_t kBudget = (size_t)8 << 30;
     if (const char *v = std::getenv("NMF_SC_CHUNK_BYTES"))
@@ -332,6 +346,7 @@ void IterativeSolver::solution_local(double *xloc)
 
 void IterativeSolver::apply_global(const double *x, double *y)
 {
+    refresh();
     apply_from_global(x, yloc_.as<double>());
     map_.assemble(yloc_.as<double>(), y, stream_, true);
 }
@@ -350,6 +365,7 @@ SolveInfo IterativeSolver::solve(double tol, int maxit)
 {
     NMF_REQUIRE(have_rhs_, NMF_ERR_STATE, "right-hand side not set");
     NMF_REQUIRE(maxit >= 0, NMF_ERR_INVALID, "maxit must be >= 0");
+    refresh();
     if (space_ == CGSpace::StaticCond) return solve_static_cond(tol, maxit);
     tol = std::max(tol, 1.0e-16);  // NekLinSysIter.cpp:57
     const int NG = map_.nglobal(), ND = map_.ndir();

@@ -81,8 +81,11 @@ class IterativeSolver {
 public:
     // ops: Helmholtz operators (FP64) whose element-local vectors,
     // concatenated in this order, are the map's local vector.  The operators
-    // must outlive the solver and keep their geometry and lambda while it
-    // is used (the diagonal is built here).
+    // must outlive the solver.  The diagonal (and, for StaticCond, the
+    // interior inverses) is built here from the operators' geometry and
+    // lambda; a later change to an operator (its version counter) makes the
+    // next call rebuild them, as the reference builds a new GlobalLinSys for
+    // a new lambda key.
     IterativeSolver(const AssemblyMap &map, const std::vector<MFOperator *> &ops, hipStream_t stream,
                     CGSpace space = CGSpace::Global);
     ~IterativeSolver();
@@ -93,6 +96,9 @@ public:
     // IProductWRTBase of the forcing), ContField::v_HelmSolve's; the global
     // CG solves with b = Assemble(bloc), the local CG starts from r = bloc
     void set_rhs_local(const double *bloc);
+    // rebuild what depends on the operators if any of them changed since
+    // the last build (called by every public entry that uses them)
+    void refresh();
     SolveInfo solve(double tol, int maxit);
     // y = A x on global device vectors, Dirichlet rows zeroed
     void apply_global(const double *x, double *y);
@@ -130,6 +136,7 @@ private:
     const AssemblyMap &map_;
     std::vector<MFOperator *> ops_;
     std::vector<int64_t> off_;
+    std::vector<uint64_t> versions_;  // the operators' versions the diagonal was built from
     hipStream_t stream_;
     CGSpace space_;
     int nb_ = 1;

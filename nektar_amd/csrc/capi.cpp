@@ -223,6 +223,41 @@ int nmf_operator_apply_physderiv(nmf_operator op, const void *in, void *out0, vo
     });
 }
 
+int nmf_operator_wsp_size(nmf_operator op, int64_t *wsp_size)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        NMF_REQUIRE(wsp_size, NMF_ERR_INVALID, "null output");
+        *wsp_size = (int64_t)op->impl->wsp_size();
+    });
+}
+
+int nmf_operator_apply_physderiv_dir(nmf_operator op, int dir, const void *in, void *out, void *wsp, void *stream)
+{
+    return guard([&] {
+        NMF_OP_CHECK();
+        MFOperator &o = *op->impl;
+        NMF_REQUIRE(o.op == OpType::PhysDeriv, NMF_ERR_INVALID, "not a PhysDeriv operator");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (wsp || o.nelmt == 0) {
+            o.apply_physderiv_dir(dir, in, out, wsp, s);
+            return;
+        }
+        // no workspace given: a stream-ordered one for this call, as the
+        // reference's Collection::ApplyOperator allocates one per call
+        void *w = nullptr;
+        const hipError_t e = hipMallocAsync(&w, o.wsp_size() * o.value_bytes(), s);
+        NMF_REQUIRE(e == hipSuccess, NMF_ERR_MEMORY, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+        try {
+            o.apply_physderiv_dir(dir, in, out, w, s);
+        } catch (...) {
+            (void)hipFreeAsync(w, s);
+            throw;
+        }
+        NMF_HIP(hipFreeAsync(w, s));
+    });
+}
+
 namespace {
 size_t in_count(const MFOperator &o) { return (size_t)o.nelmt * o.in_per_elmt(); }
 size_t out_count(const MFOperator &o) { return (size_t)o.nelmt * o.out_per_elmt(); }
@@ -813,6 +848,7 @@ int nmf_linsys_get_inv_diag(nmf_linsys l, double *d_host)
     return guard([&] {
         NMF_LS_CHECK();
         NMF_REQUIRE(d_host, NMF_ERR_INVALID, "null output");
+        l->impl->refresh();
         NMF_HIP(hipDeviceSynchronize());
         NMF_HIP(hipMemcpy(d_host, l->impl->inv_diag(), (size_t)l->impl->nglobal() * sizeof(double),
                           hipMemcpyDeviceToHost));

@@ -402,6 +402,18 @@ void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipSt
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
 }
 
+void MFOperator::apply_physderiv_dir(int dir, const void *in, void *out, void *wsp, hipStream_t s)
+{
+    NMF_REQUIRE(op == OpType::PhysDeriv, NMF_ERR_INVALID, "not a PhysDeriv operator");
+    NMF_REQUIRE(dir >= 0 && dir < 3, NMF_ERR_INVALID, "PhysDeriv direction must be 0, 1 or 2");
+    NMF_REQUIRE(nelmt == 0 || wsp, NMF_ERR_INVALID, "null workspace");
+    void *o[3];
+    char *w = static_cast<char *>(wsp);
+    const size_t half = (size_t)nelmt * nqtot * value_bytes();
+    for (int d = 0, k = 0; d < 3; ++d) o[d] = d == dir ? out : w + half * (k++);
+    apply(in, o[0], o[1], o[2], s);
+}
+
 void MFOperator::apply_mapped(const void *xg, const int *menc, void *out, hipStream_t s)
 {
     NMF_REQUIRE(can_apply_mapped(), NMF_ERR_INVALID, "mapped apply: FP64 hex Helmholtz operators only");

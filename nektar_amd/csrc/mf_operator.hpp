@@ -79,6 +79,20 @@ struct MFOperator {
     // through the signed map menc, this collection's nelmt * ncoeff entries)
     void apply_mapped(const void *xg, const int *menc, void *out, hipStream_t s);
     bool can_apply_mapped() const { return shape == Shape::Hex && op == OpType::Helmholtz && !fp32; }
+    // PhysDeriv operator()(dir, in, out, wsp) (Collections/PhysDeriv.cpp
+    // :312-317): all three derivatives, the two not asked for into wsp
+    // (wsp_size() values of the operator's precision)
+    void apply_physderiv_dir(int dir, const void *in, void *out, void *wsp, hipStream_t s);
+    // Collections::Operator::GetWspSize: workspace values of the
+    // single-direction PhysDeriv form (0 for the other operators)
+    size_t wsp_size() const { return op == OpType::PhysDeriv ? (size_t)2 * nelmt * nqtot : 0; }
+    size_t value_bytes() const { return fp32 ? sizeof(float) : sizeof(double); }
+    // the device geometry matches the host factors (no upload pending)
+    bool geometry_current() const
+    {
+        const bool needs = (needs_jac() || needs_df()) && op != OpType::BwdTrans && op != OpType::PhysInterp;
+        return !(needs && geo_dirty) && !(op == OpType::AdvDiffReact && adv_dirty);
+    }
     // IProductWRTDerivBase: three inputs at the quadrature points, one output
     void apply_iproduct_deriv(const void *in0, const void *in1, const void *in2, void *out, hipStream_t s);
 

@@ -40,31 +40,52 @@ static inline vd vfma(vd acc, vd a, vd b) { vd r; for (int l = 0; l < VW; ++l) r
 static inline vd vfmas(vd acc, vd a, double b) { vd r; for (int l = 0; l < VW; ++l) r.v[l] = fma(a.v[l], b, acc.v[l]); return r; }
 
 /* ------------------------------------------------------------------ */
-/* Per-operator tables: MatrixFree::Helper (Operator.hpp:529-641).     */
+/* Per-operator tables: MatrixFree::Helper (Operator.hpp:529-641), with
+ * modes nmd[d] and points nq[d] per direction (the runtime-size operators,
+ * Helmholtz.h:374-464, take any of them; the MODIFIED defaults have equal
+ * modes and nq = {Q, Q, Q} (hex) or {Q, Q-1, Q-1} (tet)).  nm = nmd[0]. */
 typedef struct {
-    int shape, nm, nq[3], nqtot, ncoeff;
+    int shape, nm, nmd[3], nq[3], nqtot, ncoeff;
     double *b[3], *db[3], *D[3], *w[3], *Z[3];
     double h0[64], h1[64], h2[64], h3[64];
 } tables_t;
 
-static void tables_init(tables_t *t, int shape, int nm, int nq0)
+/* Coefficients of a tet with nm0 <= nm1 <= nm2 modes: p < nm0, q < nm1 - p,
+ * r < nm2 - p - q (the packing BwdTransKernels.hpp:244-266 walks). */
+static int tet_ncoeffs3(int nm0, int nm1, int nm2)
+{
+    int n = 0;
+    for (int p = 0; p < nm0; ++p)
+        for (int q = 0; q < nm1 - p; ++q) n += nm2 - p - q;
+    return n;
+}
+
+int or_ncoeffs_aniso(int shape, const int nm[3])
+{
+    return shape == OR_HEX ? nm[0] * nm[1] * nm[2] : tet_ncoeffs3(nm[0], nm[1], nm[2]);
+}
+
+static void tables_init_full(tables_t *t, int shape, const int nm[3], const int nq[3])
 {
     t->shape = shape;
-    t->nm = nm;
-    or_default_nq(shape, nq0, t->nq);
+    t->nm = nm[0];
+    for (int d = 0; d < 3; ++d) {
+        t->nmd[d] = nm[d];
+        t->nq[d] = nq[d];
+    }
     t->nqtot = t->nq[0] * t->nq[1] * t->nq[2];
-    t->ncoeff = or_ncoeffs(shape, nm);
+    t->ncoeff = or_ncoeffs_aniso(shape, nm);
     for (int d = 0; d < 3; ++d) {
         int bt = shape == OR_HEX ? OR_MOD_A : d;
         int pt = shape == OR_HEX ? OR_GLL : d;
         int nq = t->nq[d];
-        int nmodes = or_basis_nmodes(bt, nm);
+        int nmodes = or_basis_nmodes(bt, nm[d]);
         t->b[d] = (double *)malloc((size_t)nmodes * nq * sizeof(double));
         t->db[d] = (double *)malloc((size_t)nmodes * nq * sizeof(double));
         t->D[d] = (double *)malloc((size_t)nq * nq * sizeof(double));
         t->w[d] = (double *)malloc((size_t)nq * sizeof(double));
         t->Z[d] = (double *)malloc((size_t)nq * sizeof(double));
-        or_basis(bt, nm, nq, t->b[d], t->db[d]);
+        or_basis(bt, nm[d], nq, t->b[d], t->db[d]);
         or_points(pt, nq, t->Z[d], t->w[d], t->D[d]);
         /* Operator.hpp:600-616: Radau(1,0) weights x0.5, Radau(2,0) x0.25 */
         double fac = pt == OR_RADAU_A1 ? 0.5 : pt == OR_RADAU_A2 ? 0.25 : 1.0;
@@ -507,11 +528,15 @@ typedef struct {
 
 static void ws_alloc(ws_t *w, const tables_t *T, int deformed)
 {
-    const int nqt = T->nqtot, nc = T->ncoeff, nm = T->nm;
+    const int nqt = T->nqtot, nc = T->ncoeff;
+    const size_t m0 = T->nmd[0], m1 = T->nmd[1], m2 = T->nmd[2];
+    const size_t q0 = T->nq[0], q1 = T->nq[1], q2 = T->nq[2];
     size_t big = (size_t)(nqt > nc ? nqt : nc);
-    size_t w0s = (size_t)T->nq[0] * nm * nm;
-    if ((size_t)T->nq[0] * T->nq[1] * nm > w0s) w0s = (size_t)T->nq[0] * T->nq[1] * nm;
-    if ((size_t)T->nq[1] * T->nq[2] > w0s) w0s = (size_t)T->nq[1] * T->nq[2];
+    /* bwd stages (q0 m1 m2, q1 q0 m2), iprod sums (q1 q2), tet pairs (m0 m1) */
+    size_t w0s = q0 * m1 * m2;
+    if (q0 * q1 * m2 > w0s) w0s = q0 * q1 * m2;
+    if (q1 * q2 > w0s) w0s = q1 * q2;
+    if (m0 * m1 > w0s) w0s = m0 * m1;
     if (big > w0s) w0s = big;
     w->in = (vd *)calloc(big, sizeof(vd));
     w->out = (vd *)calloc(big, sizeof(vd));
@@ -553,11 +578,11 @@ static void store_block(const vd *src, int nelmt, int b, int n, double *dst)
 
 static void bwd_block(const tables_t *T, const vd *in, ws_t *w, vd *out)
 {
-    const int nm = T->nm;
+    const int m0 = T->nmd[0], m1 = T->nmd[1], m2 = T->nmd[2];
     if (T->shape == OR_HEX)
-        bwd_hex(nm, nm, nm, T->nq[0], T->nq[1], T->nq[2], in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
+        bwd_hex(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
     else
-        bwd_tet(nm, nm, nm, T->nq[0], T->nq[1], T->nq[2], 1, in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
+        bwd_tet(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
 }
 
 static void iprod_block(const tables_t *T, int SCALE, int APPEND, int DEF,
@@ -565,12 +590,12 @@ static void iprod_block(const tables_t *T, int SCALE, int APPEND, int DEF,
                         const double *b2, const vd *jac, ws_t *w, vd *out,
                         double scale)
 {
-    const int nm = T->nm;
+    const int m0 = T->nmd[0], m1 = T->nmd[1], m2 = T->nmd[2];
     if (T->shape == OR_HEX)
-        iprod_hex(SCALE, APPEND, DEF, nm, nm, nm, T->nq[0], T->nq[1], T->nq[2], in, b0, b1, b2,
+        iprod_hex(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], in, b0, b1, b2,
                   T->w[0], T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
     else
-        iprod_tet(SCALE, APPEND, DEF, nm, nm, nm, T->nq[0], T->nq[1], T->nq[2], 1, in, b0, b1, b2,
+        iprod_tet(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, b0, b1, b2,
                   T->w[0], T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
 }
 
@@ -586,11 +611,11 @@ static void load_geom(const tables_t *T, int nelmt, int b, int deformed,
     if (df) load_block(df, nelmt, b, 9 * gq, w->df);
 }
 
-void or_bwdtrans(int shape, int nm, int nq0, int nelmt, const double *in,
+void or_bwdtrans_aniso(int shape, const int *nm, const int *nq, int nelmt, const double *in,
                  double *out)
 {
     tables_t T;
-    tables_init(&T, shape, nm, nq0);
+    tables_init_full(&T, shape, nm, nq);
     const int nblk = (nelmt + VW - 1) / VW;
 #pragma omp parallel num_threads(g_threads)
     {
@@ -607,11 +632,11 @@ void or_bwdtrans(int shape, int nm, int nq0, int nelmt, const double *in,
     tables_free(&T);
 }
 
-void or_iproduct(int shape, int nm, int nq0, int nelmt, int deformed,
+void or_iproduct_aniso(int shape, const int *nm, const int *nq, int nelmt, int deformed,
                  const double *jac, const double *in, double *out)
 {
     tables_t T;
-    tables_init(&T, shape, nm, nq0);
+    tables_init_full(&T, shape, nm, nq);
     const int nblk = (nelmt + VW - 1) / VW;
 #pragma omp parallel num_threads(g_threads)
     {
@@ -629,12 +654,12 @@ void or_iproduct(int shape, int nm, int nq0, int nelmt, int deformed,
     tables_free(&T);
 }
 
-void or_physderiv(int shape, int nm, int nq0, int nelmt, int deformed,
+void or_physderiv_aniso(int shape, const int *nm, const int *nq, int nelmt, int deformed,
                   const double *df, const double *in, double *out0,
                   double *out1, double *out2)
 {
     tables_t T;
-    tables_init(&T, shape, nm, nq0);
+    tables_init_full(&T, shape, nm, nq);
     const int nblk = (nelmt + VW - 1) / VW;
 #pragma omp parallel num_threads(g_threads)
     {
@@ -672,12 +697,12 @@ static void helm_block(const tables_t *T, int deformed, double lambda,
     iprod_block(T, 0, 1, deformed, w->d2, T->b[0], T->b[1], T->db[2], w->jac, w, w->out, 1.0);
 }
 
-void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed,
+void or_helmholtz_aniso(int shape, const int *nm, const int *nq, int nelmt, int deformed,
                   const double *jac, const double *df, double lambda,
                   const double *diff6, const double *in, double *out)
 {
     tables_t T;
-    tables_init(&T, shape, nm, nq0);
+    tables_init_full(&T, shape, nm, nq);
     const int nblk = (nelmt + VW - 1) / VW;
 #pragma omp parallel num_threads(g_threads)
     {
@@ -750,13 +775,13 @@ static void advection_pointwise(const tables_t *T, int DEF, const vd *df, const 
 /* LinearAdvectionDiffusionReaction.h:600-658 (fusedKernel3D): BwdTrans,
  * collocation derivatives, bwd += V . grad u, out = lambda IProduct(bwd),
  * diffusion metric, out += IProduct w.r.t. the three derivative bases. */
-void or_adv_diff_react(int shape, int nm, int nq0, int nelmt, int deformed,
+void or_adv_diff_react_aniso(int shape, const int *nm, const int *nq, int nelmt, int deformed,
                        const double *jac, const double *df, const double *vel,
                        double lambda, const double *diff6, const double *in,
                        double *out)
 {
     tables_t T;
-    tables_init(&T, shape, nm, nq0);
+    tables_init_full(&T, shape, nm, nq);
     const int nblk = (nelmt + VW - 1) / VW;
 #pragma omp parallel num_threads(g_threads)
     {
@@ -829,12 +854,12 @@ static void iprod_deriv_pointwise(const tables_t *T, int DEF, const vd *df,
 
 /* IProductWRTDerivBase.h:751-788 (fusedKernel3D): pointwise step, then
  * IProduct w.r.t. (db0 b1 b2), += (b0 db1 b2), += (b0 b1 db2). */
-void or_iproduct_deriv(int shape, int nm, int nq0, int nelmt, int deformed,
+void or_iproduct_deriv_aniso(int shape, const int *nm, const int *nq, int nelmt, int deformed,
                        const double *jac, const double *df, const double *in0,
                        const double *in1, const double *in2, double *out)
 {
     tables_t T;
-    tables_init(&T, shape, nm, nq0);
+    tables_init_full(&T, shape, nm, nq);
     const int nblk = (nelmt + VW - 1) / VW;
 #pragma omp parallel num_threads(g_threads)
     {
@@ -860,13 +885,13 @@ void or_iproduct_deriv(int shape, int nm, int nq0, int nelmt, int deformed,
     tables_free(&T);
 }
 
-void or_helmholtz_diag_bruteforce(int shape, int nm, int nq0, int nelmt,
+void or_helmholtz_diag_bruteforce_aniso(int shape, const int *nm, const int *nq, int nelmt,
                                   int deformed, const double *jac,
                                   const double *df, double lambda,
                                   double *diag)
 {
     tables_t T;
-    tables_init(&T, shape, nm, nq0);
+    tables_init_full(&T, shape, nm, nq);
     const int nblk = (nelmt + VW - 1) / VW;
     const int nc = T.ncoeff;
 #pragma omp parallel num_threads(g_threads)
@@ -985,4 +1010,76 @@ void or_phys_interp1d_scaled(int shape, int nq0, int nelmt, double factor,
         free(s_jir);
     }
     for (int d = 0; d < 3; ++d) free(I[d]);
+}
+
+/* The MODIFIED defaults: nm modes in every direction, nq0 points in
+ * direction 0 (or_default_nq for the others). */
+void or_bwdtrans(int shape, int nm, int nq0, int nelmt, const double *in,
+                 double *out)
+{
+    const int m[3] = {nm, nm, nm};
+    int q[3];
+    or_default_nq(shape, nq0, q);
+    or_bwdtrans_aniso(shape, m, q, nelmt, in, out);
+}
+
+void or_iproduct(int shape, int nm, int nq0, int nelmt, int deformed,
+                 const double *jac, const double *in, double *out)
+{
+    const int m[3] = {nm, nm, nm};
+    int q[3];
+    or_default_nq(shape, nq0, q);
+    or_iproduct_aniso(shape, m, q, nelmt, deformed, jac, in, out);
+}
+
+void or_physderiv(int shape, int nm, int nq0, int nelmt, int deformed,
+                  const double *df, const double *in, double *out0,
+                  double *out1, double *out2)
+{
+    const int m[3] = {nm, nm, nm};
+    int q[3];
+    or_default_nq(shape, nq0, q);
+    or_physderiv_aniso(shape, m, q, nelmt, deformed, df, in, out0, out1, out2);
+}
+
+void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed,
+                  const double *jac, const double *df, double lambda,
+                  const double *diff6, const double *in, double *out)
+{
+    const int m[3] = {nm, nm, nm};
+    int q[3];
+    or_default_nq(shape, nq0, q);
+    or_helmholtz_aniso(shape, m, q, nelmt, deformed, jac, df, lambda, diff6, in, out);
+}
+
+void or_adv_diff_react(int shape, int nm, int nq0, int nelmt, int deformed,
+                       const double *jac, const double *df, const double *vel,
+                       double lambda, const double *diff6, const double *in,
+                       double *out)
+{
+    const int m[3] = {nm, nm, nm};
+    int q[3];
+    or_default_nq(shape, nq0, q);
+    or_adv_diff_react_aniso(shape, m, q, nelmt, deformed, jac, df, vel, lambda, diff6, in, out);
+}
+
+void or_iproduct_deriv(int shape, int nm, int nq0, int nelmt, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, const double *in2, double *out)
+{
+    const int m[3] = {nm, nm, nm};
+    int q[3];
+    or_default_nq(shape, nq0, q);
+    or_iproduct_deriv_aniso(shape, m, q, nelmt, deformed, jac, df, in0, in1, in2, out);
+}
+
+void or_helmholtz_diag_bruteforce(int shape, int nm, int nq0, int nelmt,
+                                  int deformed, const double *jac,
+                                  const double *df, double lambda,
+                                  double *diag)
+{
+    const int m[3] = {nm, nm, nm};
+    int q[3];
+    or_default_nq(shape, nq0, q);
+    or_helmholtz_diag_bruteforce_aniso(shape, m, q, nelmt, deformed, jac, df, lambda, diag);
 }

@@ -89,6 +89,41 @@ void or_adv_diff_react(int shape, int nm, int nq0, int nelmt, int deformed,
                        double lambda, const double *diff6, const double *in,
                        double *out);
 
+/* The same operators with per-direction modes nm[3] and points nq[3] (the
+ * runtime-size operators, MatrixFreeOps/Helmholtz.h:374-464 and
+ * SwitchNodesPoints.h:302-343).  Hex: Modified_A on GLL in every
+ * direction; tet: Modified_A/GLL, Modified_B/Radau(1,0), Modified_C/
+ * Radau(2,0) with nm[0] <= nm[1] <= nm[2]; coefficients p < nm0, q < nm1-p,
+ * r < nm2-p-q. */
+int or_ncoeffs_aniso(int shape, const int nm[3]);
+void or_bwdtrans_aniso(int shape, const int *nm, const int *nq, int nelmt,
+                       const double *in, double *out);
+void or_iproduct_aniso(int shape, const int *nm, const int *nq, int nelmt,
+                       int deformed, const double *jac, const double *in,
+                       double *out);
+void or_physderiv_aniso(int shape, const int *nm, const int *nq, int nelmt,
+                        int deformed, const double *df, const double *in,
+                        double *out0, double *out1, double *out2);
+void or_iproduct_deriv_aniso(int shape, const int *nm, const int *nq,
+                             int nelmt, int deformed, const double *jac,
+                             const double *df, const double *in0,
+                             const double *in1, const double *in2,
+                             double *out);
+void or_helmholtz_aniso(int shape, const int *nm, const int *nq, int nelmt,
+                        int deformed, const double *jac, const double *df,
+                        double lambda, const double *diff6, const double *in,
+                        double *out);
+void or_adv_diff_react_aniso(int shape, const int *nm, const int *nq,
+                             int nelmt, int deformed, const double *jac,
+                             const double *df, const double *vel,
+                             double lambda, const double *diff6,
+                             const double *in, double *out);
+void or_helmholtz_diag_bruteforce_aniso(int shape, const int *nm,
+                                        const int *nq, int nelmt,
+                                        int deformed, const double *jac,
+                                        const double *df, double lambda,
+                                        double *diag);
+
 /* PhysInterp1DScaled (PhysInterp1DScaled.h:360-430): point values of
  * nq_in points per direction -> or_interp_nq points per direction. */
 void or_interp_nq(int shape, int nq0, double factor, int out[3]);

@@ -276,3 +276,49 @@ def splitmix_uniform(seed: int, n: int) -> np.ndarray:
         z ^= z >> 31
         out[i] = 2.0 * (z >> 11) * 2.0 ** -53 - 1.0
     return out
+
+
+# ---- per-direction modes and points (runtime-size operators) ------------
+def _i3(v):
+    return (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+def ncoeffs_aniso(shape, nm):
+    return int(lib().or_ncoeffs_aniso(shape, _i3(nm)))
+
+
+def bwdtrans_aniso(shape, nm, nq, nelmt, x):
+    out = np.zeros(nelmt * int(np.prod(nq)))
+    lib().or_bwdtrans_aniso(shape, _i3(nm), _i3(nq), nelmt, p(np.ascontiguousarray(x)), p(out))
+    return out
+
+
+def iproduct_aniso(shape, nm, nq, nelmt, deformed, jac, f):
+    out = np.zeros(nelmt * ncoeffs_aniso(shape, nm))
+    lib().or_iproduct_aniso(shape, _i3(nm), _i3(nq), nelmt, int(deformed), p(jac), p(np.ascontiguousarray(f)),
+                            p(out))
+    return out
+
+
+def physderiv_aniso(shape, nm, nq, nelmt, deformed, df, u):
+    n = nelmt * int(np.prod(nq))
+    o = [np.zeros(n) for _ in range(3)]
+    lib().or_physderiv_aniso(shape, _i3(nm), _i3(nq), nelmt, int(deformed), p(df), p(np.ascontiguousarray(u)),
+                             p(o[0]), p(o[1]), p(o[2]))
+    return o
+
+
+def iproduct_deriv_aniso(shape, nm, nq, nelmt, deformed, jac, df, f0, f1, f2):
+    out = np.zeros(nelmt * ncoeffs_aniso(shape, nm))
+    c = [np.ascontiguousarray(f, dtype=np.float64) for f in (f0, f1, f2)]
+    lib().or_iproduct_deriv_aniso(shape, _i3(nm), _i3(nq), nelmt, int(deformed), p(jac), p(df), p(c[0]), p(c[1]),
+                                  p(c[2]), p(out))
+    return out
+
+
+def helmholtz_aniso(shape, nm, nq, nelmt, deformed, jac, df, lam, x, diff6=None):
+    out = np.zeros(nelmt * ncoeffs_aniso(shape, nm))
+    d = None if diff6 is None else np.ascontiguousarray(diff6, dtype=np.float64)
+    lib().or_helmholtz_aniso(shape, _i3(nm), _i3(nq), nelmt, int(deformed), p(jac), p(df), ctypes.c_double(lam),
+                             p(d), p(np.ascontiguousarray(x)), p(out))
+    return out

@@ -190,3 +190,80 @@ def tet_dense_mass_stiff(nm, jac, df, lam, nquad=None):
     for j in range(3):
         A += (G[j].T * (w * jac)) @ G[j]
     return A, vals, w
+
+
+# ------------------------------------- per-direction sizes (runtime-size ops)
+def hex_dense_aniso(nm, nq):
+    """Phi[c, m] and its reference-direction derivatives for modes nm[3] and
+    GLL points nq[3] per direction; rows (k, j, i) with i fastest, columns
+    (r, q, p) with p fastest."""
+    Bs, Bds, Ws = [], [], []
+    for d in range(3):
+        z, w = gll(nq[d])
+        Bs.append(np.array([[modified_a(p)(zi) for p in range(nm[d])] for zi in z]))
+        Bds.append(np.array([[modified_a(p).deriv()(zi) for p in range(nm[d])] for zi in z]))
+        Ws.append(w)
+    Phi = np.kron(Bs[2], np.kron(Bs[1], Bs[0]))
+    D0 = np.kron(Bs[2], np.kron(Bs[1], Bds[0]))
+    D1 = np.kron(Bs[2], np.kron(Bds[1], Bs[0]))
+    D2 = np.kron(Bds[2], np.kron(Bs[1], Bs[0]))
+    W = np.kron(Ws[2], np.kron(Ws[1], Ws[0]))
+    return Phi, (D0, D1, D2), W
+
+
+def tet_modes_aniso(nm):
+    return [(p, q, r) for p in range(nm[0]) for q in range(nm[1] - p) for r in range(nm[2] - p - q)]
+
+
+def tet_dense_own_points(nm, nq):
+    """Tet basis values and standard-coordinate derivatives at the operator's
+    own collapsed quadrature points (GLL x Gauss-Radau(1,0) x Gauss-Radau(2,0),
+    nq[3] points, i fastest), and the weights W = w_a (w_b / 2) (w_c / 4) of
+    the Duffy-collapsed rule, for modes p < nm0, q < nm1-p, r < nm2-p-q.
+    Values come from the closed-form modified bases (with the reference's
+    singular-vertex treatment), derivatives from exact polynomial
+    differentiation and the collapsed chain rule."""
+    za, wa = gll(nq[0])
+    zb, wb = radau_m(nq[1], 1.0)
+    zc, wc = radau_m(nq[2], 2.0)
+    C, Bb, A = np.meshgrid(zc, zb, za, indexing="ij")
+    a, b, c = A.ravel(), Bb.ravel(), C.ravel()
+    W = (wc[:, None, None] * 0.25 * (wb[None, :, None] * 0.5) * wa[None, None, :]).ravel()
+    modes = tet_modes_aniso(nm)
+    one = Poly([1.0])
+    vals = np.zeros((len(a), len(modes)))
+    grads = np.zeros((3, len(a), len(modes)))
+    for m, (p, q, r) in enumerate(modes):
+        fa, fb, fc = modified_a(p), modified_b(p, q), modified_b(p + q, r)
+        if (p, q, r) == (0, 0, 1):  # top vertex: (1+c)/2
+            terms = [(one, one, HALF_P)]
+        elif p == 0 and q == 1:  # bottom vertex (r = 0) / singular edge (r >= 1)
+            terms = [(fa, fb, fc), (HALF_P, HALF_P, fc)]
+        else:
+            terms = [(fa, fb, fc)]
+        v = sum(ta(a) * tb(b) * tc(c) for ta, tb, tc in terms)
+        da = sum(ta.deriv()(a) * tb(b) * tc(c) for ta, tb, tc in terms)
+        db = sum(ta(a) * tb.deriv()(b) * tc(c) for ta, tb, tc in terms)
+        dc = sum(ta(a) * tb(b) * tc.deriv()(c) for ta, tb, tc in terms)
+        d0 = da * 4.0 / ((1 - b) * (1 - c))
+        d1 = 0.5 * (1 + a) * d0 + db * 2.0 / (1 - c)
+        d2 = 0.5 * (1 + a) * d0 + 0.5 * (1 + b) * db * 2.0 / (1 - c) + dc
+        vals[:, m] = v
+        grads[0, :, m], grads[1, :, m], grads[2, :, m] = d0, d1, d2
+    return vals, grads, W
+
+
+def dense_helmholtz(vals, grads, W, jac, df, lam, diff6=None):
+    """lam M + L from point values / reference-direction gradients (per
+    point), per-point jac [nqt] and df [nqt, 9] (df[j*3+k] = dxi_k/dx_j)."""
+    K = np.eye(3)
+    if diff6 is not None:
+        d = diff6
+        K = np.array([[d[0], d[1], d[3]], [d[1], d[2], d[4]], [d[3], d[4], d[5]]])
+    G = [sum(df[:, j * 3 + k][:, None] * grads[k] for k in range(3)) for j in range(3)]
+    A = lam * (vals.T * (W * jac)) @ vals
+    for a in range(3):
+        for b in range(3):
+            if K[a, b] != 0.0:
+                A += K[a, b] * (G[a].T * (W * jac)) @ G[b]
+    return A

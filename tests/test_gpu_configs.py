@@ -1,0 +1,157 @@
+"""The measured workloads themselves, at their full sizes, against the oracle.
+
+* Plain FP64 CG solve of -lap(u) + u = f on the 32^3 regular hex mesh of
+  [0,1]^3, P=6, Q=8 (6,967,871 unknowns): iterations within 2 of the
+  oracle's restatement of the reference CG (NekLinSysIterCG.cpp:118-372),
+  solution within 1e-8 relative, L2 error against the exact solution equal
+  to the oracle's to 1e-6 relative.
+* The batched uneven workload (64 Helmholtz applies, hex/tet, P 3..9, 1e3 to
+  1e5 elements each) in one graph submission: every FP64 and FP32 item
+  bit-identical to its operator applied alone, and the largest hex, the
+  largest tet and every P=9 item equal to the oracle (FP64 <= 1e-12, FP32
+  <= 2e-5 relative max error).
+* The serving stream (16 Poisson arrivals; meshes of 2^10..2^17 elements,
+  P 3..8, lambda, forcing frequency), FP64 and mixed precision: every
+  request converges to the FP64 relative residual 1e-10 and equals its
+  solve alone (iterations, residual, L2 error, solution norm)."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import geometry_cases as G
+from nektar_amd.workloads import batch_workload, poisson_arrivals, serve_requests
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def nmf():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import nektar_amd
+    nektar_amd.lib()
+    return nektar_amd
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_cg_32cubed_p6_against_oracle(nmf):
+    from manufactured import oracle_solve
+
+    n, nm, nq = 32, 7, 8
+    O.set_threads(16)
+    xo, info_o, l2_o = oracle_solve(n, nm, nq, 1.0)
+    s = nmf.HelmholtzSolver(n, nm, nq, 1.0)
+    assert s.nglobal - s.ndirichlet == 6_967_871
+    s.set_forcing_manufactured(1)
+    info = s.solve(mixed=False, tol=1e-10)
+    l2 = s.l2_error_manufactured(1)
+    print(f"32^3 P=6 Q=8: GPU {info.iterations} it L2 {l2:.4e}; oracle {info_o.iterations} it L2 {l2_o:.4e}; "
+          f"solution rel diff {_rel(s.solution(), xo):.2e}", flush=True)
+    assert info.converged and info.final_error < 1e-10
+    assert abs(info.iterations - info_o.iterations) <= 2
+    assert info.rhs_magnitude == pytest.approx(info_o.rhs_magnitude, rel=1e-12)
+    assert _rel(s.solution(), xo) <= 1e-8
+    assert abs(l2 - l2_o) <= 1e-6 * l2_o
+
+
+def _batch(nmf, fp32):
+    work = batch_workload()
+    dt = torch.float32 if fp32 else torch.float64
+    prec = nmf.FP32 if fp32 else nmf.FP64
+    items = []
+    for i, (shape, nm, nq, E, jac, df, lam) in enumerate(work):
+        op = nmf.GetOperatorFactory().CreateInstance(f"Helmholtz_{shape}_Regular", nm, nq, E, prec)
+        op.SetJac(jac)
+        op.SetDF(df)
+        op.SetLambda(lam)
+        xh = G.coeffs(op.input_size(), 400 + i)
+        x = torch.from_numpy(xh).to(dt).cuda()
+        y = torch.full((op.output_size(),), float("nan"), dtype=dt, device="cuda")
+        items.append(dict(op=op, shape=shape, nm=nm, nq=nq, E=E, jac=jac, df=df, lam=lam, xh=xh, x=x, y=y))
+    return items
+
+
+def _oracle_picks(items):
+    hexes = [k for k, it in enumerate(items) if it["shape"] == "Hex"]
+    tets = [k for k, it in enumerate(items) if it["shape"] == "Tet"]
+    picks = {max(hexes, key=lambda k: items[k]["E"]), max(tets, key=lambda k: items[k]["E"])}
+    picks |= {k for k, it in enumerate(items) if it["nm"] == 10}
+    # fill up to 8 with the next largest items
+    for k in sorted(range(len(items)), key=lambda k: -items[k]["E"]):
+        if len(picks) >= 8:
+            break
+        picks.add(k)
+    return sorted(picks)
+
+
+@pytest.mark.parametrize("fp32", [False, True], ids=["fp64", "fp32"])
+def test_batched_uneven_workload(nmf, fp32):
+    items = _batch(nmf, fp32)
+    assert len(items) == 64
+    assert min(it["E"] for it in items) >= 1000 and max(it["E"] for it in items) <= 100000
+    alone = []
+    for it in items:
+        y = torch.zeros_like(it["y"])
+        it["op"](it["x"], y)
+        alone.append(y)
+    torch.cuda.synchronize()
+    b = nmf.Batch(mode=nmf.Batch.GRAPH, streams=4)
+    for it in items:
+        b.add(it["op"], it["x"], it["y"])
+    info = b.info()
+    assert info["items"] == 64 and info["dofs"] == sum(it["op"].input_size() for it in items)
+    for rep in range(2):
+        for it in items:
+            it["y"].fill_(float("nan"))
+        b.run()
+        torch.cuda.synchronize()
+        for k, it in enumerate(items):
+            assert torch.equal(it["y"], alone[k]), (rep, k, it["shape"], it["nm"], it["E"])
+    O.set_threads(16)
+    picks = _oracle_picks(items)
+    assert len(picks) >= 8
+    for k in picks:
+        it = items[k]
+        shape = O.HEX if it["shape"] == "Hex" else O.TET
+        ref = O.helmholtz(shape, it["nm"], it["nq"], it["E"], False, it["jac"], it["df"], it["lam"], it["xh"])
+        err = G.rel_err(it["y"].double().cpu().numpy(), ref)
+        print(f"item {k}: {it['shape']} P={it['nm'] - 1} E={it['E']} rel err {err:.2e}", flush=True)
+        assert err <= (2e-5 if fp32 else 1e-12), (k, it["shape"], it["nm"], it["E"], err)
+
+
+def _same_alone(nmf, r, got):
+    s = nmf.HelmholtzSolver(r["n"], r["nummodes"], r["numpoints"], r["lam"])
+    s.set_forcing_manufactured(r["k"])
+    info = s.solve(mixed=bool(r["mixed"]), tol=r["tol"], maxit=r["maxit"])
+    x = s.solution()
+    l2 = s.l2_error_manufactured(r["k"])
+    assert info.converged and info.final_error < 1e-10
+    assert got["converged"] == 1
+    assert got["iterations"] == info.iterations, (r, got["iterations"], info.iterations)
+    assert got["outer_iterations"] == info.outer_iterations, r
+    assert got["final_error"] == info.final_error, r
+    assert got["l2_error"] == l2, r
+    xnorm = float(np.sqrt(np.sum(x * x)))
+    assert abs(got["solution_norm"] - xnorm) <= 1e-12 * xnorm, r
+
+
+@pytest.mark.parametrize("mixed", [0, 1], ids=["fp64", "mixed"])
+def test_serving_stream(nmf, mixed):
+    reqs = serve_requests(16)
+    assert min(r["nummodes"] for r in reqs) >= 4 and max(r["nummodes"] for r in reqs) <= 9
+    arrivals = poisson_arrivals(len(reqs), 20.0)
+    rq = [dict(r, mixed=mixed, arrival_s=float(a)) for r, a in zip(reqs, arrivals)]
+    srv = nmf.Server(hbm_budget_bytes=64 << 30, streams=4, max_active=16)
+    results, stats = srv.run(rq, validate=True)
+    assert stats["peak_active"] >= 2
+    lat = [g["latency_ms"] for g in results]
+    print(f"serving {'mixed' if mixed else 'fp64'}: {len(rq)} requests, makespan {stats['makespan_s']:.3f} s, "
+          f"p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms, "
+          f"peak active {stats['peak_active']}", flush=True)
+    for r, got in zip(rq, results):
+        assert got["unknowns"] == (r["n"] * (r["nummodes"] - 1) - 1) ** 3
+        assert got["final_error"] < 1e-10
+        _same_alone(nmf, r, got)

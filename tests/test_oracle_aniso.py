@@ -1,0 +1,126 @@
+"""Pins of the oracle's per-direction (runtime-size) operators against
+dense evaluations that share nothing with it (tests/brute.py):
+
+* hex: Kronecker products of closed-form Modified_A tables on GLL points
+  with nm[3] modes and nq[3] points per direction (any of them unequal);
+* tet: the basis (closed forms, singular-vertex treatment) evaluated at the
+  operator's own collapsed quadrature points, with the Duffy-collapsed
+  weights, standard-coordinate gradients from exact polynomial derivatives
+  and the collapsed chain rule, modes p < nm0, q < nm1-p, r < nm2-p-q.
+
+Geometry is per point (the deformed path): random J > 0 and random df at
+every quadrature point, so that an index mistake in the per-point access
+(point q of element e, factor c) changes the result.  The MODIFIED default
+sizes are included, which pins the oracle's deformed tet path on its own."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import brute as B
+import geometry_cases as G
+
+TOL = 1e-13
+NE = 3  # elements per case
+
+
+def _geom(nqt, seed):
+    rng = np.random.default_rng(seed)
+    jac = rng.uniform(0.5, 1.5, NE * nqt)
+    df = rng.uniform(-1.0, 1.0, NE * nqt * 9)
+    return jac, df
+
+
+def _elem(a, e, n):
+    return a[e * n:(e + 1) * n]
+
+
+HEX_CASES = [((3, 4, 5), (4, 5, 6)), ((4, 4, 4), (6, 5, 7)), ((2, 5, 3), (3, 5, 4)), ((5, 5, 5), (5, 5, 5)),
+             ((6, 3, 4), (9, 4, 6)), ((4, 4, 4), (5, 5, 5))]
+TET_CASES = [((3, 3, 3), (4, 3, 3)), ((4, 4, 4), (5, 4, 4)), ((5, 5, 5), (7, 6, 6)), ((3, 4, 5), (5, 5, 5)),
+             ((2, 3, 4), (4, 6, 5)), ((4, 4, 4), (6, 4, 7))]
+
+
+@pytest.mark.parametrize("nm,nq", HEX_CASES)
+def test_hex_aniso_against_dense(nm, nq):
+    Phi, Dd, W = B.hex_dense_aniso(nm, nq)
+    nqt, nc = int(np.prod(nq)), int(np.prod(nm))
+    assert O.ncoeffs_aniso(O.HEX, nm) == nc
+    jac, df = _geom(nqt, sum(nm) + 10 * sum(nq))
+    x = G.coeffs(NE * nc, 1)
+    u = O.bwdtrans_aniso(O.HEX, nm, nq, NE, x)
+    assert G.rel_err(u, np.concatenate([Phi @ _elem(x, e, nc) for e in range(NE)])) < TOL
+    f = G.coeffs(NE * nqt, 2)
+    y = O.iproduct_aniso(O.HEX, nm, nq, NE, True, jac, f)
+    yb = np.concatenate([Phi.T @ (W * _elem(jac, e, nqt) * _elem(f, e, nqt)) for e in range(NE)])
+    assert G.rel_err(y, yb) < TOL
+    d = O.physderiv_aniso(O.HEX, nm, nq, NE, True, df, f)
+    # PhysDeriv of point values: collocation derivative matrices of the
+    # point sets (exact for the interpolating polynomial), then df
+    Dc = [np.kron(np.eye(nq[2] * nq[1]), B.lagrange_deriv_matrix(B.gll(nq[0])[0]).reshape(nq[0], nq[0]).T),
+          np.kron(np.eye(nq[2]), np.kron(B.lagrange_deriv_matrix(B.gll(nq[1])[0]).reshape(nq[1], nq[1]).T,
+                                         np.eye(nq[0]))),
+          np.kron(B.lagrange_deriv_matrix(B.gll(nq[2])[0]).reshape(nq[2], nq[2]).T, np.eye(nq[1] * nq[0]))]
+    for e in range(NE):
+        fe = _elem(f, e, nqt)
+        dxi = [M @ fe for M in Dc]
+        dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
+        for j in range(3):
+            want = sum(dfe[:, j * 3 + k] * dxi[k] for k in range(3))
+            assert G.rel_err(_elem(d[j], e, nqt), want) < 1e-12
+    d6 = np.array([1.7, 0.2, 1.1, -0.3, 0.15, 0.9])
+    for diff in (None, d6):
+        h = O.helmholtz_aniso(O.HEX, nm, nq, NE, True, jac, df, 0.7, x, diff)
+        hb = np.concatenate([B.dense_helmholtz(Phi, Dd, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
+                                               0.7, diff) @ _elem(x, e, nc) for e in range(NE)])
+        assert G.rel_err(h, hb) < TOL
+
+
+@pytest.mark.parametrize("nm,nq", TET_CASES)
+def test_tet_aniso_own_points_against_dense(nm, nq):
+    vals, grads, W = B.tet_dense_own_points(nm, nq)
+    nqt, nc = int(np.prod(nq)), len(B.tet_modes_aniso(nm))
+    assert O.ncoeffs_aniso(O.TET, nm) == nc
+    jac, df = _geom(nqt, 3 * sum(nm) + sum(nq))
+    x = G.coeffs(NE * nc, 3)
+    u = O.bwdtrans_aniso(O.TET, nm, nq, NE, x)
+    assert G.rel_err(u, np.concatenate([vals @ _elem(x, e, nc) for e in range(NE)])) < TOL
+    f = G.coeffs(NE * nqt, 4)
+    y = O.iproduct_aniso(O.TET, nm, nq, NE, True, jac, f)
+    yb = np.concatenate([vals.T @ (W * _elem(jac, e, nqt) * _elem(f, e, nqt)) for e in range(NE)])
+    assert G.rel_err(y, yb) < TOL
+    # PhysDeriv of u = BwdTrans(x): the exact gradient of the expansion at the points
+    d = O.physderiv_aniso(O.TET, nm, nq, NE, True, df, u)
+    for e in range(NE):
+        xe = _elem(x, e, nc)
+        dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
+        for j in range(3):
+            want = sum(dfe[:, j * 3 + k] * (grads[k] @ xe) for k in range(3))
+            assert G.rel_err(_elem(d[j], e, nqt), want) < 1e-12
+    # IProductWRTDerivBase: sum_j int (dphi/dx_j) f_j
+    f1, f2 = G.coeffs(NE * nqt, 5), G.coeffs(NE * nqt, 6)
+    yd = O.iproduct_deriv_aniso(O.TET, nm, nq, NE, True, jac, df, f, f1, f2)
+    for e in range(NE):
+        dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
+        wj = W * _elem(jac, e, nqt)
+        want = sum((sum(dfe[:, j * 3 + k][:, None] * grads[k] for k in range(3))).T @ (wj * _elem(ff, e, nqt))
+                   for j, ff in enumerate((f, f1, f2)))
+        assert G.rel_err(_elem(yd, e, nc), want) < TOL
+    d6 = np.array([1.3, -0.1, 0.8, 0.2, 0.05, 1.4])
+    for diff in (None, d6):
+        h = O.helmholtz_aniso(O.TET, nm, nq, NE, True, jac, df, 1.9, x, diff)
+        hb = np.concatenate([B.dense_helmholtz(vals, grads, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
+                                               1.9, diff) @ _elem(x, e, nc) for e in range(NE)])
+        assert G.rel_err(h, hb) < TOL
+
+
+def test_default_sizes_match_isotropic_entry_points():
+    """The per-direction entry points at the MODIFIED defaults are the
+    isotropic ones, bit for bit."""
+    for shape, nm, nq0 in ((O.HEX, 5, 6), (O.TET, 5, 6), (O.TET, 4, 6)):
+        nq = O.default_nq(shape, nq0)
+        nqt = int(np.prod(nq))
+        jac, df = _geom(nqt, 9)
+        x = G.coeffs(NE * O.ncoeffs(shape, nm), 7)
+        a = O.helmholtz(shape, nm, nq0, NE, True, jac, df, 1.1, x)
+        b = O.helmholtz_aniso(shape, (nm, nm, nm), nq, NE, True, jac, df, 1.1, x)
+        assert np.array_equal(a, b)
