@@ -362,6 +362,19 @@ void HexSolver::begin_solve(bool mixed, double tol, int maxit)
     job_.mixed = mixed;
     tol = std::max(tol, 1.0e-16);  // NekLinSysIter.cpp:57
     job_.tol = tol;
+    try {
+        begin_solve_enqueue(mixed, tol, maxit);
+    } catch (...) {
+        // a failed set-up (graph instantiation, FP32 buffers, a launch)
+        // leaves the solver usable: drain what was enqueued, forget the job
+        (void)hipStreamSynchronize(stream_);
+        job_ = Job{};
+        throw;
+    }
+}
+
+void HexSolver::begin_solve_enqueue(bool mixed, double tol, int maxit)
+{
     NMF_HIP(hipEventRecord(ev_t0_, stream_));
     CGState *st = state_.as<CGState>();
     if (!mixed) {

@@ -191,6 +191,7 @@ private:
     void assemble_rhs(const double *f_dev);
     void enqueue_state_read();
     void enqueue_reliable_update();
+    void begin_solve_enqueue(bool mixed, double tol, int maxit);
 
     struct Job {
         bool running = false, mixed = false, pending = false, finalizing = false, finished = false;
