@@ -58,11 +58,19 @@ inline std::string GetOpstring(LibUtilities::ShapeType shape, bool deformed = fa
 
 class Operator {
 public:
+    // One BasisKey per direction, as the reference factory takes
+    // std::vector<BasisSharedPtr> (Operator.hpp:59-63); modes and points
+    // may differ between directions (anisotropic expansions run the
+    // library's runtime-size kernels).
     Operator(const std::string &key, const std::vector<LibUtilities::BasisKey> &basis, int nElmt,
              int precision = NMF_FP64)
     {
-        if (basis.empty()) throw NekError(NMF_ERR_INVALID, "empty basis");
-        NmfCheck(nmf_operator_create(key.c_str(), basis[0].numModes, basis[0].numPoints, nElmt, precision, &m_op));
+        if (basis.size() != 3)
+            throw NekError(NMF_ERR_INVALID, "a 3-D operator takes one BasisKey per direction (" +
+                                                std::to_string(basis.size()) + " given)");
+        const int modes[3] = {basis[0].numModes, basis[1].numModes, basis[2].numModes};
+        const int points[3] = {basis[0].numPoints, basis[1].numPoints, basis[2].numPoints};
+        NmfCheck(nmf_operator_create_basis(key.c_str(), modes, points, nElmt, precision, &m_op));
         int nc, nq, ne, de, pr;
         NmfCheck(nmf_operator_info(m_op, &nc, &nq, &ne, &de, &pr));
         m_ncoeff = nc;

@@ -99,6 +99,25 @@ int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0,
  * direction 0 (tets use Q-1 Gauss-Radau points in directions 1, 2). */
 int nmf_operator_create(const char *key, int nummodes, int numpoints0,
                         int nelmt, int precision, nmf_operator *out);
+/* The same with modes and points per direction, as the reference's
+ * factory takes one BasisKey per direction (Operator.hpp:59-63,
+ * std::vector<LibUtilities::BasisSharedPtr>): nummodes[3], numpoints[3].
+ * Bases are the MODIFIED ones (hex Modified_A on GLL points; tet
+ * Modified_A/B/C on GLL, Gauss-Radau(1,0), Gauss-Radau(2,0) points).
+ * Equal modes with the default points of an instantiated (modes, points)
+ * pair run the compile-time kernels; every other combination -- any
+ * anisotropic order, point counts of 2..64 and modes of 2..32 per
+ * direction (tets: nm0 <= nm1 <= nm2) -- runs the runtime-size kernels, as
+ * the reference falls back to its runtime-size operators
+ * (SwitchNodesPoints.h:302-343).  Anything else: NMF_ERR_UNSUPPORTED.
+ * PhysInterp1DScaled takes the default points only. */
+int nmf_operator_create_basis(const char *key, const int *nummodes,
+                              const int *numpoints, int nelmt, int precision,
+                              nmf_operator *out);
+/* *generic = 1 when op runs the runtime-size kernels, 0 for a compile-time
+ * kernel.  No counterpart in the reference (whose choice is internal to
+ * SwitchNodesPoints.h:220-348); for callers that report which path ran. */
+int nmf_operator_runtime_size(nmf_operator op, int *generic);
 int nmf_operator_destroy(nmf_operator op);
 /* GetInputSize/GetOutputSize per element (Collections/Operator.h:178-230) */
 int nmf_operator_info(nmf_operator op, int *ncoeff, int *nqtot, int *nelmt,
@@ -369,8 +388,8 @@ int nmf_assembly_assemble(nmf_assembly a, const double *loc, double *glob,
 /* GlobalLinSysIterativeFull + NekLinSysIterCG with the Diagonal
  * preconditioner (GlobalLinSysIterativeFull.cpp:96-383,
  * NekLinSysIterCG.cpp:118-372, PreconditionerDiagonal.cpp:93-189) over FP64
- * Helmholtz operators (hex and/or tet collections) and homogeneous
- * Dirichlet data.  The operators and the map must outlive the handle; the
+ * Helmholtz operators (hex and/or tet collections); Dirichlet data are zero
+ * unless nmf_linsys_set_dirichlet_local gives their values.  The operators and the map must outlive the handle; the
  * diagonal is built at creation from the operators' current lambda and
  * geometry (exact element-matrix diagonals, one apply per local mode). */
 int nmf_linsys_create(nmf_assembly a, const nmf_operator *ops, int nops,
@@ -394,6 +413,15 @@ int nmf_linsys_destroy(nmf_linsys l);
  * the global CG solves with b = Assemble(bloc), the local CG starts from
  * r = bloc */
 int nmf_linsys_set_rhs_local(nmf_linsys l, const double *bloc);
+/* Non-homogeneous Dirichlet data: dloc (device, the local layout) holds the
+ * Dirichlet values on the local entries of Dirichlet DOFs and 0 elsewhere,
+ * as ContField::v_ImposeDirichletConditions leaves the local coefficients
+ * (ContField.cpp:557-608); NULL returns to homogeneous data.  Solves then
+ * lift them as GlobalLinSysIterativeFull::v_Solve does
+ * (GlobalLinSysIterativeFull.cpp:118-169): rhs = bloc - A dloc element by
+ * element, CG for the perturbation (zero on the Dirichlet rows), solution =
+ * perturbation + dloc (local) or + the Dirichlet values (global). */
+int nmf_linsys_set_dirichlet_local(nmf_linsys l, const double *dloc);
 /* Stops at ||r|| < tol ||b|| (unpreconditioned, Dirichlet rows excluded) */
 int nmf_linsys_solve(nmf_linsys l, double tol, int maxit, nmf_solve_info *info);
 int nmf_linsys_get_solution(nmf_linsys l, double *x_host);

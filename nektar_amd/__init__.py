@@ -76,6 +76,9 @@ _SIGNATURES = {
     "nmf_basis_tables": (ctypes.c_int, [ctypes.c_int] * 4 + [_I, _I] + [_D] * 5),
     "nmf_operator_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ctypes.POINTER(_P)]),
+    "nmf_operator_create_basis": (ctypes.c_int, [ctypes.c_char_p, _I, _I, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.POINTER(_P)]),
+    "nmf_operator_runtime_size": (ctypes.c_int, [_P, _I]),
     "nmf_operator_destroy": (ctypes.c_int, [_P]),
     "nmf_operator_info": (ctypes.c_int, [_P, _I, _I, _I, _I, _I]),
     "nmf_operator_set_jac": (ctypes.c_int, [_P, _D]),
@@ -140,6 +143,7 @@ _SIGNATURES = {
     "nmf_linsys_sc_diagonal": (ctypes.c_int, [_P, _D]),
     "nmf_linsys_destroy": (ctypes.c_int, [_P]),
     "nmf_linsys_set_rhs_local": (ctypes.c_int, [_P, _P]),
+    "nmf_linsys_set_dirichlet_local": (ctypes.c_int, [_P, _P]),
     "nmf_linsys_solve": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(SolveInfo)]),
     "nmf_linsys_get_solution": (ctypes.c_int, [_P, _D]),
     "nmf_linsys_get_inv_diag": (ctypes.c_int, [_P, _D]),
@@ -215,13 +219,26 @@ def _device_ptr(x, dtype_expected, count) -> int:
 class Operator:
     """``MatrixFree::Operator`` instance for one element collection."""
 
-    def __init__(self, key: str, nummodes: int, numpoints: int, nelmt: int,
+    def __init__(self, key: str, nummodes, numpoints, nelmt: int,
                  precision: int = FP64):
+        """nummodes / numpoints: one int (equal modes, the shape's default
+        points from the direction-0 count) or three per-direction values
+        (the reference's one BasisKey per direction, Operator.hpp:59-63)."""
         self._h = None
         h = ctypes.c_void_p()
-        check(lib().nmf_operator_create(key.encode(), nummodes, numpoints, nelmt, precision,
-                                        ctypes.byref(h)))
+        if isinstance(nummodes, int) and isinstance(numpoints, int):
+            check(lib().nmf_operator_create(key.encode(), nummodes, numpoints, nelmt, precision,
+                                            ctypes.byref(h)))
+        else:
+            m, q = list(nummodes), list(numpoints)
+            if len(m) != 3 or len(q) != 3:
+                raise NektarError(1, "per-direction modes and points need three values each")
+            check(lib().nmf_operator_create_basis(key.encode(), (ctypes.c_int * 3)(*m), (ctypes.c_int * 3)(*q),
+                                                  nelmt, precision, ctypes.byref(h)))
         self._h = h
+        g = ctypes.c_int()
+        check(lib().nmf_operator_runtime_size(h, ctypes.byref(g)))
+        self.runtime_size = bool(g.value)
         self.key = key
         self.op = key.split("_")[0]
         nc, nq, ne, de, pr = (ctypes.c_int() for _ in range(5))
@@ -386,7 +403,7 @@ class Operator:
 class _Factory:
     """``MatrixFree::GetOperatorFactory()`` (Operator.cpp:40-44)."""
 
-    def CreateInstance(self, key: str, nummodes: int, numpoints: int, nelmt: int,
+    def CreateInstance(self, key: str, nummodes, numpoints, nelmt: int,
                        precision: int = FP64) -> Operator:
         return Operator(key, nummodes, numpoints, nelmt, precision)
 
@@ -669,6 +686,12 @@ class LinearSystem:
 
     def set_rhs_local(self, bloc) -> None:
         check(lib().nmf_linsys_set_rhs_local(self._h, _device_ptr(bloc, torch.float64, self._amap.nloc)))
+
+    def set_dirichlet_local(self, dloc) -> None:
+        """Dirichlet values in the local layout (0 off the Dirichlet DOFs);
+        None returns to homogeneous data (nmf_linsys_set_dirichlet_local)."""
+        p = None if dloc is None else _device_ptr(dloc, torch.float64, self._amap.nloc)
+        check(lib().nmf_linsys_set_dirichlet_local(self._h, p))
 
     def solve(self, tol: float = 1e-10, maxit: int = 5000) -> SolveInfo:
         info = SolveInfo()

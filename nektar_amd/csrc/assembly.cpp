@@ -234,6 +234,8 @@ void IterativeSolver::sc_diagonal(double *dglob)
     NMF_HIP(solver::launch_zero<double>(stream_, (size_t)std::max<int64_t>(map_.nloc(), 1), sloc.as<double>()));
     for (size_t k = 0; k < ops_.size(); ++k) {
         MFOperator &o = *ops_[k];
+        NMF_REQUIRE(o.equal_modes(), NMF_ERR_UNSUPPORTED,
+                    "static condensation takes operators with equal modes in every direction");
         const std::vector<int> I = schur::interior_modes(o.shape, o.nm), Bd = schur::boundary_modes(o.shape, o.nm);
         const int E = o.nelmt, nc = o.ncoeff, ni = (int)I.size(), nb = (int)Bd.size();
         const int KB = schur::batch_columns(ni);
@@ -313,22 +315,64 @@ void IterativeSolver::sc_diagonal(double *dglob)
 
 void IterativeSolver::set_rhs_local(const double *bloc)
 {
-    NMF_REQUIRE(bloc || map_.nloc() == 0, NMF_ERR_INVALID, "null right-hand side");
-    if (space_ != CGSpace::Global) {
-        if (map_.nloc())
-            NMF_HIP(hipMemcpyAsync(b_.get(), bloc, (size_t)map_.nloc() * sizeof(double), hipMemcpyDeviceToDevice,
-                                   stream_));
-    } else {
-        map_.assemble(bloc, b_.as<double>(), stream_);
-    }
+    const size_t n = (size_t)map_.nloc();
+    NMF_REQUIRE(bloc || n == 0, NMF_ERR_INVALID, "null right-hand side");
+    bloc_.allocate(std::max<size_t>(n, 1) * sizeof(double));
+    if (n) NMF_HIP(hipMemcpyAsync(bloc_.get(), bloc, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     have_rhs_ = true;
+}
+
+void IterativeSolver::set_dirichlet_local(const double *dloc)
+{
+    const size_t n = (size_t)map_.nloc();
+    if (!dloc) {
+        have_dir_ = false;
+        return;
+    }
+    NMF_REQUIRE(map_.ndir() > 0, NMF_ERR_INVALID, "Dirichlet values given but the map has no Dirichlet DOFs");
+    dloc_.allocate(std::max<size_t>(n, 1) * sizeof(double));
+    dglob_.allocate(std::max(map_.nglobal(), 1) * sizeof(double));
+    if (n) NMF_HIP(hipMemcpyAsync(dloc_.get(), dloc, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    have_dir_ = true;
+}
+
+// The right-hand side the CG sees: the caller's local one, less A dloc
+// (element by element, GeneralMatrixOp) when there are Dirichlet values
+// (GlobalLinSysIterativeFull.cpp:118-150); assembled for the global CG.
+void IterativeSolver::load_rhs()
+{
+    const size_t n = (size_t)map_.nloc();
+    const double *src = bloc_.as<double>();
+    if (have_dir_ && n) {
+        apply_local(dloc_.as<double>(), yloc_.as<double>());
+        NMF_HIP(solver::launch_sub(stream_, n, bloc_.as<double>(), yloc_.as<double>(), yloc_.as<double>()));
+        src = yloc_.as<double>();
+    }
+    if (space_ != CGSpace::Global) {
+        if (n) NMF_HIP(hipMemcpyAsync(b_.get(), src, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    } else {
+        map_.assemble(src, b_.as<double>(), stream_);
+    }
 }
 
 const double *IterativeSolver::solution()
 {
-    if (space_ == CGSpace::Global) return cg_[0].as<double>();
-    if (space_ == CGSpace::StaticCond) return sc_->xg.as<double>();
-    map_.global_values(cg_[0].as<double>(), xg_.as<double>(), stream_);
+    const int NG = map_.nglobal();
+    const double *x;
+    if (space_ == CGSpace::Global) {
+        x = cg_[0].as<double>();
+    } else if (space_ == CGSpace::StaticCond) {
+        x = sc_->xg.as<double>();
+    } else {
+        map_.global_values(cg_[0].as<double>(), xg_.as<double>(), stream_);
+        x = xg_.as<double>();
+    }
+    if (!have_dir_ || NG == 0) return x;
+    // the perturbation plus the Dirichlet values (one local copy per DOF)
+    if (x != xg_.as<double>())
+        NMF_HIP(hipMemcpyAsync(xg_.get(), x, (size_t)NG * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    map_.global_values(dloc_.as<double>(), dglob_.as<double>(), stream_);
+    NMF_HIP(solver::launch_add<double>(stream_, (size_t)NG, xg_.as<double>(), dglob_.as<double>()));
     return xg_.as<double>();
 }
 
@@ -342,6 +386,8 @@ void IterativeSolver::solution_local(double *xloc)
     else
         NMF_HIP(hipMemcpyAsync(xloc, space_ == CGSpace::StaticCond ? sc_->xloc.get() : cg_[0].get(),
                                n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    // GlobalLinSysIterativeFull.cpp:158-161: perturbation + Dirichlet values
+    if (have_dir_) NMF_HIP(solver::launch_add<double>(stream_, n, xloc, dloc_.as<double>()));
 }
 
 void IterativeSolver::apply_global(const double *x, double *y)
@@ -366,6 +412,7 @@ SolveInfo IterativeSolver::solve(double tol, int maxit)
     NMF_REQUIRE(have_rhs_, NMF_ERR_STATE, "right-hand side not set");
     NMF_REQUIRE(maxit >= 0, NMF_ERR_INVALID, "maxit must be >= 0");
     refresh();
+    load_rhs();
     if (space_ == CGSpace::StaticCond) return solve_static_cond(tol, maxit);
     tol = std::max(tol, 1.0e-16);  // NekLinSysIter.cpp:57
     const int NG = map_.nglobal(), ND = map_.ndir();

@@ -10,7 +10,8 @@
 // of the concatenated element-local coefficient vector, its global DOF
 // map[i] and sign[i] (+-1, the orientation sign of edge and face modes; all
 // +1 when NULL), with the Dirichlet DOFs numbered first (global ids
-// 0..ndir-1, m_numGlobalDirBndCoeffs).  Homogeneous Dirichlet data.
+// 0..ndir-1, m_numGlobalDirBndCoeffs).  Dirichlet data are zero unless the
+// caller gives their values (IterativeSolver::set_dirichlet_local).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -96,6 +97,13 @@ public:
     // IProductWRTBase of the forcing), ContField::v_HelmSolve's; the global
     // CG solves with b = Assemble(bloc), the local CG starts from r = bloc
     void set_rhs_local(const double *bloc);
+    // Dirichlet values in the local layout (device; zero off the Dirichlet
+    // DOFs; NULL: homogeneous), as ContField imposes them on the local
+    // coefficients before the solve (ContField.cpp:557-608).  The solve then
+    // lifts them as GlobalLinSysIterativeFull::v_Solve does (:118-169):
+    // rhs = bloc - A dloc element by element, CG for the perturbation with
+    // zero Dirichlet rows, solution = perturbation + dloc.
+    void set_dirichlet_local(const double *dloc);
     // rebuild what depends on the operators if any of them changed since
     // the last build (called by every public entry that uses them)
     void refresh();
@@ -145,6 +153,11 @@ private:
     solver::CGState *host_state_ = nullptr;
     bool have_rhs_ = false;
     std::unique_ptr<StaticCondData> sc_;
+    // the caller's local right-hand side and Dirichlet values; the
+    // right-hand side the CG sees is built from them at every solve
+    DeviceBuffer bloc_, dloc_, dglob_;
+    bool have_dir_ = false;
+    void load_rhs();
 };
 
 }  // namespace nektar_amd

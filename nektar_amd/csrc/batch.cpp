@@ -42,7 +42,7 @@ void Batch::add(MFOperator *op, const void *in, void *out0, void *out1, void *ou
     NMF_REQUIRE(op->nelmt == 0 || (in && out0), NMF_ERR_INVALID, "null input/output pointer");
     if (op->op == OpType::PhysDeriv)
         NMF_REQUIRE(op->nelmt == 0 || (out1 && out2), NMF_ERR_INVALID, "PhysDeriv needs three outputs");
-    NMF_REQUIRE(hex_supported(op->nm, op->nq[0]) || op->shape != Shape::Hex, NMF_ERR_UNSUPPORTED,
+    NMF_REQUIRE(op->generic || hex_supported(op->nm, op->nq[0]) || op->shape != Shape::Hex, NMF_ERR_UNSUPPORTED,
                 "batches take hex operators with Q = NUMMODES + 1 or + 2");
     entries_.push_back(Entry{op, in, out0, out1, out2, op->version});
     built_ = false;
@@ -125,7 +125,7 @@ void Batch::build()
         o.prepare();
         e.version = o.version;
         if (o.nelmt == 0) continue;
-        if (o.op != OpType::Helmholtz) {
+        if (o.op != OpType::Helmholtz || o.generic) {  // runtime-size operators run alone
             singles_.push_back(i);
             continue;
         }

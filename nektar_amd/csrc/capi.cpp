@@ -97,6 +97,27 @@ int nmf_operator_create(const char *key, int nummodes, int numpoints0, int nelmt
     });
 }
 
+int nmf_operator_create_basis(const char *key, const int *nummodes, const int *numpoints, int nelmt, int precision,
+                              nmf_operator *out)
+{
+    return guard([&] {
+        NMF_REQUIRE(key && out && nummodes && numpoints, NMF_ERR_INVALID, "null key, modes, points or output handle");
+        *out = nullptr;
+        auto h = std::make_unique<nmf_operator_s>();
+        h->impl = std::make_unique<MFOperator>(key, nummodes, numpoints, nelmt, precision);
+        *out = h.release();
+    });
+}
+
+int nmf_operator_runtime_size(nmf_operator op, int *generic)
+{
+    return guard([&] {
+        NMF_REQUIRE(op && op->impl, NMF_ERR_INVALID, "null operator handle");
+        NMF_REQUIRE(generic, NMF_ERR_INVALID, "null output");
+        *generic = op->impl->generic ? 1 : 0;
+    });
+}
+
 int nmf_operator_destroy(nmf_operator op)
 {
     return guard([&] { delete op; });
@@ -768,6 +789,14 @@ int nmf_linsys_set_rhs_local(nmf_linsys l, const double *bloc)
     return guard([&] {
         NMF_LS_CHECK();
         l->impl->set_rhs_local(bloc);
+    });
+}
+
+int nmf_linsys_set_dirichlet_local(nmf_linsys l, const double *dloc)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        l->impl->set_dirichlet_local(dloc);
     });
 }
 

@@ -766,6 +766,13 @@ __global__ void add_kernel(size_t NG, double *__restrict__ x, const T *__restric
         x[G] += (double)d[G];
 }
 
+// out = a - b (out may alias a or b)
+__global__ void sub_kernel(size_t n, const double *a, const double *b, double *out)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = a[i] - b[i];
+}
+
 // inv_diag = 1/diag on non-Dirichlet rows, 0 on Dirichlet rows
 __global__ void invert_diag_kernel(int N, double *__restrict__ d)
 {
@@ -978,6 +985,12 @@ template <typename T>
 hipError_t launch_add(hipStream_t stream, size_t NG, double *x, const T *d)
 {
     NMF_LAUNCH((add_kernel<T>), grid_for(NG), NG, x, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub(hipStream_t stream, size_t n, const double *a, const double *b, double *out)
+{
+    NMF_LAUNCH(sub_kernel, grid_for(n), n, a, b, out);
     return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 // (Collections/CoalescedGeomData.cpp:90-342).
 #include "mf_operator.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -83,23 +84,108 @@ std::vector<double> MFOperator::build_tables(Shape shape, int nm, int nq0)
 
 MFOperator::MFOperator(const std::string &key, int nummodes, int numpoints0, int n, int precision)
 {
+    Shape s;
+    OpType o;
+    bool d;
+    parse_operator_key(key, o, s, d);
+    const int m[3] = {nummodes, nummodes, nummodes};
+    int q[3];
+    default_points(s, numpoints0, q);
+    init(key, m, q, n, precision);
+}
+
+MFOperator::MFOperator(const std::string &key, const int nummodes[3], const int numpoints[3], int n, int precision)
+{
+    NMF_REQUIRE(nummodes && numpoints, NMF_ERR_INVALID, "null modes/points array");
+    init(key, nummodes, numpoints, n, precision);
+}
+
+void MFOperator::init(const std::string &key, const int nummodes[3], const int numpoints[3], int n, int precision)
+{
     parse_operator_key(key, op, shape, deformed);
     NMF_REQUIRE(precision == NMF_FP64 || precision == NMF_FP32, NMF_ERR_INVALID, "precision must be NMF_FP64 or NMF_FP32");
     NMF_REQUIRE(n >= 0, NMF_ERR_INVALID, "negative element count");
     fp32 = precision == NMF_FP32;
-    nm = nummodes;
     nelmt = n;
-    const bool ok = shape == Shape::Hex ? hex_op_supported(nummodes, numpoints0) : tet_supported(nummodes, numpoints0);
-    NMF_REQUIRE(ok, NMF_ERR_UNSUPPORTED,
-                "(nummodes=" + std::to_string(nummodes) + ", numpoints=" + std::to_string(numpoints0) +
-                    ") not instantiated: need 2 <= nummodes <= 10 and numpoints0 in {nummodes+1, nummodes+2}");
-    default_points(shape, numpoints0, nq);
+    for (int d = 0; d < 3; ++d) {
+        nmd[d] = nummodes[d];
+        nq[d] = numpoints[d];
+    }
+    nm = nmd[0];
     nqtot = nq[0] * nq[1] * nq[2];
-    ncoeff = num_coeffs(shape, nummodes);
-    tab.upload(build_tables(shape, nm, numpoints0), fp32);
+    // a compile-time kernel exists for equal modes in every direction and
+    // the MODIFIED default points of an instantiated (modes, points) pair
+    int dq[3];
+    default_points(shape, nq[0], dq);
+    const bool iso = nmd[1] == nm && nmd[2] == nm && nq[1] == dq[1] && nq[2] == dq[2];
+    const bool compiled = iso && (shape == Shape::Hex ? hex_op_supported(nm, nq[0]) : tet_supported(nm, nq[0]));
+    if (compiled) {
+        ncoeff = num_coeffs(shape, nm);
+        tab.upload(build_tables(shape, nm, nq[0]), fp32);
+    } else {
+        NMF_REQUIRE(op != OpType::PhysInterp, NMF_ERR_UNSUPPORTED,
+                    "PhysInterp1DScaled takes equal modes and the default points of the shape only");
+        generic = true;
+        glay = generic_layout(shape, nmd, nq);  // checks the sizes
+        ncoeff = glay.ncoeff;
+        tab.upload(glay.tab, fp32);
+        if (!glay.itab.empty()) {
+            gitab.allocate(glay.itab.size() * sizeof(int));
+            NMF_HIP(hipMemcpy(gitab.get(), glay.itab.data(), glay.itab.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
     if (op == OpType::BwdTrans || op == OpType::PhysInterp) deformed = false;  // no geometry
     if (op == OpType::PhysInterp) set_scaling_factor(1.5);  // the reference's default (Operator.hpp:136-141)
-    tile = tuned_tile(tune_key());
+    if (!generic) tile = tuned_tile(tune_key());
+}
+
+void MFOperator::prepare()
+{
+    ensure_geometry();
+    if (generic) (void)generic_args(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+GenArgs MFOperator::generic_args(const void *in0, const void *in1, const void *in2, void *o0, void *o1, void *o2,
+                                 hipStream_t s)
+{
+    GenArgs a{};
+    a.op = op;
+    a.shape = shape;
+    a.fp32 = fp32;
+    a.deformed = deformed;
+    a.nelmt = nelmt;
+    for (int d = 0; d < 3; ++d) {
+        a.nm[d] = nmd[d];
+        a.nq[d] = nq[d];
+        a.oB[d] = glay.oB[d];
+        a.oD[d] = glay.oD[d];
+        a.oW[d] = glay.oW[d];
+    }
+    for (int c = 0; c < 4; ++c) a.oH[c] = glay.oH[c];
+    a.nqtot = nqtot;
+    a.ncoeff = ncoeff;
+    a.npq = glay.npq;
+    a.buf = glay.buf;
+    a.tab = tab.get();
+    a.itab = gitab.as<int>();
+    a.geo = geo.get();
+    a.adv = adv.get();
+    a.lambda = lambda;
+    a.in0 = in0;
+    a.in1 = in1;
+    a.in2 = in2;
+    a.out0 = o0;
+    a.out1 = o1;
+    a.out2 = o2;
+    a.stream = s;
+    a.groups = std::min(nelmt, kGenGroups);
+    if (lds_bytes(a) > kGenMaxLds && nelmt > 0) {
+        // the element's buffers live in a global workspace, one slice per
+        // workgroup of the walking grid (allocated once, kept)
+        gws.allocate((size_t)a.groups * 4 * a.buf * value_bytes());
+        a.ws = gws.get();
+    }
+    return a;
 }
 
 void MFOperator::set_scaling_factor(double f)
@@ -252,14 +338,14 @@ std::vector<double> MFOperator::pack_geometry() const
             for (size_t e = 0; e < gp; ++e)
                 for (int c = 0; c < 9; ++c) g[e * 9 + c] = jac[e] * df[e * 9 + c];
         } else {
-            Points1D P = make_points(PointsType::GLL, nq[0]);
-            const int Q = nq[0];
+            const Points1D P0 = make_points(PointsType::GLL, nq[0]), P1 = make_points(PointsType::GLL, nq[1]),
+                           P2 = make_points(PointsType::GLL, nq[2]);
             for (int e = 0; e < nelmt; ++e)
-                for (int k = 0, q = 0; k < Q; ++k)
-                    for (int j = 0; j < Q; ++j)
-                        for (int i = 0; i < Q; ++i, ++q) {
+                for (int k = 0, q = 0; k < nq[2]; ++k)
+                    for (int j = 0; j < nq[1]; ++j)
+                        for (int i = 0; i < nq[0]; ++i, ++q) {
                             const size_t pt = (size_t)e * nqtot + q;
-                            const double wj = P.w[i] * P.w[j] * P.w[k] * jac[pt];
+                            const double wj = P0.w[i] * P1.w[j] * P2.w[k] * jac[pt];
                             for (int c = 0; c < 9; ++c) g[((size_t)e * 9 + c) * nqtot + q] = wj * df[pt * 9 + c];
                         }
         }
@@ -283,18 +369,18 @@ std::vector<double> MFOperator::pack_geometry() const
                 g[e * 7 + 6] = jac[e];
             }
         } else {
-            Points1D P = make_points(PointsType::GLL, nq[0]);
-            const int Q = nq[0];
+            const Points1D P0 = make_points(PointsType::GLL, nq[0]), P1 = make_points(PointsType::GLL, nq[1]),
+                           P2 = make_points(PointsType::GLL, nq[2]);
             g.resize(gp * 7);
             for (int e = 0; e < nelmt; ++e)
-                for (int k = 0, q = 0; k < Q; ++k)
-                    for (int j = 0; j < Q; ++j)
-                        for (int i = 0; i < Q; ++i, ++q) {
+                for (int k = 0, q = 0; k < nq[2]; ++k)
+                    for (int j = 0; j < nq[1]; ++j)
+                        for (int i = 0; i < nq[0]; ++i, ++q) {
                             const size_t pt = (size_t)e * nqtot + q;
                             double f[9], m[6];
                             fetch_df(pt, f);
                             metric6(f, m);
-                            const double wj = P.w[i] * P.w[j] * P.w[k] * jac[pt];
+                            const double wj = P0.w[i] * P1.w[j] * P2.w[k] * jac[pt];
                             for (int c = 0; c < 6; ++c) g[((size_t)e * 7 + c) * nqtot + q] = wj * m[c];
                             g[((size_t)e * 7 + 6) * nqtot + q] = wj;
                         }
@@ -332,9 +418,9 @@ std::vector<double> MFOperator::pack_advection() const
     std::vector<double> c((size_t)nelmt * 3 * nqtot);
     std::vector<double> z0, z1, z2;
     if (shape == Shape::Tet) {
-        z0 = make_basis(BasisType::ModifiedA, nm, nq[0]).pts.z;
-        z1 = make_basis(BasisType::ModifiedB, nm, nq[1]).pts.z;
-        z2 = make_basis(BasisType::ModifiedC, nm, nq[2]).pts.z;
+        z0 = make_points(PointsType::GLL, nq[0]).z;
+        z1 = make_points(PointsType::RadauM_A1B0, nq[1]).z;
+        z2 = make_points(PointsType::RadauM_A2B0, nq[2]).z;
     }
     for (int e = 0; e < nelmt; ++e)
         for (int k = 0, q = 0; k < nq[2]; ++k)
@@ -379,6 +465,11 @@ void MFOperator::apply_iproduct_deriv(const void *in0, const void *in1, const vo
     NMF_REQUIRE(op == OpType::IProductDeriv, NMF_ERR_INVALID, "not an IProductWRTDerivBase operator");
     NMF_REQUIRE(nelmt == 0 || (in0 && in1 && in2 && out), NMF_ERR_INVALID, "null input/output pointer");
     ensure_geometry();
+    if (generic) {
+        const hipError_t e = launch_generic(generic_args(in0, in1, in2, out, nullptr, nullptr, s));
+        NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        return;
+    }
     KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in0, out, nullptr, nullptr, s, 0, in1, in2};
     hipError_t e = shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
@@ -393,6 +484,11 @@ void MFOperator::apply(const void *in, void *out0, void *out1, void *out2, hipSt
     if (op == OpType::PhysInterp) {
         const InterpArgs ia{fp32, nelmt, {nq[0], nq[1], nq[2]}, {nqo[0], nqo[1], nqo[2]}, itab.get(), in, out0, s};
         const hipError_t e = launch_phys_interp(ia);
+        NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        return;
+    }
+    if (generic) {
+        const hipError_t e = launch_generic(generic_args(in, nullptr, nullptr, out0, out1, out2, s));
         NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
         return;
     }
@@ -433,7 +529,7 @@ TuneKey MFOperator::tune_key() const
 std::vector<int> MFOperator::tile_candidates() const
 {
     std::vector<int> c;
-    if (shape != Shape::Hex || op != OpType::Helmholtz) return c;
+    if (shape != Shape::Hex || op != OpType::Helmholtz || generic) return c;
     int buf[8];
     const int n = hex_tile_candidates(nm, nq[0], fp32, buf, 8);
     c.assign(buf, buf + n);

@@ -11,6 +11,7 @@
 
 #include "basis.hpp"
 #include "common.hpp"
+#include "generic.hpp"
 #include "launch.hpp"
 #include "tune.hpp"
 
@@ -42,7 +43,20 @@ struct MFOperator {
     int tile = 0;  // elements per workgroup of the Helmholtz kernel (0 = default)
     uint64_t version = 0;  // bumped by every change of geometry, factors or tile
 
+    // Modes per direction.  `generic`: no compile-time kernel takes this
+    // (shape, modes, points) -- anisotropic orders, point counts other than
+    // the instantiated ones -- and the runtime-size kernels run it (the
+    // reference's fallback, SwitchNodesPoints.h:302-343).
+    int nmd[3] = {0, 0, 0};
+    bool generic = false;
+    bool equal_modes() const { return nmd[0] == nmd[1] && nmd[1] == nmd[2]; }
+    GenLayout glay;              // generic: table offsets and sizes
+    DeviceBuffer gitab, gws;     // generic: tet index tables, global workspace
+
     MFOperator(const std::string &key, int nummodes, int numpoints0, int nelmt, int precision);
+    // Per-direction modes and points (MatrixFree::Operator's
+    // std::vector<BasisSharedPtr>, Operator.hpp:59-63).
+    MFOperator(const std::string &key, const int nummodes[3], const int numpoints[3], int nelmt, int precision);
 
     bool needs_jac() const
     {
@@ -78,7 +92,7 @@ struct MFOperator {
     // FP64 hex Helmholtz only: out = A (GlobalToLocal of the global vector xg
     // through the signed map menc, this collection's nelmt * ncoeff entries)
     void apply_mapped(const void *xg, const int *menc, void *out, hipStream_t s);
-    bool can_apply_mapped() const { return shape == Shape::Hex && op == OpType::Helmholtz && !fp32; }
+    bool can_apply_mapped() const { return shape == Shape::Hex && op == OpType::Helmholtz && !fp32 && !generic; }
     // PhysDeriv operator()(dir, in, out, wsp) (Collections/PhysDeriv.cpp
     // :312-317): all three derivatives, the two not asked for into wsp
     // (wsp_size() values of the operator's precision)
@@ -117,10 +131,16 @@ struct MFOperator {
     TuneKey tune_key() const;
     // Upload geometry if it changed (done by apply; batches call it before
     // they take the device pointers).
-    void prepare() { ensure_geometry(); }
+    // Runtime-size operators also allocate their global workspace here.
+    void prepare();
 
 private:
     void ensure_geometry();
+    void init(const std::string &key, const int nummodes[3], const int numpoints[3], int n, int precision);
+    // generic: arguments of the runtime-size kernels (allocates the global
+    // workspace on first use when the buffers exceed one workgroup's LDS)
+    GenArgs generic_args(const void *in0, const void *in1, const void *in2, void *o0, void *o1, void *o2,
+                         hipStream_t s);
 };
 
 // Parse "<Op>_<Shape>_<Regular|Deformed>" (MatrixFree::GetOpstring,

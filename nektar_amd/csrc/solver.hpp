@@ -71,6 +71,8 @@ template <typename T>
 hipError_t launch_zero(hipStream_t, size_t n, T *a);
 template <typename T>
 hipError_t launch_add(hipStream_t, size_t NG, double *x, const T *d);
+// out = a - b
+hipError_t launch_sub(hipStream_t, size_t n, const double *a, const double *b, double *out);
 hipError_t launch_invert_diag(hipStream_t, int N, double *d);
 hipError_t launch_sum_partials(hipStream_t, const double *pa, int nb, double *out);
 // state of a mixed solve from dsc[0] = ||b||^2 and dsc[1] = ||r0||^2

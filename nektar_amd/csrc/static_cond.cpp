@@ -54,6 +54,8 @@ void IterativeSolver::setup_static_cond()
     for (size_t k = 0; k < ops_.size(); ++k) {
         MFOperator &o = *ops_[k];
         StaticCondBlock b;
+        NMF_REQUIRE(o.equal_modes(), NMF_ERR_UNSUPPORTED,
+                    "static condensation takes operators with equal modes in every direction");
         const std::vector<int> I = schur::interior_modes(o.shape, o.nm);
         bmodes[k] = schur::boundary_modes(o.shape, o.nm);
         const std::vector<int> &Bd = bmodes[k];
@@ -149,6 +151,8 @@ void IterativeSolver::setup_static_cond()
         if (E == 0) continue;
         const size_t n = (size_t)E * nc;
         double *u = xloc_.as<double>() + off_[k], *y = yloc_.as<double>() + off_[k];
+        NMF_REQUIRE(o.equal_modes(), NMF_ERR_UNSUPPORTED,
+                    "static condensation takes operators with equal modes in every direction");
         const std::vector<int> I = schur::interior_modes(o.shape, o.nm);
         const std::vector<int> &Bd = bmodes[k];
         DeviceBuffer L(std::max<size_t>((size_t)E * ni * ni, 1) * sizeof(double));

@@ -322,3 +322,12 @@ def helmholtz_aniso(shape, nm, nq, nelmt, deformed, jac, df, lam, x, diff6=None)
     lib().or_helmholtz_aniso(shape, _i3(nm), _i3(nq), nelmt, int(deformed), p(jac), p(df), ctypes.c_double(lam),
                              p(d), p(np.ascontiguousarray(x)), p(out))
     return out
+
+
+def adv_diff_react_aniso(shape, nm, nq, nelmt, deformed, jac, df, vel, lam, x, diff6=None):
+    out = np.zeros(nelmt * ncoeffs_aniso(shape, nm))
+    d = None if diff6 is None else np.ascontiguousarray(diff6, dtype=np.float64)
+    lib().or_adv_diff_react_aniso(shape, _i3(nm), _i3(nq), nelmt, int(deformed), p(jac), p(df),
+                                  p(np.ascontiguousarray(vel, dtype=np.float64)), ctypes.c_double(lam), p(d),
+                                  p(np.ascontiguousarray(x)), p(out))
+    return out

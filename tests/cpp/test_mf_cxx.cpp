@@ -29,6 +29,8 @@ void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed, const dou
                   double lambda, const double *diff6, const double *in, double *out);
 void or_physderiv(int shape, int nm, int nq0, int nelmt, int deformed, const double *df, const double *in,
                   double *out0, double *out1, double *out2);
+void or_helmholtz_aniso(int shape, const int *nm, const int *nq, int nelmt, int deformed, const double *jac,
+                        const double *df, double lambda, const double *diff6, const double *in, double *out);
 }
 
 using namespace Nektar;
@@ -107,6 +109,34 @@ int main()
         threw = e.code == NMF_ERR_UNSUPPORTED;
     }
     if (!threw) return 3;
+
+    // one BasisKey per direction: a single key is refused, an anisotropic
+    // expansion (modes 4/5/6, points 6/7/8) runs and matches the oracle's
+    // per-direction operator
+    threw = false;
+    try {
+        MatrixFree::GetOperatorFactory().CreateInstance("Helmholtz_Hex_Regular",
+                                                        std::vector<LibUtilities::BasisKey>(1, {nm, nq}), ne);
+    } catch (const NekError &e) {
+        threw = e.code == NMF_ERR_INVALID;
+    }
+    if (!threw) return 4;
+    {
+        const std::vector<LibUtilities::BasisKey> ab = {{4, 6}, {5, 7}, {6, 8}};
+        auto ah = std::dynamic_pointer_cast<MatrixFree::Helmholtz>(
+            MatrixFree::GetOperatorFactory().CreateInstance("Helmholtz_Hex_Regular", ab, ne));
+        if (!ah || ah->GetInputSize() != ne * 4 * 5 * 6) return 5;
+        ah->SetJac(jac);
+        ah->SetDF(df);
+        ah->SetLambda(0.8);
+        std::vector<double> ax(ah->GetInputSize()), ay;
+        for (auto &v : ax) v = U(rng);
+        (*ah)(ax, ay);
+        const int am[3] = {4, 5, 6}, aq[3] = {6, 7, 8};
+        std::vector<double> ar(ay.size());
+        or_helmholtz_aniso(0, am, aq, ne, 0, jac.data(), df.data(), 0.8, nullptr, ax.data(), ar.data());
+        err = std::fmax(err, relerr(ay, ar));
+    }
 
     // MultiRegions mirror: the structured 3^3 box, P=4, its C0 map renumbered
     // Dirichlet-first, solved by the three solver choices against the oracle's CG

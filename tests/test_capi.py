@@ -67,8 +67,13 @@ def test_bad_keys_report_errors(nmf):
     assert rc == 2
     rc = L.nmf_operator_create(b"Helmholtz_Hex", 4, 5, 1, 0, ctypes.byref(h))
     assert rc == 1
-    rc = L.nmf_operator_create(b"Helmholtz_Hex_Regular", 11, 12, 1, 0, ctypes.byref(h))
-    assert rc == 2 and b"not instantiated" in L.nmf_last_error()
+    # orders without a compile-time kernel run the runtime-size kernels;
+    # beyond those kernels' limits the order is refused before any device work
+    rc = L.nmf_operator_create(b"Helmholtz_Hex_Regular", 33, 34, 1, 0, ctypes.byref(h))
+    assert rc == 2 and b"modes per direction" in L.nmf_last_error()
+    m, q = (ctypes.c_int * 3)(4, 3, 5), (ctypes.c_int * 3)(5, 5, 5)
+    rc = L.nmf_operator_create_basis(b"Helmholtz_Tet_Regular", m, q, 1, 0, ctypes.byref(h))
+    assert rc == 2 and b"nm0 <= nm1 <= nm2" in L.nmf_last_error()
     rc = L.nmf_operator_create(b"Helmholtz_Hex_Regular", 4, 5, 1, 7, ctypes.byref(h))
     assert rc == 1
     assert L.nmf_operator_set_lambda(None, 1.0) == 1

@@ -81,7 +81,18 @@ def test_fp64_cg_matches_oracle(nmf, n, nm, nq, lam, amp, seed):
     assert info.converged and info.final_error < 1e-10
     assert abs(info.iterations - info_o.iterations) <= 2, (info.iterations, info_o.iterations)
     assert info.rhs_magnitude == pytest.approx(info_o.rhs_magnitude, rel=1e-12)
-    assert _rel(s.solution(), xo) <= 1e-8
+    # the GPU solution solves the oracle's system to the stopping tolerance:
+    # its true residual with the oracle's operator ...
+    x = s.solution()
+    yl = O.helmholtz(O.HEX, nm, nq, n ** 3, P["deformed"], P["jac"], P["df"], lam, x[P["m"]])
+    y = np.zeros(s.nglobal)
+    np.add.at(y, P["m"], yl)
+    r = np.where(P["dirichlet"], 0.0, P["b"] - y)
+    assert np.sqrt(np.sum(r * r) / info_o.rhs_magnitude) < 2e-10
+    # ... and it agrees with the oracle's CG solution as far as two
+    # solutions of residual 1e-10 must (different rounding paths of the same
+    # recurrence; the condition number allows up to ~1e-7 relative)
+    assert _rel(x, xo) <= 1e-7
 
 
 @pytest.mark.parametrize("n,nm,nq,lam,amp,seed", CASES)
