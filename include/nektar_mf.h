@@ -170,6 +170,21 @@ int nmf_operator_wsp_size(nmf_operator op, int64_t *wsp_size);
  * Collection.h:148-155). */
 int nmf_operator_apply_physderiv_dir(nmf_operator op, int dir, const void *in,
                                      void *out, void *wsp, void *stream);
+/* The single-direction PhysDeriv with no device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  The two directions not asked for go to a 64 MiB static
+ * device array of the library (one per GPU, held until the process ends)
+ * instead of a workspace, so 2 * nelmt * nqtot values of the operator's
+ * precision must fit 64 MiB (NMF_ERR_UNSUPPORTED otherwise), and the
+ * operator's geometry must already be on the device: after SetDF, one
+ * ordinary apply uploads it (NMF_ERR_STATE until then).  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (different
+ * streams or threads) race; nothing in it needs clearing between calls.
+ * Results are those of nmf_operator_apply_physderiv_dir, bit for bit. */
+int nmf_operator_apply_physderiv_dir_noalloc(nmf_operator op, int dir,
+                                             const void *in, void *out,
+                                             void *stream);
 /* IProductWRTDerivBase operator()(input[3], output) (Operator.hpp:192-217,
  * IProductWRTDerivBase.h:592-825): out = sum_j IProduct w.r.t. dphi/dx_j of
  * in_j; inputs [e][nqtot] at the quadrature points, output [e][ncoeff].
