@@ -12,6 +12,9 @@ does through ContField (library/Demos/MultiRegions/Helmholtz3D.cpp).
   (ExpList::FwdTransBndConstrained, ExpList.cpp:2420; SegExp.cpp:81-180,
   QuadExp.cpp:67-160), then imposed on the element-local coefficients
   (ContField::v_ImposeDirichletConditions, ContField.cpp:557-608).
+* Tetrahedra: TriExp::v_FwdTransBndConstrained (TriExp.cpp:69-180) on the
+  triangular faces, Radau edge values interpolated to GLL points, and the
+  reference's overwrite order for shared vertices and edges.
 * ``helmholtz_solve``: ContField::v_HelmSolve (ContField.cpp:797-925) --
   rhs = -IProductWRTBase(f), the Dirichlet lifting of
   GlobalLinSysIterativeFull::v_Solve (GlobalLinSysIterativeFull.cpp:
@@ -248,53 +251,77 @@ def errors(u, exact, jac, w):
     return float(np.sqrt(np.sum(wq[None, :] * np.asarray(jac)[:, None] * err * err))), float(np.max(np.abs(err)))
 
 
+def dirichlet_problem(session: Session, var: str = "u"):
+    """Host set-up of the C0 problem of a session: element kind, local
+    vertices and coordinates, geometric factors, NUMMODES and direction-0
+    points (NUMMODES + 1, the MODIFIED default), the C0 map with the
+    Dirichlet DOFs first, the element-local Dirichlet values and the
+    physical coordinates of the quadrature points.  Meshes of
+    parallelepiped hexahedra or of tetrahedra, Dirichlet conditions."""
+    from . import SHAPE_HEX, SHAPE_TET, basis_tables
+
+    s = session
+    kinds = {k for k, _ in s.mesh.elements.values()}
+    for bcs in s.conditions.values():
+        for bc in bcs:
+            if bc.var == var and bc.kind != "D":
+                raise NotImplementedError(f"boundary condition type {bc.kind} (only Dirichlet is supported)")
+    nm = s.mesh.expansions[0]["nummodes"]
+    nq0 = nm + 1
+    regions = [rid for rid, bcs in s.conditions.items() if any(bc.var == var and bc.kind == "D" for bc in bcs)]
+    faces = [f for rid in regions for cid in s.regions[rid] for _, ids in s.mesh.composites[cid] for f in ids]
+    if kinds == {"H"}:
+        eids, lv, verts = nio.hex_elements(s.mesh)
+        jac, df = nio.hex_geometry(verts)
+        mp, sg, ng, nd = nio.hex_c0_map(lv, nm, [_face_vertices(s.mesh, f) for f in faces])
+        t = basis_tables(SHAPE_HEX, 0, nm, nq0)
+        tables = (t["bdata"].reshape(nm, nq0), t["z"], t["w"])
+        dloc = hex_dirichlet_local(s, var, lv, verts, nm, nq0, mp, sg, nd, tables)
+        X = hex_quadrature_points(verts, t["z"])
+        weights = (t["w"],)
+        tag = "Hex"
+    elif kinds == {"A"}:
+        eids, lv, verts = nio.tet_elements(s.mesh)
+        jac, df = nio.tet_geometry(verts)
+        mp, sg, ng, nd = nio.tet_c0_map(lv, nm, [frozenset(_tri_vertices(s.mesh, f)) for f in faces])
+        dloc = tet_dirichlet_local(s, var, lv, verts, nm, nq0, mp, sg, nd)
+        t = [basis_tables(SHAPE_TET, d, nm, nq0) for d in range(3)]
+        X = tet_quadrature_points(verts, t[0]["z"], t[1]["z"], t[2]["z"])
+        weights = (t[0]["w"], t[1]["w"], t[2]["w"])
+        tag = "Tet"
+    else:
+        raise NotImplementedError("Dirichlet solves take meshes of hexahedra or of tetrahedra")
+    return dict(tag=tag, eids=eids, lv=lv, verts=verts, jac=jac, df=df, nm=nm, nq0=nq0, map=mp, sign=sg,
+                nglobal=ng, ndir=nd, dloc=dloc, X=X, weights=weights)
+
+
 def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit: int = 20000,
                     solver: str = "ConjugateGradient"):
     """ContField::HelmSolve of the session's Forcing with its Dirichlet
     conditions and parameter Lambda (the Helmholtz3D demo): returns
     (L2 error, Linf error, SolveInfo, element-local solution) against the
-    session's ExactSolution at the quadrature points.  Hexahedral meshes
-    of parallelepipeds, NUMMODES per the expansion, Q = NUMMODES + 1
-    GLL points (the MODIFIED default)."""
+    session's ExactSolution at the quadrature points.  Meshes of
+    parallelepiped hexahedra or of tetrahedra, NUMMODES per the expansion,
+    the MODIFIED default points; GPU operators and CG."""
     import torch
 
-    from . import AssemblyMap, FP64, GetOperatorFactory, LinearSystem, basis_tables, SHAPE_HEX
+    from . import AssemblyMap, FP64, GetOperatorFactory, LinearSystem
 
     s = read_session(session_path)
-    kinds = {k for k, _ in s.mesh.elements.values()}
-    if kinds != {"H"}:
-        raise NotImplementedError("helmholtz_solve takes hexahedral meshes")
-    for bcs in s.conditions.values():
-        for bc in bcs:
-            if bc.var == var and bc.kind != "D":
-                raise NotImplementedError(f"boundary condition type {bc.kind} (only Dirichlet is supported)")
-    eids, lv, verts = nio.hex_elements(s.mesh)
-    jac, df = nio.hex_geometry(verts)
-    nm = s.mesh.expansions[0]["nummodes"]
-    nq, E = nm + 1, len(eids)
-    dsets = []
-    for rid, bcs in s.conditions.items():
-        if any(bc.var == var and bc.kind == "D" for bc in bcs):
-            for cid in s.regions[rid]:
-                dsets += [_face_vertices(s.mesh, f) for letter, ids in s.mesh.composites[cid] for f in ids]
-    mp, sg, ng, nd = nio.hex_c0_map(lv, nm, dsets)
-    t = basis_tables(SHAPE_HEX, 0, nm, nq)
-    B = t["bdata"].reshape(nm, nq)
-    tables = (B, t["z"], t["w"])
-    dloc = hex_dirichlet_local(s, var, lv, verts, nm, nq, mp, sg, nd, tables)
-
+    P = dirichlet_problem(s, var)
+    tag, nm, nq0, jac, df, X = P["tag"], P["nm"], P["nq0"], P["jac"], P["df"], P["X"]
+    E = len(P["eids"])
     lam = s.params.get("Lambda", 0.0)
     F = GetOperatorFactory()
-    bwd = F.CreateInstance("BwdTrans_Hex_Regular", nm, nq, E, FP64)
-    ipr = F.CreateInstance("IProduct_Hex_Regular", nm, nq, E, FP64)
+    bwd = F.CreateInstance(f"BwdTrans_{tag}_Regular", nm, nq0, E, FP64)
+    ipr = F.CreateInstance(f"IProduct_{tag}_Regular", nm, nq0, E, FP64)
     ipr.SetJac(jac)
-    helm = F.CreateInstance("Helmholtz_Hex_Regular", nm, nq, E, FP64)
+    helm = F.CreateInstance(f"Helmholtz_{tag}_Regular", nm, nq0, E, FP64)
     helm.SetJac(jac)
     helm.SetDF(df)
     helm.SetLambda(lam)
-    ls = LinearSystem(AssemblyMap(mp, sg, ng, nd), [helm], solver=solver)
+    ls = LinearSystem(AssemblyMap(P["map"], P["sign"], P["nglobal"], P["ndir"]), [helm], solver=solver)
 
-    X = hex_quadrature_points(verts, t["z"])
     names = dict(s.params, x=X[..., 0], y=X[..., 1], z=X[..., 2], t=0.0)
     f = np.broadcast_to(evaluate(s.functions["Forcing"][var], names), X.shape[:2]).astype(float).reshape(-1)
     exact = np.broadcast_to(evaluate(s.functions["ExactSolution"][var], names), X.shape[:2]).astype(float)
@@ -304,13 +331,249 @@ def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit
     bl = torch.zeros(ipr.output_size(), dtype=torch.float64, device=dev)
     ipr(fq, bl)
     ls.set_rhs_local(-bl)  # ContField.cpp:812-817: -IProductWRTBase(f)
-    ls.set_dirichlet_local(torch.from_numpy(np.ascontiguousarray(dloc)).to(dev))
+    ls.set_dirichlet_local(torch.from_numpy(np.ascontiguousarray(P["dloc"])).to(dev))
     info = ls.solve(tol=tol, maxit=maxit)
-    xl = torch.zeros(E * nm ** 3, dtype=torch.float64, device=dev)
+    xl = torch.zeros(helm.input_size(), dtype=torch.float64, device=dev)
     ls.solution_local(xl)
     uq = torch.zeros(bwd.output_size(), dtype=torch.float64, device=dev)
     bwd(xl, uq)
     torch.cuda.synchronize()
     u = uq.cpu().numpy().reshape(E, -1)
-    l2, linf = errors(u, exact, jac, t["w"])
+    if tag == "Hex":
+        l2, linf = errors(u, exact, jac, *P["weights"])
+    else:
+        l2, linf = tet_errors(u, exact, jac, *P["weights"])
     return l2, linf, info, xl.cpu().numpy()
+
+
+# ---------------------------------------------------------------------
+# Tetrahedral meshes: triangular boundary faces
+
+def _jacobi(n, a, b, x):
+    """P_n^{(a,b)}(x) by the three-term recurrence."""
+    x = np.asarray(x, dtype=float)
+    p0 = np.ones_like(x)
+    if n == 0:
+        return p0
+    p1 = 0.5 * ((a + b + 2.0) * x + (a - b))
+    for k in range(2, n + 1):
+        c = 2.0 * k + a + b
+        p0, p1 = p1, ((c - 1.0) * (c * (c - 2.0) * x + a * a - b * b) * p1
+                      - 2.0 * (k + a - 1.0) * (k + b - 1.0) * c * p0) / (2.0 * k * (k + a + b) * (c - 2.0))
+    return p1
+
+
+def _mod_a(p, z):
+    """Modified_A mode p (Basis.cpp:466-497)."""
+    m, pl = 0.5 * (1.0 - z), 0.5 * (1.0 + z)
+    return m if p == 0 else pl if p == 1 else m * pl * _jacobi(p - 2, 1.0, 1.0, z)
+
+
+def _mod_b(p, q, z):
+    """Modified_B row p, column q (Basis.cpp:499-589); Modified_C mode
+    (p, q, r) is _mod_b(p + q, r)."""
+    m, pl = 0.5 * (1.0 - z), 0.5 * (1.0 + z)
+    if p == 0:
+        return m if q == 0 else pl if q == 1 else m * pl * _jacobi(q - 2, 1.0, 1.0, z)
+    if p == 1:
+        return m if q == 0 else m * pl * _jacobi(q - 1, 1.0, 1.0, z)
+    return m ** p if q == 0 else m ** p * pl * _jacobi(q - 1, 2.0 * p - 1.0, 1.0, z)
+
+
+def _tri_basis(nm, e0, e1):
+    """Triangle modes (p < nm, q < nm - p; p slowest) at collapsed points
+    (e0, e1), the top vertex (mode 1) with its collapsed-vertex term
+    (StdTriExp's BwdTrans)."""
+    cols = [_mod_a(p, e0) * _mod_b(p, q, e1) for p in range(nm) for q in range(nm - p)]
+    cols[1] = cols[1] + _mod_a(1, e0) * _mod_b(0, 1, e1)
+    return np.stack(cols, axis=-1)
+
+
+def _tet_basis(nm, e0, e1, e2):
+    """Tet modes (the tet order) at collapsed points, with the singular
+    vertex and edge terms of BwdTransKernels.hpp:296-324."""
+    a = [_mod_a(p, e0) for p in range(nm)]
+    cols = []
+    for p in range(nm):
+        for q in range(nm - p):
+            bq = _mod_b(p, q, e1)
+            for r in range(nm - p - q):
+                cols.append(a[p] * bq * _mod_b(p + q, r, e2))
+    b00, b01 = _mod_b(0, 0, e1), _mod_b(0, 1, e1)
+    cols[1] = cols[1] + (a[0] * b01 + a[1] * b00 + a[1] * b01) * _mod_b(0, 1, e2)
+    cols[nm] = cols[nm] + a[1] * b01 * _mod_b(0, 0, e2)
+    for r in range(1, nm - 1):
+        cols[nm + r] = cols[nm + r] + a[1] * b01 * _mod_b(0, r + 1, e2)
+    return np.stack(cols, axis=-1)
+
+
+def _interp_matrix(zf, zt):
+    """Lagrange interpolation from nodes zf to points zt (LibUtilities::
+    Interp1D): I[t, f] = l_f(zt_t)."""
+    I = np.ones((len(zt), len(zf)))
+    for f in range(len(zf)):
+        for g in range(len(zf)):
+            if g != f:
+                I[:, f] *= (zt - zf[g]) / (zf[f] - zf[g])
+    return I
+
+
+def _tri_face_coeffs(nm, tab, F):
+    """TriExp::v_FwdTransBndConstrained (TriExp.cpp:69-180) of the point
+    values F[i, j] of a triangle (GLL i along edge 0, Gauss-Radau(1,0) j
+    towards the collapsed vertex 2): edge 0 from its GLL values, edges 1
+    and 2 from their Radau values interpolated to the GLL points (so vertex
+    2 takes the interpolant's value there), each edge by the 1-D
+    constrained projection, written in edge order (edge 2 last), then the
+    interior modes projected with the boundary ones held."""
+    z0, w0, z1, w1, Ba = tab
+    I = _interp_matrix(z1, z0)
+    idx = {}
+    for p in range(nm):
+        for q in range(nm - p):
+            idx[(p, q)] = len(idx)
+    c = np.zeros(len(idx))
+    c0 = _project_1d(Ba, w0, F[:, 0], F[0, 0], F[-1, 0])
+    c[idx[(0, 0)]], c[idx[(1, 0)]] = c0[0], c0[1]
+    for p in range(2, nm):
+        c[idx[(p, 0)]] = c0[p]
+    g1 = I @ F[-1, :]
+    c1 = _project_1d(Ba, w0, g1, g1[0], g1[-1])
+    c[idx[(1, 0)]], c[idx[(0, 1)]] = c1[0], c1[1]
+    for k in range(2, nm):
+        c[idx[(1, k - 1)]] = c1[k]
+    g2 = I @ F[0, :]
+    c2 = _project_1d(Ba, w0, g2, g2[0], g2[-1])
+    c[idx[(0, 0)]], c[idx[(0, 1)]] = c2[0], c2[1]
+    for k in range(2, nm):
+        c[idx[(0, k)]] = c2[k]
+    inner = [idx[(p, q)] for p in range(2, nm) for q in range(1, nm - p)]
+    if inner:
+        E0, E1 = np.meshgrid(z0, z1, indexing="ij")
+        Psi = _tri_basis(nm, E0, E1).reshape(-1, len(idx))
+        W = np.outer(w0, w1).reshape(-1)
+        M = (Psi.T * W) @ Psi
+        rhs = (Psi.T * W) @ F.reshape(-1) - M @ c
+        c[inner] = np.linalg.solve(M[np.ix_(inner, inner)], rhs[inner])
+    return c
+
+
+def _tri_vertices(mesh: nio.MeshGraph, fid: int):
+    """TriGeom's vertex order (TriGeom.cpp:59-88): vertex j is the end of
+    edge j not shared with edge j+1."""
+    e = [mesh.edges[x] for x in mesh.faces[fid][1]]
+    return [next(v for v in e[j] if v not in e[(j + 1) % 3]) for j in range(3)]
+
+
+def _tet_points(nm_unused, verts_e, e0, e1, e2):
+    """Physical coordinates of collapsed points of a straight-sided tet
+    (local vertex order; reference vertices (-1,-1,-1), (1,-1,-1),
+    (-1,1,-1), (-1,-1,1))."""
+    xi2 = e2
+    xi1 = 0.5 * (1.0 + e1) * (1.0 - e2) - 1.0
+    xi0 = 0.25 * (1.0 + e0) * (1.0 - e1) * (1.0 - e2) - 1.0
+    v = np.asarray(verts_e)
+    return (v[0] + 0.5 * (xi0 + 1.0)[..., None] * (v[1] - v[0]) + 0.5 * (xi1 + 1.0)[..., None] * (v[2] - v[0])
+            + 0.5 * (xi2 + 1.0)[..., None] * (v[3] - v[0]))
+
+
+# the element grid points on each local face: (fixed collapsed direction, index 0 or -1)
+_TET_FACE_GRID = {0: (2, 0), 1: (1, 0), 2: (0, -1), 3: (0, 0)}
+_TET_FACE_VERTS = {0: (0, 1, 2), 1: (0, 1, 3), 2: (1, 2, 3), 3: (0, 2, 3)}
+
+
+def tet_dirichlet_local(session: Session, var: str, lv, verts, nm: int, nq0: int, mp, sg, ndir: int):
+    """Element-local Dirichlet coefficients of a tetrahedral mesh, as the
+    reference leaves them before the solve: every Dirichlet face projected
+    by the triangle's constrained transform in its own frame and written
+    into the local modes of the element that owns it, faces in boundary
+    order, a later face overwriting the vertex and edge modes it shares
+    (ContField.cpp:557-590); the local copies that no face of their element
+    wrote take the value of the last face written for their DOF
+    (AssemblyMapCG.cpp:1955-2010)."""
+    from . import SHAPE_TET, basis_tables
+
+    mesh = session.mesh
+    E = len(lv)
+    t0 = basis_tables(SHAPE_TET, 0, nm, nq0)
+    t1 = basis_tables(SHAPE_TET, 1, nm, nq0)
+    t2 = basis_tables(SHAPE_TET, 2, nm, nq0)
+    z0, w0, z1, w1, z2 = t0["z"], t0["w"], t1["z"], t1["w"], t2["z"]
+    Ba = np.array([_mod_a(p, z0) for p in range(nm)])
+    tri_tab = (z0, w0, z1, w1, Ba)
+    ents = nio.tet_mode_local_entities(nm)
+    nc = len(ents)
+    owner = {}
+    for e in range(E):
+        for f, lverts in _TET_FACE_VERTS.items():
+            owner.setdefault(frozenset(int(lv[e][k]) for k in lverts), []).append((e, f))
+    loc = np.zeros(E * nc)
+    written = np.zeros(E * nc, dtype=bool)
+    last = {}
+    grids = np.meshgrid(z0, z1, z2, indexing="ij")  # [i, j, k]
+    for rid in sorted(session.conditions):
+        for bc in session.conditions[rid]:
+            if bc.var != var or bc.kind != "D":
+                continue
+            for cid in sorted(session.regions[rid]):
+                for letter, ids in mesh.composites[cid]:
+                    if letter != "F":
+                        raise NotImplementedError("Dirichlet regions of tetrahedral meshes are made of faces")
+                    for fid in ids:
+                        fv = _tri_vertices(mesh, fid)
+                        own = owner.get(frozenset(fv), [])
+                        if len(own) != 1:
+                            raise ValueError(f"boundary face {fid} belongs to {len(own)} elements")
+                        e, f = own[0]
+                        P = np.array([mesh.vertices[v] for v in fv])
+                        # the boundary condition at the face's own points
+                        E0, E1 = np.meshgrid(z0, z1, indexing="ij")
+                        X0 = 0.5 * (1.0 + E0) * (1.0 - E1)  # xi0 + 1
+                        X = P[0] + 0.5 * X0[..., None] * (P[1] - P[0]) + 0.5 * (E1 + 1.0)[..., None] * (P[2] - P[0])
+                        Fv = np.broadcast_to(evaluate(bc.value, dict(session.params, x=X[..., 0], y=X[..., 1],
+                                                                     z=X[..., 2], t=0.0)), X.shape[:2]).astype(float)
+                        c = _tri_face_coeffs(nm, tri_tab, Fv)
+                        # the element's own grid points on its face f
+                        d, side = _TET_FACE_GRID[f]
+                        sl = [slice(None)] * 3
+                        sl[d] = side
+                        g0, g1, g2 = (g[tuple(sl)].reshape(-1) for g in grids)
+                        Xe = _tet_points(nm, verts[e], g0, g1, g2)
+                        # those points in the face's collapsed coordinates
+                        A = np.stack([P[1] - P[0], P[2] - P[0]], axis=1) * 0.5
+                        xi = np.linalg.lstsq(A, (Xe - P[0]).T, rcond=None)[0].T - 1.0
+                        den = 1.0 - xi[:, 1]
+                        fe0 = np.where(den > 1e-13, 2.0 * (1.0 + xi[:, 0]) / np.where(den > 1e-13, den, 1.0) - 1.0, 0.0)
+                        gvals = _tri_basis(nm, fe0, xi[:, 1]) @ c
+                        lvf = frozenset(_TET_FACE_VERTS[f])
+                        modes = [k for k in range(nc) if ents[k] is not None and ents[k] <= lvf]
+                        Psi = _tet_basis(nm, g0, g1, g2)[:, modes]
+                        ce = np.linalg.lstsq(Psi, gvals, rcond=None)[0]
+                        for k, v in zip(modes, ce):
+                            li = e * nc + k
+                            if mp[li] >= ndir:
+                                raise ValueError("a Dirichlet face mode maps to a non-Dirichlet DOF")
+                            loc[li] = v
+                            written[li] = True
+                            last[int(mp[li])] = li
+    for li in range(E * nc):
+        g = int(mp[li])
+        if g < ndir and not written[li]:
+            src = last[g]
+            loc[li] = sg[li] * sg[src] * loc[src]
+    return loc
+
+
+def tet_quadrature_points(verts, z0, z1, z2):
+    """Physical coordinates [E, nqtot, 3] of the collapsed quadrature points
+    (direction 0 fastest) of straight-sided tets verts [E, 4, 3]."""
+    K, J, I = np.meshgrid(z2, z1, z0, indexing="ij")
+    return np.stack([_tet_points(None, v, I.reshape(-1), J.reshape(-1), K.reshape(-1)) for v in verts])
+
+
+def tet_errors(u, exact, jac, w0, w1, w2):
+    """ExpList::L2 and Linf on straight-sided tets: weights w0 x (w1/2) x
+    (w2/4) of the collapsed points (the Radau pre-scaling) times J."""
+    wq = np.einsum("k,j,i->kji", 0.25 * w2, 0.5 * w1, w0).reshape(-1)
+    err = np.asarray(u).reshape(len(jac), -1) - np.asarray(exact).reshape(len(jac), -1)
+    return float(np.sqrt(np.sum(wq[None, :] * np.asarray(jac)[:, None] * err * err))), float(np.max(np.abs(err)))

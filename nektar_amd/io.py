@@ -444,6 +444,41 @@ def _tet_index(nm):
     return idx
 
 
+def _tet_trace_modes(nm):
+    """The tet's local edges and faces with the modes that live on them
+    (StdTetExp.cpp:1459-1660): edges (start, end, modes along the edge);
+    faces (first-direction start, end, apex, [(mode, flips when the first
+    direction is reversed)]); vertex modes in local vertex order."""
+    edges = [(0, 1, [(i + 2, 0, 0) for i in range(nm - 2)]), (1, 2, [(1, i + 1, 0) for i in range(nm - 2)]),
+             (0, 2, [(0, i + 2, 0) for i in range(nm - 2)]), (0, 3, [(0, 0, i + 2) for i in range(nm - 2)]),
+             (1, 3, [(1, 0, i + 1) for i in range(nm - 2)]), (2, 3, [(0, 1, i + 1) for i in range(nm - 2)])]
+    f0 = [((i, j, 0), i % 2 == 1) for i in range(2, nm) for j in range(1, nm - i)]
+    f1 = [((i, 0, k), i % 2 == 1) for i in range(2, nm) for k in range(1, nm - i)]
+    f2 = [((1, j, k), (j + 1) % 2 == 1) for j in range(1, nm - 1) for k in range(1, nm - 1 - j)]
+    f3 = [((0, j, k), j % 2 == 1) for j in range(2, nm) for k in range(1, nm - j)]
+    faces = [(0, 1, 2, f0), (0, 1, 3, f1), (1, 2, 3, f2), (0, 2, 3, f3)]
+    vmodes = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    return edges, faces, vmodes
+
+
+def tet_mode_local_entities(nm):
+    """For every coefficient of the tet order, the set of local vertices of
+    the entity its mode belongs to (vertex, edge or face), None for
+    interior modes."""
+    ix = _tet_index(nm)
+    edges, faces, vmodes = _tet_trace_modes(nm)
+    out = [None] * len(ix)
+    for v, m in enumerate(vmodes):
+        out[ix[m]] = frozenset([v])
+    for a, b, modes in edges:
+        for m in modes:
+            out[ix[m]] = frozenset([a, b])
+    for a, b, c, modes in faces:
+        for m, _ in modes:
+            out[ix[m]] = frozenset([a, b, c])
+    return out
+
+
 def tet_c0_map(local_vertices: np.ndarray, nm: int, dirichlet_vertex_sets=()):
     """C0 map of a conforming tet mesh from the elements' global vertex ids
     local_vertices [E, 4] (TetGeom order).  Returns (map, sign, nglobal,
@@ -462,17 +497,7 @@ def tet_c0_map(local_vertices: np.ndarray, nm: int, dirichlet_vertex_sets=()):
     E = len(lv)
     ix = _tet_index(nm)
     nc = len(ix)
-    # local edges: (start vertex, end vertex, modes along it)
-    edges = [(0, 1, [(i + 2, 0, 0) for i in range(nm - 2)]), (1, 2, [(1, i + 1, 0) for i in range(nm - 2)]),
-             (0, 2, [(0, i + 2, 0) for i in range(nm - 2)]), (0, 3, [(0, 0, i + 2) for i in range(nm - 2)]),
-             (1, 3, [(1, 0, i + 1) for i in range(nm - 2)]), (2, 3, [(0, 1, i + 1) for i in range(nm - 2)])]
-    # local faces: (dir1 start, dir1 end, apex, [(mode, flips when dir1 is reversed)])
-    f0 = [((i, j, 0), i % 2 == 1) for i in range(2, nm) for j in range(1, nm - i)]
-    f1 = [((i, 0, k), i % 2 == 1) for i in range(2, nm) for k in range(1, nm - i)]
-    f2 = [((1, j, k), (j + 1) % 2 == 1) for j in range(1, nm - 1) for k in range(1, nm - 1 - j)]
-    f3 = [((0, j, k), j % 2 == 1) for j in range(2, nm) for k in range(1, nm - j)]
-    faces = [(0, 1, 2, f0), (0, 1, 3, f1), (1, 2, 3, f2), (0, 2, 3, f3)]
-    vmodes = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    edges, faces, vmodes = _tet_trace_modes(nm)
     keys: dict = {}
     apex_of: dict = {}
     nxt = [0]

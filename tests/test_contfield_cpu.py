@@ -85,3 +85,41 @@ def test_helmholtz3d_cg_hex_metrics_with_the_oracle():
     l2, linf = cf.errors(u, exact, jac, t["w"])
     assert abs(l2 - 1.11818e-07) <= 1e-12, l2
     assert abs(linf - 5.66374e-07) <= 1e-12, linf
+
+
+TET = os.path.join(HERE, "golden", "helmholtz3d", "Helmholtz3D_Tet.xml")
+
+
+def test_helmholtz3d_cg_tet_metrics_with_the_oracle():
+    """library/Demos/MultiRegions/Tests/Helmholtz3D_CG_Tet.tst (48 tets,
+    NUMMODES 5): L2 8.458e-05 (tolerance 1e-8), Linf 0.000862348 (1e-9).
+    These values carry the reference's handling of triangular boundary
+    faces (Radau edge values interpolated to GLL points, so the collapsed
+    vertex takes an extrapolated value; later faces overwrite shared
+    vertices and edges)."""
+    s = cf.read_session(TET)
+    P = cf.dirichlet_problem(s)
+    nm, nq0, E, jac, df = P["nm"], P["nq0"], len(P["eids"]), P["jac"], P["df"]
+    mp, sg, ng, nd, dloc, X = P["map"], P["sign"], P["nglobal"], P["ndir"], P["dloc"], P["X"]
+    names = dict(s.params, x=X[..., 0], y=X[..., 1], z=X[..., 2], t=0.0)
+    f = cf.evaluate(s.functions["Forcing"]["u"], names).reshape(-1)
+    exact = cf.evaluate(s.functions["ExactSolution"]["u"], names)
+    lam = s.params["Lambda"]
+    nc = O.ncoeffs(O.TET, nm)
+    rl = -O.iproduct(O.TET, nm, nq0, E, False, jac, f) - O.helmholtz(O.TET, nm, nq0, E, False, jac, df, lam, dloc)
+    A = np.zeros((ng, ng))
+    for m in range(nc):
+        x = np.zeros(E * nc)
+        x[m::nc] = 1.0
+        y = O.helmholtz(O.TET, nm, nq0, E, False, jac, df, lam, x)
+        for e in range(E):
+            i = e * nc + np.arange(nc)
+            np.add.at(A, (mp[i], np.full(nc, mp[e * nc + m])), sg[i] * y[i] * sg[e * nc + m])
+    b = np.zeros(ng)
+    np.add.at(b, mp, sg * rl)
+    xg = np.zeros(ng)
+    xg[nd:] = np.linalg.solve(A[nd:, nd:], b[nd:])
+    u = O.bwdtrans(O.TET, nm, nq0, E, sg * xg[mp] + dloc)
+    l2, linf = cf.tet_errors(u, exact, jac, *P["weights"])
+    assert abs(l2 - 8.458e-05) <= 1e-8, l2
+    assert abs(linf - 0.000862348) <= 1e-9, linf

@@ -6,7 +6,8 @@ by nektar_amd.contfield, the Dirichlet data projected as ContField does,
 rhs -IProductWRTBase(f) from the GPU operator, the lifted CG solve on the
 GPU with each of the reference's iterative solver choices, BwdTrans on the
 GPU.  The metrics must match the .tst's values within its tolerance
-(L2 1.11818e-07, Linf 5.66374e-07, 1e-12)."""
+(L2 1.11818e-07, Linf 5.66374e-07, 1e-12); likewise Helmholtz3D_CG_Tet.tst
+(48 tetrahedra, NUMMODES 5)."""
 import os
 
 import pytest
@@ -16,6 +17,7 @@ torch = pytest.importorskip("torch")
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HEX = os.path.join(HERE, "golden", "helmholtz3d", "Helmholtz3D_Hex_P6.xml")
+TET = os.path.join(HERE, "golden", "helmholtz3d", "Helmholtz3D_Tet.xml")
 
 
 @pytest.fixture(scope="module")
@@ -35,6 +37,19 @@ def test_helmholtz3d_cg_hex(nmf, solver):
     assert info.converged
     assert abs(l2 - 1.11818e-07) <= 1e-12, l2
     assert abs(linf - 5.66374e-07) <= 1e-12, linf
+
+
+@pytest.mark.parametrize("solver", ["ConjugateGradient", "ConjugateGradientLoc", "IterativeStaticCond"])
+def test_helmholtz3d_cg_tet(nmf, solver):
+    """Helmholtz3D_CG_Tet.tst: 48 tets, NUMMODES 5, the same conditions;
+    L2 8.458e-05 (tolerance 1e-8), Linf 0.000862348 (1e-9)."""
+    from nektar_amd.contfield import helmholtz_solve
+
+    l2, linf, info, _ = helmholtz_solve(TET, solver=solver, tol=1e-14)
+    print(f"Helmholtz3D_CG_Tet ({solver}): L2 {l2:.10e} Linf {linf:.10e}, {info.iterations} iterations", flush=True)
+    assert info.converged
+    assert abs(l2 - 8.458e-05) <= 1e-8, l2
+    assert abs(linf - 0.000862348) <= 1e-9, linf
 
 
 def test_repeatable(nmf):
