@@ -253,7 +253,8 @@ enum ConstFactorType {
     eFactorCoeffD11,
     eFactorCoeffD02,
     eFactorCoeffD12,
-    eFactorCoeffD22
+    eFactorCoeffD22,
+    eFactorConst  // PhysInterp1DScaled's scaling factor (Collections/PhysInterp1DScaled.cpp:158-186)
 };
 using FactorMap = std::map<ConstFactorType, double>;
 

@@ -2,7 +2,9 @@
 // interface (include/nektar_amd/MatrixFreeOps.hpp), checked against the
 // oracle.  Exercises factory keys, SetJac/SetDF, UpdateFactors (lambda and
 // a constant diffusion tensor), Helmholtz and PhysDeriv on host arrays and
-// the error path; then the MultiRegions mirror (include/nektar_amd/
+// the error path; anisotropic BasisKeys; the Collections::Operator mirror
+// (include/nektar_amd/Collections.hpp: OperatorKey factory, the 5-argument
+// and single-direction forms); then the MultiRegions mirror (include/nektar_amd/
 // GlobalLinSys.hpp): AssemblyMapCG::Assemble and the three solver choices
 // against the oracle's CG on a 3^3 box.  Prints "OK <max rel err>" and
 // exits 0 on success.
@@ -11,6 +13,7 @@
 #include <random>
 #include <vector>
 
+#include "nektar_amd/Collections.hpp"
 #include "nektar_amd/GlobalLinSys.hpp"
 #include "nektar_amd/MatrixFreeOps.hpp"
 
@@ -29,6 +32,8 @@ void or_helmholtz(int shape, int nm, int nq0, int nelmt, int deformed, const dou
                   double lambda, const double *diff6, const double *in, double *out);
 void or_physderiv(int shape, int nm, int nq0, int nelmt, int deformed, const double *df, const double *in,
                   double *out0, double *out1, double *out2);
+void or_iproduct_deriv(int shape, int nm, int nq0, int nelmt, int deformed, const double *jac, const double *df,
+                       const double *in0, const double *in1, const double *in2, double *out);
 void or_helmholtz_aniso(int shape, const int *nm, const int *nq, int nelmt, int deformed, const double *jac,
                         const double *df, double lambda, const double *diff6, const double *in, double *out);
 }
@@ -136,6 +141,67 @@ int main()
         std::vector<double> ar(ay.size());
         or_helmholtz_aniso(0, am, aq, ne, 0, jac.data(), df.data(), 0.8, nullptr, ax.data(), ar.data());
         err = std::fmax(err, relerr(ay, ar));
+    }
+
+    // Collections::Operator: OperatorKey factory, the 5-argument and the
+    // single-direction forms, UpdateFactors, GetWspSize, unsupported keys
+    {
+        Collections::CollectionExpansion ce;
+        ce.shape = LibUtilities::eHexahedron;
+        ce.basis = basis;
+        ce.nElmt = ne;
+        ce.jac = jac;
+        ce.df = df;
+        auto &CF = Collections::GetOperatorFactory();
+        auto ch = CF.CreateInstance(
+            Collections::OperatorKey(LibUtilities::eHexahedron, Collections::eHelmholtz, Collections::eMatrixFree,
+                                     false),
+            ce, Collections::FactorMap{{Collections::eFactorLambda, 0.6}});
+        if (ch->GetNumElmt() != (unsigned)ne || ch->GetWspSize() != 0 || ch->GetCoeffSize() != ch->GetInputSize())
+            return 6;
+        std::vector<double> cin(ch->GetInputSize()), c0, c1, c2, wsp, cref(ch->GetOutputSize());
+        for (auto &v : cin) v = U(rng);
+        (*ch)(cin, c0, c1, c2, wsp);
+        or_helmholtz(0, nm, nq, ne, 0, jac.data(), df.data(), 0.6, nullptr, cin.data(), cref.data());
+        err = std::fmax(err, relerr(c0, cref));
+        auto cp = CF.CreateInstance(Collections::OperatorKey(LibUtilities::eHexahedron, Collections::ePhysDeriv,
+                                                             Collections::eMatrixFree, false),
+                                    ce);
+        if (cp->GetWspSize() != 2 * cp->GetPhysSize()) return 7;
+        std::vector<double> pu(cp->GetInputSize()), p0, p1, p2, pdir;
+        for (auto &v : pu) v = U(rng);
+        (*cp)(pu, p0, p1, p2, wsp);
+        std::vector<double> q0(pu.size()), q1(pu.size()), q2(pu.size());
+        or_physderiv(0, nm, nq, ne, 0, df.data(), pu.data(), q0.data(), q1.data(), q2.data());
+        err = std::fmax(err, std::fmax(relerr(p0, q0), std::fmax(relerr(p1, q1), relerr(p2, q2))));
+        (*cp)(1, pu, pdir, wsp);
+        if (pdir != p1) return 8;  // the same derivative, bit for bit
+        // IProductWRTDerivBase: inputs in entries 0-2, output in entry 3
+        auto cd = CF.CreateInstance(Collections::OperatorKey(LibUtilities::eHexahedron,
+                                                             Collections::eIProductWRTDerivBase,
+                                                             Collections::eMatrixFree, false),
+                                    ce);
+        std::vector<double> e1 = q1, e2 = q2, e3;
+        (*cd)(q0, e1, e2, e3, wsp);
+        std::vector<double> dref(cd->GetOutputSize());
+        or_iproduct_deriv(0, nm, nq, ne, 0, jac.data(), df.data(), q0.data(), q1.data(), q2.data(), dref.data());
+        err = std::fmax(err, relerr(e3, dref));
+        bool threw = false;
+        try {
+            CF.CreateInstance(Collections::OperatorKey(LibUtilities::eHexahedron, Collections::eHelmholtz,
+                                                       Collections::eSumFac, false),
+                              ce);
+        } catch (const NekError &e) {
+            threw = e.code == NMF_ERR_UNSUPPORTED;
+        }
+        if (!threw) return 9;
+        threw = false;
+        try {
+            (*ch)(0, cin, c0, wsp);
+        } catch (const NekError &) {
+            threw = true;
+        }
+        if (!threw) return 10;
     }
 
     // MultiRegions mirror: the structured 3^3 box, P=4, its C0 map renumbered
