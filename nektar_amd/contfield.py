@@ -137,12 +137,14 @@ def _project_1d(B, w, f, v0, v1):
     return c
 
 
-def _project_quad(B, w, F):
+def _project_quad(B, w, F, Jf=None):
     """Modified_A x Modified_A coefficients C[a, b] of F[i, j] (points of
     direction a fastest along i) by the constrained projection of a
     quadrilateral (QuadExp.cpp:67-160): corners from the corner values,
     each edge by the 1-D projection of its values, the interior modes
-    L2-projected with the boundary ones held."""
+    L2-projected with the boundary ones held, in the inner product of the
+    face's quadrature with its area element Jf[i, j] (constant, and then
+    cancelling, on parallelogram faces)."""
     nm = B.shape[0]
     C = np.zeros((nm, nm))
     C[0, 0], C[1, 0], C[0, 1], C[1, 1] = F[0, 0], F[-1, 0], F[0, -1], F[-1, -1]
@@ -152,12 +154,25 @@ def _project_quad(B, w, F):
     C[1, :] = _project_1d(B, w, F[-1, :], F[-1, 0], F[-1, -1])
     if nm > 2:
         B2 = np.einsum("ai,bj->abij", B, B).reshape(nm * nm, -1)
-        W2 = np.outer(w, w).reshape(-1)
+        W2 = np.outer(w, w).reshape(-1) * (1.0 if Jf is None else np.asarray(Jf).reshape(-1))
         M = (B2 * W2) @ B2.T
         rhs = (B2 * W2) @ F.reshape(-1) - M @ C.reshape(-1)
         inner = np.array([a * nm + b for a in range(2, nm) for b in range(2, nm)])
         C.reshape(-1)[inner] = np.linalg.solve(M[np.ix_(inner, inner)], rhs[inner])
     return C
+
+
+def _hex_map(v, xi):
+    """Trilinear map of one hexahedron (vertices v [8, 3], local order) at
+    reference points xi [n, 3]: x [n, 3] and dx/dxi [n, 3, 3]."""
+    s = nio.HEX_REF
+    f = 1.0 + xi[:, None, :] * s[None, :, :]
+    N = np.prod(f, axis=2) / 8.0
+    dN = np.empty(f.shape)
+    for d in range(3):
+        o = [c for c in range(3) if c != d]
+        dN[..., d] = s[None, :, d] * f[..., o[0]] * f[..., o[1]] / 8.0
+    return N @ v, np.einsum("nvd,vk->nkd", dN, v)
 
 
 def _hex_face_frames(lv_e):
@@ -206,17 +221,18 @@ def hex_dirichlet_local(session: Session, var: str, lv, verts, nm: int, nq: int,
                             raise ValueError(f"boundary face {fid} belongs to {len(own)} elements")
                         e, (d, s) = own[0]
                         a, b = [k for k in range(3) if k != d]
-                        # face points (xi_a = z_i, xi_b = z_j, xi_d = s) in space
+                        # face points (xi_a = z_i, xi_b = z_j, xi_d = s) of the
+                        # trilinear map, and the face's area element
                         xi = np.zeros((nq, nq, 3))
                         xi[:, :, a] = z[:, None]
                         xi[:, :, b] = z[None, :]
                         xi[:, :, d] = s
-                        A = np.stack([verts[e, 1] - verts[e, 0], verts[e, 3] - verts[e, 0],
-                                      verts[e, 4] - verts[e, 0]], axis=1) / 2
-                        X = verts[e, 0] + (xi + 1.0) @ A.T
+                        X, dX = _hex_map(verts[e], xi.reshape(-1, 3))
+                        X = X.reshape(nq, nq, 3)
+                        Jf = np.linalg.norm(np.cross(dX[:, :, a], dX[:, :, b]), axis=1).reshape(nq, nq)
                         F = evaluate(bc.value, dict(session.params, x=X[..., 0], y=X[..., 1], z=X[..., 2], t=0.0))
                         F = np.broadcast_to(F, X.shape[:2]).astype(float)
-                        C = _project_quad(B, w, F)
+                        C = _project_quad(B, w, F, Jf)
                         md = 0 if s < 0 else 1
                         for pa in range(nm):
                             for pb in range(nm):
@@ -234,21 +250,20 @@ def hex_dirichlet_local(session: Session, var: str, lv, verts, nm: int, nq: int,
 
 
 def hex_quadrature_points(verts, z):
-    """Physical coordinates [E, Q^3, 3] of the GLL points z of
-    parallelepiped hexahedra verts [E, 8, 3]: x = v0 + A (xi + 1), point
-    (i, j, k) at index (k Q + j) Q + i."""
-    xi = np.stack(np.meshgrid(z, z, z, indexing="ij"), axis=-1).transpose(2, 1, 0, 3).reshape(-1, 3)
-    A = np.stack([verts[:, 1] - verts[:, 0], verts[:, 3] - verts[:, 0], verts[:, 4] - verts[:, 0]], axis=2) / 2
-    return verts[:, None, 0, :] + np.einsum("ekd,qd->eqk", A, xi + 1.0)
+    """Physical coordinates [E, Q^3, 3] of the GLL points z of (trilinear)
+    hexahedra verts [E, 8, 3], point (i, j, k) at index (k Q + j) Q + i."""
+    return nio.hex_map_points(verts, z)[0]
 
 
 def errors(u, exact, jac, w):
     """ExpList::L2 and Linf (ExpList.cpp:3836-3863) of u - exact at the
-    quadrature points of hexahedra with one Jacobian each: sqrt(sum_e
-    sum_q W_q J_e (u - exact)^2) and max |u - exact|."""
+    quadrature points of hexahedra: sqrt(sum_e sum_q W_q J (u - exact)^2)
+    and max |u - exact|, J per element or per point."""
     wq = np.einsum("k,j,i->kji", w, w, w).reshape(-1)
-    err = np.asarray(u).reshape(len(jac), -1) - np.asarray(exact).reshape(len(jac), -1)
-    return float(np.sqrt(np.sum(wq[None, :] * np.asarray(jac)[:, None] * err * err))), float(np.max(np.abs(err)))
+    E = np.asarray(u).size // wq.size
+    err = np.asarray(u).reshape(E, -1) - np.asarray(exact).reshape(E, -1)
+    J = np.asarray(jac).reshape(E, -1)
+    return float(np.sqrt(np.sum(wq[None, :] * J * err * err))), float(np.max(np.abs(err)))
 
 
 def dirichlet_problem(session: Session, var: str = "u"):
@@ -256,8 +271,9 @@ def dirichlet_problem(session: Session, var: str = "u"):
     vertices and coordinates, geometric factors, NUMMODES and direction-0
     points (NUMMODES + 1, the MODIFIED default), the C0 map with the
     Dirichlet DOFs first, the element-local Dirichlet values and the
-    physical coordinates of the quadrature points.  Meshes of
-    parallelepiped hexahedra or of tetrahedra, Dirichlet conditions."""
+    physical coordinates of the quadrature points.  Meshes of hexahedra
+    (per-point factors when any is not a parallelepiped) or of tetrahedra,
+    Dirichlet conditions."""
     from . import SHAPE_HEX, SHAPE_TET, basis_tables
 
     s = session
@@ -272,9 +288,10 @@ def dirichlet_problem(session: Session, var: str = "u"):
     faces = [f for rid in regions for cid in s.regions[rid] for _, ids in s.mesh.composites[cid] for f in ids]
     if kinds == {"H"}:
         eids, lv, verts = nio.hex_elements(s.mesh)
-        jac, df = nio.hex_geometry(verts)
-        mp, sg, ng, nd = nio.hex_c0_map(lv, nm, [_face_vertices(s.mesh, f) for f in faces])
         t = basis_tables(SHAPE_HEX, 0, nm, nq0)
+        deformed = not bool(np.all(nio.hex_is_parallelepiped(verts)))
+        jac, df = nio.hex_geometry_points(verts, t["z"]) if deformed else nio.hex_geometry(verts)
+        mp, sg, ng, nd = nio.hex_c0_map(lv, nm, [_face_vertices(s.mesh, f) for f in faces])
         tables = (t["bdata"].reshape(nm, nq0), t["z"], t["w"])
         dloc = hex_dirichlet_local(s, var, lv, verts, nm, nq0, mp, sg, nd, tables)
         X = hex_quadrature_points(verts, t["z"])
@@ -289,34 +306,38 @@ def dirichlet_problem(session: Session, var: str = "u"):
         X = tet_quadrature_points(verts, t[0]["z"], t[1]["z"], t[2]["z"])
         weights = (t[0]["w"], t[1]["w"], t[2]["w"])
         tag = "Tet"
+        deformed = False
     else:
         raise NotImplementedError("Dirichlet solves take meshes of hexahedra or of tetrahedra")
-    return dict(tag=tag, eids=eids, lv=lv, verts=verts, jac=jac, df=df, nm=nm, nq0=nq0, map=mp, sign=sg,
+    return dict(tag=tag, deformed=deformed, eids=eids, lv=lv, verts=verts, jac=jac, df=df, nm=nm, nq0=nq0,
+                map=mp, sign=sg,
                 nglobal=ng, ndir=nd, dloc=dloc, X=X, weights=weights)
 
 
 def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit: int = 20000,
                     solver: str = "ConjugateGradient"):
     """ContField::HelmSolve of the session's Forcing with its Dirichlet
-    conditions and parameter Lambda (the Helmholtz3D demo): returns
+    conditions and parameter Lambda (the Helmholtz3D demo; session_path may
+    also be a Session already read): returns
     (L2 error, Linf error, SolveInfo, element-local solution) against the
     session's ExactSolution at the quadrature points.  Meshes of
-    parallelepiped hexahedra or of tetrahedra, NUMMODES per the expansion,
+    (trilinear) hexahedra or of tetrahedra, NUMMODES per the expansion,
     the MODIFIED default points; GPU operators and CG."""
     import torch
 
     from . import AssemblyMap, FP64, GetOperatorFactory, LinearSystem
 
-    s = read_session(session_path)
+    s = session_path if isinstance(session_path, Session) else read_session(session_path)
     P = dirichlet_problem(s, var)
     tag, nm, nq0, jac, df, X = P["tag"], P["nm"], P["nq0"], P["jac"], P["df"], P["X"]
     E = len(P["eids"])
     lam = s.params.get("Lambda", 0.0)
     F = GetOperatorFactory()
+    geo = "Deformed" if P["deformed"] else "Regular"
     bwd = F.CreateInstance(f"BwdTrans_{tag}_Regular", nm, nq0, E, FP64)
-    ipr = F.CreateInstance(f"IProduct_{tag}_Regular", nm, nq0, E, FP64)
+    ipr = F.CreateInstance(f"IProduct_{tag}_{geo}", nm, nq0, E, FP64)
     ipr.SetJac(jac)
-    helm = F.CreateInstance(f"Helmholtz_{tag}_Regular", nm, nq0, E, FP64)
+    helm = F.CreateInstance(f"Helmholtz_{tag}_{geo}", nm, nq0, E, FP64)
     helm.SetJac(jac)
     helm.SetDF(df)
     helm.SetLambda(lam)

@@ -10,7 +10,8 @@
   element's reference map and so the meaning of its coefficients.
 * ``hex_geometry``: Jacobian and derivative factors of straight-sided
   (parallelepiped) hexahedra, one set per element (GeomFactors.cpp:150-477
-  for the regular case).
+  for the regular case); ``hex_geometry_points``: the same at every
+  quadrature point of trilinear hexahedra (the deformed case).
 * ``tet_local_vertices`` / ``tet_geometry``: the same for tetrahedra
   (TetGeom.cpp:113-300).
 * ``hex_c0_map``: the C0 local-to-global map with orientation signs of a
@@ -284,11 +285,53 @@ def hex_geometry(verts: np.ndarray, tol: float = 1e-12):
     pred = verts[:, :1, :] + np.einsum("eki,vi->evk", A, HEX_REF + 1)
     scale = np.max(np.abs(verts)) + 1.0
     if np.max(np.abs(pred - verts)) > tol * scale:
-        raise NotImplementedError("curved or non-parallelepiped hexahedra need per-point factors")
+        raise NotImplementedError("non-parallelepiped hexahedra need per-point factors (hex_geometry_points)")
     det = np.linalg.det(A)
     inv = np.linalg.inv(A)  # inv[e, k, j] = dxi_k / dx_j
     df = np.transpose(inv, (0, 2, 1)).reshape(-1, 9)  # df[j*3 + k]
     return np.abs(det), np.ascontiguousarray(df.reshape(-1))
+
+
+def hex_is_parallelepiped(verts: np.ndarray, tol: float = 1e-12) -> np.ndarray:
+    """Per element: the trilinear map has no xi-product terms, the regular
+    geometry type of HexGeom::v_CalcGeomType (HexGeom.cpp:83-153)."""
+    verts = np.asarray(verts, dtype=float)
+    A = np.stack([verts[:, 1] - verts[:, 0], verts[:, 3] - verts[:, 0], verts[:, 4] - verts[:, 0]], axis=2) / 2
+    pred = verts[:, :1, :] + np.einsum("eki,vi->evk", A, HEX_REF + 1)
+    scale = np.max(np.abs(verts)) + 1.0 if verts.size else 1.0
+    return np.max(np.abs(pred - verts), axis=(1, 2)) <= tol * scale
+
+
+def hex_map_points(verts: np.ndarray, z: np.ndarray):
+    """The trilinear map of hexahedra verts [E, 8, 3] at the tensor points
+    (z_i, z_j, z_k), index (k Q + j) Q + i: coordinates x [E, Q^3, 3] and
+    derivatives dx[E, Q^3, 3 (x_k), 3 (xi_d)]."""
+    verts = np.asarray(verts, dtype=float)
+    z = np.asarray(z, dtype=float)
+    K, J, I = np.meshgrid(z, z, z, indexing="ij")
+    xi = np.stack([I.reshape(-1), J.reshape(-1), K.reshape(-1)], axis=1)  # [Q^3, 3]
+    s = HEX_REF  # [8, 3] vertex signs
+    f = 1.0 + xi[:, None, :] * s[None, :, :]  # [Q^3, 8, 3]
+    N = np.prod(f, axis=2) / 8.0  # shape functions [Q^3, 8]
+    dN = np.empty(f.shape)
+    for d in range(3):
+        others = [c for c in range(3) if c != d]
+        dN[..., d] = s[None, :, d] * f[..., others[0]] * f[..., others[1]] / 8.0
+    x = np.einsum("qv,evk->eqk", N, verts)
+    dx = np.einsum("qvd,evk->eqkd", dN, verts)
+    return x, dx
+
+
+def hex_geometry_points(verts: np.ndarray, z: np.ndarray):
+    """(jac, df) at every quadrature point of trilinear hexahedra (the
+    deformed factors of GeomFactors.cpp:150-477): jac[e*Q^3 + q] =
+    |det(dx/dxi)|, df[(e*Q^3 + q)*9 + j*3 + k] = dxi_k/dx_j."""
+    _, dx = hex_map_points(verts, z)
+    E, nq3 = dx.shape[:2]
+    A = dx.reshape(-1, 3, 3)  # A[k, d] = dx_k / dxi_d
+    inv = np.linalg.inv(A)  # inv[d, k] = dxi_d / dx_k
+    df = np.transpose(inv, (0, 2, 1)).reshape(-1, 9)  # df[k*3 + d]
+    return np.abs(np.linalg.det(A)), np.ascontiguousarray(df.reshape(-1))
 
 
 def hex_c0_map(local_vertices: np.ndarray, nm: int, dirichlet_vertex_sets=()):

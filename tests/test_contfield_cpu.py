@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
+import geometry_cases as G
 from nektar_amd import contfield as cf
 from nektar_amd import io as nio
 
@@ -123,3 +124,80 @@ def test_helmholtz3d_cg_tet_metrics_with_the_oracle():
     l2, linf = cf.tet_errors(u, exact, jac, *P["weights"])
     assert abs(l2 - 8.458e-05) <= 1e-8, l2
     assert abs(linf - 0.000862348) <= 1e-9, linf
+
+
+def perturbed_hex_session(seed=3, amp=0.06):
+    """The Helmholtz3D_CG_Hex session with every vertex coordinate that is
+    not on the box boundary moved by U[-amp, amp]: trilinear elements whose
+    boundary faces stay in the box planes but are no longer parallelograms."""
+    s = cf.read_session(HEX)
+    rng = np.random.default_rng(seed)
+    for v in sorted(s.mesh.vertices):
+        x = s.mesh.vertices[v].copy()
+        inner = (x > 1e-12) & (x < 1 - 1e-12)
+        x[inner] += rng.uniform(-amp, amp, int(inner.sum()))
+        s.mesh.vertices[v] = x
+    return s
+
+
+def dense_solve(s, P):
+    """The lifted C0 solve with the oracle's operators and a dense solve;
+    returns the element-local solution and the L2 / Linf errors."""
+    nm, nq, E, jac, df, X = P["nm"], P["nq0"], len(P["eids"]), P["jac"], P["df"], P["X"]
+    mp, sg, ng, nd, dloc, deformed = P["map"], P["sign"], P["nglobal"], P["ndir"], P["dloc"], P["deformed"]
+    nc = nm ** 3
+    names = dict(s.params, x=X[..., 0], y=X[..., 1], z=X[..., 2], t=0.0)
+    f = cf.evaluate(s.functions["Forcing"]["u"], names).reshape(-1)
+    exact = cf.evaluate(s.functions["ExactSolution"]["u"], names)
+    lam = s.params["Lambda"]
+    rl = -O.iproduct(O.HEX, nm, nq, E, deformed, jac, f) - O.helmholtz(O.HEX, nm, nq, E, deformed, jac, df, lam,
+                                                                        dloc)
+    A = np.zeros((ng, ng))
+    for m in range(nc):
+        x = np.zeros(E * nc)
+        x[m::nc] = 1.0
+        y = O.helmholtz(O.HEX, nm, nq, E, deformed, jac, df, lam, x)
+        for e in range(E):
+            i = e * nc + np.arange(nc)
+            np.add.at(A, (mp[i], np.full(nc, mp[e * nc + m])), sg[i] * y[i] * sg[e * nc + m])
+    b = np.zeros(ng)
+    np.add.at(b, mp, sg * rl)
+    xg = np.zeros(ng)
+    xg[nd:] = np.linalg.solve(A[nd:, nd:], b[nd:])
+    xl = sg * xg[mp] + dloc
+    u = O.bwdtrans(O.HEX, nm, nq, E, xl)
+    return xl, cf.errors(u, exact, jac, P["weights"][0])
+
+
+def test_trilinear_hexahedra_from_files():
+    """Per-point factors of trilinear hexahedra equal the oracle's
+    (GeomFactors' deformed case), the boundary projection on faces that are
+    not parallelograms is orthogonal in the face's area-weighted inner
+    product, and the lifted solve converges to the exact solution."""
+    import nektar_amd as nmf
+
+    s = perturbed_hex_session()
+    P = cf.dirichlet_problem(s)
+    assert P["deformed"]
+    nm, nq, verts = P["nm"], P["nq0"], P["verts"]
+    jo, dfo = O.hex_geom(np.ascontiguousarray(verts.reshape(-1)), len(verts), nq, True)
+    assert G.rel_err(P["jac"], jo) < 1e-14 and G.rel_err(P["df"], dfo) < 1e-14
+    # one boundary face: xi_2 = -1 of element 0 (z = 0 plane, a general quadrilateral)
+    t = nmf.basis_tables(nmf.SHAPE_HEX, 0, nm, nq)
+    B, z, w = t["bdata"].reshape(nm, nq), t["z"], t["w"]
+    xi = np.zeros((nq, nq, 3))
+    xi[:, :, 0], xi[:, :, 1], xi[:, :, 2] = z[:, None], z[None, :], -1.0
+    X, dX = cf._hex_map(verts[0], xi.reshape(-1, 3))
+    Jf = np.linalg.norm(np.cross(dX[:, :, 0], dX[:, :, 1]), axis=1).reshape(nq, nq)
+    assert np.ptp(Jf) > 1e-3 * np.max(Jf)  # not a parallelogram
+    F = np.sin(3 * X[:, 0]) * np.cos(2 * X[:, 1]).reshape(-1)
+    F = F.reshape(nq, nq)
+    C = cf._project_quad(B, w, F, Jf)
+    g = B.T @ C @ B
+    for a in range(2, nm):
+        for b in range(2, nm):
+            phi = np.outer(B[a], B[b])
+            assert abs(np.sum(np.outer(w, w) * Jf * (F - g) * phi)) < 1e-14 * np.sum(np.outer(w, w) * Jf)
+    # the solve
+    _, (l2, linf) = dense_solve(s, P)
+    assert l2 < 5e-7 and linf < 2e-6, (l2, linf)  # 1.3e-07 and 6.1e-07

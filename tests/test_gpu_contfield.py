@@ -52,6 +52,24 @@ def test_helmholtz3d_cg_tet(nmf, solver):
     assert abs(linf - 0.000862348) <= 1e-9, linf
 
 
+def test_trilinear_hexahedra(nmf):
+    """The same session with its inner vertex coordinates perturbed
+    (trilinear elements, per-point geometry, boundary faces that are not
+    parallelograms): the GPU solve equals the oracle's dense solve."""
+    import numpy as np
+
+    import test_contfield_cpu as T
+    from nektar_amd.contfield import dirichlet_problem, helmholtz_solve
+
+    s = T.perturbed_hex_session()
+    xd, (l2d, linfd) = T.dense_solve(s, dirichlet_problem(s))
+    l2, linf, info, xl = helmholtz_solve(s, tol=1e-14)
+    print(f"trilinear hexes: L2 {l2:.10e} (dense {l2d:.10e}), Linf {linf:.10e} (dense {linfd:.10e})", flush=True)
+    assert info.converged
+    assert abs(l2 - l2d) <= 1e-9 * l2d and abs(linf - linfd) <= 1e-9 * linfd
+    assert np.max(np.abs(xl - xd)) <= 1e-10 * np.max(np.abs(xd))
+
+
 def test_repeatable(nmf):
     """Two solves of the same session give the same solution bit for bit."""
     import numpy as np
