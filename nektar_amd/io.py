@@ -614,10 +614,18 @@ def c0_helmholtz_smoothing(session_path: str, fld_path: str, length: float, tol:
 
     mesh = read_mesh_xml(session_path)
     kinds = {k for k, _ in mesh.elements.values()}
+    geo = "Regular"
     if kinds == {"H"}:
         eids, lv, verts = hex_elements(mesh)
         tag, shape_name = "Hex", "Hexahedron"
-        jac, df = hex_geometry(verts)
+        if np.all(hex_is_parallelepiped(verts)):
+            jac, df = hex_geometry(verts)
+        else:  # trilinear elements: factors at every quadrature point
+            from . import SHAPE_HEX, basis_tables
+
+            nq_ = numpoints or mesh.expansions[0]["nummodes"] + 1
+            jac, df = hex_geometry_points(verts, basis_tables(SHAPE_HEX, 0, mesh.expansions[0]["nummodes"], nq_)["z"])
+            geo = "Deformed"
     elif kinds == {"A"}:
         eids, lv, verts = tet_elements(mesh)
         tag, shape_name = "Tet", "Tetrahedron"
@@ -635,9 +643,9 @@ def c0_helmholtz_smoothing(session_path: str, fld_path: str, length: float, tol:
     lam = (2.0 * np.pi / length) ** 2
     F = GetOperatorFactory()
     bwd = F.CreateInstance(f"BwdTrans_{tag}_Regular", nm, nq, E, FP64)
-    ipr = F.CreateInstance(f"IProduct_{tag}_Regular", nm, nq, E, FP64)
+    ipr = F.CreateInstance(f"IProduct_{tag}_{geo}", nm, nq, E, FP64)
     ipr.SetJac(jac)
-    helm = F.CreateInstance(f"Helmholtz_{tag}_Regular", nm, nq, E, FP64)
+    helm = F.CreateInstance(f"Helmholtz_{tag}_{geo}", nm, nq, E, FP64)
     helm.SetJac(jac)
     helm.SetDF(df)
     helm.SetLambda(lam)
