@@ -124,3 +124,38 @@ def test_default_sizes_match_isotropic_entry_points():
         a = O.helmholtz(shape, nm, nq0, NE, True, jac, df, 1.1, x)
         b = O.helmholtz_aniso(shape, (nm, nm, nm), nq, NE, True, jac, df, 1.1, x)
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shape_case", ["tet", "hex"])
+def test_planted_per_point_index_mistake_fails(shape_case):
+    """The dense pin is sensitive to the per-point geometry index: the
+    oracle given factors with two quadrature points of one element swapped
+    (what a point-index mistake inside it would read) no longer matches the
+    dense evaluation built from the true factors."""
+    if shape_case == "tet":
+        nm, nq = (4, 4, 4), (5, 4, 4)
+        vals, grads, W = B.tet_dense_own_points(nm, nq)
+        shape = O.TET
+    else:
+        nm, nq = (3, 4, 5), (4, 5, 6)
+        Phi, Dd, W = B.hex_dense_aniso(nm, nq)
+        shape = O.HEX
+    nqt = int(np.prod(nq))
+    nc = O.ncoeffs_aniso(shape, nm)
+    jac, df = _geom(nqt, 31)
+    x = G.coeffs(NE * nc, 8)
+    good = O.helmholtz_aniso(shape, nm, nq, NE, True, jac, df, 1.9, x)
+    for planted in ("jac", "df"):
+        j2, d2 = jac.copy(), df.copy()
+        e, q0, q1 = 1, 2, nqt - 3
+        if planted == "jac":
+            j2[e * nqt + q0], j2[e * nqt + q1] = jac[e * nqt + q1], jac[e * nqt + q0]
+        else:
+            a, b = (e * nqt + q0) * 9, (e * nqt + q1) * 9
+            d2[a:a + 9], d2[b:b + 9] = df[b:b + 9], df[a:a + 9]
+        bad = O.helmholtz_aniso(shape, nm, nq, NE, True, j2, d2, 1.9, x)
+        assert G.rel_err(bad, good) > 1e-6, planted
+    if shape_case == "tet":
+        hb = np.concatenate([B.dense_helmholtz(vals, grads, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
+                                               1.9) @ _elem(x, e, nc) for e in range(NE)])
+        assert G.rel_err(good, hb) < TOL
