@@ -133,8 +133,7 @@ def bench_ops(N, torch, steps=20, warmup=3):
                     op.SetLambda(1.0)
                 if key == "LinearAdvectionDiffusionReaction":
                     op.SetAdvectionVelocities(vel)
-                if key == "Helmholtz":
-                    op.autotune()
+                tile = op.autotune()[0] if op.tile_candidates() else 0
                 dt = torch.float64 if prec == N.FP64 else torch.float32
                 x = torch.rand(op.input_size(), dtype=dt, device="cuda") * 2 - 1
                 y = torch.zeros(op.output_size(), dtype=dt, device="cuda")
@@ -151,7 +150,7 @@ def bench_ops(N, torch, steps=20, warmup=3):
                 else:
                     byts, flops = E * (nqt + nc + 1) * sv, None
                 r = {"us": round(us, 2), "alg_gbs": round(byts / us / 1e3, 1),
-                     "hbm_frac": round(byts / us / 1e3 / 8000.0, 3)}
+                     "hbm_frac": round(byts / us / 1e3 / 8000.0, 3), "tile": tile}
                 if flops:
                     r["tflops"] = round(flops / us / 1e6, 2)
                     r["fp_frac"] = round(flops / us / 1e6 / (PEAK_FP64_TFLOPS if sv == 8 else 157.3), 3)

@@ -145,7 +145,7 @@ void Batch::build()
         groups_[it->second].members.push_back(i);
     }
     for (auto &g : groups_) {
-        const int epb = g.shape == Shape::Hex ? hex_tile_elems(g.nq, g.tile) : tet_block_elems(g.nq);
+        const int epb = g.shape == Shape::Hex ? hex_tile_elems(g.nq, g.tile) : tet_tile_elems(g.nq, g.tile);
         int block0 = 0;
         std::vector<GroupItem<double>> i64;
         std::vector<GroupItem<float>> i32;
@@ -254,11 +254,13 @@ void Batch::run(hipStream_t s)
 
 void Batch::autotune(hipStream_t s)
 {
-    std::map<std::tuple<int, int, int, int>, std::vector<MFOperator *>> kinds;
+    // one choice per kind (shape, order, points, precision, geometry),
+    // timed on the kind's largest operator
+    std::map<std::tuple<int, int, int, int, int>, std::vector<MFOperator *>> kinds;
     for (const auto &e : entries_) {
         MFOperator *o = e.op;
-        if (o->shape != Shape::Hex || o->op != OpType::Helmholtz || o->nelmt == 0) continue;
-        auto &v = kinds[std::make_tuple(o->nm, o->nq[0], (int)o->fp32, (int)o->deformed)];
+        if (o->op != OpType::Helmholtz || o->nelmt == 0 || o->tile_candidates().empty()) continue;
+        auto &v = kinds[std::make_tuple((int)o->shape, o->nm, o->nq[0], (int)o->fp32, (int)o->deformed)];
         if (std::find(v.begin(), v.end(), o) == v.end()) v.push_back(o);
     }
     for (auto &kv : kinds) {

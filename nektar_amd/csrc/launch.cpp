@@ -52,6 +52,16 @@ int hex_tile_candidates(int nm, int nq, bool fp32, int *out, int max)
     }
 }
 
+int tet_tile_candidates(int nq0, int *out, int max)
+{
+    const int cand[2] = {tet_default_tpe(nq0), tet_default_tpe(nq0) == tet_packed_tpe(nq0) ? tet_wave_tpe(nq0)
+                                                                                          : tet_packed_tpe(nq0)};
+    int n = 0;
+    for (int c = 0; c < 2 && n < max; ++c)
+        if (c == 0 || cand[1] != cand[0]) out[n++] = cand[c];
+    return n;
+}
+
 hipError_t launch_hex_grouped(int nm, int nq, const GroupArgs &a)
 {
     switch (nm) {

@@ -81,11 +81,25 @@ struct GroupArgs {
     int tile = 0;       // hex: elements per workgroup (0 = default)
 };
 
-// Elements per workgroup: hex default fill (tile 0) or the given tile;
-// tets have one shape.  The kernels assert they use the same numbers.
+// Elements per workgroup: hex default fill (tile 0) or the given tile.
+// A tet tile is the number of work-items per element: packed
+// (nq0*(nq0-1), elements share waves) or wave-aligned (rounded up to a
+// multiple of 64); workgroups hold 256/tile elements (at least one).  The
+// kernels assert they use the same numbers.
 constexpr int hex_fill_elems(int nq) { return 256 / (nq * nq) > 0 ? 256 / (nq * nq) : 1; }
-constexpr int tet_block_elems(int nq0) { return 256 / (nq0 * (nq0 - 1)) > 0 ? 256 / (nq0 * (nq0 - 1)) : 1; }
 inline int hex_tile_elems(int nq, int tile) { return tile > 0 ? tile : hex_fill_elems(nq); }
+constexpr int tet_packed_tpe(int nq0) { return nq0 * (nq0 - 1); }
+constexpr int tet_wave_tpe(int nq0) { return (nq0 * (nq0 - 1) + 63) / 64 * 64; }
+// default: wave-aligned where one element fills most of a wave (Q = 8:
+// 56 of 64 work-items), packed otherwise (measured, FP64 and FP32 apply)
+constexpr int tet_default_tpe(int nq0) { return nq0 == 8 ? tet_wave_tpe(nq0) : tet_packed_tpe(nq0); }
+constexpr int tet_tile_elems(int nq0, int tile)
+{
+    return 256 / (tile > 0 ? tile : tet_default_tpe(nq0)) > 0 ? 256 / (tile > 0 ? tile : tet_default_tpe(nq0)) : 1;
+}
+// tet tile candidates (work-items per element) in the order the autotuner
+// tries them
+int tet_tile_candidates(int nq0, int *out, int max);
 
 hipError_t launch_hex_grouped(int nm, int nq, const GroupArgs &a);
 

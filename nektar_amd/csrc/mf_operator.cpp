@@ -68,6 +68,15 @@ std::vector<double> MFOperator::build_tables(Shape shape, int nm, int nq0)
     for (double z : b1.pts.z) t.push_back(0.5 * (1.0 + z));
     for (double z : b1.pts.z) t.push_back(2.0 / (1.0 - z));
     for (double z : b2.pts.z) t.push_back(2.0 / (1.0 - z));
+    // transposed copies, so that every contraction reads one contiguous row
+    // per output: B0T[i*nm + p] = B0[p*nq0 + i], DdT[i*nqd + l] = Dd[l*nqd + i]
+    for (int i = 0; i < nq0; ++i)
+        for (int p = 0; p < nm; ++p) t.push_back(b0.bdata[p * nq0 + i]);
+    for (auto *b : {&b0, &b1, &b2}) {
+        const int n = (int)b->pts.w.size();
+        for (int i = 0; i < n; ++i)
+            for (int l = 0; l < n; ++l) t.push_back(b->pts.D[l * n + i]);
+    }
     // zero-padded B1 and B2 rows that the kernels stage into LDS: from a
     // multiple of 4, rows of pitch rp = nq1 rounded up to 4, each table
     // followed by nm zero rows (tet::Dims::oB1P, RP, NB1R, NB2R)
@@ -470,7 +479,7 @@ void MFOperator::apply_iproduct_deriv(const void *in0, const void *in1, const vo
         NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
         return;
     }
-    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in0, out, nullptr, nullptr, s, 0, in1, in2};
+    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in0, out, nullptr, nullptr, s, tile, in1, in2};
     hipError_t e = shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);
     NMF_REQUIRE(e == hipSuccess, NMF_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
 }
@@ -529,9 +538,10 @@ TuneKey MFOperator::tune_key() const
 std::vector<int> MFOperator::tile_candidates() const
 {
     std::vector<int> c;
-    if (shape != Shape::Hex || op != OpType::Helmholtz || generic) return c;
-    int buf[8];
-    const int n = hex_tile_candidates(nm, nq[0], fp32, buf, 8);
+    if (generic || op == OpType::PhysInterp) return c;
+    int buf[8], n = 0;
+    if (shape == Shape::Hex && op == OpType::Helmholtz) n = hex_tile_candidates(nm, nq[0], fp32, buf, 8);
+    if (shape == Shape::Tet) n = tet_tile_candidates(nq[0], buf, 8);
     c.assign(buf, buf + n);
     return c;
 }
@@ -554,17 +564,24 @@ int MFOperator::autotune(hipStream_t s, std::vector<double> *times_us)
     if (times_us) times_us->clear();
     if (cand.empty() || nelmt == 0) return 0;
     ensure_geometry();
+    if (op == OpType::AdvDiffReact && !adv.get()) return 0;  // no advection velocities yet
     const size_t es = fp32 ? sizeof(float) : sizeof(double);
-    DeviceBuffer in((size_t)nelmt * ncoeff * es), out((size_t)nelmt * ncoeff * es);
+    // three inputs and three outputs of the larger of the coefficient and
+    // point counts cover every operator's arguments
+    const size_t vals = (size_t)nelmt * std::max(ncoeff, nqtot);
+    DeviceBuffer in(3 * vals * es), out(3 * vals * es);
     // every byte 0x3F: all values ~4.8e-4 (FP64) / ~0.75 (FP32), normal and
     // non-zero, so the timed arithmetic draws the power of real data
-    NMF_HIP(hipMemsetAsync(in.get(), 0x3F, (size_t)nelmt * ncoeff * es, s));
-    KernelArgs a{op, fp32, deformed, nelmt, tab.get(), geo.get(), lambda, in.get(), out.get(), nullptr, nullptr, s, 0};
+    NMF_HIP(hipMemsetAsync(in.get(), 0x3F, 3 * vals * es, s));
+    char *ib = static_cast<char *>(in.get()), *ob = static_cast<char *>(out.get());
+    KernelArgs a{op,       fp32, deformed,          nelmt, tab.get(), geo.get(), lambda, ib, ob, ob + vals * es,
+                 ob + 2 * vals * es, s, 0, ib + vals * es, ib + 2 * vals * es};
+    a.adv = adv.get();
     const int best = time_tiles(
         s, cand,
         [&](int t) {
             a.tile = t;
-            return launch_hex(nm, nq[0], a);
+            return shape == Shape::Hex ? launch_hex(nm, nq[0], a) : launch_tet(nm, nq[0], a);
         },
         10, times_us);
     tile = best;

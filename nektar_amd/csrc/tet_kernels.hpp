@@ -13,8 +13,11 @@
 //
 // Table buffer (T): B0[NM*NQ0] B1[NMB*NQ1] B2[NC*NQ2] D0[NQ0^2] D1[NQ1^2]
 //   D2[NQ2^2] W0[NQ0] W1[NQ1] W2[NQ2] H0[NQ0] H1[NQ1] H2[NQ1] H3[NQ2]
+//   B0T[NQ0*NM] D0T[NQ0^2] D1T[NQ1^2] D2T[NQ2^2]
 //   (W1, W2 carry the 0.5 / 0.25 Radau pre-scaling, Operator.hpp:600-616;
-//    H* are the half-space factors of HelmholtzKernels.hpp:1368-1389.)
+//    H* are the half-space factors of HelmholtzKernels.hpp:1368-1389;
+//    the *T tables are transposed copies: B0T[i*NM + p] = B0[p*NQ0 + i],
+//    DdT[i*NQd + l] = Dd[l*NQd + i].)
 //   then, from a multiple of 4, zero-padded B1 and B2 rows of pitch RP that
 //   each kernel stages into LDS (Dims::oB1P, Dims::LTAB).
 // Geometry: IProduct jac[e] | jac[e*NQT+q]; PhysDeriv df[e*9+d] |
@@ -24,6 +27,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "contract.hpp"
 #include "launch.hpp"
 
 namespace nektar_amd {
@@ -35,13 +39,13 @@ struct Dims {
     static constexpr int NQT = NQ0 * NQ1 * NQ2;
     static constexpr int NMB = NM * (NM + 1) / 2;
     static constexpr int NC = NM * (NM + 1) * (NM + 2) / 6;
-    static constexpr int TPE = NQ0 * NQ1;
-    static constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;
     static constexpr int oB0 = 0, oB1 = oB0 + NM * NQ0, oB2 = oB1 + NMB * NQ1;
     static constexpr int oD0 = oB2 + NC * NQ2, oD1 = oD0 + NQ0 * NQ0, oD2 = oD1 + NQ1 * NQ1;
     static constexpr int oW0 = oD2 + NQ2 * NQ2, oW1 = oW0 + NQ0, oW2 = oW1 + NQ1;
     static constexpr int oH0 = oW2 + NQ2, oH1 = oH0 + NQ0, oH2 = oH1 + NQ1, oH3 = oH2 + NQ1;
-    static constexpr int TABSIZE = oH3 + NQ2;
+    static constexpr int oB0T = oH3 + NQ2, oD0T = oB0T + NQ0 * NM, oD1T = oD0T + NQ0 * NQ0;
+    static constexpr int oD2T = oD1T + NQ1 * NQ1;
+    static constexpr int TABSIZE = oD2T + NQ2 * NQ2;
     // Zero-padded copies of B1 and B2 after the tables above, staged into
     // LDS by every kernel that runs the collapsed contractions: rows of
     // pitch RP (a multiple of 4, so rows start on 16-byte boundaries in
@@ -68,10 +72,36 @@ struct Slab {
     static_assert((Dm::NMB + NM - 1) * Dm::NQ2 <= SIZE, "unpredicated pair reads must stay inside the slab");
 };
 
-template <int NM, int NQ>
-constexpr int block_size()
+// Work-items per element (TPE) of a launch.  Packed: TPE = NQ0*NQ1, so
+// that elements share waves and every stage ends at a workgroup barrier.
+// Wave-aligned: TPE rounded up to a multiple of 64; at TPE = 64 every
+// element is one wave, its stages exchange data through LDS inside the
+// wave only, and the waves of a workgroup never wait for each other (the
+// work-items past NQ0*NQ1 idle in the per-column stages).
+template <int NM, int NQ, int TPE_>
+struct Tile {
+    using Dm = Dims<NM, NQ>;
+    static constexpr int TPE = TPE_;
+    static constexpr int NCOL = Dm::NQ0 * Dm::NQ1;
+    static_assert(TPE >= NCOL, "a tile needs one work-item per (i,j) column");
+    static constexpr int EPB = 256 / TPE > 0 ? 256 / TPE : 1;
+    static constexpr int BLOCK = EPB * TPE;
+    static constexpr bool WAVE = TPE == 64;
+};
+
+// LDS ordering of an element's stages.  Inside one wave the LDS operations
+// complete in issue order, so a wave-aligned element needs no hardware
+// barrier, only a compiler fence that keeps its loads after its stores.
+template <bool WAVE>
+__device__ __forceinline__ void esync()
 {
-    return Dims<NM, NQ>::EPB * Dims<NM, NQ>::TPE;
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
 }
 
 // (p, q) pair index t -> p, q (pairs ordered p-major, q < NM - p)
@@ -110,6 +140,16 @@ template <typename T, int NM, int NQ>
 struct Tab {
     using Dm = Dims<NM, NQ>;
     const T *__restrict__ b;
+    // contiguous table rows through opaque() (contract.hpp): one row per
+    // output of a contraction, loaded one ahead by contract()
+    __device__ __forceinline__ const_ptr<T> B0row(int p) const { return opaque(b + Dm::oB0 + p * Dm::NQ0); }
+    __device__ __forceinline__ const_ptr<T> B0Trow(int i) const { return opaque(b + Dm::oB0T + i * NM); }
+    __device__ __forceinline__ const_ptr<T> D0row(int i) const { return opaque(b + Dm::oD0 + i * Dm::NQ0); }
+    __device__ __forceinline__ const_ptr<T> D1row(int j) const { return opaque(b + Dm::oD1 + j * Dm::NQ1); }
+    __device__ __forceinline__ const_ptr<T> D2row(int k) const { return opaque(b + Dm::oD2 + k * Dm::NQ2); }
+    __device__ __forceinline__ const_ptr<T> D0Trow(int i) const { return opaque(b + Dm::oD0T + i * Dm::NQ0); }
+    __device__ __forceinline__ const_ptr<T> D1Trow(int j) const { return opaque(b + Dm::oD1T + j * Dm::NQ1); }
+    __device__ __forceinline__ const_ptr<T> D2Trow(int k) const { return opaque(b + Dm::oD2T + k * Dm::NQ2); }
     __device__ __forceinline__ T B0(int p, int i) const { return b[Dm::oB0 + p * Dm::NQ0 + i]; }
     __device__ __forceinline__ T B1(int m, int j) const { return b[Dm::oB1 + m * Dm::NQ1 + j]; }
     __device__ __forceinline__ T B2(int m, int k) const { return b[Dm::oB2 + m * Dm::NQ2 + k]; }
@@ -143,6 +183,24 @@ struct LTab {
     const T *l;
     __device__ __forceinline__ const T *B1(int m) const { return l + m * LP; }
     __device__ __forceinline__ const T *B2(int m) const { return l + (Dm::NB1R + m) * LP; }
+};
+
+// Work split of the (p,q) pair stages over TPE work-items.  With room for
+// two work-items per pair, the forward r-contraction gives each a half of
+// the k range (KW values from k = kg*KW, whole 16-byte units inside the
+// staged row) and the transpose k-contraction gives each a half of the r
+// range (RW rows from r = rg*RW); otherwise one work-item per pair does it
+// all.
+template <typename T, int NM, int NQ, int TPE>
+struct PairSplit {
+    using Dm = Dims<NM, NQ>;
+    using L = LTab<T, NM, NQ>;
+    static constexpr int KW2 = ((Dm::NQ2 + 1) / 2 + L::VW - 1) / L::VW * L::VW;
+    static constexpr bool TWO = 2 * Dm::NMB <= TPE;
+    static constexpr int KG = (TWO && 2 * KW2 <= L::LP) ? 2 : 1;
+    static constexpr int KW = KG == 2 ? KW2 : Dm::NQ2;
+    static constexpr int RG = TWO ? 2 : 1;
+    static constexpr int RW = (NM + RG - 1) / RG;
 };
 
 // N consecutive values of a staged row, read as 16-byte vectors (rows are
@@ -190,16 +248,19 @@ __device__ __forceinline__ void stage_tables(const T *__restrict__ tabp, T *lt)
 // The triangular loops run over all NM rows: the coefficients (or partial
 // sums) beyond a pair's length are zero, and the rows they meet are real
 // modes of the next pair or the zero padding, so no lane needs a branch.
-template <typename T, int NM, int NQ>
+template <typename T, int NM, int NQ, int TPE>
 __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
                                               const T *__restrict__ xin, bool valid, int t, T *S1, T *S2, T *S3,
                                               T (&u)[NQ - 1])
 {
     using Dm = Dims<NM, NQ>;
+    using Ti = Tile<NM, NQ, TPE>;
+    using S = PairSplit<T, NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
-    if (t < Dm::NMB) {
+    if (t < Dm::NMB * S::KG) {
+        const int pr = t / S::KG, kg = t % S::KG;
         int p, q;
-        pair_of<NM>(t, p, q);
+        pair_of<NM>(pr, p, q);
         const int base = coeff_base<NM>(p, q);
         const int len = NM - p - q;
         T x[NM];
@@ -212,20 +273,23 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const L
                 x[r] = r < len ? xv : T(0);
             }
         }
-        T s[NQ2];
+        T s[S::KW];
 #pragma unroll
-        for (int k = 0; k < NQ2; ++k) s[k] = T(0);
+        for (int k = 0; k < S::KW; ++k) s[k] = T(0);
 #pragma unroll
         for (int r = 0; r < NM; ++r) {
-            T b[NQ2];
-            load_row(lt.B2(base + r), b);
+            T b[S::KW];
+            load_row(lt.B2(base + r) + kg * S::KW, b);
 #pragma unroll
-            for (int k = 0; k < NQ2; ++k) s[k] = fma(x[r], b[k], s[k]);
+            for (int k = 0; k < S::KW; ++k) s[k] = fma(x[r], b[k], s[k]);
         }
 #pragma unroll
-        for (int k = 0; k < NQ2; ++k) S1[t * NQ2 + k] = s[k];
+        for (int kk = 0; kk < S::KW; ++kk) {
+            const int k = kg * S::KW + kk;
+            if (S::KG == 1 || k < NQ2) S1[pr * NQ2 + k] = s[kk];
+        }
     }
-    __syncthreads();
+    esync<Ti::WAVE>();
     if (t < NM * NQ2) {
         const int p = t % NM, k = t / NM;
         const int pb = modeb<NM>(p, 0);
@@ -250,7 +314,7 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const L
 #pragma unroll
         for (int j = 0; j < NQ1; ++j) S2[(k * NQ1 + j) * NM + p] = s[j];
     }
-    __syncthreads();
+    esync<Ti::WAVE>();
     // singular-mode coefficients (broadcast reads)
     T c_top = T(0), c_bot = T(0), c_edge[NM > 2 ? NM - 2 : 1];
     if (valid) {
@@ -264,31 +328,30 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const L
         T f[NM];
 #pragma unroll
         for (int p = 0; p < NM; ++p) f[p] = S2[(k * NQ1 + j) * NM + p];
-        // correction weight along (i): top uses B0_0 B1_1 + B0_1 (B1_0 + B1_1),
-        // bottom and edge use B0_1 B1_1 (separable in i)
+        // the corrections are separable in i: the top vertex adds
+        // c_top B2_1 (B0_0 B1_1 + B0_1 (B1_0 + B1_1)), the bottom vertex and
+        // the singular edge add B0_1 B1_1 edge(k), so they fold into the
+        // p = 0 and p = 1 partial sums
         const T b10 = lt.B1(0)[j], b11 = lt.B1(1)[j];
         T edge = c_bot * lt.B2(0)[k];
 #pragma unroll
         for (int r = 1; r < NM - 1; ++r) edge = fma(c_edge[r - 1], lt.B2(r + 1)[k], edge);
-        const T b21 = lt.B2(1)[k];
+        const T tb = c_top * lt.B2(1)[k];
+        f[0] = fma(tb, b11, f[0]);
+        f[1] = fma(tb, b10 + b11, fma(edge, b11, f[1]));
+        T o[NQ0];
+        contract<false>(f, [&](int i) { return tab.B0Trow(i); }, o);
 #pragma unroll
-        for (int i = 0; i < NQ0; ++i) {
-            T s = T(0);
-#pragma unroll
-            for (int p = 0; p < NM; ++p) s = fma(f[p], tab.B0(p, i), s);
-            T top = tab.B0(0, i) * b11;
-            top = fma(tab.B0(1, i), b10, top);
-            top = fma(tab.B0(1, i), b11, top);
-            s = fma(top * b21, c_top, s);
-            s = fma(tab.B0(1, i) * b11, edge, s);
-            S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = s;
-        }
+        for (int i = 0; i < NQ0; ++i) S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = o[i];
     }
-    __syncthreads();
-    {
+    esync<Ti::WAVE>();
+    if (t < Ti::NCOL) {
         const int i = t % NQ0, j = t / NQ0;
 #pragma unroll
         for (int k = 0; k < NQ2; ++k) u[k] = S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) u[k] = T(0);
     }
 }
 
@@ -298,33 +361,32 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const L
 // corrections  : TOP(k), BOT(k)          -> S3[k], S3[NQ2 + k]
 // Stage I2 (p,k): sum_j B1 s            -> S1[pair*NQ2 + k]
 // Stage I3 (pairs): sum_k B2            -> out
-template <typename T, int NM, int NQ>
+template <typename T, int NM, int NQ, int TPE>
 __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
                                                   const T (&y)[NQ - 1], bool valid, int t, T *S1, T *S2, T *S3,
                                                   T *__restrict__ xout)
 {
     using Dm = Dims<NM, NQ>;
+    using Ti = Tile<NM, NQ, TPE>;
+    using S = PairSplit<T, NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
-    {
+    if (t < Ti::NCOL) {
         const int i = t % NQ0, j = t / NQ0;
 #pragma unroll
         for (int k = 0; k < NQ2; ++k) S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = y[k];
     }
-    __syncthreads();
+    esync<Ti::WAVE>();
     if (t < NQ1 * NQ2) {
         const int j = t % NQ1, k = t / NQ1;
         T v[NQ0];
 #pragma unroll
         for (int i = 0; i < NQ0; ++i) v[i] = S3[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i];
+        T o[NM];
+        contract<false>(v, [&](int p) { return tab.B0row(p); }, o);
 #pragma unroll
-        for (int p = 0; p < NM; ++p) {
-            T s = v[0] * tab.B0(p, 0);
-#pragma unroll
-            for (int i = 1; i < NQ0; ++i) s = fma(v[i], tab.B0(p, i), s);
-            S2[(k * NQ1 + j) * NM + p] = s;
-        }
+        for (int p = 0; p < NM; ++p) S2[(k * NQ1 + j) * NM + p] = o[p];
     }
-    __syncthreads();
+    esync<Ti::WAVE>();
     if (t < NQ2) {
         const int k = t;
         T top = T(0), bot = T(0);
@@ -355,18 +417,22 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
             if (q < NM - p) S1[(pb + q) * NQ2 + k] = s;
         }
     }
-    __syncthreads();
-    if (t < Dm::NMB && valid) {
+    esync<Ti::WAVE>();
+    if (t < Dm::NMB * S::RG && valid) {
+        const int pr = t / S::RG, rg = t % S::RG;
         int p, q;
-        pair_of<NM>(t, p, q);
+        pair_of<NM>(pr, p, q);
         const int base = coeff_base<NM>(p, q);
         const int len = NM - p - q;
         T v[NQ2];
 #pragma unroll
-        for (int k = 0; k < NQ2; ++k) v[k] = S1[t * NQ2 + k];
+        for (int k = 0; k < NQ2; ++k) v[k] = S1[pr * NQ2 + k];
         const bool p00 = (p == 0 && q == 0), p01 = (p == 0 && q == 1);
+        // rows rg*RW .. rg*RW+RW-1 (past len: the next pair's rows or the
+        // zero rows after the table, read but not stored)
 #pragma unroll
-        for (int r = 0; r < NM; ++r) {
+        for (int rr = 0; rr < S::RW; ++rr) {
+            const int r = rg * S::RW + rr;
             T b[NQ2];
             load_row(lt.B2(base + r), b);
             T s = v[0] * b[0];
@@ -391,43 +457,53 @@ __device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, con
 }
 
 // ===================================================================
-template <typename T, int NM, int NQ>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void bwdtrans_kernel(
-    int nelmt, const T *__restrict__ tabp, const T *__restrict__ in, T *__restrict__ out)
+// Kernel entry points: TPE work-items per element (Tile), workgroups of
+// Tile<NM, NQ, TPE>::BLOCK work-items.  Work-items past the NQ0*NQ1
+// columns compute on the last column's indices (every read in bounds) and
+// store nothing of it.
+template <typename T, int NM, int NQ, int TPE>
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void bwdtrans_kernel(int nelmt, const T *__restrict__ tabp,
+                                                                           const T *__restrict__ in,
+                                                                           T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    using Ti = Tile<NM, NQ, TPE>;
+    __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
     const LTab<T, NM, NQ> lt{ltab};
-    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
-    const int e = blockIdx.x * Dm::EPB + le;
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt;
     T u[NQ - 1];
-    bwd_to_column<T, NM, NQ>(tab, lt, in + (size_t)e * Dm::NC, valid, t, lds[le][0], lds[le][1], lds[le][0], u);
-    if (valid) {
+    bwd_to_column<T, NM, NQ, TPE>(tab, lt, in + (size_t)e * Dm::NC, valid, t, lds[le][0], lds[le][1], lds[le][0],
+                                  u);
+    if (valid && t < Ti::NCOL) {
         const int i = t % Dm::NQ0, j = t / Dm::NQ0;
 #pragma unroll
         for (int k = 0; k < Dm::NQ2; ++k) out[(size_t)e * Dm::NQT + (k * Dm::NQ1 + j) * Dm::NQ0 + i] = u[k];
     }
 }
 
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_kernel(
-    int nelmt, const T *__restrict__ tabp, const T *__restrict__ jac, const T *__restrict__ in,
-    T *__restrict__ out)
+template <typename T, int NM, int NQ, bool DEF, int TPE>
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void iproduct_kernel(int nelmt, const T *__restrict__ tabp,
+                                                                           const T *__restrict__ jac,
+                                                                           const T *__restrict__ in,
+                                                                           T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    using Ti = Tile<NM, NQ, TPE>;
+    __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
     const LTab<T, NM, NQ> lt{ltab};
-    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
-    const int e = blockIdx.x * Dm::EPB + le;
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt;
-    const int i = t % Dm::NQ0, j = t / Dm::NQ0;
+    const int tc = min(t, Ti::NCOL - 1);
+    const int i = tc % Dm::NQ0, j = tc / Dm::NQ0;
     const T wij = tab.W0(i) * tab.W1(j);
     T jv = T(0);
     if (!DEF && valid) jv = jac[e];
@@ -439,7 +515,8 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_kernel(
         const T jq = DEF ? (valid ? jac[(size_t)e * Dm::NQT + c] : T(0)) : jv;
         y[k] = fv * jq * (wij * tab.W2(k));
     }
-    iprod_from_column<T, NM, NQ>(tab, lt, y, valid, t, lds[le][0], lds[le][1], lds[le][0], out + (size_t)e * Dm::NC);
+    iprod_from_column<T, NM, NQ, TPE>(tab, lt, y, valid, t, lds[le][0], lds[le][1], lds[le][0],
+                                      out + (size_t)e * Dm::NC);
 }
 
 // collocation derivatives along i (lines (j,k)) and j (lines (i,k)) from A
@@ -450,72 +527,59 @@ __device__ __forceinline__ void deriv01(const Tab<T, NM, NQ> &tab, int t, const 
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
     if (t < NQ1 * NQ2) {
         const int j = t % NQ1, k = t / NQ1;
-        T v[NQ0];
+        T v[NQ0], o[NQ0];
 #pragma unroll
         for (int l = 0; l < NQ0; ++l) v[l] = A[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + l];
+        contract<false>(v, [&](int i) { return tab.D0Trow(i); }, o);
 #pragma unroll
-        for (int i = 0; i < NQ0; ++i) {
-            T s = v[0] * tab.D0(0, i);
-#pragma unroll
-            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(l, i), s);
-            Bs[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = s;
-        }
+        for (int i = 0; i < NQ0; ++i) Bs[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = o[i];
     }
     if (t < NQ0 * NQ2) {
         const int i = t % NQ0, k = t / NQ0;
-        T v[NQ1];
+        T v[NQ1], o[NQ1];
 #pragma unroll
         for (int l = 0; l < NQ1; ++l) v[l] = A[(k * NQ1 + l) * Slab<T, NM, NQ>::PI + i];
+        contract<false>(v, [&](int j) { return tab.D1Trow(j); }, o);
 #pragma unroll
-        for (int j = 0; j < NQ1; ++j) {
-            T s = v[0] * tab.D1(0, j);
-#pragma unroll
-            for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(l, j), s);
-            C[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = s;
-        }
+        for (int j = 0; j < NQ1; ++j) C[(k * NQ1 + j) * Slab<T, NM, NQ>::PI + i] = o[j];
     }
 }
 
 template <typename T, int NM, int NQ>
 __device__ __forceinline__ void deriv2_column(const Tab<T, NM, NQ> &tab, const T (&u)[NQ - 1], T (&d2)[NQ - 1])
 {
-    constexpr int NQ2 = NQ - 1;
-#pragma unroll
-    for (int k = 0; k < NQ2; ++k) {
-        T s = u[0] * tab.D2(0, k);
-#pragma unroll
-        for (int l = 1; l < NQ2; ++l) s = fma(u[l], tab.D2(l, k), s);
-        d2[k] = s;
-    }
+    contract<false>(u, [&](int k) { return tab.D2Trow(k); }, d2);
 }
 
 // PhysDeriv: PhysDerivKernels.hpp:436-507 (tensor) + :198-259 (collapsed
 // chain rule) + :326-340 (df)
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
+template <typename T, int NM, int NQ, bool DEF, int TPE>
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void physderiv_kernel(
     int nelmt, const T *__restrict__ tabp, const T *__restrict__ df, const T *__restrict__ in,
     T *__restrict__ out0, T *__restrict__ out1, T *__restrict__ out2)
 {
     using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][3][Slab<T, NM, NQ>::SIZE];
+    using Ti = Tile<NM, NQ, TPE>;
+    __shared__ T lds[Ti::EPB][3][Slab<T, NM, NQ>::SIZE];
     const Tab<T, NM, NQ> tab{tabp};
-    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
-    const int e = blockIdx.x * Dm::EPB + le;
-    const bool valid = e < nelmt;
-    const int i = t % Dm::NQ0, j = t / Dm::NQ0;
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * Ti::EPB + le;
+    const bool valid = e < nelmt, col = t < Ti::NCOL;
+    const int tc = min(t, Ti::NCOL - 1);
+    const int i = tc % Dm::NQ0, j = tc / Dm::NQ0;
     T *A = lds[le][0], *Bs = lds[le][1], *C = lds[le][2];
     T u[NQ - 1], d2[NQ - 1];
 #pragma unroll
     for (int k = 0; k < Dm::NQ2; ++k) {
         const int c = (k * Dm::NQ1 + j) * Dm::NQ0 + i;
         u[k] = valid ? in[(size_t)e * Dm::NQT + c] : T(0);
-        A[(k * Dm::NQ1 + j) * Slab<T, NM, NQ>::PI + i] = u[k];
+        if (col) A[(k * Dm::NQ1 + j) * Slab<T, NM, NQ>::PI + i] = u[k];
     }
     deriv2_column<T, NM, NQ>(tab, u, d2);
-    __syncthreads();
+    esync<Ti::WAVE>();
     deriv01<T, NM, NQ>(tab, t, A, Bs, C);
-    __syncthreads();
-    if (!valid) return;
+    esync<Ti::WAVE>();
+    if (!valid || !col) return;
     T g[9];
     if (!DEF) {
 #pragma unroll
@@ -562,96 +626,82 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void physderiv_kernel(
 // (each line belongs to one work-item, which reads it whole before writing
 // it), so the element needs two slabs.  The caller's writes of X and Y
 // must not be followed by a barrier yet.
-template <typename T, int NM, int NQ>
+template <typename T, int NM, int NQ, int TPE>
 __device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
                                                          bool valid, int t, int e, T (&y)[NQ - 1],
                                                          const T (&v2)[NQ - 1], T *X, T *Y, T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
+    using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2;
-    const int i = t % NQ0, j = t / NQ0;
-#pragma unroll
-    for (int k = 0; k < NQ2; ++k) {
-        T s = y[k];
-#pragma unroll
-        for (int l = 0; l < NQ2; ++l) s = fma(tab.D2(k, l), v2[l], s);
-        y[k] = s;
-    }
-    __syncthreads();
+    const int tc = min(t, Ti::NCOL - 1);
+    const int i = tc % NQ0, j = tc / NQ0;
+    contract<true>(v2, [&](int k) { return tab.D2row(k); }, y);
+    esync<Ti::WAVE>();
     if (t < NQ1 * NQ2) {  // D0^T along i, lines (j,k) of Y, in place
         const int jj = t % NQ1, k = t / NQ1;
         T *line = Y + (k * NQ1 + jj) * Slab<T, NM, NQ>::PI;
-        T v[NQ0];
+        T v[NQ0], o[NQ0];
 #pragma unroll
         for (int l = 0; l < NQ0; ++l) v[l] = line[l];
+        contract<false>(v, [&](int ii) { return tab.D0row(ii); }, o);
 #pragma unroll
-        for (int ii = 0; ii < NQ0; ++ii) {
-            T s = v[0] * tab.D0(ii, 0);
-#pragma unroll
-            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(ii, l), s);
-            line[ii] = s;
-        }
+        for (int ii = 0; ii < NQ0; ++ii) line[ii] = o[ii];
     }
     if (t < NQ0 * NQ2) {  // D1^T along j, lines (i,k) of X, in place
         const int ii = t % NQ0, k = t / NQ0;
-        T v[NQ1];
+        T v[NQ1], o[NQ1];
 #pragma unroll
         for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * Slab<T, NM, NQ>::PI + ii];
+        contract<false>(v, [&](int jj) { return tab.D1row(jj); }, o);
 #pragma unroll
-        for (int jj = 0; jj < NQ1; ++jj) {
-            T s = v[0] * tab.D1(jj, 0);
-#pragma unroll
-            for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(jj, l), s);
-            X[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = s;
-        }
+        for (int jj = 0; jj < NQ1; ++jj) X[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = o[jj];
     }
-    __syncthreads();
+    esync<Ti::WAVE>();
 #pragma unroll
     for (int k = 0; k < NQ2; ++k) {
         const int c = (k * NQ1 + j) * Slab<T, NM, NQ>::PI + i;
         y[k] += Y[c] + X[c];
     }
-    iprod_from_column<T, NM, NQ>(tab, lt, y, valid, t, X, Y, X, out + (size_t)e * Dm::NC);
+    iprod_from_column<T, NM, NQ, TPE>(tab, lt, y, valid, t, X, Y, X, out + (size_t)e * Dm::NC);
 }
 
 // One workgroup of the fused Helmholtz apply: elements blk*EPB ..
 // blk*EPB+EPB-1 of an operator with nelmt elements.
-template <typename T, int NM, int NQ, bool DEF, bool ADV = false>
+template <typename T, int NM, int NQ, bool DEF, int TPE, bool ADV = false>
 __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__restrict__ tabp,
                                                 const T *__restrict__ geo, T lambda, const T *__restrict__ in,
                                                 T *__restrict__ out, T (*lds)[2][Slab<T, NM, NQ>::SIZE], const T *ltab,
                                                 const T *__restrict__ adv = nullptr)
 {
     using Dm = Dims<NM, NQ>;
+    using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
     const Tab<T, NM, NQ> tab{tabp};
     const LTab<T, NM, NQ> lt{ltab};
-    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
-    const int e = blk * Dm::EPB + le;
-    const bool valid = e < nelmt;
-    const int i = t % NQ0, j = t / NQ0;
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blk * Ti::EPB + le;
+    const bool valid = e < nelmt, col = t < Ti::NCOL;
+    const int tc = min(t, Ti::NCOL - 1);
+    const int i = tc % NQ0, j = tc / NQ0;
     T *X = lds[le][0], *Y = lds[le][1];
 
     // X holds the quadrature values after the transform; Y <- D0 X along
     // the (j,k) lines, then D1 in place on the (i,k) lines of X
     T u[NQ - 1], d2[NQ - 1];
-    bwd_to_column<T, NM, NQ>(tab, lt, in + (size_t)e * Dm::NC, valid, t, X, Y, X, u);
+    bwd_to_column<T, NM, NQ, TPE>(tab, lt, in + (size_t)e * Dm::NC, valid, t, X, Y, X, u);
     deriv2_column<T, NM, NQ>(tab, u, d2);
     if (t < NQ1 * NQ2) {
         const int jj = t % NQ1, k = t / NQ1;
         const T *line = X + (k * NQ1 + jj) * Slab<T, NM, NQ>::PI;
-        T v[NQ0];
+        T v[NQ0], o[NQ0];
 #pragma unroll
         for (int l = 0; l < NQ0; ++l) v[l] = line[l];
+        contract<false>(v, [&](int ii) { return tab.D0Trow(ii); }, o);
 #pragma unroll
-        for (int ii = 0; ii < NQ0; ++ii) {
-            T s = v[0] * tab.D0(0, ii);
-#pragma unroll
-            for (int l = 1; l < NQ0; ++l) s = fma(v[l], tab.D0(l, ii), s);
-            Y[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = s;
-        }
+        for (int ii = 0; ii < NQ0; ++ii) Y[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = o[ii];
     }
-    __syncthreads();
+    esync<Ti::WAVE>();
     // advection coefficients of this column: loads issued here, so that
     // their latency hides behind the direction-1 pass
     T cadv[3][ADV ? NQ2 : 1];
@@ -666,35 +716,80 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
     }
     if (t < NQ0 * NQ2) {
         const int ii = t % NQ0, k = t / NQ0;
-        T v[NQ1];
+        T v[NQ1], o[NQ1];
 #pragma unroll
         for (int l = 0; l < NQ1; ++l) v[l] = X[(k * NQ1 + l) * Slab<T, NM, NQ>::PI + ii];
+        contract<false>(v, [&](int jj) { return tab.D1Trow(jj); }, o);
 #pragma unroll
-        for (int jj = 0; jj < NQ1; ++jj) {
-            T s = v[0] * tab.D1(0, jj);
-#pragma unroll
-            for (int l = 1; l < NQ1; ++l) s = fma(v[l], tab.D1(l, jj), s);
-            X[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = s;
-        }
+        for (int jj = 0; jj < NQ1; ++jj) X[(k * NQ1 + jj) * Slab<T, NM, NQ>::PI + ii] = o[jj];
     }
-    __syncthreads();
+    esync<Ti::WAVE>();
 
     T y[NQ - 1], v2[NQ - 1];
-    {
+    const T h0 = tab.H0(i), h1 = tab.H1(j), h2 = tab.H2(j);
+    if constexpr (!DEF) {
+        // Affine element: df and J are constants, and the metric terms of
+        // the collapsed directions factor into a per-column part and the
+        // direction-2 factors h3(k), W2(k):
+        //   A_c = df[3c] + h0 (df[3c+1] + df[3c+2]),  E_c = df[3c+1] + h1 df[3c+2]
+        //   g0 = h2^2 h3^2 sum A^2,  g3 = h2 h3^2 sum A E,  g4 = h2 h3 sum df[3c+2] A
+        //   g1 = h3^2 sum E^2,       g5 = h3 sum df[3c+2] E, g2 = sum df[3c+2]^2
+        // so each point needs the three products with (a0, a1, a2) only.
         T g[10];
-        if (!DEF) {
 #pragma unroll
-            for (int c = 0; c < 10; ++c) g[c] = valid ? geo[(size_t)e * 10 + c] : T(0);
+        for (int c = 0; c < 10; ++c) g[c] = valid ? geo[(size_t)e * 10 + c] : T(0);
+        T ga = T(0), gb = T(0), gc = T(0), gd = T(0), ge = T(0), gf = T(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T A = fma(h0, g[3 * c + 1] + g[3 * c + 2], g[3 * c]);
+            const T E = fma(h1, g[3 * c + 2], g[3 * c + 1]);
+            ga = fma(A, A, ga);
+            gb = fma(A, E, gb);
+            gc = fma(g[3 * c + 2], A, gc);
+            gd = fma(E, E, gd);
+            ge = fma(g[3 * c + 2], E, ge);
+            gf = fma(g[3 * c + 2], g[3 * c + 2], gf);
         }
-        const T h0 = tab.H0(i), h1 = tab.H1(j), h2 = tab.H2(j);
+        const T cw = tab.W0(i) * tab.W1(j) * g[9];
+        const T cwh2 = cw * h2;
+        const T q00 = cwh2 * h2 * ga, q01 = cwh2 * gb, q02 = cwh2 * gc;
+        const T q11 = cw * gd, q12 = cw * ge, q22 = cw * gf, lm = lambda * cw;
+#pragma unroll
+        for (int k = 0; k < NQ2; ++k) {
+            const int cl = (k * NQ1 + j) * Slab<T, NM, NQ>::PI + i;
+            const T w2 = tab.W2(k), h3 = tab.H3(k);
+            const T s1 = w2 * h3, s2 = s1 * h3;
+            const T a0 = Y[cl], a1 = X[cl], a2 = d2[k];
+            T m0 = q00 * a0;
+            m0 = fma(q01, a1, m0);
+            T m1 = q01 * a0;
+            m1 = fma(q11, a1, m1);
+            T m2 = q02 * a0;
+            m2 = fma(q12, a1, m2);
+            const T n1 = s1 * a2, n0 = w2 * a2;
+            const T o0 = fma(q02, n1, s2 * m0);
+            const T o1 = fma(q12, n1, s2 * m1);
+            v2[k] = fma(q22, n0, s1 * m2);
+            if (col) {
+                Y[cl] = o0;
+                X[cl] = o1;
+            }
+            T m = u[k];
+            if constexpr (ADV) {
+                m = fma(cadv[0][k], a0, m);
+                m = fma(cadv[1][k], a1, m);
+                m = fma(cadv[2][k], a2, m);
+            }
+            y[k] = (lm * w2) * m;
+        }
+    } else {
+        T g[10];
         const T wij = tab.W0(i) * tab.W1(j);
 #pragma unroll
         for (int k = 0; k < NQ2; ++k) {
             const int c = (k * NQ1 + j) * NQ0 + i, cl = (k * NQ1 + j) * Slab<T, NM, NQ>::PI + i;
-            if (DEF) {
 #pragma unroll
-                for (int cc = 0; cc < 10; ++cc) g[cc] = valid ? geo[((size_t)e * 10 + cc) * NQT + c] : T(0);
-            }
+            for (int cc = 0; cc < 10; ++cc) g[cc] = valid ? geo[((size_t)e * 10 + cc) * NQT + c] : T(0);
             const T h3 = tab.H3(k);
             const T h2h3 = h2 * h3, h1h3 = h1 * h3, h0h2h3 = h0 * h2h3;
             T t1 = h0h2h3 * (g[1] + g[2]);
@@ -738,8 +833,10 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             T o2 = g4 * a0;
             o2 = fma(g5, a1, o2);
             o2 = fma(g2, a2, o2);
-            Y[cl] = wj * o0;
-            X[cl] = wj * o1;
+            if (col) {
+                Y[cl] = wj * o0;
+                X[cl] = wj * o1;
+            }
             v2[k] = wj * o2;
             T m = u[k];
             if constexpr (ADV) {
@@ -750,19 +847,20 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
             y[k] = (lambda * wj) * m;
         }
     }
-    grad_transpose_to_coeffs<T, NM, NQ>(tab, lt, valid, t, e, y, v2, X, Y, out);
+    grad_transpose_to_coeffs<T, NM, NQ, TPE>(tab, lt, valid, t, e, y, v2, X, Y, out);
 }
 
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
-    int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, T lambda, const T *__restrict__ in,
-    T *__restrict__ out)
+template <typename T, int NM, int NQ, bool DEF, int TPE>
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void helmholtz_kernel(int nelmt, const T *__restrict__ tabp,
+                                                                            const T *__restrict__ geo, T lambda,
+                                                                            const T *__restrict__ in,
+                                                                            T *__restrict__ out)
 {
-    using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    using Ti = Tile<NM, NQ, TPE>;
+    __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
-    helmholtz_block<T, NM, NQ, DEF>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab);
+    helmholtz_block<T, NM, NQ, DEF, TPE>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab);
 }
 
 // LinearAdvectionDiffusionReaction (LinearAdvectionDiffusionReaction.h
@@ -772,33 +870,33 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_kernel(
 // c_0 = h2 h3 (a_0 + h0 (a_1 + a_2)), c_1 = h3 (a_1 + h1 a_2), c_2 = a_2,
 // a_d = sum_j V_j df[j*3+d], packed per point at set-up:
 // adv[(e*3 + d)*NQT + q].
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void adv_diff_react_kernel(
+template <typename T, int NM, int NQ, bool DEF, int TPE>
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void adv_diff_react_kernel(
     int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, const T *__restrict__ adv, T lambda,
     const T *__restrict__ in, T *__restrict__ out)
 {
-    using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    using Ti = Tile<NM, NQ, TPE>;
+    __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
-    helmholtz_block<T, NM, NQ, DEF, true>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab, adv);
+    helmholtz_block<T, NM, NQ, DEF, TPE, true>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab, adv);
 }
 
 // Grouped form: several operators of the same kind in one launch; block b
 // belongs to the item whose [block0, next block0) range holds it.
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_grouped_kernel(
+template <typename T, int NM, int NQ, bool DEF, int TPE>
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void helmholtz_grouped_kernel(
     int count, const GroupItem<T> *__restrict__ items, const T *__restrict__ tabp)
 {
-    using Dm = Dims<NM, NQ>;
-    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    using Ti = Tile<NM, NQ, TPE>;
+    __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const int b = blockIdx.x;
     int it = 0;
     while (it + 1 < count && b >= items[it + 1].block0) ++it;
     const GroupItem<T> I = items[it];
-    helmholtz_block<T, NM, NQ, DEF>(b - I.block0, I.nelmt, tabp, I.geo, I.lambda, I.in, I.out, lds, ltab);
+    helmholtz_block<T, NM, NQ, DEF, TPE>(b - I.block0, I.nelmt, tabp, I.geo, I.lambda, I.in, I.out, lds, ltab);
 }
 
 // IProductWRTDerivBase for tets (IProductWRTDerivBase.h:751-788, pointwise
@@ -806,22 +904,24 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void helmholtz_grouped_kern
 // mapped to the collapsed directions t0 <- f0 (t0 + f1 (t1 + t2)),
 // t1 <- f2 (t1 + f3 t2) with f0 = h2 h3, f1 = h0, f2 = h3, f3 = h1, then
 // weighted by W J.  Geometry packed as for the Helmholtz kernel (df, J).
-template <typename T, int NM, int NQ, bool DEF>
-__global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
+template <typename T, int NM, int NQ, bool DEF, int TPE>
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void iproduct_deriv_kernel(
     int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, const T *__restrict__ in0,
     const T *__restrict__ in1, const T *__restrict__ in2, T *__restrict__ out)
 {
     using Dm = Dims<NM, NQ>;
+    using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
-    __shared__ T lds[Dm::EPB][2][Slab<T, NM, NQ>::SIZE];
+    __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
     const LTab<T, NM, NQ> lt{ltab};
-    const int le = threadIdx.x / Dm::TPE, t = threadIdx.x % Dm::TPE;
-    const int e = blockIdx.x * Dm::EPB + le;
-    const bool valid = e < nelmt;
-    const int i = t % NQ0, j = t / NQ0;
+    const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
+    const int e = blockIdx.x * Ti::EPB + le;
+    const bool valid = e < nelmt, col = t < Ti::NCOL;
+    const int tc = min(t, Ti::NCOL - 1);
+    const int i = tc % NQ0, j = tc / NQ0;
     T *X = lds[le][0], *Y = lds[le][1];
     T g[10];
     if (!DEF) {
@@ -853,12 +953,14 @@ __global__ __launch_bounds__((block_size<NM, NQ>())) void iproduct_deriv_kernel(
         t0 = fma(t1 + t2, h0, t0) * (h2 * h3);
         t1 = fma(t2, h1, t1) * h3;
         const T wj = wij * tab.W2(k) * g[9];
-        Y[cl] = wj * t0;
-        X[cl] = wj * t1;
+        if (col) {
+            Y[cl] = wj * t0;
+            X[cl] = wj * t1;
+        }
         v2[k] = wj * t2;
         y[k] = T(0);
     }
-    grad_transpose_to_coeffs<T, NM, NQ>(tab, lt, valid, t, e, y, v2, X, Y, out);
+    grad_transpose_to_coeffs<T, NM, NQ, TPE>(tab, lt, valid, t, e, y, v2, X, Y, out);
 }
 
 }  // namespace tet
