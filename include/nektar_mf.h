@@ -205,10 +205,12 @@ int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in,
 /* ---------------------------------------------------------------------
  * Workgroup tiles and the autotuner.  No counterpart in the reference,
  * whose kernels have one shape per vector width (VecDataType in
- * MatrixFreeOps/Operator.hpp:56-57); here the Helmholtz kernels are
- * instantiated for several tiles (elements per workgroup) and the
- * fastest is measured per (key, nummodes, numpoints, precision) on the
- * caller's data.  The choice is remembered for the rest of the process:
+ * MatrixFreeOps/Operator.hpp:56-57); here the hex Helmholtz kernels are
+ * instantiated for several tiles (elements per workgroup) and every tet
+ * kernel for two (work-items per element: nq0*(nq0-1), elements sharing
+ * waves, or that rounded up to a multiple of 64, one element per wave at
+ * nq0 <= 8), and the fastest is measured per (key, nummodes, numpoints,
+ * precision) on the caller's data.  The choice is remembered for the rest of the process:
  * operators and solvers created later for the same kind start from it.
  * tile 0 = the library default.  Results do not depend on the tile.
  * ------------------------------------------------------------------- */
@@ -251,8 +253,8 @@ int nmf_batch_add_physderiv(nmf_batch b, nmf_operator op, const void *in,
 int nmf_batch_set_mode(nmf_batch b, int mode);
 /* streams 1..8 (default 4) */
 int nmf_batch_set_streams(nmf_batch b, int streams);
-/* Autotune each hex Helmholtz kind on its largest member; every member of
- * the kind takes the chosen tile. */
+/* Autotune each Helmholtz kind (hex or tet) on its largest member; every
+ * member of the kind takes the chosen tile. */
 int nmf_batch_autotune(nmf_batch b, void *stream);
 int nmf_batch_run(nmf_batch b, void *stream);
 /* entries, grouped launches, and total element-local input DOFs */
@@ -437,6 +439,19 @@ int nmf_linsys_set_rhs_local(nmf_linsys l, const double *bloc);
  * element, CG for the perturbation (zero on the Dirichlet rows), solution =
  * perturbation + dloc (local) or + the Dirichlet values (global). */
 int nmf_linsys_set_dirichlet_local(nmf_linsys l, const double *dloc);
+/* Robin conditions du/dn + alpha u = g: the boundary mass that
+ * GlobalLinSysIterativeFull adds after every element operator apply
+ * (AddRobinTraceContribution, GlobalLinSysIterativeFull.cpp:134-150 and
+ * 219-233; the robin BC info of ContField), as rows of the local vector:
+ * y[rows[k]] += sum_{m = rowptr[k]}^{rowptr[k+1]-1} vals[m] x[cols[m]]
+ * (host arrays; local indices; distinct rows; vals = int_face alpha
+ * phi_i phi_j summed over an element's Robin faces).  It enters the
+ * matrix multiply, the Dirichlet lifting and the Jacobi diagonal; g goes
+ * into the right-hand side as for Neumann faces.  nrows = 0 removes it.
+ * NMF_ERR_UNSUPPORTED with IterativeStaticCond. */
+int nmf_linsys_set_robin_local(nmf_linsys l, int64_t nrows, const int64_t *rows,
+                               const int64_t *rowptr, const int64_t *cols,
+                               const double *vals);
 /* Stops at ||r|| < tol ||b|| (unpreconditioned, Dirichlet rows excluded) */
 int nmf_linsys_solve(nmf_linsys l, double tol, int maxit, nmf_solve_info *info);
 int nmf_linsys_get_solution(nmf_linsys l, double *x_host);

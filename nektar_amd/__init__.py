@@ -144,6 +144,7 @@ _SIGNATURES = {
     "nmf_linsys_destroy": (ctypes.c_int, [_P]),
     "nmf_linsys_set_rhs_local": (ctypes.c_int, [_P, _P]),
     "nmf_linsys_set_dirichlet_local": (ctypes.c_int, [_P, _P]),
+    "nmf_linsys_set_robin_local": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P]),
     "nmf_linsys_solve": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(SolveInfo)]),
     "nmf_linsys_get_solution": (ctypes.c_int, [_P, _D]),
     "nmf_linsys_get_inv_diag": (ctypes.c_int, [_P, _D]),
@@ -692,6 +693,20 @@ class LinearSystem:
         None returns to homogeneous data (nmf_linsys_set_dirichlet_local)."""
         p = None if dloc is None else _device_ptr(dloc, torch.float64, self._amap.nloc)
         check(lib().nmf_linsys_set_dirichlet_local(self._h, p))
+
+    def set_robin_local(self, rows, rowptr, cols, vals) -> None:
+        """Robin boundary mass rows of the local vector: y[rows[k]] +=
+        sum_{m in [rowptr[k], rowptr[k+1])} vals[m] x[cols[m]]
+        (nmf_linsys_set_robin_local); empty rows remove it."""
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        if rows.size and (rowptr.size != rows.size + 1 or cols.size != vals.size or cols.size != rowptr[-1]):
+            raise ValueError("Robin arrays of inconsistent sizes")
+        self._robin = (rows, rowptr, cols, vals)  # kept alive for the call
+        check(lib().nmf_linsys_set_robin_local(self._h, rows.size, rows.ctypes.data, rowptr.ctypes.data,
+                                               cols.ctypes.data, vals.ctypes.data))
 
     def solve(self, tol: float = 1e-10, maxit: int = 5000) -> SolveInfo:
         info = SolveInfo()

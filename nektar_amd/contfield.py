@@ -249,6 +249,84 @@ def hex_dirichlet_local(session: Session, var: str, lv, verts, nm: int, nq: int,
     return dloc
 
 
+def hex_weak_local(session: Session, var: str, lv, verts, nm: int, nq: int, tables):
+    """Neumann and Robin terms of a hexahedral mesh in the element-local
+    layout.  Every Neumann or Robin face adds int_face g phi_i dS to the
+    local right-hand side of the element that owns it (the boundary
+    expansion's IProductWRTBase of g, DisContField.cpp:4060-4106, added to
+    the forcing at ContField.cpp:823-853); every Robin face (du/dn +
+    alpha u = g, PRIMCOEFF = alpha) also adds its face mass
+    int_face alpha phi_i phi_j dS to the operator (AddRobinTraceContribution
+    on the robin BC info, GlobalLinSysIterativeFull.cpp:134-150, 219-233).
+    The face integrals use the element's own GLL points on the face (the
+    boundary expansion's points) and the face's area element.
+    Returns (b [E*nm^3], robin rows (rows, rowptr, cols, vals) or None)."""
+    mesh = session.mesh
+    B, z, w = tables
+    E = len(lv)
+    nc = nm ** 3
+    owner = {}
+    for e in range(E):
+        for key, frame in _hex_face_frames(lv[e]).items():
+            owner.setdefault(key, []).append((e, frame))
+    b = np.zeros(E * nc)
+    robin = {}
+    for rid, bcs in session.conditions.items():
+        for bc in bcs:
+            if bc.var != var or bc.kind not in ("N", "R"):
+                continue
+            for cid in session.regions[rid]:
+                for letter, ids in mesh.composites[cid]:
+                    if letter != "F":
+                        raise NotImplementedError("boundary regions of hexahedral meshes are made of faces")
+                    for fid in ids:
+                        own = owner.get(_face_vertices(mesh, fid), [])
+                        if len(own) != 1:
+                            raise ValueError(f"boundary face {fid} belongs to {len(own)} elements")
+                        e, (d, sd) = own[0]
+                        a, c = [k for k in range(3) if k != d]
+                        xi = np.zeros((nq, nq, 3))
+                        xi[:, :, a] = z[:, None]
+                        xi[:, :, c] = z[None, :]
+                        xi[:, :, d] = sd
+                        X, dX = _hex_map(verts[e], xi.reshape(-1, 3))
+                        X = X.reshape(nq, nq, 3)
+                        Jf = np.linalg.norm(np.cross(dX[:, :, a], dX[:, :, c]), axis=1).reshape(nq, nq)
+                        names = dict(session.params, x=X[..., 0], y=X[..., 1], z=X[..., 2], t=0.0)
+                        g = np.broadcast_to(evaluate(bc.value, names), (nq, nq)).astype(float)
+                        W2 = np.outer(w, w) * Jf
+                        md = 0 if sd < 0 else 1
+                        li = np.empty((nm, nm), dtype=np.int64)
+                        for pa in range(nm):
+                            for pb in range(nm):
+                                idx = [0, 0, 0]
+                                idx[a], idx[c], idx[d] = pa, pb, md
+                                li[pa, pb] = e * nc + (idx[2] * nm + idx[1]) * nm + idx[0]
+                        b[li] += B @ (W2 * g) @ B.T
+                        if bc.kind == "R":
+                            if bc.primcoeff is None:
+                                raise ValueError(f"Robin condition of region {rid} without PRIMCOEFF")
+                            al = np.broadcast_to(evaluate(bc.primcoeff, names), (nq, nq)).astype(float)
+                            Phi = np.einsum("ai,bj->abij", B, B).reshape(nm * nm, nq * nq)
+                            Mf = (Phi * (W2 * al).reshape(-1)) @ Phi.T
+                            flat = li.reshape(-1)
+                            for r in range(nm * nm):
+                                row = robin.setdefault(int(flat[r]), {})
+                                for q in range(nm * nm):
+                                    row[int(flat[q])] = row.get(int(flat[q]), 0.0) + Mf[r, q]
+    if not robin:
+        return b, None
+    rows = np.array(sorted(robin), dtype=np.int64)
+    rowptr = np.zeros(rows.size + 1, dtype=np.int64)
+    cols, vals = [], []
+    for k, r in enumerate(rows):
+        cs = sorted(robin[int(r)])
+        cols += cs
+        vals += [robin[int(r)][q] for q in cs]
+        rowptr[k + 1] = len(cols)
+    return b, (rows, rowptr, np.array(cols, dtype=np.int64), np.array(vals))
+
+
 def hex_quadrature_points(verts, z):
     """Physical coordinates [E, Q^3, 3] of the GLL points z of (trilinear)
     hexahedra verts [E, 8, 3], point (i, j, k) at index (k Q + j) Q + i."""
@@ -280,8 +358,10 @@ def dirichlet_problem(session: Session, var: str = "u"):
     kinds = {k for k, _ in s.mesh.elements.values()}
     for bcs in s.conditions.values():
         for bc in bcs:
-            if bc.var == var and bc.kind != "D":
-                raise NotImplementedError(f"boundary condition type {bc.kind} (only Dirichlet is supported)")
+            if bc.var == var and bc.kind not in ("D", "N", "R"):
+                raise NotImplementedError(f"boundary condition type {bc.kind} (Dirichlet, Neumann and Robin are)")
+            if bc.var == var and bc.kind != "D" and kinds != {"H"}:
+                raise NotImplementedError("Neumann and Robin conditions are supported on hexahedral meshes")
     nm = s.mesh.expansions[0]["nummodes"]
     nq0 = nm + 1
     regions = [rid for rid, bcs in s.conditions.items() if any(bc.var == var and bc.kind == "D" for bc in bcs)]
@@ -294,6 +374,7 @@ def dirichlet_problem(session: Session, var: str = "u"):
         mp, sg, ng, nd = nio.hex_c0_map(lv, nm, [_face_vertices(s.mesh, f) for f in faces])
         tables = (t["bdata"].reshape(nm, nq0), t["z"], t["w"])
         dloc = hex_dirichlet_local(s, var, lv, verts, nm, nq0, mp, sg, nd, tables)
+        bweak, robin = hex_weak_local(s, var, lv, verts, nm, nq0, tables)
         X = hex_quadrature_points(verts, t["z"])
         weights = (t["w"],)
         tag = "Hex"
@@ -307,11 +388,12 @@ def dirichlet_problem(session: Session, var: str = "u"):
         weights = (t[0]["w"], t[1]["w"], t[2]["w"])
         tag = "Tet"
         deformed = False
+        bweak, robin = np.zeros(len(lv) * (nm * (nm + 1) * (nm + 2) // 6)), None
     else:
         raise NotImplementedError("Dirichlet solves take meshes of hexahedra or of tetrahedra")
     return dict(tag=tag, deformed=deformed, eids=eids, lv=lv, verts=verts, jac=jac, df=df, nm=nm, nq0=nq0,
                 map=mp, sign=sg,
-                nglobal=ng, ndir=nd, dloc=dloc, X=X, weights=weights)
+                nglobal=ng, ndir=nd, dloc=dloc, bweak=bweak, robin=robin, X=X, weights=weights)
 
 
 def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit: int = 20000,
@@ -351,7 +433,13 @@ def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit
     fq = torch.from_numpy(np.ascontiguousarray(f)).to(dev)
     bl = torch.zeros(ipr.output_size(), dtype=torch.float64, device=dev)
     ipr(fq, bl)
-    ls.set_rhs_local(-bl)  # ContField.cpp:812-817: -IProductWRTBase(f)
+    # ContField.cpp:812-853: -IProductWRTBase(f) plus the Neumann and Robin
+    # boundary integrals
+    if P["robin"] is not None:
+        ls.set_robin_local(*P["robin"])
+    if np.any(P["bweak"]):
+        bl = bl - torch.from_numpy(P["bweak"]).to(dev)
+    ls.set_rhs_local(-bl)
     ls.set_dirichlet_local(torch.from_numpy(np.ascontiguousarray(P["dloc"])).to(dev))
     info = ls.solve(tol=tol, maxit=maxit)
     xl = torch.zeros(helm.input_size(), dtype=torch.float64, device=dev)

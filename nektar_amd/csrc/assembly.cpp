@@ -172,6 +172,52 @@ void IterativeSolver::apply_local(const double *xloc, double *yloc)
 {
     for (size_t k = 0; k < ops_.size(); ++k)
         ops_[k]->apply(xloc + off_[k], yloc + off_[k], nullptr, nullptr, stream_);
+    add_robin(xloc, false, yloc);
+}
+
+void IterativeSolver::add_robin(const double *x, bool global, double *yloc)
+{
+    if (nrobin_ == 0) return;
+    NMF_HIP(solver::launch_robin(stream_, nrobin_, rrow_.as<int64_t>(), rptr_.as<int64_t>(), rcol_.as<int64_t>(),
+                                 rval_.as<double>(), global ? map_.signed_map() : nullptr, x, yloc));
+}
+
+void IterativeSolver::set_robin_local(int64_t nrows, const int64_t *rows, const int64_t *rowptr,
+                                      const int64_t *cols, const double *vals)
+{
+    NMF_REQUIRE(nrows >= 0, NMF_ERR_INVALID, "negative Robin row count");
+    NMF_REQUIRE(nrows == 0 || space_ != CGSpace::StaticCond, NMF_ERR_UNSUPPORTED,
+                "Robin conditions are not available with static condensation");
+    if (nrows == 0) {
+        nrobin_ = 0;
+        build_diagonal();
+        return;
+    }
+    NMF_REQUIRE(rows && rowptr && cols && vals, NMF_ERR_INVALID, "null Robin array");
+    const int64_t n = map_.nloc(), nnz = rowptr[nrows];
+    NMF_REQUIRE(rowptr[0] == 0 && nnz >= 0, NMF_ERR_INVALID, "Robin row pointers must start at 0");
+    std::vector<char> seen((size_t)n, 0);
+    for (int64_t k = 0; k < nrows; ++k) {
+        NMF_REQUIRE(rows[k] >= 0 && rows[k] < n, NMF_ERR_INVALID, "Robin row outside the local vector");
+        NMF_REQUIRE(!seen[(size_t)rows[k]], NMF_ERR_INVALID, "Robin rows must be distinct");
+        seen[(size_t)rows[k]] = 1;
+        NMF_REQUIRE(rowptr[k + 1] >= rowptr[k], NMF_ERR_INVALID, "Robin row pointers must not decrease");
+        for (int64_t m = rowptr[k]; m < rowptr[k + 1]; ++m)
+            NMF_REQUIRE(cols[m] >= 0 && cols[m] < n, NMF_ERR_INVALID, "Robin column outside the local vector");
+    }
+    NMF_HIP(hipStreamSynchronize(stream_));
+    rrow_.allocate((size_t)nrows * sizeof(int64_t));
+    rptr_.allocate((size_t)(nrows + 1) * sizeof(int64_t));
+    rcol_.allocate((size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t));
+    rval_.allocate((size_t)std::max<int64_t>(nnz, 1) * sizeof(double));
+    NMF_HIP(hipMemcpy(rrow_.get(), rows, (size_t)nrows * sizeof(int64_t), hipMemcpyHostToDevice));
+    NMF_HIP(hipMemcpy(rptr_.get(), rowptr, (size_t)(nrows + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (nnz) {
+        NMF_HIP(hipMemcpy(rcol_.get(), cols, (size_t)nnz * sizeof(int64_t), hipMemcpyHostToDevice));
+        NMF_HIP(hipMemcpy(rval_.get(), vals, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice));
+    }
+    nrobin_ = nrows;
+    build_diagonal();
 }
 
 void IterativeSolver::apply_from_global(const double *xg, double *yloc)
@@ -186,6 +232,7 @@ void IterativeSolver::apply_from_global(const double *xg, double *yloc)
             o.apply(xloc_.as<double>() + off_[k], yloc + off_[k], nullptr, nullptr, stream_);
         }
     }
+    add_robin(xg, true, yloc);
 }
 
 // PreconditionerDiagonal.cpp:93-125: the diagonal of every element matrix,
@@ -207,6 +254,9 @@ void IterativeSolver::build_diagonal()
             NMF_HIP(solver::launch_take_mode(stream_, (size_t)o.nelmt, o.ncoeff, m, y, dloc + off_[k]));
         }
     }
+    if (nrobin_)  // the Robin mass's diagonal entries
+        NMF_HIP(solver::launch_robin_diag(stream_, nrobin_, rrow_.as<int64_t>(), rptr_.as<int64_t>(),
+                                          rcol_.as<int64_t>(), rval_.as<double>(), dloc));
     map_.assemble(dloc, invd_.as<double>(), stream_, false, nullptr, nullptr, true);
     NMF_HIP(solver::launch_invert_diag_dir(stream_, map_.nglobal(), map_.ndir(), invd_.as<double>()));
     NMF_HIP(hipStreamSynchronize(stream_));

@@ -104,6 +104,19 @@ public:
     // rhs = bloc - A dloc element by element, CG for the perturbation with
     // zero Dirichlet rows, solution = perturbation + dloc.
     void set_dirichlet_local(const double *dloc);
+    // The Robin boundary mass of Helmholtz solves with Robin conditions
+    // du/dn + alpha u = g: what GlobalLinSysIterativeFull adds to every
+    // element operator apply (AddRobinTraceContribution,
+    // GlobalLinSysIterativeFull.cpp:134-150 in the Dirichlet lifting and
+    // :219-233 in the matrix multiply), the face mass matrices
+    // int_face alpha phi_i phi_j of the elements with Robin faces, given as
+    // rows of the local vector: for k < nrows, y[rows[k]] +=
+    // sum_{m in [rowptr[k], rowptr[k+1])} vals[m] x[cols[m]] (host arrays,
+    // local indices, rows distinct).  nrows = 0 removes it.  The Jacobi
+    // diagonal takes the entries with cols[m] = rows[k].  Not available with
+    // static condensation.
+    void set_robin_local(int64_t nrows, const int64_t *rows, const int64_t *rowptr, const int64_t *cols,
+                         const double *vals);
     // rebuild what depends on the operators if any of them changed since
     // the last build (called by every public entry that uses them)
     void refresh();
@@ -157,6 +170,10 @@ private:
     // right-hand side the CG sees is built from them at every solve
     DeviceBuffer bloc_, dloc_, dglob_;
     bool have_dir_ = false;
+    // Robin rows (set_robin_local): row index, row start, column, value
+    DeviceBuffer rrow_, rptr_, rcol_, rval_;
+    int64_t nrobin_ = 0;
+    void add_robin(const double *x, bool global, double *yloc);
     void load_rhs();
 };
 

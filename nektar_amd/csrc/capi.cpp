@@ -800,6 +800,15 @@ int nmf_linsys_set_dirichlet_local(nmf_linsys l, const double *dloc)
     });
 }
 
+int nmf_linsys_set_robin_local(nmf_linsys l, int64_t nrows, const int64_t *rows, const int64_t *rowptr,
+                               const int64_t *cols, const double *vals)
+{
+    return guard([&] {
+        NMF_LS_CHECK();
+        l->impl->set_robin_local(nrows, rows, rowptr, cols, vals);
+    });
+}
+
 int nmf_linsys_solve(nmf_linsys l, double tol, int maxit, nmf_solve_info *info)
 {
     return guard([&] {

@@ -773,6 +773,39 @@ __global__ void sub_kernel(size_t n, const double *a, const double *b, double *o
         out[i] = a[i] - b[i];
 }
 
+// Robin boundary mass rows: y[rows[k]] += sum_m vals[m] x(cols[m]), with
+// x local, or global through the signed map (entry c >= 0: +x[c], c < 0:
+// -x[~c]); one work-item per row (rows distinct, so no two write one entry)
+__global__ void robin_kernel(int64_t nrows, const int64_t *__restrict__ rows, const int64_t *__restrict__ rowptr,
+                             const int64_t *__restrict__ cols, const double *__restrict__ vals,
+                             const int *__restrict__ smap, const double *__restrict__ x, double *__restrict__ y)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nrows; k += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int64_t m = rowptr[k]; m < rowptr[k + 1]; ++m) {
+            double xv;
+            if (smap) {
+                const int g = smap[cols[m]];
+                xv = g >= 0 ? x[g] : -x[~g];
+            } else {
+                xv = x[cols[m]];
+            }
+            s = fma(vals[m], xv, s);
+        }
+        y[rows[k]] += s;
+    }
+}
+
+// d[rows[k]] += the row's diagonal entry (cols[m] == rows[k])
+__global__ void robin_diag_kernel(int64_t nrows, const int64_t *__restrict__ rows,
+                                  const int64_t *__restrict__ rowptr, const int64_t *__restrict__ cols,
+                                  const double *__restrict__ vals, double *__restrict__ d)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nrows; k += (int64_t)gridDim.x * blockDim.x)
+        for (int64_t m = rowptr[k]; m < rowptr[k + 1]; ++m)
+            if (cols[m] == rows[k]) d[rows[k]] += vals[m];
+}
+
 // inv_diag = 1/diag on non-Dirichlet rows, 0 on Dirichlet rows
 __global__ void invert_diag_kernel(int N, double *__restrict__ d)
 {
@@ -991,6 +1024,22 @@ hipError_t launch_add(hipStream_t stream, size_t NG, double *x, const T *d)
 hipError_t launch_sub(hipStream_t stream, size_t n, const double *a, const double *b, double *out)
 {
     NMF_LAUNCH(sub_kernel, grid_for(n), n, a, b, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_robin(hipStream_t stream, int64_t nrows, const int64_t *rows, const int64_t *rowptr,
+                        const int64_t *cols, const double *vals, const int *smap, const double *x, double *y)
+{
+    if (nrows <= 0) return hipSuccess;
+    NMF_LAUNCH(robin_kernel, grid_for((size_t)nrows), nrows, rows, rowptr, cols, vals, smap, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_robin_diag(hipStream_t stream, int64_t nrows, const int64_t *rows, const int64_t *rowptr,
+                             const int64_t *cols, const double *vals, double *d)
+{
+    if (nrows <= 0) return hipSuccess;
+    NMF_LAUNCH(robin_diag_kernel, grid_for((size_t)nrows), nrows, rows, rowptr, cols, vals, d);
     return hipGetLastError();
 }
 

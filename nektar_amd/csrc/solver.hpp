@@ -73,6 +73,13 @@ template <typename T>
 hipError_t launch_add(hipStream_t, size_t NG, double *x, const T *d);
 // out = a - b
 hipError_t launch_sub(hipStream_t, size_t n, const double *a, const double *b, double *out);
+// Robin boundary mass rows (k_solver.hip robin_kernel): y[rows[k]] +=
+// sum vals[m] x(cols[m]), x local (smap NULL) or global through the signed
+// map; and their diagonal entries added to d
+hipError_t launch_robin(hipStream_t, int64_t nrows, const int64_t *rows, const int64_t *rowptr, const int64_t *cols,
+                        const double *vals, const int *smap, const double *x, double *y);
+hipError_t launch_robin_diag(hipStream_t, int64_t nrows, const int64_t *rows, const int64_t *rowptr,
+                             const int64_t *cols, const double *vals, double *d);
 hipError_t launch_invert_diag(hipStream_t, int N, double *d);
 hipError_t launch_sum_partials(hipStream_t, const double *pa, int nb, double *out);
 // state of a mixed solve from dsc[0] = ||b||^2 and dsc[1] = ||r0||^2

@@ -7,7 +7,8 @@ rhs -IProductWRTBase(f) from the GPU operator, the lifted CG solve on the
 GPU with each of the reference's iterative solver choices, BwdTrans on the
 GPU.  The metrics must match the .tst's values within its tolerance
 (L2 1.11818e-07, Linf 5.66374e-07, 1e-12); likewise Helmholtz3D_CG_Tet.tst
-(48 tetrahedra, NUMMODES 5)."""
+(48 tetrahedra, NUMMODES 5) and Helmholtz3D_CG_Hex_AllBCs.tst (Neumann and
+Robin faces)."""
 import os
 
 import pytest
@@ -50,6 +51,43 @@ def test_helmholtz3d_cg_tet(nmf, solver):
     assert info.converged
     assert abs(l2 - 8.458e-05) <= 1e-8, l2
     assert abs(linf - 0.000862348) <= 1e-9, linf
+
+
+ALLBCS = os.path.join(HERE, "golden", "helmholtz3d", "Helmholtz3D_Hex_AllBCs_P6.xml")
+
+
+@pytest.mark.parametrize("solver", ["ConjugateGradient", "ConjugateGradientLoc"])
+def test_helmholtz3d_cg_hex_allbcs(nmf, solver):
+    """Helmholtz3D_CG_Hex_AllBCs.tst: 27 hexahedra, NUMMODES 4, Dirichlet,
+    Neumann and Robin (PRIMCOEFF 2, 3) faces; the reference prints
+    L2 0.000416575 and Linf 0.000871589 (tolerance 1e-12 on six digits).
+    The Robin face mass runs in the GPU matrix multiply, the lifting and
+    the Jacobi diagonal."""
+    from nektar_amd.contfield import helmholtz_solve
+
+    l2, linf, info, _ = helmholtz_solve(ALLBCS, solver=solver, tol=1e-14)
+    print(f"Helmholtz3D_CG_Hex_AllBCs ({solver}): L2 {l2:.10e} Linf {linf:.10e}, {info.iterations} iterations",
+          flush=True)
+    assert info.converged
+    assert f"{l2:.6g}" == "0.000416575", l2
+    assert f"{linf:.6g}" == "0.000871589", linf
+
+
+def test_allbcs_matches_dense_and_rejects_static_condensation(nmf):
+    """The GPU solution equals the oracle's dense solve with the Robin mass;
+    static condensation refuses Robin rows."""
+    import numpy as np
+
+    import test_contfield_cpu as T
+    from nektar_amd import contfield as cf
+
+    s = cf.read_session(ALLBCS)
+    xd, _ = T.dense_solve(s, cf.dirichlet_problem(s))
+    _, _, info, xl = cf.helmholtz_solve(s, tol=1e-14)
+    assert info.converged
+    assert np.max(np.abs(xl - xd)) <= 1e-10 * np.max(np.abs(xd))
+    with pytest.raises(nmf.NektarError):
+        cf.helmholtz_solve(s, solver="IterativeStaticCond", tol=1e-14)
 
 
 def test_trilinear_hexahedra(nmf):
