@@ -122,6 +122,14 @@ def test_mixed_precision_cg(nmf, n, nm, nq, lam, amp, seed):
     r = (b - ax.cpu().numpy())[interior]
     assert np.sqrt(r @ r / (b @ b)) < 1e-10
     assert imx.iterations <= 3 * i64.iterations + 20
+    # error against the FP64 CPU oracle's CG solution of the same system
+    # (the last case is the curved P=7 Q=9 mesh of the mixed-precision
+    # configuration at a size the oracle solves quickly)
+    P = _oracle_problem(n, nm, nq, lam, amp, seed)
+    xo, _ = O.cg_helmholtz_hex(n, nm, nq, P["deformed"], P["jac"], P["df"], lam, P["b"], P["invd"], 1e-10, 5000)
+    err = _rel(s.solution(), xo)
+    print(f"mixed vs FP64 oracle ({n}^3 P={nm - 1} Q={nq}, amp {amp}): max rel diff {err:.3e}", flush=True)
+    assert err <= 1e-7
 
 
 def test_solve_is_bitwise_reproducible(nmf):
