@@ -14,8 +14,8 @@
 // their coalesced geometric factors; here CollectionExpansion carries the
 // same information: one BasisKey per direction, the element count and the
 // geometric factors per element (regular) or per quadrature point
-// (deformed).  Only the MatrixFree implementation of hexahedra and
-// tetrahedra (not nodal) is provided; other keys throw NekError with
+// (deformed).  Only the MatrixFree implementation of hexahedra,
+// tetrahedra and prisms (not nodal) is provided; other keys throw NekError with
 // NMF_ERR_UNSUPPORTED, as the reference's factory fails for keys nobody
 // registered.
 #pragma once

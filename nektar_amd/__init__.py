@@ -32,7 +32,7 @@ __all__ = [
 ]
 
 FP64, FP32 = 0, 1
-SHAPE_HEX, SHAPE_TET = 0, 1
+SHAPE_HEX, SHAPE_TET, SHAPE_PRISM = 0, 1, 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnektar_amd.so")
@@ -181,7 +181,7 @@ def _dptr(a: np.ndarray):
 
 
 def GetOpstring(shape: str, deformed: bool = False) -> str:
-    """``MatrixFree::GetOpstring`` (Operator.cpp:46-90) for Hex/Tet."""
+    """``MatrixFree::GetOpstring`` (Operator.cpp:46-90) for Hex/Tet/Prism."""
     return "_" + shape + ("_Deformed" if deformed else "_Regular")
 
 

@@ -220,25 +220,29 @@ Basis1D make_basis(BasisType t, int nm, int nq)
 
 int num_coeffs(Shape s, int nm)
 {
+    if (s == Shape::Prism) return nm * nm * (nm + 1) / 2;
     return s == Shape::Hex ? nm * nm * nm : nm * (nm + 1) * (nm + 2) / 6;
 }
 
+// the MODIFIED defaults: hex Q, Q, Q; tet Q, Q-1, Q-1; prism Q, Q, Q-1
 void default_points(Shape s, int nq0, int nq[3])
 {
     nq[0] = nq0;
-    nq[1] = s == Shape::Hex ? nq0 : nq0 - 1;
+    nq[1] = s == Shape::Tet ? nq0 - 1 : nq0;
     nq[2] = s == Shape::Hex ? nq0 : nq0 - 1;
 }
 
 BasisType default_basis(Shape s, int dir)
 {
     if (s == Shape::Hex) return BasisType::ModifiedA;
+    if (s == Shape::Prism) return dir == 2 ? BasisType::ModifiedB : BasisType::ModifiedA;
     return dir == 0 ? BasisType::ModifiedA : dir == 1 ? BasisType::ModifiedB : BasisType::ModifiedC;
 }
 
 PointsType default_points_type(Shape s, int dir)
 {
     if (s == Shape::Hex) return PointsType::GLL;
+    if (s == Shape::Prism) return dir == 2 ? PointsType::RadauM_A1B0 : PointsType::GLL;
     return dir == 0 ? PointsType::GLL : dir == 1 ? PointsType::RadauM_A1B0 : PointsType::RadauM_A2B0;
 }
 

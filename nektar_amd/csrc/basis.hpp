@@ -16,7 +16,7 @@ namespace nektar_amd {
 
 enum class PointsType { GLL = 0, RadauM_A1B0 = 1, RadauM_A2B0 = 2 };
 enum class BasisType { ModifiedA = 0, ModifiedB = 1, ModifiedC = 2 };
-enum class Shape { Hex = 0, Tet = 1 };
+enum class Shape { Hex = 0, Tet = 1, Prism = 2 };
 
 struct Points1D {
     int nq = 0;

@@ -12,6 +12,10 @@ int generic_ncoeffs(Shape s, const int nm[3])
 {
     if (s == Shape::Hex) return nm[0] * nm[1] * nm[2];
     int n = 0;
+    if (s == Shape::Prism) {
+        for (int p = 0; p < nm[0]; ++p) n += nm[1] * (nm[2] - p);
+        return n;
+    }
     for (int p = 0; p < nm[0]; ++p)
         for (int q = 0; q < nm[1] - p; ++q) n += nm[2] - p - q;
     return n;
@@ -31,6 +35,9 @@ void generic_check(Shape s, const int nm[3], const int nq[3])
     if (s == Shape::Tet)
         NMF_REQUIRE(nm[0] <= nm[1] && nm[1] <= nm[2], NMF_ERR_UNSUPPORTED,
                     "tet modes must satisfy nm0 <= nm1 <= nm2 (the collapsed Modified_A/B/C expansion)");
+    if (s == Shape::Prism)
+        NMF_REQUIRE(nm[0] <= nm[2], NMF_ERR_UNSUPPORTED,
+                    "prism modes must satisfy nm0 <= nm2 (the collapsed Modified_A/A/B expansion)");
 }
 
 GenLayout generic_layout(Shape s, const int nm[3], const int nq[3])
@@ -53,7 +60,8 @@ GenLayout generic_layout(Shape s, const int nm[3], const int nq[3])
         for (double w : b[d].pts.w) L.tab.push_back(f * w);
     }
     // tet half-space factors h0 = (1+z0)/2, h1 = (1+z1)/2, h2 = 2/(1-z1),
-    // h3 = 2/(1-z2) (HelmholtzKernels.hpp:1368-1389); unused for hexes
+    // h3 = 2/(1-z2) (HelmholtzKernels.hpp:1368-1389); prisms use h0 and h3
+    // (the reference's prism h0, h1, :1390-1403); unused for hexes
     L.oH[0] = (int)L.tab.size();
     for (double z : b[0].pts.z) L.tab.push_back(0.5 * (1.0 + z));
     L.oH[1] = (int)L.tab.size();
@@ -61,7 +69,7 @@ GenLayout generic_layout(Shape s, const int nm[3], const int nq[3])
     L.oH[2] = (int)L.tab.size();
     for (double z : b[1].pts.z) L.tab.push_back(s == Shape::Tet ? 2.0 / (1.0 - z) : 0.0);
     L.oH[3] = (int)L.tab.size();
-    for (double z : b[2].pts.z) L.tab.push_back(s == Shape::Tet ? 2.0 / (1.0 - z) : 0.0);
+    for (double z : b[2].pts.z) L.tab.push_back(s != Shape::Hex ? 2.0 / (1.0 - z) : 0.0);
 
     const int m0 = nm[0], m1 = nm[1], m2 = nm[2], n0 = nq[0], n1 = nq[1], n2 = nq[2];
     L.nqtot = n0 * n1 * n2;
@@ -69,6 +77,29 @@ GenLayout generic_layout(Shape s, const int nm[3], const int nq[3])
     size_t buf = (size_t)L.nqtot;
     if (s == Shape::Hex) {
         buf = std::max({buf, (size_t)m2 * m1 * n0, (size_t)m2 * n1 * n0});
+    } else if (s == Shape::Prism) {
+        // columns (p, q) in coefficient order (p-major): first coefficient,
+        // first Modified_B mode of (p, r = 0), r count nm2 - p, p; per
+        // coefficient its p, column and Modified_B mode (p, r)
+        std::vector<int> cP, cPair, cM2, pC0, pM20, pNR, pP;
+        int c = 0, col = 0, mode_pr = 0;
+        for (int p = 0; p < m0; ++p) {
+            for (int q = 0; q < m1; ++q, ++col) {
+                pC0.push_back(c);
+                pM20.push_back(mode_pr);
+                pNR.push_back(m2 - p);
+                pP.push_back(p);
+                for (int r = 0; r < m2 - p; ++r, ++c) {
+                    cP.push_back(p);
+                    cPair.push_back(col);
+                    cM2.push_back(mode_pr + r);
+                }
+            }
+            mode_pr += m2 - p;
+        }
+        L.npq = col;
+        for (auto *v : {&cP, &cPair, &cM2, &pC0, &pM20, &pNR, &pP}) L.itab.insert(L.itab.end(), v->begin(), v->end());
+        buf = std::max({buf, (size_t)L.npq * n2, (size_t)m0 * n2 * n1});
     } else {
         // coefficient lists: p, (p,q) pair and Modified_C mode of each
         // coefficient; pair lists: first coefficient, first mode, r count, p

@@ -48,13 +48,15 @@ inline size_t lds_bytes(const GenArgs &a) { return (size_t)4 * a.buf * (a.fp32 ?
 // tet index tables and the sizes the kernels need.
 struct GenLayout {
     std::vector<double> tab;
-    std::vector<int> itab;  // tets: cP | cPair | cM2 | pC0 | pM20 | pNR | pP
+    std::vector<int> itab;  // tets, prisms: cP | cPair | cM2 | pC0 | pM20 | pNR | pP
     int oB[3], oD[3], oW[3], oH[4];
     int npq = 0, buf = 0, nqtot = 0, ncoeff = 0;
 };
 
 // Coefficients of the expansion: hex nm0*nm1*nm2; tet with nm0 <= nm1 <= nm2:
-// p < nm0, q < nm1 - p, r < nm2 - p - q (BwdTransKernels.hpp:244-266).
+// p < nm0, q < nm1 - p, r < nm2 - p - q (BwdTransKernels.hpp:244-266);
+// prism with nm0 <= nm2: p < nm0, q < nm1, r < nm2 - p
+// (StdPrismExp.cpp:1947-1956, BwdTransKernels.hpp:345-360).
 int generic_ncoeffs(Shape s, const int nm[3]);
 // Throws NMF_ERR_UNSUPPORTED for shapes the runtime-size kernels do not take.
 void generic_check(Shape s, const int nm[3], const int nq[3]);

@@ -34,6 +34,9 @@ using std::fma;
 namespace nektar_amd {
 namespace gen {
 
+// element shapes of the stages below (Shape's values)
+constexpr int SH_HEX = 0, SH_TET = 1, SH_PRISM = 2;
+
 template <typename T>
 struct Ctx {
     const GenArgs &a;
@@ -241,6 +244,76 @@ NMF_GD void tet_reverse(const Ctx<T> &c, const TetIdx &ix, const T *Y, T *X1, T 
     }
 }
 
+// ---- prism transforms (BwdTransKernels.hpp:334-405, IProductKernels.hpp:519-655)
+// The same index tables as the tet (TetIdx): columns (p, q) in coefficient
+// order with their first coefficient, first Modified_B mode (p, r = 0),
+// r count nm2 - p and p; per coefficient its p, column and Modified_B mode.
+// The collapsed-edge modes (0, q, 1) carry an extra B0_1 B1_q B2_1 term
+// (the reference's `correct`), which enters column (1, q) with B2 mode 1.
+template <typename T>
+NMF_GD void prism_forward(const Ctx<T> &c, const TetIdx &ix, const T *__restrict__ x, T *X1, T *X2, T *dst)
+{
+    const int m0 = c.a.nm[0], m1 = c.a.nm[1], m2 = c.a.nm[2], npq = c.a.npq;
+    const int n0 = c.a.nq[0], n1 = c.a.nq[1], n2 = c.a.nq[2];
+    // fpq(k) = sum_r B2(mode(p, r), k) x(p, q, r) (+ the (0, q, 1) term for p = 1)
+    for (int o = gtid(); o < npq * n2; o += gnt()) {
+        const int k = o % n2, col = o / n2;
+        const int c0 = ix.pC0[col], mc = ix.pM20[col], nr = ix.pNR[col], p = ix.pP[col];
+        T s = T(0);
+        for (int r = 0; r < nr; ++r) s = fma(c.B(2, mc + r, k), x[c0 + r], s);
+        if (p == 1) s = fma(c.B(2, 1, k), x[(col - m1) * m2 + 1], s);
+        X1[o] = s;  // [col*n2 + k]
+    }
+    gsync();
+    // fp(j, k) = sum_q B1(q, j) fpq(k)
+    for (int o = gtid(); o < m0 * n2 * n1; o += gnt()) {
+        const int j = o % n1, k = (o / n1) % n2, p = o / (n1 * n2);
+        T s = T(0);
+        for (int q = 0; q < m1; ++q) s = fma(c.B(1, q, j), X1[(p * m1 + q) * n2 + k], s);
+        X2[o] = s;  // [(p*n2 + k)*n1 + j]
+    }
+    gsync();
+    for (int o = gtid(); o < n2 * n1 * n0; o += gnt()) {
+        const int i = o % n0, j = (o / n0) % n1, k = o / (n0 * n1);
+        T s = T(0);
+        for (int p = 0; p < m0; ++p) s = fma(c.B(0, p, i), X2[(p * n2 + k) * n1 + j], s);
+        dst[o] = s;
+    }
+}
+
+template <typename T>
+NMF_GD void prism_reverse(const Ctx<T> &c, const TetIdx &ix, const T *Y, T *X1, T *X2, T *__restrict__ out)
+{
+    const int m0 = c.a.nm[0], m1 = c.a.nm[1], npq = c.a.npq, nc = c.a.ncoeff;
+    const int n0 = c.a.nq[0], n1 = c.a.nq[1], n2 = c.a.nq[2];
+    // A_p(j, k) = sum_i B0(p, i) y(i, j, k)
+    for (int o = gtid(); o < m0 * n2 * n1; o += gnt()) {
+        const int j = o % n1, k = (o / n1) % n2, p = o / (n1 * n2);
+        T s = T(0);
+        for (int i = 0; i < n0; ++i) s = fma(c.B(0, p, i), Y[(k * n1 + j) * n0 + i], s);
+        X1[o] = s;  // [(p*n2 + k)*n1 + j]
+    }
+    gsync();
+    // A_pq(k) = sum_j B1(q, j) A_p(j, k)
+    for (int o = gtid(); o < npq * n2; o += gnt()) {
+        const int k = o % n2, col = o / n2, p = ix.pP[col], q = col - p * m1;
+        T s = T(0);
+        for (int j = 0; j < n1; ++j) s = fma(c.B(1, q, j), X1[(p * n2 + k) * n1 + j], s);
+        X2[o] = s;  // [col*n2 + k]
+    }
+    gsync();
+    // out(p, q, r) = sum_k B2(mode(p, r), k) A_pq(k); (0, q, 1) also takes
+    // sum_k B2(1, k) A_1q(k)
+    for (int o = gtid(); o < nc; o += gnt()) {
+        const int col = ix.cPair[o], mc = ix.cM2[o];
+        T s = T(0);
+        for (int k = 0; k < n2; ++k) s = fma(c.B(2, mc, k), X2[col * n2 + k], s);
+        if (ix.cP[o] == 0 && mc == 1)
+            for (int k = 0; k < n2; ++k) s = fma(c.B(2, 1, k), X2[(m1 + col) * n2 + k], s);
+        out[o] = s;
+    }
+}
+
 // ---- pointwise stages -------------------------------------------------
 // geometry of point q of element e: regular g[e*G + c], deformed
 // g[(e*G + c)*nqtot + q]
@@ -264,7 +337,7 @@ NMF_GFI Half<T> half_at(const Ctx<T> &c, int i, int j, int k)
 // Helmholtz / LinearAdvectionDiffusionReaction pointwise stage: in X0 = u,
 // X1..X3 = derivatives along the reference (collapsed) directions; out
 // X0 = weighted mass integrand, X1..X3 = weighted flux integrands.
-template <typename T, bool TET>
+template <typename T, int SH>
 NMF_GD void helmholtz_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T *X3)
 {
     const GenArgs &a = c.a;
@@ -284,7 +357,7 @@ NMF_GD void helmholtz_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T *
             m = fma(ac[2 * a.nqtot], a2, m);
         }
         T t0, t1, t2, mass;
-        if constexpr (!TET) {
+        if constexpr (SH == SH_HEX) {
             // packed J m_de (regular, weights applied here) or W J m_de
             T gg[7];
             for (int cc = 0; cc < 7; ++cc) gg[cc] = geo_at(a, g, 7, e, cc, q);
@@ -305,6 +378,47 @@ NMF_GD void helmholtz_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T *
             } else {
                 mass = (lambda * gg[6]) * m;
             }
+        } else if constexpr (SH == SH_PRISM) {
+            // DiffusionCoeffPrismKernel (HelmholtzKernels.hpp:918-1119):
+            // s_c = h1 (h0 df[3c+2] + df[3c]), h0 = (1+z0)/2, h1 = 2/(1-z2)
+            T f[9];
+            for (int cc = 0; cc < 9; ++cc) f[cc] = geo_at(a, g, 10, e, cc, q);
+            const T jw = geo_at(a, g, 10, e, 9, q) * w;
+            const T h0 = c.H(0, i), h1 = c.H(3, k);
+            const T s1 = h1 * fma(h0, f[2], f[0]);
+            const T s2 = h1 * fma(h0, f[5], f[3]);
+            const T s3 = h1 * fma(h0, f[8], f[6]);
+            T g0 = s1 * s1;
+            g0 = fma(s2, s2, g0);
+            g0 = fma(s3, s3, g0);
+            T g3 = f[1] * s1;
+            g3 = fma(f[4], s2, g3);
+            g3 = fma(f[7], s3, g3);
+            T g4 = f[2] * s1;
+            g4 = fma(f[5], s2, g4);
+            g4 = fma(f[8], s3, g4);
+            T g1 = f[1] * f[1];
+            g1 = fma(f[4], f[4], g1);
+            g1 = fma(f[7], f[7], g1);
+            T g2 = f[2] * f[2];
+            g2 = fma(f[5], f[5], g2);
+            g2 = fma(f[8], f[8], g2);
+            T g5 = f[1] * f[2];
+            g5 = fma(f[4], f[5], g5);
+            g5 = fma(f[7], f[8], g5);
+            t0 = g0 * a0;
+            t0 = fma(g3, a1, t0);
+            t0 = fma(g4, a2, t0);
+            t1 = g3 * a0;
+            t1 = fma(g1, a1, t1);
+            t1 = fma(g5, a2, t1);
+            t2 = g4 * a0;
+            t2 = fma(g5, a1, t2);
+            t2 = fma(g2, a2, t2);
+            t0 *= jw;
+            t1 *= jw;
+            t2 *= jw;
+            mass = lambda * (jw * m);
         } else {
             // DiffusionCoeffTetKernel (HelmholtzKernels.hpp:700-913), metric
             // g0..g5 of the collapsed derivatives
@@ -367,7 +481,7 @@ NMF_GD void helmholtz_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T *
 // PhysDeriv pointwise stage: collapsed chain rule for tets
 // (PhysDerivKernels.hpp:198-259) and d/dx_j = sum_d df[j*3+d] d/dxi_d
 // (:326-340); writes the three outputs of element e.
-template <typename T, bool TET>
+template <typename T, int SH>
 NMF_GD void physderiv_pointwise(const Ctx<T> &c, int e, const T *X1, const T *X2, const T *X3)
 {
     const GenArgs &a = c.a;
@@ -378,7 +492,13 @@ NMF_GD void physderiv_pointwise(const Ctx<T> &c, int e, const T *X1, const T *X2
     T *o2 = static_cast<T *>(a.out2) + (size_t)e * a.nqtot;
     for (int q = gtid(); q < a.nqtot; q += gnt()) {
         T d0 = X1[q], d1 = X2[q], d2 = X3[q];
-        if constexpr (TET) {
+        if constexpr (SH == SH_PRISM) {
+            // PhysDerivKernels.hpp:293-301: d0 *= 2/(1-z2), d2 += (1+z0)/2 d0
+            const int i = q % n0, k = q / (n0 * n1);
+            d0 = d0 * c.H(3, k);
+            d2 = fma(c.H(0, i), d0, d2);
+        }
+        if constexpr (SH == SH_TET) {
             const int i = q % n0, j = (q / n0) % n1, k = q / (n0 * n1);
             const Half<T> h = half_at(c, i, j, k);
             d0 = d0 * (h.h2 * h.h3);
@@ -407,7 +527,7 @@ NMF_GD void physderiv_pointwise(const Ctx<T> &c, int e, const T *X1, const T *X2
 // IProductWRTDerivBase pointwise stage (IProductWRTDerivBaseKernels.hpp:
 // 194-335): t_d = sum_j df[j*3+d] in_j, mapped to the collapsed directions
 // for tets, weighted; X0 = 0.
-template <typename T, bool TET>
+template <typename T, int SH>
 NMF_GD void iprod_deriv_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T *X3)
 {
     const GenArgs &a = c.a;
@@ -421,7 +541,7 @@ NMF_GD void iprod_deriv_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T
         const T w = c.W(0, i) * c.W(1, j) * c.W(2, k);
         const T f0 = in0[q], f1 = in1[q], f2 = in2[q];
         T t0, t1, t2;
-        if constexpr (!TET) {
+        if constexpr (SH == SH_HEX) {
             // packed J df (regular, weights here) or W J df
             T gg[9];
             for (int cc = 0; cc < 9; ++cc) gg[cc] = geo_at(a, g, 9, e, cc, q);
@@ -446,11 +566,16 @@ NMF_GD void iprod_deriv_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T
             t0 = d[0] * f0 + d[3] * f1 + d[6] * f2;
             t1 = d[1] * f0 + d[4] * f1 + d[7] * f2;
             t2 = d[2] * f0 + d[5] * f1 + d[8] * f2;
-            const Half<T> h = half_at(c, i, j, k);
-            t0 = fma(t1 + t2, h.h0, t0);
-            t0 = t0 * (h.h2 * h.h3);
-            t1 = fma(t2, h.h1, t1);
-            t1 = t1 * h.h3;
+            if constexpr (SH == SH_PRISM) {
+                // IProductWRTDerivBaseKernels.hpp:391-403: t0 <- h1 (t0 + h0 t2)
+                t0 = c.H(3, k) * fma(c.H(0, i), t2, t0);
+            } else {
+                const Half<T> h = half_at(c, i, j, k);
+                t0 = fma(t1 + t2, h.h0, t0);
+                t0 = t0 * (h.h2 * h.h3);
+                t1 = fma(t2, h.h1, t1);
+                t1 = t1 * h.h3;
+            }
             t0 *= jw;
             t1 *= jw;
             t2 *= jw;
@@ -463,7 +588,7 @@ NMF_GD void iprod_deriv_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T
 }
 
 // IProductWRTBase pointwise stage: y = W J f
-template <typename T, bool TET>
+template <typename T, int SH>
 NMF_GD void iprod_pointwise(const Ctx<T> &c, int e, T *X0)
 {
     const GenArgs &a = c.a;
@@ -480,17 +605,18 @@ NMF_GD void iprod_pointwise(const Ctx<T> &c, int e, T *X0)
 }
 
 // ---- one element -------------------------------------------------------
-template <typename T, bool TET>
+template <typename T, int SH>
 NMF_GD void element(const GenArgs &a, int e, T *X0, T *X1, T *X2, T *X3)
 {
     const Ctx<T> c{a, static_cast<const T *>(a.tab)};
-    const TetIdx ix = TET ? TetIdx(a.itab, a.ncoeff, a.npq) : TetIdx();
+    const TetIdx ix = SH != SH_HEX ? TetIdx(a.itab, a.ncoeff, a.npq) : TetIdx();
     const OpType op = a.op;
     const bool coeff_in = op == OpType::BwdTrans || op == OpType::Helmholtz || op == OpType::AdvDiffReact;
     if (coeff_in) {
         const T *x = static_cast<const T *>(a.in0) + (size_t)e * a.ncoeff;
         T *dst = op == OpType::BwdTrans ? static_cast<T *>(a.out0) + (size_t)e * a.nqtot : X0;
-        if constexpr (TET) tet_forward(c, ix, x, X1, X2, dst);
+        if constexpr (SH == SH_TET) tet_forward(c, ix, x, X1, X2, dst);
+        else if constexpr (SH == SH_PRISM) prism_forward(c, ix, x, X1, X2, dst);
         else hex_forward(c, x, X1, X2, dst);
         if (op == OpType::BwdTrans) return;
     } else if (op == OpType::PhysDeriv) {
@@ -499,25 +625,26 @@ NMF_GD void element(const GenArgs &a, int e, T *X0, T *X1, T *X2, T *X3)
     }
     gsync();
     if (op == OpType::IProduct) {
-        iprod_pointwise<T, TET>(c, e, X0);
+        iprod_pointwise<T, SH>(c, e, X0);
     } else if (op == OpType::IProductDeriv) {
-        iprod_deriv_pointwise<T, TET>(c, e, X0, X1, X2, X3);
+        iprod_deriv_pointwise<T, SH>(c, e, X0, X1, X2, X3);
         gsync();
         transposed_derivatives(c, X0, X1, X2, X3);
     } else {
         derivatives(c, X0, X1, X2, X3);
         gsync();
         if (op == OpType::PhysDeriv) {
-            physderiv_pointwise<T, TET>(c, e, X1, X2, X3);
+            physderiv_pointwise<T, SH>(c, e, X1, X2, X3);
             return;
         }
-        helmholtz_pointwise<T, TET>(c, e, X0, X1, X2, X3);
+        helmholtz_pointwise<T, SH>(c, e, X0, X1, X2, X3);
         gsync();
         transposed_derivatives(c, X0, X1, X2, X3);
     }
     gsync();
     T *out = static_cast<T *>(a.out0) + (size_t)e * a.ncoeff;
-    if constexpr (TET) tet_reverse(c, ix, X0, X1, X2, X3, out);
+    if constexpr (SH == SH_TET) tet_reverse(c, ix, X0, X1, X2, X3, out);
+    else if constexpr (SH == SH_PRISM) prism_reverse(c, ix, X0, X1, X2, out);
     else hex_reverse(c, X0, X1, X2, out);
 }
 

@@ -31,36 +31,36 @@ namespace gen {
 constexpr int kThreads = 256;
 
 // Intermediates in LDS: one element per workgroup.
-template <typename T, bool TET>
+template <typename T, int SH>
 __global__ __launch_bounds__(kThreads) void lds_kernel(GenArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *X = reinterpret_cast<T *>(smem);
-    element<T, TET>(a, blockIdx.x, X, X + a.buf, X + 2 * a.buf, X + 3 * a.buf);
+    element<T, SH>(a, blockIdx.x, X, X + a.buf, X + 2 * a.buf, X + 3 * a.buf);
 }
 
 // Intermediates in the global workspace: a.groups workgroups walk the
 // elements, each with its own four buffers.
-template <typename T, bool TET>
+template <typename T, int SH>
 __global__ __launch_bounds__(kThreads) void global_kernel(GenArgs a)
 {
     T *X = static_cast<T *>(a.ws) + (size_t)blockIdx.x * 4 * a.buf;
     for (int e = blockIdx.x; e < a.nelmt; e += gridDim.x) {
-        element<T, TET>(a, e, X, X + a.buf, X + 2 * a.buf, X + 3 * a.buf);
+        element<T, SH>(a, e, X, X + a.buf, X + 2 * a.buf, X + 3 * a.buf);
         __syncthreads();  // the buffers are reused by the next element
     }
 }
 
-template <typename T, bool TET>
+template <typename T, int SH>
 hipError_t launch_t(const GenArgs &a)
 {
     if (a.nelmt == 0) return hipSuccess;
     const size_t lds = lds_bytes(a);
     if (lds <= kMaxLds) {
-        hipLaunchKernelGGL((lds_kernel<T, TET>), dim3(a.nelmt), dim3(kThreads), lds, a.stream, a);
+        hipLaunchKernelGGL((lds_kernel<T, SH>), dim3(a.nelmt), dim3(kThreads), lds, a.stream, a);
     } else {
         if (!a.ws) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((global_kernel<T, TET>), dim3(a.groups), dim3(kThreads), 0, a.stream, a);
+        hipLaunchKernelGGL((global_kernel<T, SH>), dim3(a.groups), dim3(kThreads), 0, a.stream, a);
     }
     return hipGetLastError();
 }
@@ -69,8 +69,11 @@ hipError_t launch_t(const GenArgs &a)
 
 hipError_t launch_generic(const GenArgs &a)
 {
-    if (a.shape == Shape::Hex) return a.fp32 ? gen::launch_t<float, false>(a) : gen::launch_t<double, false>(a);
-    return a.fp32 ? gen::launch_t<float, true>(a) : gen::launch_t<double, true>(a);
+    if (a.shape == Shape::Hex)
+        return a.fp32 ? gen::launch_t<float, gen::SH_HEX>(a) : gen::launch_t<double, gen::SH_HEX>(a);
+    if (a.shape == Shape::Prism)
+        return a.fp32 ? gen::launch_t<float, gen::SH_PRISM>(a) : gen::launch_t<double, gen::SH_PRISM>(a);
+    return a.fp32 ? gen::launch_t<float, gen::SH_TET>(a) : gen::launch_t<double, gen::SH_TET>(a);
 }
 
 }  // namespace nektar_amd

@@ -31,7 +31,8 @@ void parse_operator_key(const std::string &key, OpType &op, Shape &shape, bool &
                                              "Helmholtz, LinearAdvectionDiffusionReaction, PhysInterp1DScaled)");
     if (s == "Hex") shape = Shape::Hex;
     else if (s == "Tet") shape = Shape::Tet;
-    else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet)");
+    else if (s == "Prism") shape = Shape::Prism;
+    else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet, Prism)");
     if (g == "Regular") deformed = false;
     else if (g == "Deformed") deformed = true;
     else throw Error(NMF_ERR_INVALID, "geometry tag '" + g + "' must be Regular or Deformed");
@@ -127,7 +128,10 @@ void MFOperator::init(const std::string &key, const int nummodes[3], const int n
     int dq[3];
     default_points(shape, nq[0], dq);
     const bool iso = nmd[1] == nm && nmd[2] == nm && nq[1] == dq[1] && nq[2] == dq[2];
-    const bool compiled = iso && (shape == Shape::Hex ? hex_op_supported(nm, nq[0]) : tet_supported(nm, nq[0]));
+    // (prisms always run the runtime-size kernels)
+    const bool compiled = iso && (shape == Shape::Hex   ? hex_op_supported(nm, nq[0])
+                                  : shape == Shape::Tet ? tet_supported(nm, nq[0])
+                                                        : false);
     if (compiled) {
         ncoeff = num_coeffs(shape, nm);
         tab.upload(build_tables(shape, nm, nq[0]), fp32);
@@ -426,10 +430,10 @@ std::vector<double> MFOperator::pack_advection() const
 {
     std::vector<double> c((size_t)nelmt * 3 * nqtot);
     std::vector<double> z0, z1, z2;
-    if (shape == Shape::Tet) {
-        z0 = make_points(PointsType::GLL, nq[0]).z;
-        z1 = make_points(PointsType::RadauM_A1B0, nq[1]).z;
-        z2 = make_points(PointsType::RadauM_A2B0, nq[2]).z;
+    if (shape != Shape::Hex) {
+        z0 = make_points(default_points_type(shape, 0), nq[0]).z;
+        z1 = make_points(default_points_type(shape, 1), nq[1]).z;
+        z2 = make_points(default_points_type(shape, 2), nq[2]).z;
     }
     for (int e = 0; e < nelmt; ++e)
         for (int k = 0, q = 0; k < nq[2]; ++k)
@@ -447,6 +451,12 @@ std::vector<double> MFOperator::pack_advection() const
                         a[0] = h2 * h3 * (a0 + h0 * (a1 + a2));
                         a[1] = h3 * (a1 + h1 * a2);
                         a[2] = a2;
+                    } else if (shape == Shape::Prism) {
+                        // AdvectionPrismKernel (LinearAdvectionDiffusionReactionKernels.hpp:
+                        // 384-470): d/dxi0 = h1 d0, d/dxi2 = h0 h1 d0 + d2 with
+                        // h0 = (1+z0)/2, h1 = 2/(1-z2)
+                        const double h0 = 0.5 * (1.0 + z0[i]), h1 = 2.0 / (1.0 - z2[k]);
+                        a[0] = h1 * (a[0] + h0 * a[2]);
                     }
                     for (int d = 0; d < 3; ++d) c[((size_t)e * 3 + d) * nqtot + q] = a[d];
                 }

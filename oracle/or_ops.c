@@ -9,8 +9,12 @@
  * TEST INFRASTRUCTURE ONLY (parity oracle and CPU baseline).
  *
  * Sources restated (library/MatrixFreeOps/):
- *   BwdTransKernels.hpp:160-227 (hex), :231-330 (tet)
- *   IProductKernels.hpp:44-64 (ScaleAppend), :270-350 (hex), :354-514 (tet)
+ *   BwdTransKernels.hpp:160-227 (hex), :231-330 (tet), :334-405 (prism)
+ *   IProductKernels.hpp:44-64 (ScaleAppend), :270-350 (hex), :354-514 (tet),
+ *                       :519-655 (prism)
+ *   HelmholtzKernels.hpp:918-1119 (prism metric), :1390-1403 (prism h0, h1)
+ *   IProductWRTDerivBaseKernels.hpp:338-411 (prism),
+ *   LinearAdvectionDiffusionReactionKernels.hpp:384-470 (prism)
  *   PhysDerivKernels.hpp:184-345 (PhysDeriv3DKernel), :436-507 (tensor)
  *   HelmholtzKernels.hpp:389-696 (hex metric), :700-913 (tet metric),
  *                         :1354-1425 (tet half-space factors h0..h3)
@@ -60,9 +64,21 @@ static int tet_ncoeffs3(int nm0, int nm1, int nm2)
     return n;
 }
 
+/* Coefficients of a prism with nm0 <= nm2: p < nm0, q < nm1, r < nm2 - p
+ * (StdPrismExp::GetMode, StdPrismExp.cpp:1947-1956; the order
+ * BwdTransKernels.hpp:345-360 walks). */
+static int prism_ncoeffs3(int nm0, int nm1, int nm2)
+{
+    int n = 0;
+    for (int p = 0; p < nm0; ++p) n += nm1 * (nm2 - p);
+    return n;
+}
+
 int or_ncoeffs_aniso(int shape, const int nm[3])
 {
-    return shape == OR_HEX ? nm[0] * nm[1] * nm[2] : tet_ncoeffs3(nm[0], nm[1], nm[2]);
+    return shape == OR_HEX   ? nm[0] * nm[1] * nm[2]
+           : shape == OR_TET ? tet_ncoeffs3(nm[0], nm[1], nm[2])
+                             : prism_ncoeffs3(nm[0], nm[1], nm[2]);
 }
 
 static void tables_init_full(tables_t *t, int shape, const int nm[3], const int nq[3])
@@ -75,9 +91,11 @@ static void tables_init_full(tables_t *t, int shape, const int nm[3], const int 
     }
     t->nqtot = t->nq[0] * t->nq[1] * t->nq[2];
     t->ncoeff = or_ncoeffs_aniso(shape, nm);
+    /* prism: Modified_A, Modified_A, Modified_B on GLL, GLL, Radau(1,0) */
+    static const int prism_bt[3] = {OR_MOD_A, OR_MOD_A, OR_MOD_B}, prism_pt[3] = {OR_GLL, OR_GLL, OR_RADAU_A1};
     for (int d = 0; d < 3; ++d) {
-        int bt = shape == OR_HEX ? OR_MOD_A : d;
-        int pt = shape == OR_HEX ? OR_GLL : d;
+        int bt = shape == OR_HEX ? OR_MOD_A : shape == OR_TET ? d : prism_bt[d];
+        int pt = shape == OR_HEX ? OR_GLL : shape == OR_TET ? d : prism_pt[d];
         int nq = t->nq[d];
         int nmodes = or_basis_nmodes(bt, nm[d]);
         t->b[d] = (double *)malloc((size_t)nmodes * nq * sizeof(double));
@@ -98,6 +116,12 @@ static void tables_init_full(tables_t *t, int shape, const int nm[3], const int 
             t->h1[j] = 0.5 * (1 + t->Z[1][j]);
             t->h2[j] = 2.0 / (1 - t->Z[1][j]);
         }
+        for (int k = 0; k < t->nq[2]; ++k) t->h3[k] = 2.0 / (1 - t->Z[2][k]);
+    }
+    if (shape == OR_PRISM) {
+        /* HelmholtzKernels.hpp:1390-1403 (prism): h0[i] = (1 + z0)/2 and
+         * h1[k] = 2/(1 - z2), kept here in h0 and h3 */
+        for (int i = 0; i < t->nq[0]; ++i) t->h0[i] = 0.5 * (1 + t->Z[0][i]);
         for (int k = 0; k < t->nq[2]; ++k) t->h3[k] = 2.0 / (1 - t->Z[2][k]);
     }
 }
@@ -195,6 +219,48 @@ static void bwd_tet(int nm0, int nm1, int nm2, int nq0, int nq1, int nq2,
                     }
                 }
                 out[cnt_kji] = tmp;
+            }
+        }
+    }
+}
+
+/* BwdTransKernels.hpp:334-405 (BwdTransPrismKernel): B2 indexed by the
+ * (p, r) pair, the collapsed-edge correction adds mode (0, q, 1) times
+ * B0_1 B1_q B2_1 */
+static void bwd_prism(int nm0, int nm1, int nm2, int nq0, int nq1, int nq2,
+                      int correct, const vd *in, const double *b0,
+                      const double *b1, const double *b2, vd *fpq, vd *fp,
+                      vd *out)
+{
+    for (int k = 0, cnt_kji = 0; k < nq2; ++k) {
+        int mode_pqr = 0, mode_pq = 0, mode_pr = 0;
+        for (int p = 0; p < nm0; ++p) {
+            for (int q = 0; q < nm1; ++q, ++mode_pq) {
+                vd prod = vz();
+                for (int r = 0; r < nm2 - p; ++r, ++mode_pqr)
+                    prod = vfmas(prod, in[mode_pqr], b2[(mode_pr + r) * nq2 + k]);
+                fpq[mode_pq] = prod;
+            }
+            mode_pr += nm2 - p;
+        }
+        for (int j = 0; j < nq1; ++j) {
+            mode_pq = 0;
+            for (int p = 0; p < nm0; ++p) {
+                vd prod = vz();
+                for (int q = 0; q < nm1; ++q, ++mode_pq) prod = vfmas(prod, fpq[mode_pq], b1[q * nq1 + j]);
+                fp[p] = prod;
+            }
+            for (int i = 0; i < nq0; ++i, ++cnt_kji) {
+                vd val = vz();
+                for (int p = 0; p < nm0; ++p) val = vfmas(val, fp[p], b0[p * nq0 + i]);
+                if (correct) {
+                    const double basis_2 = b2[nq2 + k], basis_0 = b0[nq0 + i];
+                    for (int q = 0; q < nm1; ++q) {
+                        const double basis_1 = b1[q * nq1 + j];
+                        val = vfma(val, vs(basis_2 * basis_1), vmuls(in[q * nm2 + 1], basis_0));
+                    }
+                }
+                out[cnt_kji] = val;
             }
         }
     }
@@ -327,6 +393,65 @@ static void iprod_tet(int SCALE, int APPEND, int DEF, int nm0, int nm1,
     }
 }
 
+/* IProductKernels.hpp:519-655 (IProductPrismKernel) */
+static void iprod_prism(int SCALE, int APPEND, int DEF, int nm0, int nm1,
+                        int nm2, int nq0, int nq1, int nq2, int correct,
+                        const vd *in, const double *b0, const double *b1,
+                        const double *b2, const double *w0, const double *w1,
+                        const double *w2, const vd *jac, vd *sums_kj,
+                        vd *sums_k, vd *out, double scale)
+{
+    int mode_pr = 0, mode_pqr = 0;
+    for (int p = 0; p < nm0; ++p) {
+        int cnt_kji = 0, cnt_kj = 0;
+        for (int k = 0; k < nq2; ++k)
+            for (int j = 0; j < nq1; ++j, ++cnt_kj) {
+                vd sum_kj = vz();
+                for (int i = 0; i < nq0; ++i, ++cnt_kji) {
+                    vd jv = DEF ? jac[nq0 * nq1 * k + nq0 * j + i] : jac[0];
+                    vd prod = vmuls(vmuls(jv, b0[nq0 * p + i]), w0[i]);
+                    sum_kj = vfma(sum_kj, prod, in[cnt_kji]);
+                }
+                sums_kj[cnt_kj] = sum_kj;
+            }
+        for (int q = 0; q < nm1; ++q) {
+            cnt_kj = 0;
+            for (int k = 0; k < nq2; ++k) {
+                vd sum_k = vz();
+                for (int j = 0; j < nq1; ++j, ++cnt_kj)
+                    sum_k = vfmas(sum_k, sums_kj[cnt_kj], b1[q * nq1 + j] * w1[j]);
+                sums_k[k] = sum_k;
+            }
+            for (int r = 0; r < nm2 - p; ++r, ++mode_pqr) {
+                vd sum_k = vz();
+                for (int k = 0; k < nq2; ++k) sum_k = vfmas(sum_k, sums_k[k], b2[(mode_pr + r) * nq2 + k] * w2[k]);
+                scale_append(SCALE, APPEND, &out[mode_pqr], sum_k, scale);
+            }
+        }
+        mode_pr += nm2 - p;
+    }
+    if (correct) {
+        /* singular edge: phi_1q1 added to phi_0q1 */
+        vd corr[64];
+        for (int q = 0; q < nm1; ++q) corr[q] = vz();
+        for (int k = 0, cnt = 0; k < nq2; ++k) {
+            vd kw = vs(w2[k]);
+            if (!DEF) kw = vmul(kw, jac[0]);
+            for (int j = 0; j < nq1; ++j) {
+                vd kjw = vmuls(kw, w1[j]);
+                for (int i = 0; i < nq0; ++i, ++cnt) {
+                    vd prod = vmul(vmuls(kjw, w0[i]), in[cnt]);
+                    if (DEF) prod = vmul(prod, jac[k * nq1 * nq0 + j * nq0 + i]);
+                    const double basis_2 = b2[nq2 + k], basis_0 = b0[nq0 + i];
+                    for (int q = 0; q < nm1; ++q)
+                        corr[q] = vfma(corr[q], vs(basis_2 * b1[q * nq1 + j]), vmuls(prod, basis_0));
+                }
+            }
+        }
+        for (int q = 0; q < nm1; ++q) scale_append(SCALE, 1, &out[nm2 * q + 1], corr[q], scale);
+    }
+}
+
 /* PhysDerivKernels.hpp:436-507 (column-major D: D[k*nq+i]) */
 static void deriv_tensor(int nq0, int nq1, int nq2, const vd *in,
                          const double *D0, const double *D1, const double *D2,
@@ -394,6 +519,17 @@ static void physderiv_chain(int shape, int DEF, int nq0, int nq1, int nq2,
                     d2[e] = vadd(o, d2[e]);
                 }
             }
+    }
+    if (shape == OR_PRISM) {
+        /* PhysDerivKernels.hpp:293-301: d0 *= 2/(1 - z2), d2 += (1 + z0)/2 d0 */
+        for (int k = 0, e = 0; k < nq2; ++k) {
+            const double x2 = 2.0 / (1.0 - Z2[k]);
+            for (int j = 0; j < nq1; ++j)
+                for (int i = 0; i < nq0; ++i, ++e) {
+                    d0[e] = vmuls(d0[e], x2);
+                    d2[e] = vfmas(d2[e], d0[e], 0.5 * (1.0 + Z0[i]));
+                }
+        }
     }
     vd f[9];
     if (!DEF) for (int d = 0; d < 9; ++d) f[d] = df[d];
@@ -520,6 +656,70 @@ static void diffusion_tet(int DEF, const tables_t *T, int cvd,
     }
 }
 
+/* HelmholtzKernels.hpp:918-1119 (DiffusionCoeffPrismKernel, no variable
+ * diffusion): tmp_c = h1 (h0 df[3c+2] + df[3c]) */
+static void diffusion_prism(int DEF, const tables_t *T, int cvd,
+                            const double *dd, const vd *df, vd *d0, vd *d1,
+                            vd *d2)
+{
+    const int nq0 = T->nq[0], nq1 = T->nq[1], nq2 = T->nq[2];
+    double d00 = 1, d01 = 0, d11 = 1, d02 = 0, d12 = 0, d22 = 1;
+    if (cvd) { d00 = dd[0]; d01 = dd[1]; d11 = dd[2]; d02 = dd[3]; d12 = dd[4]; d22 = dd[5]; }
+    vd f[9];
+    if (!DEF) for (int d = 0; d < 9; ++d) f[d] = df[d];
+    for (int k = 0, cnt = 0; k < nq2; ++k) {
+        const double h1 = T->h3[k];
+        for (int j = 0; j < nq1; ++j)
+            for (int i = 0; i < nq0; ++i, ++cnt) {
+                const double h0 = T->h0[i];
+                if (DEF) for (int d = 0; d < 9; ++d) f[d] = df[cnt * 9 + d];
+                vd t1 = vmuls(vfmas(f[0], f[2], h0), h1);
+                vd t2 = vmuls(vfmas(f[3], f[5], h0), h1);
+                vd t3 = vmuls(vfmas(f[6], f[8], h0), h1);
+                vd g0, g1, g2, g3, g4, g5;
+                if (!cvd) {
+                    g0 = vmul(t1, t1); g0 = vfma(g0, t2, t2); g0 = vfma(g0, t3, t3);
+                    g3 = vmul(f[1], t1); g3 = vfma(g3, f[4], t2); g3 = vfma(g3, f[7], t3);
+                    g4 = vmul(f[2], t1); g4 = vfma(g4, f[5], t2); g4 = vfma(g4, f[8], t3);
+                    g1 = vmul(f[1], f[1]); g1 = vfma(g1, f[4], f[4]); g1 = vfma(g1, f[7], f[7]);
+                    g2 = vmul(f[2], f[2]); g2 = vfma(g2, f[5], f[5]); g2 = vfma(g2, f[8], f[8]);
+                    g5 = vmul(f[1], f[2]); g5 = vfma(g5, f[4], f[5]); g5 = vfma(g5, f[7], f[8]);
+                } else {
+                    vd s0 = vmuls(t1, d00); s0 = vfmas(s0, t2, d01); s0 = vfmas(s0, t3, d02);
+                    vd s1 = vmuls(t1, d01); s1 = vfmas(s1, t2, d11); s1 = vfmas(s1, t3, d12);
+                    vd s2 = vmuls(t1, d02); s2 = vfmas(s2, t2, d12); s2 = vfmas(s2, t3, d22);
+                    vd s3 = vmuls(f[1], d00); s3 = vfmas(s3, f[4], d01); s3 = vfmas(s3, f[7], d02);
+                    vd s4 = vmuls(f[1], d01); s4 = vfmas(s4, f[4], d11); s4 = vfmas(s4, f[7], d12);
+                    vd s5 = vmuls(f[1], d02); s5 = vfmas(s5, f[4], d12); s5 = vfmas(s5, f[7], d22);
+                    vd s6 = vmuls(f[2], d00); s6 = vfmas(s6, f[5], d01); s6 = vfmas(s6, f[8], d02);
+                    vd s7 = vmuls(f[2], d01); s7 = vfmas(s7, f[5], d11); s7 = vfmas(s7, f[8], d12);
+                    vd s8 = vmuls(f[2], d02); s8 = vfmas(s8, f[5], d12); s8 = vfmas(s8, f[8], d22);
+                    g0 = vmul(s0, t1); g0 = vfma(g0, s1, t2); g0 = vfma(g0, s2, t3);
+                    g3 = vmul(s0, f[1]); g3 = vfma(g3, s1, f[4]); g3 = vfma(g3, s2, f[7]);
+                    g4 = vmul(s0, f[2]); g4 = vfma(g4, s1, f[5]); g4 = vfma(g4, s2, f[8]);
+                    g1 = vmul(s3, f[1]); g1 = vfma(g1, s4, f[4]); g1 = vfma(g1, s5, f[7]);
+                    g5 = vmul(s3, f[2]); g5 = vfma(g5, s4, f[5]); g5 = vfma(g5, s5, f[8]);
+                    g2 = vmul(s6, f[2]); g2 = vfma(g2, s7, f[5]); g2 = vfma(g2, s8, f[8]);
+                }
+                vd a0 = d0[cnt], a1 = d1[cnt], a2 = d2[cnt];
+                vd o = vmul(g0, a0); o = vfma(o, g3, a1); o = vfma(o, g4, a2); d0[cnt] = o;
+                o = vmul(g3, a0); o = vfma(o, g1, a1); o = vfma(o, g5, a2); d1[cnt] = o;
+                o = vmul(g4, a0); o = vfma(o, g5, a1); o = vfma(o, g2, a2); d2[cnt] = o;
+            }
+    }
+}
+
+static void diffusion(int DEF, const tables_t *T, int cvd, const double *dd, const vd *df, vd *d0, vd *d1,
+                      vd *d2)
+{
+    if (T->shape == OR_HEX)
+        diffusion_hex(DEF, T->nqtot, cvd, dd, df, d0, d1, d2);
+    else if (T->shape == OR_TET)
+        diffusion_tet(DEF, T, cvd, dd, df, d0, d1, d2);
+    else
+        diffusion_prism(DEF, T, cvd, dd, df, d0, d1, d2);
+}
+
 /* ------------------------------------------------------------------ */
 /* Block workspace and per-block dispatch                              */
 typedef struct {
@@ -581,8 +781,11 @@ static void bwd_block(const tables_t *T, const vd *in, ws_t *w, vd *out)
     const int m0 = T->nmd[0], m1 = T->nmd[1], m2 = T->nmd[2];
     if (T->shape == OR_HEX)
         bwd_hex(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
-    else
+    else if (T->shape == OR_TET)
         bwd_tet(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
+    else
+        bwd_prism(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, T->b[0], T->b[1], T->b[2], w->w0, w->w1,
+                  out);
 }
 
 static void iprod_block(const tables_t *T, int SCALE, int APPEND, int DEF,
@@ -594,9 +797,15 @@ static void iprod_block(const tables_t *T, int SCALE, int APPEND, int DEF,
     if (T->shape == OR_HEX)
         iprod_hex(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], in, b0, b1, b2,
                   T->w[0], T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
-    else
+    else if (T->shape == OR_TET)
         iprod_tet(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, b0, b1, b2,
                   T->w[0], T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
+    else
+        /* the correction also runs with derivative tables: it then adds the
+         * derivative of the collapsed-edge term (Helmholtz.h:564-599 passes
+         * correct with every table set) */
+        iprod_prism(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, b0, b1, b2, T->w[0],
+                    T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
 }
 
 static int g_threads = 1;
@@ -688,10 +897,7 @@ static void helm_block(const tables_t *T, int deformed, double lambda,
     bwd_block(T, w->in, w, w->bwd);
     iprod_block(T, 1, 0, deformed, w->bwd, T->b[0], T->b[1], T->b[2], w->jac, w, w->out, lambda);
     deriv_tensor(T->nq[0], T->nq[1], T->nq[2], w->bwd, T->D[0], T->D[1], T->D[2], w->d0, w->d1, w->d2);
-    if (T->shape == OR_HEX)
-        diffusion_hex(deformed, T->nqtot, diff6 != NULL, diff6, w->df, w->d0, w->d1, w->d2);
-    else
-        diffusion_tet(deformed, T, diff6 != NULL, diff6, w->df, w->d0, w->d1, w->d2);
+    diffusion(deformed, T, diff6 != NULL, diff6, w->df, w->d0, w->d1, w->d2);
     iprod_block(T, 0, 1, deformed, w->d0, T->db[0], T->b[1], T->b[2], w->jac, w, w->out, 1.0);
     iprod_block(T, 0, 1, deformed, w->d1, T->b[0], T->db[1], T->b[2], w->jac, w, w->out, 1.0);
     iprod_block(T, 0, 1, deformed, w->d2, T->b[0], T->b[1], T->db[2], w->jac, w, w->out, 1.0);
@@ -743,6 +949,13 @@ static void advection_pointwise(const tables_t *T, int DEF, const vd *df, const 
                     for (int c = 0; c < 9; ++c) f[c] = df[(size_t)cnt * 9 + c];
                 const vd vx = vel[cnt * 3], vy = vel[cnt * 3 + 1], vzz = vel[cnt * 3 + 2];
                 vd m0 = d0[cnt], m1 = d1[cnt], m2 = d2[cnt];
+                if (T->shape == OR_PRISM) {
+                    /* AdvectionPrismKernel (LinearAdvectionDiffusionReactionKernels.hpp:
+                     * 384-470): metric0 = h1 d0, metric2 = h0 metric0 + d2 */
+                    const double hp0 = T->h0[i], hp1 = T->h3[k];
+                    m0 = vmuls(d0[cnt], hp1);
+                    m2 = vadd(vmuls(m0, hp0), d2[cnt]);
+                }
                 if (tet) {
                     const double h0 = 0.5 * (1.0 + T->Z[0][i]);
                     const vd metric0 = vmuls(d0[cnt], h2h3);
@@ -797,10 +1010,7 @@ void or_adv_diff_react_aniso(int shape, const int *nm, const int *nq, int nelmt,
             deriv_tensor(T.nq[0], T.nq[1], T.nq[2], w.bwd, T.D[0], T.D[1], T.D[2], w.d0, w.d1, w.d2);
             advection_pointwise(&T, deformed, w.df, v, w.d0, w.d1, w.d2, w.bwd);
             iprod_block(&T, 1, 0, deformed, w.bwd, T.b[0], T.b[1], T.b[2], w.jac, &w, w.out, lambda);
-            if (T.shape == OR_HEX)
-                diffusion_hex(deformed, T.nqtot, diff6 != NULL, diff6, w.df, w.d0, w.d1, w.d2);
-            else
-                diffusion_tet(deformed, &T, diff6 != NULL, diff6, w.df, w.d0, w.d1, w.d2);
+            diffusion(deformed, &T, diff6 != NULL, diff6, w.df, w.d0, w.d1, w.d2);
             iprod_block(&T, 0, 1, deformed, w.d0, T.db[0], T.b[1], T.b[2], w.jac, &w, w.out, 1.0);
             iprod_block(&T, 0, 1, deformed, w.d1, T.b[0], T.db[1], T.b[2], w.jac, &w, w.out, 1.0);
             iprod_block(&T, 0, 1, deformed, w.d2, T.b[0], T.b[1], T.db[2], w.jac, &w, w.out, 1.0);
@@ -843,6 +1053,12 @@ static void iprod_deriv_pointwise(const tables_t *T, int DEF, const vd *df,
                     t0 = vmuls(t0, f0);
                     t1 = vfmas(t1, t2, f3);
                     t1 = vmuls(t1, f2);
+                } else if (T->shape == OR_PRISM) {
+                    /* IProductWRTDerivBaseKernels.hpp:391-403: t0 <- f t0 + f (h0 t2),
+                     * f = 2/(1 - z2) */
+                    const double hf1 = 0.5 * (1.0 + T->Z[0][i]);
+                    t0 = vmuls(t0, f2);
+                    t0 = vfmas(t0, vmuls(t2, hf1), f2);
                 }
                 o0[cnt] = t0;
                 o1[cnt] = t1;

@@ -31,7 +31,7 @@ extern "C" {
 #endif
 
 /* ---- shapes / point types ------------------------------------------- */
-enum { OR_HEX = 0, OR_TET = 1 };
+enum { OR_HEX = 0, OR_TET = 1, OR_PRISM = 2 };
 enum { OR_GLL = 0, OR_RADAU_A1 = 1, OR_RADAU_A2 = 2 };
 enum { OR_MOD_A = 0, OR_MOD_B = 1, OR_MOD_C = 2 };
 

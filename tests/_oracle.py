@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_POLYLIB = os.path.join(ROOT, "oracle", "_ref", "libpolylib_ref.so")
 
-HEX, TET = 0, 1
+HEX, TET, PRISM = 0, 1, 2
 GLL, RADAU_A1, RADAU_A2 = 0, 1, 2
 MOD_A, MOD_B, MOD_C = 0, 1, 2
 
@@ -46,10 +46,14 @@ def set_threads(n: int):
 
 
 def ncoeffs(shape, nm):
+    if shape == PRISM:
+        return nm * nm * (nm + 1) // 2
     return nm ** 3 if shape == HEX else nm * (nm + 1) * (nm + 2) // 6
 
 
 def default_nq(shape, nq0):
+    if shape == PRISM:
+        return (nq0, nq0, nq0 - 1)
     return (nq0, nq0, nq0) if shape == HEX else (nq0, nq0 - 1, nq0 - 1)
 
 

@@ -253,6 +253,47 @@ def tet_dense_own_points(nm, nq):
     return vals, grads, W
 
 
+def prism_modes_aniso(nm):
+    """Prism modes in the reference's order (StdPrismExp::GetMode): p < nm0,
+    q < nm1, r < nm2 - p, r fastest."""
+    return [(p, q, r) for p in range(nm[0]) for q in range(nm[1]) for r in range(nm[2] - p)]
+
+
+def prism_dense_own_points(nm, nq):
+    """Prism basis values and standard-coordinate derivatives at the
+    operator's own collapsed points (GLL x GLL x Gauss-Radau(1,0), nq[3]
+    points, i fastest) and the weights W = w_a w_b (w_c / 2) of the collapsed
+    rule (xi0 = (1 + a)(1 - c)/2 - 1, xi1 = b, xi2 = c).  Mode (p, q, r) is
+    psi^A_p(a) psi^A_q(b) psi^B_pr(c); the collapsed-edge modes (0, q, 1)
+    are psi^A_q(b) (1 + c)/2, independent of a.  Derivatives from exact
+    polynomial differentiation and the collapsed chain rule
+    d/dxi0 = 2/(1 - c) d/da, d/dxi2 = (1 + a)/(1 - c) d/da + d/dc."""
+    za, wa = gll(nq[0])
+    zb, wb = gll(nq[1])
+    zc, wc = radau_m(nq[2], 1.0)
+    C, Bb, A = np.meshgrid(zc, zb, za, indexing="ij")
+    a, b, c = A.ravel(), Bb.ravel(), C.ravel()
+    W = (wc[:, None, None] * 0.5 * wb[None, :, None] * wa[None, None, :]).ravel()
+    modes = prism_modes_aniso(nm)
+    one = Poly([1.0])
+    vals = np.zeros((len(a), len(modes)))
+    grads = np.zeros((3, len(a), len(modes)))
+    for m, (p, q, r) in enumerate(modes):
+        if p == 0 and r == 1:
+            ta, tb, tc = one, modified_a(q), HALF_P
+        else:
+            ta, tb, tc = modified_a(p), modified_a(q), modified_b(p, r)
+        v = ta(a) * tb(b) * tc(c)
+        da = ta.deriv()(a) * tb(b) * tc(c)
+        db = ta(a) * tb.deriv()(b) * tc(c)
+        dc = ta(a) * tb(b) * tc.deriv()(c)
+        vals[:, m] = v
+        grads[0, :, m] = da * 2.0 / (1 - c)
+        grads[1, :, m] = db
+        grads[2, :, m] = da * (1 + a) / (1 - c) + dc
+    return vals, grads, W
+
+
 def dense_helmholtz(vals, grads, W, jac, df, lam, diff6=None):
     """lam M + L from point values / reference-direction gradients (per
     point), per-point jac [nqt] and df [nqt, 9] (df[j*3+k] = dxi_k/dx_j)."""

@@ -49,8 +49,10 @@ void run(GenArgs a)
 {
     for (int e = 0; e < a.nelmt; ++e) {
         std::vector<double> X0(a.buf), X1(a.buf), X2(a.buf), X3(a.buf);
-        if (a.shape == Shape::Hex) gen::element<double, false>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
-        else gen::element<double, true>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
+        if (a.shape == Shape::Hex) gen::element<double, gen::SH_HEX>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
+        else if (a.shape == Shape::Prism)
+            gen::element<double, gen::SH_PRISM>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
+        else gen::element<double, gen::SH_TET>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
     }
 }
 
@@ -69,14 +71,17 @@ int main()
         {Shape::Hex, {4, 4, 4}, {8, 8, 8}, 2},   {Shape::Tet, {3, 4, 5}, {5, 5, 5}, 5},
         {Shape::Tet, {2, 3, 4}, {4, 6, 5}, 4},   {Shape::Tet, {4, 4, 4}, {6, 4, 7}, 3},
         {Shape::Tet, {5, 5, 5}, {7, 6, 6}, 3},   {Shape::Tet, {2, 2, 2}, {3, 2, 2}, 4},
-        {Shape::Tet, {6, 6, 9}, {4, 3, 3}, 2},
+        {Shape::Tet, {6, 6, 9}, {4, 3, 3}, 2},   {Shape::Prism, {3, 3, 3}, {4, 4, 3}, 5},
+        {Shape::Prism, {3, 4, 5}, {5, 4, 5}, 4}, {Shape::Prism, {4, 2, 4}, {5, 6, 5}, 3},
+        {Shape::Prism, {2, 2, 2}, {3, 3, 2}, 4}, {Shape::Prism, {6, 6, 6}, {8, 8, 7}, 2},
     };
     double worst = 0.0;
     std::mt19937_64 rng(11);
     std::uniform_real_distribution<double> U(-1.0, 1.0), Upos(0.5, 1.5);
     for (const Case &c : cases) {
         const GenLayout L = generic_layout(c.shape, c.nm, c.nq);
-        const int S = c.shape == Shape::Hex ? 0 : 1, E = c.E, nqt = L.nqtot, nc = L.ncoeff;
+        // S: the oracle's shape (0 hex, 1 tet, 2 prism); tets and prisms pack df and J
+        const int S = (int)c.shape, E = c.E, nqt = L.nqtot, nc = L.ncoeff;
         const Basis1D P0 = make_basis(default_basis(c.shape, 0), c.nm[0], c.nq[0]),
                       P1 = make_basis(default_basis(c.shape, 1), c.nm[1], c.nq[1]),
                       P2 = make_basis(default_basis(c.shape, 2), c.nm[2], c.nq[2]);
@@ -127,7 +132,10 @@ int main()
                     const double *v = &vel[((size_t)e * nqt + q) * 3];
                     double a3[3];
                     for (int d = 0; d < 3; ++d) a3[d] = v[0] * f[d] + v[1] * f[3 + d] + v[2] * f[6 + d];
-                    if (S) {
+                    if (S == 2) {
+                        const double h0 = 0.5 * (1 + P0.pts.z[i]), h1 = 2 / (1 - P2.pts.z[k]);
+                        a3[0] = h1 * (a3[0] + h0 * a3[2]);
+                    } else if (S) {
                         const double h0 = 0.5 * (1 + P0.pts.z[i]), h1 = 0.5 * (1 + P1.pts.z[j]),
                                      h2 = 2 / (1 - P1.pts.z[j]), h3 = 2 / (1 - P2.pts.z[k]);
                         const double b0 = a3[0], b1 = a3[1], b2 = a3[2];

@@ -4,7 +4,8 @@ on dense evaluations.  The reference runs these shapes through its
 runtime-size operator3D (MatrixFreeOps/SwitchNodesPoints.h:302-343,
 Helmholtz.h:374-464).
 
-Cases: anisotropic modes and points on hexes and tets, isotropic orders
+Cases: anisotropic modes and points on hexes, tets and prisms (which run
+the runtime-size kernels at every size), isotropic orders
 without a compile-time kernel (hex P=11 / Q=14, Q = 2 nm; tet P=10), and
 sizes whose buffers exceed one workgroup's LDS, which run from the global
 workspace (including more elements than workgroups of that grid).  Every
@@ -37,6 +38,11 @@ HEX = [((3, 4, 5), (4, 5, 6), 37), ((4, 4, 4), (6, 5, 7), 20), ((6, 3, 4), (9, 4
        ((11, 9, 10), (13, 13, 13), 1100)]
 TET = [((3, 4, 5), (5, 5, 5), 37), ((2, 3, 4), (4, 6, 5), 21), ((4, 4, 4), (6, 4, 7), 19),
        ((5, 5, 5), (7, 7, 7), 16), ((11, 11, 11), (12, 11, 11), 10), ((8, 10, 12), (16, 14, 14), 1030)]
+# prisms always run the runtime-size kernels: the MODIFIED defaults (P=3, 6,
+# 9), anisotropic sizes, and a size from the global workspace with more
+# elements than workgroups
+PRISM = [((4, 4, 4), (5, 5, 4), 37), ((7, 7, 7), (8, 8, 7), 64), ((3, 4, 5), (5, 4, 5), 15),
+         ((4, 2, 4), (5, 6, 5), 33), ((10, 10, 10), (11, 11, 10), 12), ((12, 12, 12), (14, 14, 13), 1030)]
 
 
 @pytest.fixture(scope="module")
@@ -79,8 +85,12 @@ def _run(nmf, key, nm, nq, E, prec, jac, df, ins, outs, lam=None, diff=None, vel
     return [t.double().cpu().numpy() for t in tout]
 
 
-def _collapse_factor(nmf, nq):
-    """max 2/(1-z1) * max 2/(1-z2) over the tet's Radau points"""
+def _collapse_factor(nmf, nq, shape="Tet"):
+    """max 2/(1-z1) * max 2/(1-z2) over the tet's Radau points; max
+    2/(1-z2) over the prism's"""
+    if shape == "Prism":
+        z2 = nmf.basis_tables(nmf.SHAPE_PRISM, 2, 2, nq[2] + 1)["z"]
+        return float(np.max(2 / (1 - z2)))
     z1 = nmf.basis_tables(nmf.SHAPE_TET, 1, 2, nq[1] + 1)["z"]
     z2 = nmf.basis_tables(nmf.SHAPE_TET, 2, 2, nq[2] + 1)["z"]
     return float(np.max(2 / (1 - z1)) * np.max(2 / (1 - z2)))
@@ -89,20 +99,20 @@ def _collapse_factor(nmf, nq):
 def _check(got, want, prec, nmf, what):
     nm = what[1]
     tol = 5e-5 if prec == nmf.FP32 else (1e-12 if max(nm) <= 10 else 1e-11)
-    if what[0] == "Tet" and what[-1] == "PhysDeriv":
-        tol *= max(1.0, _collapse_factor(nmf, what[2]) / 10)
+    if what[0] in ("Tet", "Prism") and what[-1] == "PhysDeriv":
+        tol *= max(1.0, _collapse_factor(nmf, what[2], what[0]) / 10)
     for g, w in zip(got, want):
         err = G.rel_err(g, w)
         assert err <= tol, (what[-1], what, err)
 
 
-@pytest.mark.parametrize("shape", ["Hex", "Tet"])
+@pytest.mark.parametrize("shape", ["Hex", "Tet", "Prism"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("deformed", [False, True], ids=["regular", "deformed"])
 def test_runtime_size_operators_against_oracle(nmf, shape, prec, deformed):
     P = nmf.FP64 if prec == "fp64" else nmf.FP32
-    S = O.HEX if shape == "Hex" else O.TET
-    cases = HEX if shape == "Hex" else TET
+    S = {"Hex": O.HEX, "Tet": O.TET, "Prism": O.PRISM}[shape]
+    cases = {"Hex": HEX, "Tet": TET, "Prism": PRISM}[shape]
     geo = "Deformed" if deformed else "Regular"
     for ci, (nm, nq, E) in enumerate(cases):
         if prec == "fp32" and E > 100:
@@ -157,3 +167,9 @@ def test_compiled_and_runtime_choice(nmf):
         F.CreateInstance("Helmholtz_Tet_Regular", [5, 4, 5], [6, 5, 5], 4)  # nm0 <= nm1 <= nm2
     with pytest.raises(nmf.NektarError):
         F.CreateInstance("Helmholtz_Hex_Regular", [33, 4, 4], [34, 6, 6], 4)
+    # prisms: runtime-size at the defaults too, nm0 <= nm2, no PhysInterp1DScaled
+    assert F.CreateInstance("Helmholtz_Prism_Regular", 7, 8, 4).runtime_size
+    with pytest.raises(nmf.NektarError):
+        F.CreateInstance("Helmholtz_Prism_Regular", [5, 4, 4], [6, 6, 5], 4)
+    with pytest.raises(nmf.NektarError):
+        F.CreateInstance("PhysInterp1DScaled_Prism_Regular", 5, 6, 4)

@@ -159,3 +159,78 @@ def test_planted_per_point_index_mistake_fails(shape_case):
         hb = np.concatenate([B.dense_helmholtz(vals, grads, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
                                                1.9) @ _elem(x, e, nc) for e in range(NE)])
         assert G.rel_err(good, hb) < TOL
+
+
+PRISM_CASES = [((3, 3, 3), (4, 4, 3)), ((4, 4, 4), (5, 5, 4)), ((3, 4, 5), (5, 4, 5)), ((2, 3, 3), (4, 3, 3)),
+               ((5, 5, 5), (7, 7, 6)), ((4, 2, 4), (5, 6, 5))]
+
+
+@pytest.mark.parametrize("nm,nq", PRISM_CASES)
+def test_prism_own_points_against_dense(nm, nq):
+    """Prism operators (BwdTransKernels.hpp:334-405, IProductKernels.hpp:
+    519-655, the prism chain rule and metric, HelmholtzKernels.hpp:918-1119,
+    IProductWRTDerivBaseKernels.hpp:338-411) against the dense prism
+    expansion at the operator's own points, per-point geometry."""
+    vals, grads, W = B.prism_dense_own_points(nm, nq)
+    nqt, nc = int(np.prod(nq)), len(B.prism_modes_aniso(nm))
+    assert O.ncoeffs_aniso(O.PRISM, nm) == nc
+    jac, df = _geom(nqt, 5 * sum(nm) + sum(nq))
+    x = G.coeffs(NE * nc, 13)
+    u = O.bwdtrans_aniso(O.PRISM, nm, nq, NE, x)
+    assert G.rel_err(u, np.concatenate([vals @ _elem(x, e, nc) for e in range(NE)])) < TOL
+    f = G.coeffs(NE * nqt, 14)
+    y = O.iproduct_aniso(O.PRISM, nm, nq, NE, True, jac, f)
+    yb = np.concatenate([vals.T @ (W * _elem(jac, e, nqt) * _elem(f, e, nqt)) for e in range(NE)])
+    assert G.rel_err(y, yb) < TOL
+    d = O.physderiv_aniso(O.PRISM, nm, nq, NE, True, df, u)
+    for e in range(NE):
+        xe = _elem(x, e, nc)
+        dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
+        for j in range(3):
+            want = sum(dfe[:, j * 3 + k] * (grads[k] @ xe) for k in range(3))
+            assert G.rel_err(_elem(d[j], e, nqt), want) < 1e-12
+    f1, f2 = G.coeffs(NE * nqt, 15), G.coeffs(NE * nqt, 16)
+    yd = O.iproduct_deriv_aniso(O.PRISM, nm, nq, NE, True, jac, df, f, f1, f2)
+    for e in range(NE):
+        dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
+        wj = W * _elem(jac, e, nqt)
+        want = sum((sum(dfe[:, j * 3 + k][:, None] * grads[k] for k in range(3))).T @ (wj * _elem(ff, e, nqt))
+                   for j, ff in enumerate((f, f1, f2)))
+        assert G.rel_err(_elem(yd, e, nc), want) < TOL
+    d6 = np.array([1.3, -0.1, 0.8, 0.2, 0.05, 1.4])
+    for diff in (None, d6):
+        h = O.helmholtz_aniso(O.PRISM, nm, nq, NE, True, jac, df, 1.9, x, diff)
+        hb = np.concatenate([B.dense_helmholtz(vals, grads, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
+                                               1.9, diff) @ _elem(x, e, nc) for e in range(NE)])
+        assert G.rel_err(h, hb) < TOL
+    # LinearAdvectionDiffusionReaction: lambda M + L plus lambda (phi_i, V . grad u)
+    vel = G.coeffs(NE * nqt * 3, 17)
+    a = O.adv_diff_react_aniso(O.PRISM, nm, nq, NE, True, jac, df, vel, 1.9, x)
+    for e in range(NE):
+        xe = _elem(x, e, nc)
+        dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
+        ve = _elem(vel, e, 3 * nqt).reshape(nqt, 3)
+        gx = [sum(dfe[:, j * 3 + k] * (grads[k] @ xe) for k in range(3)) for j in range(3)]
+        adv = sum(ve[:, j] * gx[j] for j in range(3))
+        want = B.dense_helmholtz(vals, grads, W, _elem(jac, e, nqt), dfe, 1.9) @ xe + 1.9 * vals.T @ (
+            W * _elem(jac, e, nqt) * adv)
+        assert G.rel_err(_elem(a, e, nc), want) < TOL
+
+
+def test_prism_regular_geometry_and_defaults():
+    """Per-element geometry gives the per-point result with constant
+    factors, and the isotropic entry points use the prism defaults
+    (Q, Q, Q-1)."""
+    nm, nq0 = 4, 5
+    nq = O.default_nq(O.PRISM, nq0)
+    assert nq == (5, 5, 4)
+    nqt = int(np.prod(nq))
+    rng = np.random.default_rng(3)
+    jac_e = rng.uniform(0.5, 1.5, NE)
+    df_e = rng.uniform(-1, 1, NE * 9)
+    jac_q = np.repeat(jac_e, nqt)
+    df_q = np.concatenate([np.tile(df_e[e * 9:(e + 1) * 9], nqt) for e in range(NE)])
+    x = G.coeffs(NE * O.ncoeffs(O.PRISM, nm), 18)
+    a = O.helmholtz(O.PRISM, nm, nq0, NE, False, jac_e, df_e, 0.7, x)
+    b = O.helmholtz_aniso(O.PRISM, (nm, nm, nm), nq, NE, True, jac_q, df_q, 0.7, x)
+    assert G.rel_err(a, b) < 1e-13
