@@ -323,6 +323,38 @@ def bench_batch(N, torch, steps, warmup):
             r[f"{mtag}_us"] = round(ms * 1e3, 1)
             r[f"{mtag}_gdofs"] = round(info["dofs"] / (ms * 1e-3) / 1e9, 2)
         r["graph_over_per_item"] = round(r["per_item_us"] / r["graph_us"], 3)
+        # each kind (shape, P, Q) alone as one grouped launch: its roofline
+        # fraction, and how the sum of the kinds compares with the graph
+        # (the graph runs kinds on 4 streams)
+        kinds = {}
+        for idx, (shape, nm, nq, E, _, _, _) in enumerate(work):
+            kinds.setdefault((shape, nm, nq), []).append(idx)
+        peak = PEAK_FP64_TFLOPS if prec == N.FP64 else 157.3
+        per, total = {}, 0.0
+        for (shape, nm, nq), idxs in sorted(kinds.items()):
+            bk = N.Batch(mode=N.Batch.GROUPED, streams=1)
+            for i in idxs:
+                bk.add(ops[i], *arrays[i])
+            for _ in range(warmup):
+                bk.run()
+            torch.cuda.synchronize()
+            s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s0.record()
+            for _ in range(steps):
+                bk.run()
+            s1.record()
+            torch.cuda.synchronize()
+            us = s0.elapsed_time(s1) / steps * 1e3
+            fpe = hex_flops_per_elem(nm, nq) if shape == "Hex" else tet_flops_per_elem(nm, nq)
+            fl = sum(work[i][3] for i in idxs) * fpe
+            per[f"{shape}_P{nm - 1}_Q{nq}"] = {"items": len(idxs), "elements": sum(work[i][3] for i in idxs),
+                                               "us": round(us, 1), "tflops": round(fl / us / 1e6, 2),
+                                               "fp_frac": round(fl / us / 1e6 / peak, 3)}
+            total += us
+            del bk
+        r["per_kind"] = per
+        r["sum_of_kinds_us"] = round(total, 1)
+        r["sum_of_kinds_over_graph"] = round(total / r["graph_us"], 3)
         res[tag] = r
         del b, ops, arrays
     res["fp32_over_fp64"] = round(res["fp64"]["graph_us"] / res["fp32"]["graph_us"], 3)

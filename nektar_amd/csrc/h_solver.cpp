@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "basis.hpp"
 #include "mf_operator.hpp"
@@ -81,6 +82,12 @@ size_t HexSolver::estimate_bytes(int n, int nm, int nq, bool deformed, bool mixe
     size_t b = 8 * (NG * (5 + 6) + nloc + geo);
     if (mixed) b += 4 * (NG * 6 + nloc + geo);
     b += 8 * nel * (size_t)nq * nq * nq;  // forcing / quadrature scratch
+    // the small buffers, so that the estimate bounds device_bytes(): mesh
+    // vertices, points and weights, FP64 + FP32 tables, the diagonal's
+    // tables, CG state and the partial sums of the reductions
+    const size_t tab = (size_t)2 * nq * nq + 2 * (size_t)nm * nq + nq;
+    b += 24 * (size_t)(n + 1) * (n + 1) * (n + 1) + 16 * (size_t)nq + 12 * tab + 24 * (size_t)nm * nq;
+    b += sizeof(CGState) + 4 * (size_t)solver::kRedBlocks * sizeof(double) + 8 * sizeof(double);
     return b;
 }
 
@@ -275,10 +282,17 @@ void HexSolver::apply_global_fp64(const double *x, double *y)
     NMF_HIP(solver::launch_assemble<double>(stream_, n_, nm_, N_, sloc64_.as<double>(), y, nullptr, nullptr, nullptr));
 }
 
+std::mutex &graph_capture_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
 void HexSolver::capture_iterations(bool fp32)
 {
     const int slot = fp32 ? 1 : 0;
     if (graph_[slot]) return;
+    std::lock_guard<std::mutex> lock(graph_capture_mutex());
     CGState *st = state_.as<CGState>();
     const int *done = &st->done;
     hipGraph_t g;
@@ -325,6 +339,7 @@ void HexSolver::ensure_fp32()
 void HexSolver::capture_pcg32()
 {
     if (graph_[2]) return;
+    std::lock_guard<std::mutex> lock(graph_capture_mutex());
     CGState *st = state_.as<CGState>();
     const int *done = &st->done;
     float *x = cg32_[0].as<float>(), *r = cg32_[1].as<float>(), *p = cg32_[2].as<float>(), *q = cg32_[3].as<float>();

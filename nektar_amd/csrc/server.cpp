@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <future>
 #include <thread>
 
 namespace nektar_amd {
@@ -81,69 +82,114 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, bool
         Pooled *p;
     };
     std::deque<int> queue;
-    std::vector<Active> active;
+    std::vector<Active> active, starting;  // starting: solver still being built
+    std::vector<int> overtaken(reqs.size(), 0);
     size_t next = 0, finished = 0;
     const auto t0 = Clock::now();
+
+    // the set-up that follows a solver becoming available: forcing, first
+    // chunk of the solve, bookkeeping
+    auto launch = [&](int ri, Pooled *p) {
+        const ServeRequest &r = reqs[ri];
+        ServeResult &res = out[ri];
+        p->solver->set_forcing_manufactured(r.k);
+        p->solver->begin_solve(r.mixed != 0, r.tol, r.maxit);
+        p->bytes = std::max<int64_t>(p->bytes, (int64_t)p->solver->device_bytes());
+        res.unknowns = (int64_t)p->solver->nglobal() - (int64_t)p->solver->ndirichlet();
+        active.push_back({ri, p});
+        st.peak_active = std::max(st.peak_active, (int)active.size());
+        st.peak_bytes = std::max(st.peak_bytes, used_bytes());
+    };
+    // admission of one request; false if its memory does not fit now
+    auto admit = [&](int ri) {
+        const ServeRequest &r = reqs[ri];
+        const int64_t est = (int64_t)HexSolver::estimate_bytes(r.n, r.nummodes, r.numpoints, false, r.mixed != 0);
+        Pooled *reuse = nullptr;
+        for (auto &q : pool_)
+            if (!q->busy && q->solver && q->n == r.n && q->nm == r.nummodes && q->nq == r.numpoints) {
+                reuse = q.get();
+                break;
+            }
+        const int64_t need = reuse ? std::max<int64_t>(0, est - reuse->bytes) : est;
+        if (reuse) reuse->busy = true;  // keep it out of eviction
+        if (!make_room(need, st)) {
+            if (reuse) reuse->busy = false;
+            return false;
+        }
+        ServeResult &res = out[ri];
+        res.arrival_s = r.arrival_s;
+        res.admit_s = seconds_since(t0);
+        if (reuse) {
+            reuse->last_use = ++tick_;
+            ++stream_load_[reuse->stream];
+            res.stream = reuse->stream;
+            reuse->solver->set_lambda(r.lambda);
+            res.reused_solver = 1;
+            ++st.solvers_reused;
+            launch(ri, reuse);
+            return true;
+        }
+        // a new solver, built on a worker thread on the least-loaded stream;
+        // its memory is reserved (the estimate) from now on
+        const int s = (int)(std::min_element(stream_load_.begin(), stream_load_.end()) - stream_load_.begin());
+        auto np = std::make_unique<Pooled>();
+        np->n = r.n;
+        np->nm = r.nummodes;
+        np->nq = r.numpoints;
+        np->busy = true;
+        np->stream = s;
+        np->bytes = est;
+        np->last_use = ++tick_;
+        const hipStream_t hs = streams_[s];
+        const int n = r.n, nm = r.nummodes, nq = r.numpoints;
+        const double lam = r.lambda;
+        np->building = std::async(std::launch::async, [n, nm, nq, lam, hs] {
+            // never beside a graph capture (which the allocations would break)
+            std::lock_guard<std::mutex> lock(graph_capture_mutex());
+            return std::make_unique<HexSolver>(HexBoxMesh::make(n, 0.0, 0), nm, nq, lam, hs);
+        });
+        ++stream_load_[s];
+        res.stream = s;
+        starting.push_back({ri, np.get()});
+        pool_.push_back(std::move(np));
+        ++st.solvers_created;
+        return true;
+    };
+
     while (finished < reqs.size()) {
         const double now = seconds_since(t0);
         while (next < order.size() && reqs[order[next]].arrival_s <= now) queue.push_back(order[next++]);
 
-        // admission: FIFO, while a slot is free and the memory fits
-        while (!queue.empty() && (int)active.size() < max_active_) {
-            const int ri = queue.front();
-            const ServeRequest &r = reqs[ri];
-            const int64_t est = (int64_t)HexSolver::estimate_bytes(r.n, r.nummodes, r.numpoints, false, r.mixed != 0);
-            Pooled *reuse = nullptr;
-            for (auto &q : pool_)
-                if (!q->busy && q->n == r.n && q->nm == r.nummodes && q->nq == r.numpoints) {
-                    reuse = q.get();
-                    break;
-                }
-            const int64_t need = reuse ? std::max<int64_t>(0, est - reuse->bytes) : est;
-            if (reuse) reuse->busy = true;  // keep it out of eviction
-            const bool fits = make_room(need, st);
-            if (!fits) {
-                if (reuse) reuse->busy = false;
-                break;  // wait for memory (FIFO: no overtaking)
+        // admission: first fit in arrival order while a slot is free; the
+        // head can be passed at most kMaxOvertakes times
+        for (auto it = queue.begin(); it != queue.end() && (int)(active.size() + starting.size()) < max_active_;) {
+            const bool head = it == queue.begin();
+            if (!head && overtaken[queue.front()] >= kMaxOvertakes) break;
+            const int ri = *it;
+            if (!admit(ri)) {
+                ++it;
+                continue;
             }
-            ServeResult &res = out[ri];
-            Pooled *p = reuse;
-            if (!p) {
-                const int s = (int)(std::min_element(stream_load_.begin(), stream_load_.end()) - stream_load_.begin());
-                auto np = std::make_unique<Pooled>();
-                np->solver = std::make_unique<HexSolver>(HexBoxMesh::make(r.n, 0.0, 0), r.nummodes, r.numpoints,
-                                                         r.lambda, streams_[s]);
-                np->n = r.n;
-                np->nm = r.nummodes;
-                np->nq = r.numpoints;
-                np->busy = true;
-                np->stream = s;
-                np->bytes = (int64_t)np->solver->device_bytes();
-                p = np.get();
-                pool_.push_back(std::move(np));
-                ++st.solvers_created;
-            } else {
-                p->solver->set_lambda(r.lambda);
-                res.reused_solver = 1;
-                ++st.solvers_reused;
+            if (!head) ++overtaken[queue.front()];
+            it = queue.erase(it);
+        }
+
+        bool progressed = false;
+        // solvers whose construction has finished start their solve
+        for (size_t a = 0; a < starting.size();) {
+            Pooled &p = *starting[a].p;
+            if (p.building.wait_for(std::chrono::seconds(0)) != std::future_status::ready) {
+                ++a;
+                continue;
             }
-            p->last_use = ++tick_;
-            ++stream_load_[p->stream];
-            p->solver->set_forcing_manufactured(r.k);
-            p->solver->begin_solve(r.mixed != 0, r.tol, r.maxit);
-            p->bytes = std::max<int64_t>(p->bytes, (int64_t)p->solver->device_bytes());
-            res.arrival_s = r.arrival_s;
-            res.admit_s = seconds_since(t0);
-            res.stream = p->stream;
-            res.unknowns = (int64_t)p->solver->nglobal() - (int64_t)p->solver->ndirichlet();
-            active.push_back({ri, p});
-            queue.pop_front();
-            st.peak_active = std::max(st.peak_active, (int)active.size());
-            st.peak_bytes = std::max(st.peak_bytes, used_bytes());
+            p.solver = p.building.get();  // rethrows a failed construction
+            p.bytes = (int64_t)p.solver->device_bytes();
+            launch(starting[a].req, &p);
+            starting.erase(starting.begin() + a);
+            progressed = true;
         }
 
         // progress: enqueue the next chunk of every solve whose state is in
-        bool progressed = false;
         for (size_t a = 0; a < active.size();) {
             Pooled &p = *active[a].p;
             if (!p.solver->advance(false)) {
@@ -172,7 +218,7 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, bool
             ++finished;
         }
         if (!progressed) {
-            if (active.empty() && queue.empty() && next < order.size()) {
+            if (active.empty() && starting.empty() && queue.empty() && next < order.size()) {
                 const double wait = reqs[order[next]].arrival_s - seconds_since(t0);
                 if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
             } else {

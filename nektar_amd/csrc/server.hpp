@@ -3,12 +3,17 @@
 //
 // Requests (structured hex box, order, lambda, forcing frequency, tolerance,
 // precision) arrive at given times.  One host loop
-//   * queues arrivals (FIFO),
-//   * admits the head of the queue while a slot is free and its device
-//     memory fits the HBM budget -- solvers (geometry, diagonal, Krylov
-//     vectors, captured iterations) stay resident in a pool keyed by mesh
-//     shape and are reused with a new lambda; idle ones are evicted
-//     least-recently-used when memory is needed,
+//   * queues arrivals in arrival order,
+//   * admits queued requests while a slot is free and their device memory
+//     fits the HBM budget, first fit: a request that does not fit yet lets
+//     later (smaller) ones pass, each at most kMaxOvertakes times, after
+//     which it waits for memory with no one passing -- solvers (geometry,
+//     diagonal, Krylov vectors, captured iterations) stay resident in a pool
+//     keyed by mesh shape and are reused with a new lambda; idle ones are
+//     evicted least-recently-used when memory is needed,
+//   * builds new solvers on worker threads (allocation, geometry and the
+//     diagonal end in a synchronisation of the solver's own stream), so the
+//     loop keeps advancing the admitted solves meanwhile,
 //   * advances every admitted solve without blocking (HexSolver::advance):
 //     each solve's next chunk is enqueued as soon as the previous one has
 //     reported its state, on one of a few streams, so the work of many
@@ -19,6 +24,7 @@
 #pragma once
 
 #include <cstdint>
+#include <future>
 #include <memory>
 #include <vector>
 
@@ -61,7 +67,8 @@ public:
 
 private:
     struct Pooled {
-        std::unique_ptr<HexSolver> solver;
+        std::unique_ptr<HexSolver> solver;   // null while it is being built
+        std::future<std::unique_ptr<HexSolver>> building;
         int n, nm, nq;
         int64_t bytes;
         bool busy;
@@ -71,6 +78,7 @@ private:
     int64_t used_bytes() const;
     bool make_room(int64_t need, ServeStats &st);
 
+    static constexpr int kMaxOvertakes = 8;
     int64_t budget_;
     int max_active_;
     std::vector<hipStream_t> streams_;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -15,6 +16,12 @@
 #include "tune.hpp"
 
 namespace nektar_amd {
+
+// Held while a solver captures its iterations into a hipGraph, and by
+// anything that builds a solver on another thread: allocations,
+// synchronous copies and stream synchronisation elsewhere in the process
+// invalidate a capture in progress.
+std::mutex &graph_capture_mutex();
 namespace solver {
 
 constexpr int kBlock = 256;

@@ -5,6 +5,8 @@ final residual, L2 error against the manufactured solution and the norm of
 the solution vector are compared for exact equality with a plain solve of
 the same request on a fresh solver (the solve's arithmetic does not depend
 on what runs beside it; a reused pooled solver gets a new lambda)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -62,6 +64,21 @@ def test_same_alone_and_in_the_mix(nmf):
         # exact solution has L2 norm 0.35
         assert 0.0 < got["l2_error"] < 0.1
     check_same(nmf, reqs, results)
+
+
+def test_estimate_bounds_device_bytes(nmf):
+    """Admission reserves the estimate while a solver is still being built,
+    so the estimate must bound what the solver then holds."""
+    for r in mixed_requests():
+        for mixed in (False, True):
+            s = nmf.HelmholtzSolver(r["n"], r["nummodes"], r["numpoints"], r["lam"])
+            s.set_forcing_manufactured(1)
+            s.solve(mixed=mixed, tol=1e-6)
+            sizes = [ctypes.c_int64() for _ in range(4)]
+            nmf.check(nmf.lib().nmf_solver_sizes(s._h, *[ctypes.byref(v) for v in sizes]))
+            held = sizes[3].value  # after the solve (the mixed solve adds its FP32 buffers)
+            est = nmf.HelmholtzSolver.estimate_bytes(r["n"], r["nummodes"], r["numpoints"], False, mixed)
+            assert held <= est, (r, mixed, held, est)
 
 
 def test_budget_admission_and_eviction(nmf):
