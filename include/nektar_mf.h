@@ -217,8 +217,13 @@ int nmf_operator_apply_physderiv_host(nmf_operator op, const double *in,
  * nq0 <= 8), and the fastest is measured per (key, nummodes, numpoints,
  * precision) on the caller's data.  The choice is remembered for the rest of the process:
  * operators and solvers created later for the same kind start from it.
- * tile 0 = the library default.  Results do not depend on the tile.
+ * tile 0 = the library default.  Results do not depend on the tile,
+ * except that hex Helmholtz tiles NMF_TILE_MFMA + E (E = 1, 2, 4 elements
+ * per workgroup, NUMMODES >= 5) run the per-direction products on the
+ * matrix cores (MFMA), which sum in fours: their results agree with the
+ * vector tiles to rounding and are bit-identical among themselves.
  * ------------------------------------------------------------------- */
+#define NMF_TILE_MFMA 1024
 /* Tiles available to op (count 0 when its kernel has a single shape). */
 int nmf_operator_tile_candidates(nmf_operator op, int *tiles, int max,
                                  int *count);

@@ -33,6 +33,8 @@ __all__ = [
 
 FP64, FP32 = 0, 1
 SHAPE_HEX, SHAPE_TET, SHAPE_PRISM = 0, 1, 2
+# hex Helmholtz tiles TILE_MFMA + E: matrix-core kernels, E elements per workgroup
+TILE_MFMA = 1024
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnektar_amd.so")

@@ -23,6 +23,8 @@ void clear_last_error() { g_last_error.clear(); }
 
 using namespace nektar_amd;
 
+static_assert(kMfmaTile == NMF_TILE_MFMA, "matrix-core tile code differs from the header's");
+
 extern "C" {
 
 const char *nmf_last_error(void) { return g_last_error.c_str(); }

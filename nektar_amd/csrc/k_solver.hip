@@ -28,24 +28,37 @@ __device__ __forceinline__ bool on_boundary(int gx, int gy, int gz, int N)
 }
 
 // Block-wide deterministic sum of K values per thread; result in out[] of
-// thread 0.
+// thread 0.  Each 64-lane wave folds its values with a fixed xor butterfly
+// (cross-lane moves, no LDS round trips), lane 0 of every wave parks the
+// wave's sums in LDS, and thread 0 adds the kBlock/64 wave sums in wave
+// order: one barrier instead of log2(kBlock) + 1.  The summation order is
+// fixed, so repeated runs give identical bits.  A caller that runs two
+// block_sum<K> of the same K back to back separates them with a barrier.
 template <int K>
 __device__ __forceinline__ void block_sum(double (&v)[K], double *out)
 {
-    __shared__ double red[K][kBlock];
+    constexpr int W = 64, NW = kBlock / W;
+    static_assert(kBlock % W == 0, "block must be whole waves");
 #pragma unroll
-    for (int c = 0; c < K; ++c) red[c][threadIdx.x] = v[c];
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
+    for (int c = 0; c < K; ++c) {
 #pragma unroll
-            for (int c = 0; c < K; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-        }
-        __syncthreads();
+        for (int o = W / 2; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o, W);
     }
+    __shared__ double red[K][NW];
+    const int lane = threadIdx.x % W, wv = threadIdx.x / W;
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) red[c][wv] = v[c];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < K; ++c) out[c] = red[c][0];
+        for (int c = 0; c < K; ++c) {
+            double s = red[c][0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) s += red[c][w];
+            out[c] = s;
+        }
     }
 }
 

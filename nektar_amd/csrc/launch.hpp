@@ -87,7 +87,14 @@ struct GroupArgs {
 // multiple of 64); workgroups hold 256/tile elements (at least one).  The
 // kernels assert they use the same numbers.
 constexpr int hex_fill_elems(int nq) { return 256 / (nq * nq) > 0 ? 256 / (nq * nq) : 1; }
-inline int hex_tile_elems(int nq, int tile) { return tile > 0 ? tile : hex_fill_elems(nq); }
+// Hex Helmholtz tiles kMfmaTile + E select the matrix-core kernels with E
+// elements per 256-work-item workgroup; they are compiled for NUMMODES >=
+// kMfmaMinModes (where the per-direction products are dense enough).
+constexpr int kMfmaTile = 1024, kMfmaMinModes = 5;
+inline int hex_tile_elems(int nq, int tile)
+{
+    return tile >= kMfmaTile ? tile - kMfmaTile : (tile > 0 ? tile : hex_fill_elems(nq));
+}
 constexpr int tet_packed_tpe(int nq0) { return nq0 * (nq0 - 1); }
 constexpr int tet_wave_tpe(int nq0) { return (nq0 * (nq0 - 1) + 63) / 64 * 64; }
 // default: wave-aligned where one element fills most of a wave (Q = 8:

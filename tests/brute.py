@@ -259,7 +259,7 @@ def prism_modes_aniso(nm):
     return [(p, q, r) for p in range(nm[0]) for q in range(nm[1]) for r in range(nm[2] - p)]
 
 
-def prism_dense_own_points(nm, nq):
+def prism_dense_own_points(nm, nq, correct=True):
     """Prism basis values and standard-coordinate derivatives at the
     operator's own collapsed points (GLL x GLL x Gauss-Radau(1,0), nq[3]
     points, i fastest) and the weights W = w_a w_b (w_c / 2) of the collapsed
@@ -267,7 +267,10 @@ def prism_dense_own_points(nm, nq):
     psi^A_p(a) psi^A_q(b) psi^B_pr(c); the collapsed-edge modes (0, q, 1)
     are psi^A_q(b) (1 + c)/2, independent of a.  Derivatives from exact
     polynomial differentiation and the collapsed chain rule
-    d/dxi0 = 2/(1 - c) d/da, d/dxi2 = (1 + a)/(1 - c) d/da + d/dc."""
+    d/dxi0 = 2/(1 - c) d/da, d/dxi2 = (1 + a)/(1 - c) d/da + d/dc.
+    correct=False leaves the collapsed-edge modes as the plain products
+    psi^A_0(a) psi^A_q(b) psi^B_01(c), the expansion an operator without the
+    collapsed-edge term computes."""
     za, wa = gll(nq[0])
     zb, wb = gll(nq[1])
     zc, wc = radau_m(nq[2], 1.0)
@@ -279,7 +282,7 @@ def prism_dense_own_points(nm, nq):
     vals = np.zeros((len(a), len(modes)))
     grads = np.zeros((3, len(a), len(modes)))
     for m, (p, q, r) in enumerate(modes):
-        if p == 0 and r == 1:
+        if correct and p == 0 and r == 1:
             ta, tb, tc = one, modified_a(q), HALF_P
         else:
             ta, tb, tc = modified_a(p), modified_a(q), modified_b(p, r)

@@ -217,6 +217,25 @@ def test_prism_own_points_against_dense(nm, nq):
         assert G.rel_err(_elem(a, e, nc), want) < TOL
 
 
+@pytest.mark.parametrize("nm,nq", PRISM_CASES[:3])
+def test_prism_without_edge_term_fails_the_pin(nm, nq):
+    """The dense pin sees the collapsed-edge term (BwdTransKernels.hpp:
+    387-397): the expansion without it differs from the oracle by far more
+    than the pin's tolerance, in BwdTrans and in the Helmholtz matrix."""
+    good, grads, W = B.prism_dense_own_points(nm, nq)
+    bad, bgrads, _ = B.prism_dense_own_points(nm, nq, correct=False)
+    nqt, nc = int(np.prod(nq)), len(B.prism_modes_aniso(nm))
+    x = G.coeffs(NE * nc, 13)
+    u = O.bwdtrans_aniso(O.PRISM, nm, nq, NE, x)
+    ub = np.concatenate([bad @ _elem(x, e, nc) for e in range(NE)])
+    assert G.rel_err(u, ub) > 1e-3
+    jac, df = _geom(nqt, 5 * sum(nm) + sum(nq))
+    h = O.helmholtz_aniso(O.PRISM, nm, nq, NE, True, jac, df, 1.9, x)
+    hb = np.concatenate([B.dense_helmholtz(bad, bgrads, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
+                                           1.9) @ _elem(x, e, nc) for e in range(NE)])
+    assert G.rel_err(h, hb) > 1e-3
+
+
 def test_prism_regular_geometry_and_defaults():
     """Per-element geometry gives the per-point result with constant
     factors, and the isotropic entry points use the prism defaults
