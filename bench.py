@@ -159,6 +159,49 @@ def bench_ops(N, torch, steps=20, warmup=3):
     return out
 
 
+def bench_mfma(N, torch, steps=10, warmup=2):
+    """Vector-ALU against matrix-core (MFMA) hex Helmholtz kernels at high
+    order: P = 7, 8, 9 (NUMMODES 8, 9, 10), Q = P + 2, 13,824 elements
+    (24^3), FP64 and FP32, regular geometry, and the deformed (per-point
+    geometry) kernel at P = 7.  Every tile is timed by the autotuner; the
+    fastest tile of each family is reported with its algorithmic TFLOP/s
+    and fraction of the precision's peak."""
+    import numpy as np
+
+    E = 24 ** 3
+    rng = np.random.default_rng(8)
+    out = {}
+    for nm, deformed in ((8, False), (9, False), (10, False), (8, True)):
+        nq = nm + 1
+        if deformed:
+            nqt = nq ** 3
+            jac = rng.uniform(0.5, 1.5, E * nqt) * 1e-4
+            df = (np.tile(np.eye(3).reshape(-1), E * nqt) + 0.2 * rng.uniform(-1, 1, E * nqt * 9)) * 40.0
+        else:
+            M = (np.eye(3)[None] + 0.25 * rng.uniform(-1, 1, (E, 3, 3))) * 0.05
+            jac = np.abs(np.linalg.det(M))
+            df = np.ascontiguousarray(np.transpose(np.linalg.inv(M), (0, 2, 1))).reshape(-1)
+        F = hex_flops_per_elem(nm, nq) * E
+        for prec, tag in ((N.FP64, "fp64"), (N.FP32, "fp32")):
+            key = "Helmholtz_Hex_" + ("Deformed" if deformed else "Regular")
+            op = N.GetOperatorFactory().CreateInstance(key, nm, nq, E, prec)
+            op.SetJac(jac)
+            op.SetDF(df)
+            op.SetLambda(1.0)
+            best, times = op.autotune()
+            peak = PEAK_FP64_TFLOPS if prec == N.FP64 else 157.3
+            r = {"best_tile": best}
+            for fam, sel in (("valu", lambda t: t < N.TILE_MFMA), ("mfma", lambda t: t >= N.TILE_MFMA)):
+                cand = {t: v for t, v in times.items() if sel(t)}
+                t = min(cand, key=cand.get)
+                r[fam] = {"tile": t, "us": round(cand[t], 2), "tflops": round(F / cand[t] / 1e6, 2),
+                          "fp_frac": round(F / cand[t] / 1e6 / peak, 3)}
+            r["mfma_over_valu"] = round(r["valu"]["us"] / r["mfma"]["us"], 3)
+            out[f"P{nm - 1}_Q{nq}_{'deformed' if deformed else 'regular'}_{tag}"] = r
+            del op
+    return out
+
+
 def rotated_box_map(n, nm, seed=9):
     """The 32^3-type structured box with every element in a random local
     frame (axis permutation and reflections): map and sign of every local
@@ -495,6 +538,8 @@ def main():
                     help="skip the general assembly solve (32^3 P=6 box in random element frames)")
     ap.add_argument("--no-ops", action="store_true",
                     help="skip the per-operator timings (hex and tet, 32,768 elements, P=6)")
+    ap.add_argument("--no-mfma", action="store_true",
+                    help="skip the vector against matrix-core (MFMA) Helmholtz comparison at P = 7..9")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 512 MiB ten times (PMC byte-counter calibration)")
@@ -550,11 +595,16 @@ def main():
     value = ndof / (ms64 * 1e-3) / 1e9
     achieved = F / (ms64 * 1e-3) / 1e12
     traffic = read_traffic(f"helmholtz_fp64_tile{tile64}")
-    # the kernel is compute-bound (17.4 flop/B) on FP64 vector FMAs (no MFMA);
+    # the kernel is compute-bound (17.4 flop/B); the tile the autotuner chose
+    # runs on FP64 vector FMAs or on the matrix cores (tiles >= TILE_MFMA);
     # MI355X's FP64 vector and FP64 matrix peaks are the same 78.6 TFLOP/s
-    roofline = {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    on_mfma = tile64 >= N.TILE_MFMA
+    kname = (f"hexmf::helmholtz_kernel<double,7,8,false,{tile64 - N.TILE_MFMA}>" if on_mfma
+             else f"hex::helmholtz_kernel<double,7,8,false,{tile64}>")
+    roofline = {"bound": "mfma" if on_mfma else "valu_fp64", "achieved": round(achieved, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "kernel": f"hex::helmholtz_kernel<double,7,8,false,{tile64}>",
+                "kernel": kname,
                 "flops_per_launch": F,
                 "alg_bytes_per_launch": hex_bytes_per_elem(nm, nq, False, 8) * ne}
 
@@ -589,6 +639,8 @@ def main():
 
     if not args.no_ops:
         extra["ops_p6_32768"] = bench_ops(N, torch)
+    if not args.no_mfma:
+        extra["mfma_p7_p9"] = bench_mfma(N, torch)
     if not args.no_general:
         extra["general_32_p6"] = bench_general(N, torch)
     if not args.no_curved:

@@ -132,6 +132,9 @@ template <int AX, int NA, int NB, int E, int NQ>
 struct Panel {
     static constexpr int NL = NA * NB, NLINES = E * NL, NPAN = (NLINES + 15) / 16;
     static constexpr int NPW = (NPAN + kWaves - 1) / kWaves;
+    // panel it of the wave exists (wave-uniform; true for every it below
+    // NPAN / kWaves, so only the last iteration can branch)
+    static NMF_MD bool has(int wave, int it) { return it < NPAN / kWaves || wave + kWaves * it < NPAN; }
     bool lv;  // this lane's line exists (false on every lane of a panel past the last)
     int e, xa, xb, base;
     NMF_MD Panel(int wave, int lane, int it)
@@ -181,10 +184,9 @@ NMF_MD void store(T *dst, int base, bool lv, int g, const typename Mma<T>::v4 &a
 
 // One contraction along axis AX for every panel of the wave: src -> dst
 // (src == dst allowed: a panel writes only its own lines).  All panels'
-// inputs are loaded before the first product, with no branch between
-// panels, so the loads overlap and the products of different panels
-// interleave; a panel past the last line has no valid lane and only
-// multiplies zeros.
+// inputs are loaded before the first product, so the loads overlap and
+// the products of different panels interleave; a wave skips the products
+// of a panel slot past the last panel.
 template <typename T, int AX, int NA, int NB, int E, int NQ, int NIN, int NOUT, int KS>
 NMF_MD void stage(const T (&M)[KS], const T *src, T *dst, int wave, int lane)
 {
@@ -203,6 +205,7 @@ NMF_MD void stage(const T (&M)[KS], const T *src, T *dst, int wave, int lane)
     }
 #pragma unroll
     for (int it = 0; it < NPW; ++it) {
+        if (!P::has(wave, it)) break;
         const typename Mma<T>::v4 zero = {T(0), T(0), T(0), T(0)};
         store<T, NOUT, SD>(dst, base[it], lv[it], g, chain<T, KS>(M, b[it], zero));
     }
@@ -291,9 +294,18 @@ NMF_MD void helmholtz_block(const T *__restrict__ tabp, const T *__restrict__ ge
             load_panel<T, KM, NM, NQ * NQ>(A, pn.base, pn.lv, g, b[it]);
         }
 #pragma unroll
-        for (int it = 0; it < N2; ++it) ureg[it] = chain<T, KM>(Ab, b[it], zero);
+        for (int it = 0; it < N2; ++it) {
+            ureg[it] = zero;
+            greg[it] = zero;
+        }
 #pragma unroll
         for (int it = 0; it < N2; ++it) {
+            if (!P2::has(wave, it)) break;
+            ureg[it] = chain<T, KM>(Ab, b[it], zero);
+        }
+#pragma unroll
+        for (int it = 0; it < N2; ++it) {
+            if (!P2::has(wave, it)) break;
             greg[it] = chain<T, KQ>(Ad, ureg[it], zero);
             store<T, NQ, NQ * NQ>(B, base2[it], lv2[it], g, ureg[it]);
         }
@@ -383,7 +395,10 @@ NMF_MD void helmholtz_block(const T *__restrict__ tabp, const T *__restrict__ ge
             }
         }
 #pragma unroll
-        for (int it = 0; it < N1; ++it) store<T, NQ, NQ>(C, base[it], lv[it], g, chain<T, KQ>(At, b[it], acc[it]));
+        for (int it = 0; it < N1; ++it) {
+            if (!P::has(wave, it)) break;
+            store<T, NQ, NQ>(C, base[it], lv[it], g, chain<T, KQ>(At, b[it], acc[it]));
+        }
     }
     barrier();
     // axis 2: z = D2^T V2 + (axis-0 and axis-1 parts) + mass term, then
@@ -399,9 +414,15 @@ NMF_MD void helmholtz_block(const T *__restrict__ tabp, const T *__restrict__ ge
             }
         }
 #pragma unroll
-        for (int it = 0; it < N2; ++it) z[it] = chain<T, KQ>(At, greg[it], z[it]);
+        for (int it = 0; it < N2; ++it) {
+            if (!P2::has(wave, it)) break;
+            z[it] = chain<T, KQ>(At, greg[it], z[it]);
+        }
 #pragma unroll
-        for (int it = 0; it < N2; ++it) store<T, NM, NQ * NQ>(B, base2[it], lv2[it], g, chain<T, KQ>(Ai, z[it], zero));
+        for (int it = 0; it < N2; ++it) {
+            if (!P2::has(wave, it)) break;
+            store<T, NM, NQ * NQ>(B, base2[it], lv2[it], g, chain<T, KQ>(Ai, z[it], zero));
+        }
     }
     barrier();
     // j -> q along axis 1, lines (i, r): B -> A
