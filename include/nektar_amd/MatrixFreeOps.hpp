@@ -35,7 +35,7 @@ inline void NmfCheck(int rc)
 }
 
 namespace LibUtilities {
-enum ShapeType { eHexahedron = 0, eTetrahedron = 1, ePrism = 2 };
+enum ShapeType { eHexahedron = 0, eTetrahedron = 1, ePrism = 2, ePyramid = 3 };
 
 // The part of BasisKey the matrix-free operators read: modes and points
 // of a MODIFIED expansion (Modified_A on GLL for hex; A/B/C on GLL and
@@ -51,7 +51,10 @@ namespace MatrixFree {
 inline std::string GetOpstring(LibUtilities::ShapeType shape, bool deformed = false)
 {
     std::string s = "_";
-    s += shape == LibUtilities::eHexahedron ? "Hex" : shape == LibUtilities::eTetrahedron ? "Tet" : "Prism";
+    s += shape == LibUtilities::eHexahedron    ? "Hex"
+         : shape == LibUtilities::eTetrahedron ? "Tet"
+         : shape == LibUtilities::ePrism       ? "Prism"
+                                               : "Pyr";
     s += deformed ? "_Deformed" : "_Regular";
     return s;
 }

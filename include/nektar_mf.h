@@ -54,6 +54,7 @@ extern "C" {
 #define NMF_SHAPE_HEX 0
 #define NMF_SHAPE_TET 1
 #define NMF_SHAPE_PRISM 2
+#define NMF_SHAPE_PYR 3
 
 typedef struct nmf_operator_s *nmf_operator;
 typedef struct nmf_solver_s *nmf_solver;
@@ -96,12 +97,15 @@ int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0,
  * by GetOpstring, Op in {BwdTrans, IProduct, IProductWRTDerivBase,
  * PhysDeriv, Helmholtz, LinearAdvectionDiffusionReaction,
  * PhysInterp1DScaled},
- * Shape in {Hex, Tet, Prism}.  nummodes = P+1 per direction, numpoints0 =
- * Q in direction 0 (tets use Q-1 Gauss-Radau points in directions 1, 2,
- * prisms in direction 2).  Prisms (Modified_A, Modified_A, Modified_B;
- * BwdTransKernels.hpp:334-405, IProductKernels.hpp:519-655,
- * HelmholtzKernels.hpp:918-1119) always run the runtime-size kernels;
- * PhysInterp1DScaled takes hexes and tets only. */
+ * Shape in {Hex, Tet, Prism, Pyr}.  nummodes = P+1 per direction,
+ * numpoints0 = Q in direction 0 (tets use Q-1 Gauss-Radau points in
+ * directions 1, 2, prisms and pyramids in direction 2).  Prisms
+ * (Modified_A, Modified_A, Modified_B; BwdTransKernels.hpp:334-405,
+ * IProductKernels.hpp:519-655, HelmholtzKernels.hpp:918-1119) and pyramids
+ * (Modified_A, Modified_A, ModifiedPyr_C on Gauss-Radau(2,0) points, equal
+ * modes in the three directions; BwdTransKernels.hpp:407-489,
+ * IProductKernels.hpp:660-815, HelmholtzKernels.hpp:1119-1326) always run
+ * the runtime-size kernels; PhysInterp1DScaled takes hexes and tets only. */
 int nmf_operator_create(const char *key, int nummodes, int numpoints0,
                         int nelmt, int precision, nmf_operator *out);
 /* The same with modes and points per direction, as the reference's

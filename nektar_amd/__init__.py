@@ -32,7 +32,7 @@ __all__ = [
 ]
 
 FP64, FP32 = 0, 1
-SHAPE_HEX, SHAPE_TET, SHAPE_PRISM = 0, 1, 2
+SHAPE_HEX, SHAPE_TET, SHAPE_PRISM, SHAPE_PYR = 0, 1, 2, 3
 # hex Helmholtz tiles TILE_MFMA + E: matrix-core kernels, E elements per workgroup
 TILE_MFMA = 1024
 
@@ -183,7 +183,7 @@ def _dptr(a: np.ndarray):
 
 
 def GetOpstring(shape: str, deformed: bool = False) -> str:
-    """``MatrixFree::GetOpstring`` (Operator.cpp:46-90) for Hex/Tet/Prism."""
+    """``MatrixFree::GetOpstring`` (Operator.cpp:46-90) for Hex/Tet/Prism/Pyr."""
     return "_" + shape + ("_Deformed" if deformed else "_Regular")
 
 

@@ -11,6 +11,7 @@
 // Basis.cpp:466-645 (Modified_A/B/C) with dbdata = D * bdata (:493-495).
 #include "basis.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -165,6 +166,17 @@ double modified_b(int p, int q, double x)
     return mp * pl * jacobi(q - 1, 2.0 * p - 1.0, 1.0, x);
 }
 
+// ModifiedPyr_C entry of pyramid mode (p, q, r) at x (Basis.cpp:646-741):
+// Modified_B row max(p, q) when p or q is a vertex index (<= 1), else
+// ((1-x)/2)^(p+q-2), times (1+x)/2 P_{r-1}^{(2p+2q-3, 1)}(x) for r >= 1.
+double modified_pyr_c(int p, int q, int r, double x)
+{
+    if (p <= 1 || q <= 1) return modified_b(std::max(p, q), r, x);
+    const double mp = std::pow(0.5 * (1.0 - x), p + q - 2);
+    if (r == 0) return mp;
+    return mp * 0.5 * (1.0 + x) * jacobi(r - 1, 2.0 * p + 2.0 * q - 3.0, 1.0, x);
+}
+
 double modified_a(int p, double x)
 {
     const double m = 0.5 * (1.0 - x), pl = 0.5 * (1.0 + x);
@@ -198,6 +210,15 @@ Basis1D make_basis(BasisType t, int nm, int nq)
         for (int p = 0; p < nm; ++p)
             for (int q = 0; q < nm - p; ++q, ++m)
                 for (int i = 0; i < nq; ++i) B.bdata[m * nq + i] = modified_b(p, q, z[i]);
+    } else if (t == BasisType::ModifiedPyrC) {
+        // pyramid modes (p, q, r), r < nm - max(p, q), r fastest
+        B.nmodes = nm * (nm + 1) * (2 * nm + 1) / 6;
+        B.bdata.resize((size_t)B.nmodes * nq);
+        size_t m = 0;
+        for (int p = 0; p < nm; ++p)
+            for (int q = 0; q < nm; ++q)
+                for (int r = 0; r < nm - std::max(p, q); ++r, ++m)
+                    for (int i = 0; i < nq; ++i) B.bdata[m * nq + i] = modified_pyr_c(p, q, r, z[i]);
     } else {
         B.nmodes = nm * (nm + 1) * (nm + 2) / 6;
         B.bdata.resize((size_t)B.nmodes * nq);
@@ -221,10 +242,12 @@ Basis1D make_basis(BasisType t, int nm, int nq)
 int num_coeffs(Shape s, int nm)
 {
     if (s == Shape::Prism) return nm * nm * (nm + 1) / 2;
+    if (s == Shape::Pyr) return nm * (nm + 1) * (2 * nm + 1) / 6;
     return s == Shape::Hex ? nm * nm * nm : nm * (nm + 1) * (nm + 2) / 6;
 }
 
-// the MODIFIED defaults: hex Q, Q, Q; tet Q, Q-1, Q-1; prism Q, Q, Q-1
+// the MODIFIED defaults: hex Q, Q, Q; tet Q, Q-1, Q-1; prism and pyramid
+// Q, Q, Q-1
 void default_points(Shape s, int nq0, int nq[3])
 {
     nq[0] = nq0;
@@ -236,6 +259,7 @@ BasisType default_basis(Shape s, int dir)
 {
     if (s == Shape::Hex) return BasisType::ModifiedA;
     if (s == Shape::Prism) return dir == 2 ? BasisType::ModifiedB : BasisType::ModifiedA;
+    if (s == Shape::Pyr) return dir == 2 ? BasisType::ModifiedPyrC : BasisType::ModifiedA;
     return dir == 0 ? BasisType::ModifiedA : dir == 1 ? BasisType::ModifiedB : BasisType::ModifiedC;
 }
 
@@ -243,6 +267,7 @@ PointsType default_points_type(Shape s, int dir)
 {
     if (s == Shape::Hex) return PointsType::GLL;
     if (s == Shape::Prism) return dir == 2 ? PointsType::RadauM_A1B0 : PointsType::GLL;
+    if (s == Shape::Pyr) return dir == 2 ? PointsType::RadauM_A2B0 : PointsType::GLL;  // MeshGraph.cpp:1754-1770
     return dir == 0 ? PointsType::GLL : dir == 1 ? PointsType::RadauM_A1B0 : PointsType::RadauM_A2B0;
 }
 

@@ -15,8 +15,8 @@
 namespace nektar_amd {
 
 enum class PointsType { GLL = 0, RadauM_A1B0 = 1, RadauM_A2B0 = 2 };
-enum class BasisType { ModifiedA = 0, ModifiedB = 1, ModifiedC = 2 };
-enum class Shape { Hex = 0, Tet = 1, Prism = 2 };
+enum class BasisType { ModifiedA = 0, ModifiedB = 1, ModifiedC = 2, ModifiedPyrC = 3 };
+enum class Shape { Hex = 0, Tet = 1, Prism = 2, Pyr = 3 };
 
 struct Points1D {
     int nq = 0;

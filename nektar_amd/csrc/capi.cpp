@@ -72,10 +72,11 @@ int nmf_basis_tables(int shape, int dir, int nummodes, int numpoints0, int *nq_d
                      double *bdata, double *dbdata, double *z, double *w, double *D)
 {
     return guard([&] {
-        NMF_REQUIRE(shape == NMF_SHAPE_HEX || shape == NMF_SHAPE_TET || shape == NMF_SHAPE_PRISM,
-                    NMF_ERR_UNSUPPORTED, "shape must be hex, tet or prism");
+        NMF_REQUIRE(shape == NMF_SHAPE_HEX || shape == NMF_SHAPE_TET || shape == NMF_SHAPE_PRISM ||
+                        shape == NMF_SHAPE_PYR,
+                    NMF_ERR_UNSUPPORTED, "shape must be hex, tet, prism or pyramid");
         NMF_REQUIRE(dir >= 0 && dir < 3, NMF_ERR_INVALID, "dir must be 0..2");
-        const Shape s = shape == NMF_SHAPE_HEX ? Shape::Hex : shape == NMF_SHAPE_TET ? Shape::Tet : Shape::Prism;
+        const Shape s = static_cast<Shape>(shape);
         int nq[3];
         default_points(s, numpoints0, nq);
         Basis1D b = make_basis(default_basis(s, dir), nummodes, nq[dir]);

@@ -16,6 +16,11 @@ int generic_ncoeffs(Shape s, const int nm[3])
         for (int p = 0; p < nm[0]; ++p) n += nm[1] * (nm[2] - p);
         return n;
     }
+    if (s == Shape::Pyr) {
+        for (int p = 0; p < nm[0]; ++p)
+            for (int q = 0; q < nm[1]; ++q) n += nm[2] - std::max(p, q);
+        return n;
+    }
     for (int p = 0; p < nm[0]; ++p)
         for (int q = 0; q < nm[1] - p; ++q) n += nm[2] - p - q;
     return n;
@@ -38,6 +43,9 @@ void generic_check(Shape s, const int nm[3], const int nq[3])
     if (s == Shape::Prism)
         NMF_REQUIRE(nm[0] <= nm[2], NMF_ERR_UNSUPPORTED,
                     "prism modes must satisfy nm0 <= nm2 (the collapsed Modified_A/A/B expansion)");
+    if (s == Shape::Pyr)  // BwdTrans3DWorkspace's precondition (BwdTransKernels.hpp:537-544)
+        NMF_REQUIRE(nm[0] == nm[1] && nm[1] == nm[2], NMF_ERR_UNSUPPORTED,
+                    "pyramid modes must be equal in the three directions (ModifiedPyr_C is built for nm0 = nm1 = nm2)");
 }
 
 GenLayout generic_layout(Shape s, const int nm[3], const int nq[3])
@@ -61,7 +69,8 @@ GenLayout generic_layout(Shape s, const int nm[3], const int nq[3])
     }
     // tet half-space factors h0 = (1+z0)/2, h1 = (1+z1)/2, h2 = 2/(1-z1),
     // h3 = 2/(1-z2) (HelmholtzKernels.hpp:1368-1389); prisms use h0 and h3
-    // (the reference's prism h0, h1, :1390-1403); unused for hexes
+    // (the reference's prism h0, h1, :1390-1403), pyramids h0, h1 and h3
+    // (the reference's pyramid h0, h1, h2, :1405-1423); unused for hexes
     L.oH[0] = (int)L.tab.size();
     for (double z : b[0].pts.z) L.tab.push_back(0.5 * (1.0 + z));
     L.oH[1] = (int)L.tab.size();
@@ -100,6 +109,29 @@ GenLayout generic_layout(Shape s, const int nm[3], const int nq[3])
         L.npq = col;
         for (auto *v : {&cP, &cPair, &cM2, &pC0, &pM20, &pNR, &pP}) L.itab.insert(L.itab.end(), v->begin(), v->end());
         buf = std::max({buf, (size_t)L.npq * n2, (size_t)m0 * n2 * n1});
+    } else if (s == Shape::Pyr) {
+        // columns (p, q), p-major, each with its first coefficient, first
+        // ModifiedPyr_C mode (the coefficient index: the table runs in
+        // coefficient order), r count nm2 - max(p, q) and p; per coefficient
+        // its p, column and ModifiedPyr_C mode
+        std::vector<int> cP, cPair, cM2, pC0, pM20, pNR, pP;
+        int c = 0, col = 0;
+        for (int p = 0; p < m0; ++p)
+            for (int q = 0; q < m1; ++q, ++col) {
+                const int nr = m2 - std::max(p, q);
+                pC0.push_back(c);
+                pM20.push_back(c);
+                pNR.push_back(nr);
+                pP.push_back(p);
+                for (int r = 0; r < nr; ++r, ++c) {
+                    cP.push_back(p);
+                    cPair.push_back(col);
+                    cM2.push_back(c);
+                }
+            }
+        L.npq = col;
+        for (auto *v : {&cP, &cPair, &cM2, &pC0, &pM20, &pNR, &pP}) L.itab.insert(L.itab.end(), v->begin(), v->end());
+        buf = std::max({buf, (size_t)L.npq * n2, (size_t)m0 * n2 * n1, (size_t)n2});
     } else {
         // coefficient lists: p, (p,q) pair and Modified_C mode of each
         // coefficient; pair lists: first coefficient, first mode, r count, p

@@ -35,7 +35,7 @@ namespace nektar_amd {
 namespace gen {
 
 // element shapes of the stages below (Shape's values)
-constexpr int SH_HEX = 0, SH_TET = 1, SH_PRISM = 2;
+constexpr int SH_HEX = 0, SH_TET = 1, SH_PRISM = 2, SH_PYR = 3;
 
 template <typename T>
 struct Ctx {
@@ -314,6 +314,93 @@ NMF_GD void prism_reverse(const Ctx<T> &c, const TetIdx &ix, const T *Y, T *X1, 
     }
 }
 
+// ---- pyramid transforms (BwdTransKernels.hpp:407-489, IProductKernels.hpp:660-815)
+// Index tables as the tet (TetIdx): columns (p, q), p-major, with their
+// first coefficient, first ModifiedPyr_C mode, r count nm2 - max(p, q) and
+// p; per coefficient its p, column and mode.  Directions 0 and 1 are
+// Modified_A (indexed by p and q).  The top vertex (coefficient 1) also
+// carries (B0_0 B1_1 + B0_1 B1_0 + B0_1 B1_1) B2_1 (the reference's
+// `correct`).
+template <typename T>
+NMF_GD void pyr_forward(const Ctx<T> &c, const TetIdx &ix, const T *__restrict__ x, T *X1, T *X2, T *dst)
+{
+    const int m0 = c.a.nm[0], m1 = c.a.nm[1], npq = c.a.npq;
+    const int n0 = c.a.nq[0], n1 = c.a.nq[1], n2 = c.a.nq[2];
+    // fpq(k) = sum_r B2(mode(p, q, r), k) x(p, q, r)
+    for (int o = gtid(); o < npq * n2; o += gnt()) {
+        const int k = o % n2, col = o / n2;
+        const int c0 = ix.pC0[col], mc = ix.pM20[col], nr = ix.pNR[col];
+        T s = T(0);
+        for (int r = 0; r < nr; ++r) s = fma(c.B(2, mc + r, k), x[c0 + r], s);
+        X1[o] = s;  // [col*n2 + k]
+    }
+    gsync();
+    // fp(j, k) = sum_q B1(q, j) fpq(k)
+    for (int o = gtid(); o < m0 * n2 * n1; o += gnt()) {
+        const int j = o % n1, k = (o / n1) % n2, p = o / (n1 * n2);
+        T s = T(0);
+        for (int q = 0; q < m1; ++q) s = fma(c.B(1, q, j), X1[(p * m1 + q) * n2 + k], s);
+        X2[o] = s;  // [(p*n2 + k)*n1 + j]
+    }
+    gsync();
+    const T xtop = x[1];
+    for (int o = gtid(); o < n2 * n1 * n0; o += gnt()) {
+        const int i = o % n0, j = (o / n0) % n1, k = o / (n0 * n1);
+        T s = T(0);
+        for (int p = 0; p < m0; ++p) s = fma(c.B(0, p, i), X2[(p * n2 + k) * n1 + j], s);
+        const T b00 = c.B(0, 0, i), b01 = c.B(0, 1, i), b10 = c.B(1, 0, j), b11 = c.B(1, 1, j);
+        T t = b00 * b11;
+        t = fma(b01, b10, t);
+        t = fma(b01, b11, t);
+        s = fma(xtop, t * c.B(2, 1, k), s);
+        dst[o] = s;
+    }
+}
+
+template <typename T>
+NMF_GD void pyr_reverse(const Ctx<T> &c, const TetIdx &ix, const T *Y, T *X1, T *X2, T *X3, T *__restrict__ out)
+{
+    const int m0 = c.a.nm[0], m1 = c.a.nm[1], npq = c.a.npq, nc = c.a.ncoeff;
+    const int n0 = c.a.nq[0], n1 = c.a.nq[1], n2 = c.a.nq[2];
+    // A_p(j, k) = sum_i B0(p, i) y(i, j, k)
+    for (int o = gtid(); o < m0 * n2 * n1; o += gnt()) {
+        const int j = o % n1, k = (o / n1) % n2, p = o / (n1 * n2);
+        T s = T(0);
+        for (int i = 0; i < n0; ++i) s = fma(c.B(0, p, i), Y[(k * n1 + j) * n0 + i], s);
+        X1[o] = s;  // [(p*n2 + k)*n1 + j]
+    }
+    gsync();
+    // A_pq(k) = sum_j B1(q, j) A_p(j, k); the top-vertex sum per k
+    for (int o = gtid(); o < npq * n2 + n2; o += gnt()) {
+        if (o < npq * n2) {
+            const int k = o % n2, col = o / n2, p = ix.pP[col], q = col - p * m1;
+            T s = T(0);
+            for (int j = 0; j < n1; ++j) s = fma(c.B(1, q, j), X1[(p * n2 + k) * n1 + j], s);
+            X2[o] = s;  // [col*n2 + k]
+        } else {
+            // B1(1) A_0 + B1(0) A_1 + B1(1) A_1
+            const int k = o - npq * n2;
+            const T *a0 = X1 + (0 * n2 + k) * n1, *a1 = X1 + (1 * n2 + k) * n1;
+            T s = T(0);
+            for (int j = 0; j < n1; ++j) {
+                s = fma(c.B(1, 1, j), a0[j], s);
+                s = fma(c.B(1, 0, j), a1[j], s);
+                s = fma(c.B(1, 1, j), a1[j], s);
+            }
+            X3[k] = s;
+        }
+    }
+    gsync();
+    for (int o = gtid(); o < nc; o += gnt()) {
+        const int col = ix.cPair[o], mc = ix.cM2[o];
+        T s = T(0);
+        for (int k = 0; k < n2; ++k) s = fma(c.B(2, mc, k), X2[col * n2 + k], s);
+        if (o == 1)
+            for (int k = 0; k < n2; ++k) s = fma(c.B(2, 1, k), X3[k], s);
+        out[o] = s;
+    }
+}
+
 // ---- pointwise stages -------------------------------------------------
 // geometry of point q of element e: regular g[e*G + c], deformed
 // g[(e*G + c)*nqtot + q]
@@ -378,6 +465,49 @@ NMF_GD void helmholtz_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T *
             } else {
                 mass = (lambda * gg[6]) * m;
             }
+        } else if constexpr (SH == SH_PYR) {
+            // DiffusionCoeffPyrKernel (HelmholtzKernels.hpp:1119-1326):
+            // s0..2 = h2 df[0,3,6] + h0 h2 df[2,5,8], s3..5 = h2 df[1,4,7] +
+            // h1 h2 df[2,5,8], h0 = (1+z0)/2, h1 = (1+z1)/2, h2 = 2/(1-z2)
+            T f[9];
+            for (int cc = 0; cc < 9; ++cc) f[cc] = geo_at(a, g, 10, e, cc, q);
+            const T jw = geo_at(a, g, 10, e, 9, q) * w;
+            const T h2 = c.H(3, k), h0h2 = c.H(0, i) * h2, h1h2 = c.H(1, j) * h2;
+            const T s0 = fma(h0h2, f[2], h2 * f[0]), s1 = fma(h0h2, f[5], h2 * f[3]),
+                    s2 = fma(h0h2, f[8], h2 * f[6]);
+            const T s3 = fma(h1h2, f[2], h2 * f[1]), s4 = fma(h1h2, f[5], h2 * f[4]),
+                    s5 = fma(h1h2, f[8], h2 * f[7]);
+            T g0 = s0 * s0;
+            g0 = fma(s1, s1, g0);
+            g0 = fma(s2, s2, g0);
+            T g1 = s3 * s3;
+            g1 = fma(s4, s4, g1);
+            g1 = fma(s5, s5, g1);
+            T g2 = f[2] * f[2];
+            g2 = fma(f[5], f[5], g2);
+            g2 = fma(f[8], f[8], g2);
+            T g3 = s0 * s3;
+            g3 = fma(s1, s4, g3);
+            g3 = fma(s2, s5, g3);
+            T g4 = f[2] * s0;
+            g4 = fma(f[5], s1, g4);
+            g4 = fma(f[8], s2, g4);
+            T g5 = f[2] * s3;
+            g5 = fma(f[5], s4, g5);
+            g5 = fma(f[8], s5, g5);
+            t0 = g0 * a0;
+            t0 = fma(g3, a1, t0);
+            t0 = fma(g4, a2, t0);
+            t1 = g3 * a0;
+            t1 = fma(g1, a1, t1);
+            t1 = fma(g5, a2, t1);
+            t2 = g4 * a0;
+            t2 = fma(g5, a1, t2);
+            t2 = fma(g2, a2, t2);
+            t0 *= jw;
+            t1 *= jw;
+            t2 *= jw;
+            mass = lambda * (jw * m);
         } else if constexpr (SH == SH_PRISM) {
             // DiffusionCoeffPrismKernel (HelmholtzKernels.hpp:918-1119):
             // s_c = h1 (h0 df[3c+2] + df[3c]), h0 = (1+z0)/2, h1 = 2/(1-z2)
@@ -498,6 +628,15 @@ NMF_GD void physderiv_pointwise(const Ctx<T> &c, int e, const T *X1, const T *X2
             d0 = d0 * c.H(3, k);
             d2 = fma(c.H(0, i), d0, d2);
         }
+        if constexpr (SH == SH_PYR) {
+            // PhysDerivKernels.hpp:301-311: d0, d1 *= 2/(1-z2),
+            // d2 += (1+z0)/2 d0 + (1+z1)/2 d1
+            const int i = q % n0, j = (q / n0) % n1, k = q / (n0 * n1);
+            d0 = d0 * c.H(3, k);
+            d1 = d1 * c.H(3, k);
+            d2 = fma(c.H(0, i), d0, d2);
+            d2 = fma(c.H(1, j), d1, d2);
+        }
         if constexpr (SH == SH_TET) {
             const int i = q % n0, j = (q / n0) % n1, k = q / (n0 * n1);
             const Half<T> h = half_at(c, i, j, k);
@@ -569,6 +708,11 @@ NMF_GD void iprod_deriv_pointwise(const Ctx<T> &c, int e, T *X0, T *X1, T *X2, T
             if constexpr (SH == SH_PRISM) {
                 // IProductWRTDerivBaseKernels.hpp:391-403: t0 <- h1 (t0 + h0 t2)
                 t0 = c.H(3, k) * fma(c.H(0, i), t2, t0);
+            } else if constexpr (SH == SH_PYR) {
+                // IProductWRTDerivBaseKernels.hpp:469-488: t0 <- h2 (t0 + h0 t2),
+                // t1 <- h2 (t1 + h1 t2)
+                t0 = c.H(3, k) * fma(c.H(0, i), t2, t0);
+                t1 = c.H(3, k) * fma(c.H(1, j), t2, t1);
             } else {
                 const Half<T> h = half_at(c, i, j, k);
                 t0 = fma(t1 + t2, h.h0, t0);
@@ -617,6 +761,7 @@ NMF_GD void element(const GenArgs &a, int e, T *X0, T *X1, T *X2, T *X3)
         T *dst = op == OpType::BwdTrans ? static_cast<T *>(a.out0) + (size_t)e * a.nqtot : X0;
         if constexpr (SH == SH_TET) tet_forward(c, ix, x, X1, X2, dst);
         else if constexpr (SH == SH_PRISM) prism_forward(c, ix, x, X1, X2, dst);
+        else if constexpr (SH == SH_PYR) pyr_forward(c, ix, x, X1, X2, dst);
         else hex_forward(c, x, X1, X2, dst);
         if (op == OpType::BwdTrans) return;
     } else if (op == OpType::PhysDeriv) {
@@ -645,6 +790,7 @@ NMF_GD void element(const GenArgs &a, int e, T *X0, T *X1, T *X2, T *X3)
     T *out = static_cast<T *>(a.out0) + (size_t)e * a.ncoeff;
     if constexpr (SH == SH_TET) tet_reverse(c, ix, X0, X1, X2, X3, out);
     else if constexpr (SH == SH_PRISM) prism_reverse(c, ix, X0, X1, X2, out);
+    else if constexpr (SH == SH_PYR) pyr_reverse(c, ix, X0, X1, X2, X3, out);
     else hex_reverse(c, X0, X1, X2, out);
 }
 

@@ -73,6 +73,8 @@ hipError_t launch_generic(const GenArgs &a)
         return a.fp32 ? gen::launch_t<float, gen::SH_HEX>(a) : gen::launch_t<double, gen::SH_HEX>(a);
     if (a.shape == Shape::Prism)
         return a.fp32 ? gen::launch_t<float, gen::SH_PRISM>(a) : gen::launch_t<double, gen::SH_PRISM>(a);
+    if (a.shape == Shape::Pyr)
+        return a.fp32 ? gen::launch_t<float, gen::SH_PYR>(a) : gen::launch_t<double, gen::SH_PYR>(a);
     return a.fp32 ? gen::launch_t<float, gen::SH_TET>(a) : gen::launch_t<double, gen::SH_TET>(a);
 }
 

@@ -32,7 +32,8 @@ void parse_operator_key(const std::string &key, OpType &op, Shape &shape, bool &
     if (s == "Hex") shape = Shape::Hex;
     else if (s == "Tet") shape = Shape::Tet;
     else if (s == "Prism") shape = Shape::Prism;
-    else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet, Prism)");
+    else if (s == "Pyr") shape = Shape::Pyr;
+    else throw Error(NMF_ERR_UNSUPPORTED, "shape '" + s + "' is not provided (Hex, Tet, Prism, Pyr)");
     if (g == "Regular") deformed = false;
     else if (g == "Deformed") deformed = true;
     else throw Error(NMF_ERR_INVALID, "geometry tag '" + g + "' must be Regular or Deformed");
@@ -128,7 +129,7 @@ void MFOperator::init(const std::string &key, const int nummodes[3], const int n
     int dq[3];
     default_points(shape, nq[0], dq);
     const bool iso = nmd[1] == nm && nmd[2] == nm && nq[1] == dq[1] && nq[2] == dq[2];
-    // (prisms always run the runtime-size kernels)
+    // (prisms and pyramids always run the runtime-size kernels)
     const bool compiled = iso && (shape == Shape::Hex   ? hex_op_supported(nm, nq[0])
                                   : shape == Shape::Tet ? tet_supported(nm, nq[0])
                                                         : false);
@@ -451,6 +452,14 @@ std::vector<double> MFOperator::pack_advection() const
                         a[0] = h2 * h3 * (a0 + h0 * (a1 + a2));
                         a[1] = h3 * (a1 + h1 * a2);
                         a[2] = a2;
+                    } else if (shape == Shape::Pyr) {
+                        // AdvectionPyrKernel (LinearAdvectionDiffusionReactionKernels.hpp:
+                        // 475-560): d/dxi0 = h2 d0, d/dxi1 = h2 d1, d/dxi2 =
+                        // h0 h2 d0 + h1 h2 d1 + d2 with h0 = (1+z0)/2,
+                        // h1 = (1+z1)/2, h2 = 2/(1-z2)
+                        const double h0 = 0.5 * (1.0 + z0[i]), h1 = 0.5 * (1.0 + z1[j]), h2 = 2.0 / (1.0 - z2[k]);
+                        a[0] = h2 * (a[0] + h0 * a[2]);
+                        a[1] = h2 * (a[1] + h1 * a[2]);
                     } else if (shape == Shape::Prism) {
                         // AdvectionPrismKernel (LinearAdvectionDiffusionReactionKernels.hpp:
                         // 384-470): d/dxi0 = h1 d0, d/dxi2 = h0 h1 d0 + d2 with

@@ -74,11 +74,22 @@ static int prism_ncoeffs3(int nm0, int nm1, int nm2)
     return n;
 }
 
+/* Coefficients of a pyramid: p < nm0, q < nm1, r < nm2 - max(p, q)
+ * (StdPyrExp, the order BwdTransKernels.hpp:419-446 walks) */
+static int pyr_ncoeffs3(int nm0, int nm1, int nm2)
+{
+    int n = 0;
+    for (int p = 0; p < nm0; ++p)
+        for (int q = 0; q < nm1; ++q) n += nm2 - (p > q ? p : q);
+    return n;
+}
+
 int or_ncoeffs_aniso(int shape, const int nm[3])
 {
-    return shape == OR_HEX   ? nm[0] * nm[1] * nm[2]
-           : shape == OR_TET ? tet_ncoeffs3(nm[0], nm[1], nm[2])
-                             : prism_ncoeffs3(nm[0], nm[1], nm[2]);
+    return shape == OR_HEX     ? nm[0] * nm[1] * nm[2]
+           : shape == OR_TET   ? tet_ncoeffs3(nm[0], nm[1], nm[2])
+           : shape == OR_PRISM ? prism_ncoeffs3(nm[0], nm[1], nm[2])
+                               : pyr_ncoeffs3(nm[0], nm[1], nm[2]);
 }
 
 static void tables_init_full(tables_t *t, int shape, const int nm[3], const int nq[3])
@@ -91,11 +102,14 @@ static void tables_init_full(tables_t *t, int shape, const int nm[3], const int 
     }
     t->nqtot = t->nq[0] * t->nq[1] * t->nq[2];
     t->ncoeff = or_ncoeffs_aniso(shape, nm);
-    /* prism: Modified_A, Modified_A, Modified_B on GLL, GLL, Radau(1,0) */
+    /* prism: Modified_A, Modified_A, Modified_B on GLL, GLL, Radau(1,0);
+     * pyramid: Modified_A, Modified_A, ModifiedPyr_C on GLL, GLL, Radau(2,0)
+     * (MeshGraph.cpp:1754-1770) */
     static const int prism_bt[3] = {OR_MOD_A, OR_MOD_A, OR_MOD_B}, prism_pt[3] = {OR_GLL, OR_GLL, OR_RADAU_A1};
+    static const int pyr_bt[3] = {OR_MOD_A, OR_MOD_A, OR_MOD_PYR_C}, pyr_pt[3] = {OR_GLL, OR_GLL, OR_RADAU_A2};
     for (int d = 0; d < 3; ++d) {
-        int bt = shape == OR_HEX ? OR_MOD_A : shape == OR_TET ? d : prism_bt[d];
-        int pt = shape == OR_HEX ? OR_GLL : shape == OR_TET ? d : prism_pt[d];
+        int bt = shape == OR_HEX ? OR_MOD_A : shape == OR_TET ? d : shape == OR_PRISM ? prism_bt[d] : pyr_bt[d];
+        int pt = shape == OR_HEX ? OR_GLL : shape == OR_TET ? d : shape == OR_PRISM ? prism_pt[d] : pyr_pt[d];
         int nq = t->nq[d];
         int nmodes = or_basis_nmodes(bt, nm[d]);
         t->b[d] = (double *)malloc((size_t)nmodes * nq * sizeof(double));
@@ -122,6 +136,13 @@ static void tables_init_full(tables_t *t, int shape, const int nm[3], const int 
         /* HelmholtzKernels.hpp:1390-1403 (prism): h0[i] = (1 + z0)/2 and
          * h1[k] = 2/(1 - z2), kept here in h0 and h3 */
         for (int i = 0; i < t->nq[0]; ++i) t->h0[i] = 0.5 * (1 + t->Z[0][i]);
+        for (int k = 0; k < t->nq[2]; ++k) t->h3[k] = 2.0 / (1 - t->Z[2][k]);
+    }
+    if (shape == OR_PYR) {
+        /* HelmholtzKernels.hpp:1405-1423 (pyramid): h0[i] = (1 + z0)/2,
+         * h1[j] = (1 + z1)/2, h2[k] = 2/(1 - z2), kept here in h0, h1, h3 */
+        for (int i = 0; i < t->nq[0]; ++i) t->h0[i] = 0.5 * (1 + t->Z[0][i]);
+        for (int j = 0; j < t->nq[1]; ++j) t->h1[j] = 0.5 * (1 + t->Z[1][j]);
         for (int k = 0; k < t->nq[2]; ++k) t->h3[k] = 2.0 / (1 - t->Z[2][k]);
     }
 }
@@ -259,6 +280,54 @@ static void bwd_prism(int nm0, int nm1, int nm2, int nq0, int nq1, int nq2,
                         const double basis_1 = b1[q * nq1 + j];
                         val = vfma(val, vs(basis_2 * basis_1), vmuls(in[q * nm2 + 1], basis_0));
                     }
+                }
+                out[cnt_kji] = val;
+            }
+        }
+    }
+}
+
+/* BwdTransKernels.hpp:407-489 (BwdTransPyrKernel): B2 indexed by the mode
+ * (p, q, r); the top vertex (coefficient 1) also carries
+ * (B0_0 B1_1 + B0_1 B1_0 + B0_1 B1_1) B2_1 */
+static void bwd_pyr(int nm0, int nm1, int nm2, int nq0, int nq1, int nq2,
+                    int correct, const vd *in, const double *b0,
+                    const double *b1, const double *b2, vd *fpq, vd *fp,
+                    vd *out)
+{
+    for (int k = 0, cnt_kji = 0; k < nq2; ++k) {
+        int cnt_pqr = 0, mode_pqr = 0, mode_pq = 0;
+        for (int p = 0; p < nm0; ++p) {
+            for (int q = 0; q < p; ++q, ++mode_pq) {
+                vd prod = vz();
+                for (int r = 0; r < nm2 - p; ++r, ++mode_pqr, ++cnt_pqr)
+                    prod = vfmas(prod, in[cnt_pqr], b2[mode_pqr * nq2 + k]);
+                fpq[mode_pq] = prod;
+            }
+            for (int q = p; q < nm1; ++q, ++mode_pq) {
+                vd prod = vz();
+                for (int r = 0; r < nm2 - q; ++r, ++mode_pqr, ++cnt_pqr)
+                    prod = vfmas(prod, in[cnt_pqr], b2[mode_pqr * nq2 + k]);
+                fpq[mode_pq] = prod;
+            }
+            for (int q = nm1; q < nm2; ++q) mode_pqr += nm2 - q;
+        }
+        for (int j = 0; j < nq1; ++j) {
+            mode_pq = 0;
+            for (int p = 0; p < nm0; ++p) {
+                vd prod = vz();
+                for (int q = 0; q < nm1; ++q, ++mode_pq) prod = vfmas(prod, fpq[mode_pq], b1[q * nq1 + j]);
+                fp[p] = prod;
+            }
+            for (int i = 0; i < nq0; ++i, ++cnt_kji) {
+                vd val = vz();
+                for (int p = 0; p < nm0; ++p) val = vfmas(val, fp[p], b0[p * nq0 + i]);
+                if (correct) {
+                    double t = b0[i] * b1[nq1 + j];
+                    t = fma(b0[nq0 + i], b1[j], t);
+                    t = fma(b0[nq0 + i], b1[nq1 + j], t);
+                    t = t * b2[nq2 + k];
+                    val = vfmas(val, in[1], t);
                 }
                 out[cnt_kji] = val;
             }
@@ -452,6 +521,65 @@ static void iprod_prism(int SCALE, int APPEND, int DEF, int nm0, int nm1,
     }
 }
 
+/* IProductKernels.hpp:660-815 (IProductPyrKernel) */
+static void iprod_pyr(int SCALE, int APPEND, int DEF, int nm0, int nm1,
+                      int nm2, int nq0, int nq1, int nq2, int correct,
+                      const vd *in, const double *b0, const double *b1,
+                      const double *b2, const double *w0, const double *w1,
+                      const double *w2, const vd *jac, vd *sums_kj,
+                      vd *sums_k, vd *out, double scale)
+{
+    int mode2 = 0, mode_pqr = 0;
+    for (int p = 0; p < nm0; ++p) {
+        int cnt_kji = 0, cnt_kj = 0;
+        for (int k = 0; k < nq2; ++k)
+            for (int j = 0; j < nq1; ++j, ++cnt_kj) {
+                vd sum_kj = vz();
+                for (int i = 0; i < nq0; ++i, ++cnt_kji) {
+                    vd jv = DEF ? jac[nq0 * nq1 * k + nq0 * j + i] : jac[0];
+                    vd prod = vmuls(vmuls(jv, b0[nq0 * p + i]), w0[i]);
+                    sum_kj = vfma(sum_kj, prod, in[cnt_kji]);
+                }
+                sums_kj[cnt_kj] = sum_kj;
+            }
+        for (int q = 0; q < nm1; ++q) {
+            cnt_kj = 0;
+            for (int k = 0; k < nq2; ++k) {
+                vd sum_k = vz();
+                for (int j = 0; j < nq1; ++j, ++cnt_kj)
+                    sum_k = vfmas(sum_k, sums_kj[cnt_kj], b1[q * nq1 + j] * w1[j]);
+                sums_k[k] = sum_k;
+            }
+            const int nr = nm2 - (q < p ? p : q);
+            for (int r = 0; r < nr; ++r, ++mode2, ++mode_pqr) {
+                vd sum_k = vz();
+                for (int k = 0; k < nq2; ++k) sum_k = vfmas(sum_k, sums_k[k], b2[mode2 * nq2 + k] * w2[k]);
+                scale_append(SCALE, APPEND, &out[mode_pqr], sum_k, scale);
+            }
+        }
+        for (int q = nm1; q < nm2; ++q) mode2 += nm2 - q;
+    }
+    if (correct) {
+        /* top vertex, appended point by point as the reference does */
+        for (int k = 0, cnt = 0; k < nq2; ++k) {
+            vd tq2 = vs(w2[k]);
+            if (!DEF) tq2 = vmul(tq2, jac[0]);
+            for (int j = 0; j < nq1; ++j) {
+                vd tq1 = vmuls(tq2, w1[j]);
+                for (int i = 0; i < nq0; ++i, ++cnt) {
+                    vd tq = vmuls(tq1, w0[i]);
+                    if (DEF) tq = vmul(tq, jac[k * nq0 * nq1 + j * nq0 + i]);
+                    double t = b0[i] * b1[nq1 + j];
+                    t = fma(b0[nq0 + i], b1[j], t);
+                    t = fma(b0[nq0 + i], b1[nq1 + j], t);
+                    t = t * b2[nq2 + k];
+                    scale_append(SCALE, 1, &out[1], vmul(vmuls(in[cnt], t), tq), scale);
+                }
+            }
+        }
+    }
+}
+
 /* PhysDerivKernels.hpp:436-507 (column-major D: D[k*nq+i]) */
 static void deriv_tensor(int nq0, int nq1, int nq2, const vd *in,
                          const double *D0, const double *D1, const double *D2,
@@ -519,6 +647,22 @@ static void physderiv_chain(int shape, int DEF, int nq0, int nq1, int nq2,
                     d2[e] = vadd(o, d2[e]);
                 }
             }
+    }
+    if (shape == OR_PYR) {
+        /* PhysDerivKernels.hpp:301-311: d0, d1 *= 2/(1 - z2),
+         * d2 += (1 + z0)/2 d0 + (1 + z1)/2 d1 */
+        for (int k = 0, e = 0; k < nq2; ++k) {
+            const double x2 = 2.0 / (1.0 - Z2[k]);
+            for (int j = 0; j < nq1; ++j) {
+                const double x1 = 0.5 * (1.0 + Z1[j]);
+                for (int i = 0; i < nq0; ++i, ++e) {
+                    d0[e] = vmuls(d0[e], x2);
+                    d1[e] = vmuls(d1[e], x2);
+                    d2[e] = vfmas(d2[e], d0[e], 0.5 * (1.0 + Z0[i]));
+                    d2[e] = vfmas(d2[e], d1[e], x1);
+                }
+            }
+        }
     }
     if (shape == OR_PRISM) {
         /* PhysDerivKernels.hpp:293-301: d0 *= 2/(1 - z2), d2 += (1 + z0)/2 d0 */
@@ -709,6 +853,65 @@ static void diffusion_prism(int DEF, const tables_t *T, int cvd,
     }
 }
 
+/* HelmholtzKernels.hpp:1119-1326 (DiffusionCoeffPyrKernel, no variable
+ * diffusion): tmp0..2 = h2 df[0,3,6] + h0 h2 df[2,5,8],
+ * tmp3..5 = h2 df[1,4,7] + h1 h2 df[2,5,8] */
+static void diffusion_pyr(int DEF, const tables_t *T, int cvd,
+                          const double *dd, const vd *df, vd *d0, vd *d1,
+                          vd *d2)
+{
+    const int nq0 = T->nq[0], nq1 = T->nq[1], nq2 = T->nq[2];
+    double d00 = 1, d01 = 0, d11 = 1, d02 = 0, d12 = 0, d22 = 1;
+    if (cvd) { d00 = dd[0]; d01 = dd[1]; d11 = dd[2]; d02 = dd[3]; d12 = dd[4]; d22 = dd[5]; }
+    vd f[9];
+    if (!DEF) for (int d = 0; d < 9; ++d) f[d] = df[d];
+    for (int k = 0, cnt = 0; k < nq2; ++k) {
+        const double h2 = T->h3[k];
+        for (int j = 0; j < nq1; ++j) {
+            const double h1 = T->h1[j], h1h2 = h1 * h2;
+            for (int i = 0; i < nq0; ++i, ++cnt) {
+                const double h0 = T->h0[i], h0h2 = h0 * h2;
+                if (DEF) for (int d = 0; d < 9; ++d) f[d] = df[cnt * 9 + d];
+                vd t0 = vfmas(vmuls(f[0], h2), f[2], h0h2);
+                vd t1 = vfmas(vmuls(f[3], h2), f[5], h0h2);
+                vd t2 = vfmas(vmuls(f[6], h2), f[8], h0h2);
+                vd t3 = vfmas(vmuls(f[1], h2), f[2], h1h2);
+                vd t4 = vfmas(vmuls(f[4], h2), f[5], h1h2);
+                vd t5 = vfmas(vmuls(f[7], h2), f[8], h1h2);
+                vd g0, g1, g2, g3, g4, g5;
+                if (!cvd) {
+                    g0 = vmul(t0, t0); g0 = vfma(g0, t1, t1); g0 = vfma(g0, t2, t2);
+                    g1 = vmul(t3, t3); g1 = vfma(g1, t4, t4); g1 = vfma(g1, t5, t5);
+                    g2 = vmul(f[2], f[2]); g2 = vfma(g2, f[5], f[5]); g2 = vfma(g2, f[8], f[8]);
+                    g3 = vmul(t0, t3); g3 = vfma(g3, t1, t4); g3 = vfma(g3, t2, t5);
+                    g4 = vmul(f[2], t0); g4 = vfma(g4, f[5], t1); g4 = vfma(g4, f[8], t2);
+                    g5 = vmul(f[2], t3); g5 = vfma(g5, f[5], t4); g5 = vfma(g5, f[8], t5);
+                } else {
+                    vd s0 = vmuls(t0, d00); s0 = vfmas(s0, t1, d01); s0 = vfmas(s0, t2, d02);
+                    vd s1 = vmuls(t0, d01); s1 = vfmas(s1, t1, d11); s1 = vfmas(s1, t2, d12);
+                    vd s2 = vmuls(t0, d02); s2 = vfmas(s2, t1, d12); s2 = vfmas(s2, t2, d22);
+                    vd s3 = vmuls(t3, d00); s3 = vfmas(s3, t4, d01); s3 = vfmas(s3, t5, d02);
+                    vd s4 = vmuls(t3, d01); s4 = vfmas(s4, t4, d11); s4 = vfmas(s4, t5, d12);
+                    vd s5 = vmuls(t3, d02); s5 = vfmas(s5, t4, d12); s5 = vfmas(s5, t5, d22);
+                    vd s6 = vmuls(f[2], d00); s6 = vfmas(s6, f[5], d01); s6 = vfmas(s6, f[8], d02);
+                    vd s7 = vmuls(f[2], d01); s7 = vfmas(s7, f[5], d11); s7 = vfmas(s7, f[8], d12);
+                    vd s8 = vmuls(f[2], d02); s8 = vfmas(s8, f[5], d12); s8 = vfmas(s8, f[8], d22);
+                    g0 = vmul(s0, t0); g0 = vfma(g0, s1, t1); g0 = vfma(g0, s2, t2);
+                    g3 = vmul(s0, t3); g3 = vfma(g3, s1, t4); g3 = vfma(g3, s2, t5);
+                    g4 = vmul(s0, f[2]); g4 = vfma(g4, s1, f[5]); g4 = vfma(g4, s2, f[8]);
+                    g1 = vmul(s3, t3); g1 = vfma(g1, s4, t4); g1 = vfma(g1, s5, t5);
+                    g5 = vmul(s3, f[2]); g5 = vfma(g5, s4, f[5]); g5 = vfma(g5, s5, f[8]);
+                    g2 = vmul(s6, f[2]); g2 = vfma(g2, s7, f[5]); g2 = vfma(g2, s8, f[8]);
+                }
+                vd a0 = d0[cnt], a1 = d1[cnt], a2 = d2[cnt];
+                vd o = vmul(g0, a0); o = vfma(o, g3, a1); o = vfma(o, g4, a2); d0[cnt] = o;
+                o = vmul(g3, a0); o = vfma(o, g1, a1); o = vfma(o, g5, a2); d1[cnt] = o;
+                o = vmul(g4, a0); o = vfma(o, g5, a1); o = vfma(o, g2, a2); d2[cnt] = o;
+            }
+        }
+    }
+}
+
 static void diffusion(int DEF, const tables_t *T, int cvd, const double *dd, const vd *df, vd *d0, vd *d1,
                       vd *d2)
 {
@@ -716,8 +919,10 @@ static void diffusion(int DEF, const tables_t *T, int cvd, const double *dd, con
         diffusion_hex(DEF, T->nqtot, cvd, dd, df, d0, d1, d2);
     else if (T->shape == OR_TET)
         diffusion_tet(DEF, T, cvd, dd, df, d0, d1, d2);
-    else
+    else if (T->shape == OR_PRISM)
         diffusion_prism(DEF, T, cvd, dd, df, d0, d1, d2);
+    else
+        diffusion_pyr(DEF, T, cvd, dd, df, d0, d1, d2);
 }
 
 /* ------------------------------------------------------------------ */
@@ -783,9 +988,11 @@ static void bwd_block(const tables_t *T, const vd *in, ws_t *w, vd *out)
         bwd_hex(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
     else if (T->shape == OR_TET)
         bwd_tet(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
-    else
+    else if (T->shape == OR_PRISM)
         bwd_prism(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, T->b[0], T->b[1], T->b[2], w->w0, w->w1,
                   out);
+    else
+        bwd_pyr(m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, T->b[0], T->b[1], T->b[2], w->w0, w->w1, out);
 }
 
 static void iprod_block(const tables_t *T, int SCALE, int APPEND, int DEF,
@@ -800,12 +1007,15 @@ static void iprod_block(const tables_t *T, int SCALE, int APPEND, int DEF,
     else if (T->shape == OR_TET)
         iprod_tet(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, b0, b1, b2,
                   T->w[0], T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
-    else
+    else if (T->shape == OR_PRISM)
         /* the correction also runs with derivative tables: it then adds the
          * derivative of the collapsed-edge term (Helmholtz.h:564-599 passes
          * correct with every table set) */
         iprod_prism(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, b0, b1, b2, T->w[0],
                     T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
+    else
+        iprod_pyr(SCALE, APPEND, DEF, m0, m1, m2, T->nq[0], T->nq[1], T->nq[2], 1, in, b0, b1, b2, T->w[0],
+                  T->w[1], T->w[2], jac, w->w0, w->w1, out, scale);
 }
 
 static int g_threads = 1;
@@ -949,6 +1159,16 @@ static void advection_pointwise(const tables_t *T, int DEF, const vd *df, const 
                     for (int c = 0; c < 9; ++c) f[c] = df[(size_t)cnt * 9 + c];
                 const vd vx = vel[cnt * 3], vy = vel[cnt * 3 + 1], vzz = vel[cnt * 3 + 2];
                 vd m0 = d0[cnt], m1 = d1[cnt], m2 = d2[cnt];
+                if (T->shape == OR_PYR) {
+                    /* AdvectionPyrKernel (LinearAdvectionDiffusionReactionKernels.hpp:
+                     * 475-560): metric0 = h2 d0, metric1 = h2 d1,
+                     * metric2 = d2 + h0 h2 d0 + h1 h2 d1 */
+                    const double hy2 = T->h3[k], hy0h2 = T->h0[i] * hy2, hy1h2 = T->h1[j] * hy2;
+                    m0 = vmuls(d0[cnt], hy2);
+                    m1 = vmuls(d1[cnt], hy2);
+                    m2 = vfmas(d2[cnt], d0[cnt], hy0h2);
+                    m2 = vfmas(m2, d1[cnt], hy1h2);
+                }
                 if (T->shape == OR_PRISM) {
                     /* AdvectionPrismKernel (LinearAdvectionDiffusionReactionKernels.hpp:
                      * 384-470): metric0 = h1 d0, metric2 = h0 metric0 + d2 */
@@ -1053,6 +1273,14 @@ static void iprod_deriv_pointwise(const tables_t *T, int DEF, const vd *df,
                     t0 = vmuls(t0, f0);
                     t1 = vfmas(t1, t2, f3);
                     t1 = vmuls(t1, f2);
+                } else if (T->shape == OR_PYR) {
+                    /* IProductWRTDerivBaseKernels.hpp:469-488: t0 <- f t0 + f (h0 t2),
+                     * t1 <- f t1 + f (h1 t2), f = 2/(1 - z2) */
+                    const double hf1 = 0.5 * (1.0 + T->Z[0][i]), hf2 = 0.5 * (1.0 + z1);
+                    t0 = vmuls(t0, f2);
+                    t0 = vfmas(t0, vmuls(t2, hf1), f2);
+                    t1 = vmuls(t1, f2);
+                    t1 = vfmas(t1, vmuls(t2, hf2), f2);
                 } else if (T->shape == OR_PRISM) {
                     /* IProductWRTDerivBaseKernels.hpp:391-403: t0 <- f t0 + f (h0 t2),
                      * f = 2/(1 - z2) */

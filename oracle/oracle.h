@@ -31,9 +31,9 @@ extern "C" {
 #endif
 
 /* ---- shapes / point types ------------------------------------------- */
-enum { OR_HEX = 0, OR_TET = 1, OR_PRISM = 2 };
+enum { OR_HEX = 0, OR_TET = 1, OR_PRISM = 2, OR_PYR = 3 };
 enum { OR_GLL = 0, OR_RADAU_A1 = 1, OR_RADAU_A2 = 2 };
-enum { OR_MOD_A = 0, OR_MOD_B = 1, OR_MOD_C = 2 };
+enum { OR_MOD_A = 0, OR_MOD_B = 1, OR_MOD_C = 2, OR_MOD_PYR_C = 3 };
 
 /* ---- Polylib restatement (or_polylib.c) ----------------------------- */
 void or_jacobfd(int np, const double *z, double *poly, double *polyd, int n,

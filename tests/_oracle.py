@@ -12,9 +12,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_POLYLIB = os.path.join(ROOT, "oracle", "_ref", "libpolylib_ref.so")
 
-HEX, TET, PRISM = 0, 1, 2
+HEX, TET, PRISM, PYR = 0, 1, 2, 3
 GLL, RADAU_A1, RADAU_A2 = 0, 1, 2
-MOD_A, MOD_B, MOD_C = 0, 1, 2
+MOD_A, MOD_B, MOD_C, MOD_PYR_C = 0, 1, 2, 3
 
 _D = ctypes.POINTER(ctypes.c_double)
 _lib = None
@@ -48,11 +48,13 @@ def set_threads(n: int):
 def ncoeffs(shape, nm):
     if shape == PRISM:
         return nm * nm * (nm + 1) // 2
+    if shape == PYR:
+        return nm * (nm + 1) * (2 * nm + 1) // 6
     return nm ** 3 if shape == HEX else nm * (nm + 1) * (nm + 2) // 6
 
 
 def default_nq(shape, nq0):
-    if shape == PRISM:
+    if shape in (PRISM, PYR):
         return (nq0, nq0, nq0 - 1)
     return (nq0, nq0, nq0) if shape == HEX else (nq0, nq0 - 1, nq0 - 1)
 

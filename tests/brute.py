@@ -297,6 +297,68 @@ def prism_dense_own_points(nm, nq, correct=True):
     return vals, grads, W
 
 
+def pyr_modes(nm):
+    """Pyramid modes in the reference's order (BwdTransKernels.hpp:419-446):
+    p < nm, q < nm, r < nm - max(p, q), r fastest."""
+    return [(p, q, r) for p in range(nm) for q in range(nm) for r in range(nm - max(p, q))]
+
+
+def pyr_c(p, q, r):
+    """Direction-2 factor of pyramid mode (p, q, r), from its definition in
+    Basis.cpp:646-741: Modified_B row max(p, q) when p or q is a vertex
+    index (<= 1), else ((1-z)/2)^(p+q-2) for r = 0 and
+    ((1-z)/2)^(p+q-2) (1+z)/2 P_{r-1}^{(2p+2q-3, 1)} for r >= 1."""
+    if p <= 1 or q <= 1:
+        return modified_b(max(p, q), r)
+    s = p + q - 2
+    if r == 0:
+        return HALF_M ** s
+    return HALF_M ** s * HALF_P * _poly_jacobi(r - 1, 2.0 * p + 2.0 * q - 3.0, 1.0)
+
+
+def pyr_dense_own_points(nm, nq, correct=True):
+    """Pyramid basis values and standard-coordinate derivatives at the
+    operator's own collapsed points (GLL x GLL x Gauss-Radau(2,0), i
+    fastest) and the weights W = w_a w_b (w_c / 4) of the collapsed rule
+    (xi0 = (1 + a)(1 - c)/2 - 1, xi1 = (1 + b)(1 - c)/2 - 1, xi2 = c).
+    Mode (p, q, r) is psi^A_p(a) psi^A_q(b) psi^C_pqr(c); the top vertex
+    (0, 0, 1) is (1 + c)/2 on its own (the reference adds
+    (psi_0 psi_1 + psi_1 psi_0 + psi_1 psi_1)(1+c)/2 to psi_0 psi_0 (1+c)/2).
+    correct=False leaves it as that single product.  The a and b
+    derivatives are exact; the c derivative is that of the factor's
+    interpolant through the c points (the reference differentiates by
+    collocation, dbdata = D bdata, Basis.cpp:736-739, and the factors
+    ((1-c)/2)^(p+q-2) reach degree 2nm-4, above what nq2 points interpolate
+    exactly).  Chain rule d/dxi0 = 2/(1-c) d/da, d/dxi1 = 2/(1-c) d/db,
+    d/dxi2 = (1+a)/(1-c) d/da + (1+b)/(1-c) d/db + d/dc."""
+    za, wa = gll(nq[0])
+    zb, wb = gll(nq[1])
+    zc, wc = radau_m(nq[2], 2.0)
+    C, Bb, A = np.meshgrid(zc, zb, za, indexing="ij")
+    a, b, c = A.ravel(), Bb.ravel(), C.ravel()
+    W = (wc[:, None, None] * 0.25 * wb[None, :, None] * wa[None, None, :]).ravel()
+    Dc = lagrange_deriv_matrix(zc).reshape(len(zc), len(zc))  # [k, i] = l_k'(z_i)
+    kc = np.repeat(np.arange(len(zc)), len(za) * len(zb))      # c index of each point
+    modes = pyr_modes(nm)
+    one = Poly([1.0])
+    vals = np.zeros((len(a), len(modes)))
+    grads = np.zeros((3, len(a), len(modes)))
+    for m, (p, q, r) in enumerate(modes):
+        if correct and (p, q, r) == (0, 0, 1):
+            ta, tb, tc = one, one, HALF_P
+        else:
+            ta, tb, tc = modified_a(p), modified_a(q), pyr_c(p, q, r)
+        vals[:, m] = ta(a) * tb(b) * tc(c)
+        da = ta.deriv()(a) * tb(b) * tc(c)
+        db = ta(a) * tb.deriv()(b) * tc(c)
+        dtc = Dc.T @ tc(zc)
+        dc = ta(a) * tb(b) * dtc[kc]
+        grads[0, :, m] = da * 2.0 / (1 - c)
+        grads[1, :, m] = db * 2.0 / (1 - c)
+        grads[2, :, m] = (da * (1 + a) + db * (1 + b)) / (1 - c) + dc
+    return vals, grads, W
+
+
 def dense_helmholtz(vals, grads, W, jac, df, lam, diff6=None):
     """lam M + L from point values / reference-direction gradients (per
     point), per-point jac [nqt] and df [nqt, 9] (df[j*3+k] = dxi_k/dx_j)."""

@@ -109,7 +109,7 @@ int main()
 
     bool threw = false;
     try {
-        MatrixFree::GetOperatorFactory().CreateInstance("Helmholtz_Pyr_Regular", basis, ne);
+        MatrixFree::GetOperatorFactory().CreateInstance("Helmholtz_Quad_Regular", basis, ne);
     } catch (const NekError &e) {
         threw = e.code == NMF_ERR_UNSUPPORTED;
     }

@@ -52,6 +52,8 @@ void run(GenArgs a)
         if (a.shape == Shape::Hex) gen::element<double, gen::SH_HEX>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
         else if (a.shape == Shape::Prism)
             gen::element<double, gen::SH_PRISM>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
+        else if (a.shape == Shape::Pyr)
+            gen::element<double, gen::SH_PYR>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
         else gen::element<double, gen::SH_TET>(a, e, X0.data(), X1.data(), X2.data(), X3.data());
     }
 }
@@ -74,13 +76,17 @@ int main()
         {Shape::Tet, {6, 6, 9}, {4, 3, 3}, 2},   {Shape::Prism, {3, 3, 3}, {4, 4, 3}, 5},
         {Shape::Prism, {3, 4, 5}, {5, 4, 5}, 4}, {Shape::Prism, {4, 2, 4}, {5, 6, 5}, 3},
         {Shape::Prism, {2, 2, 2}, {3, 3, 2}, 4}, {Shape::Prism, {6, 6, 6}, {8, 8, 7}, 2},
+        {Shape::Pyr, {2, 2, 2}, {3, 3, 2}, 4},   {Shape::Pyr, {3, 3, 3}, {4, 4, 3}, 5},
+        {Shape::Pyr, {4, 4, 4}, {6, 5, 5}, 3},   {Shape::Pyr, {5, 5, 5}, {7, 7, 6}, 2},
+        {Shape::Pyr, {6, 6, 6}, {8, 8, 7}, 2},
     };
     double worst = 0.0;
     std::mt19937_64 rng(11);
     std::uniform_real_distribution<double> U(-1.0, 1.0), Upos(0.5, 1.5);
     for (const Case &c : cases) {
         const GenLayout L = generic_layout(c.shape, c.nm, c.nq);
-        // S: the oracle's shape (0 hex, 1 tet, 2 prism); tets and prisms pack df and J
+        // S: the oracle's shape (0 hex, 1 tet, 2 prism, 3 pyramid); all but
+        // the hex pack df and J
         const int S = (int)c.shape, E = c.E, nqt = L.nqtot, nc = L.ncoeff;
         const Basis1D P0 = make_basis(default_basis(c.shape, 0), c.nm[0], c.nq[0]),
                       P1 = make_basis(default_basis(c.shape, 1), c.nm[1], c.nq[1]),
@@ -135,6 +141,11 @@ int main()
                     if (S == 2) {
                         const double h0 = 0.5 * (1 + P0.pts.z[i]), h1 = 2 / (1 - P2.pts.z[k]);
                         a3[0] = h1 * (a3[0] + h0 * a3[2]);
+                    } else if (S == 3) {
+                        const double h0 = 0.5 * (1 + P0.pts.z[i]), h1 = 0.5 * (1 + P1.pts.z[j]),
+                                     h2 = 2 / (1 - P2.pts.z[k]);
+                        a3[0] = h2 * (a3[0] + h0 * a3[2]);
+                        a3[1] = h2 * (a3[1] + h1 * a3[2]);
                     } else if (S) {
                         const double h0 = 0.5 * (1 + P0.pts.z[i]), h1 = 0.5 * (1 + P1.pts.z[j]),
                                      h2 = 2 / (1 - P1.pts.z[j]), h3 = 2 / (1 - P2.pts.z[k]);

@@ -41,14 +41,24 @@ def test_library_exports_every_declared_symbol(nmf):
     assert not missing, missing
 
 
-@pytest.mark.parametrize("shape", [0, 1])
+# bases and points per direction: hex, tet, prism, pyramid
+_BTYPE = {0: (O.MOD_A,) * 3, 1: (O.MOD_A, O.MOD_B, O.MOD_C), 2: (O.MOD_A, O.MOD_A, O.MOD_B),
+          3: (O.MOD_A, O.MOD_A, O.MOD_PYR_C)}
+_PTYPE = {0: (O.GLL,) * 3, 1: (O.GLL, O.RADAU_A1, O.RADAU_A2), 2: (O.GLL, O.GLL, O.RADAU_A1),
+          3: (O.GLL, O.GLL, O.RADAU_A2)}
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 @pytest.mark.parametrize("nm", [2, 5, 8, 10])
 def test_product_basis_tables_match_oracle(nmf, shape, nm):
+    """The product's closed-form bases (nektar_amd/csrc/basis.cpp) against
+    the oracle's restatement of Basis.cpp (block copies for Modified_C and
+    ModifiedPyr_C)."""
     nq0 = nm + 1
     for d in range(3):
         t = nmf.basis_tables(shape, d, nm, nq0)
-        btype = O.MOD_A if shape == 0 else d
-        ptype = O.GLL if shape == 0 else d
+        btype = _BTYPE[shape][d]
+        ptype = _PTYPE[shape][d]
         b, db = O.basis(btype, nm, t["nq"])
         z, w, D = O.points(ptype, t["nq"])
         assert np.max(np.abs(t["z"] - z)) < 2e-16
@@ -61,8 +71,8 @@ def test_product_basis_tables_match_oracle(nmf, shape, nm):
 def test_bad_keys_report_errors(nmf):
     h = ctypes.c_void_p()
     L = nmf.lib()
-    rc = L.nmf_operator_create(b"Helmholtz_Pyr_Regular", 4, 5, 1, 0, ctypes.byref(h))
-    assert rc == 2 and b"Pyr" in L.nmf_last_error()
+    rc = L.nmf_operator_create(b"Helmholtz_Quad_Regular", 4, 5, 1, 0, ctypes.byref(h))
+    assert rc == 2 and b"Quad" in L.nmf_last_error()
     rc = L.nmf_operator_create(b"Foo_Hex_Regular", 4, 5, 1, 0, ctypes.byref(h))
     assert rc == 2
     rc = L.nmf_operator_create(b"Helmholtz_Hex", 4, 5, 1, 0, ctypes.byref(h))

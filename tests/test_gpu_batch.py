@@ -168,7 +168,14 @@ def test_batch_autotune_keeps_results(nmf):
     b.run()
     torch.cuda.synchronize()
     for k, it in enumerate(items):
-        assert np.array_equal(it.got()[0], ref[k][0])
+        if it.op.tile < nmf.TILE_MFMA:
+            # a vector-ALU tile computes the same arithmetic as the default
+            assert np.array_equal(it.got()[0], ref[k][0])
+        else:
+            # a matrix-core tile rounds differently: equal to its own solo
+            # apply bit for bit, to the default tile's within rounding
+            assert np.array_equal(it.got()[0], it.alone()[0])
+            assert np.max(np.abs(it.got()[0] - ref[k][0])) <= 1e-12 * np.max(np.abs(ref[k][0]))
         if it.shape == O.HEX:
             assert it.op.tile in it.op.tile_candidates()
     nmf.clear_tuning()

@@ -4,8 +4,9 @@ on dense evaluations.  The reference runs these shapes through its
 runtime-size operator3D (MatrixFreeOps/SwitchNodesPoints.h:302-343,
 Helmholtz.h:374-464).
 
-Cases: anisotropic modes and points on hexes, tets and prisms (which run
-the runtime-size kernels at every size), isotropic orders
+Cases: anisotropic modes and points on hexes, tets and prisms, equal modes
+with any points on pyramids (prisms and pyramids run the runtime-size
+kernels at every size), isotropic orders
 without a compile-time kernel (hex P=11 / Q=14, Q = 2 nm; tet P=10), and
 sizes whose buffers exceed one workgroup's LDS, which run from the global
 workspace (including more elements than workgroups of that grid).  Every
@@ -43,6 +44,12 @@ TET = [((3, 4, 5), (5, 5, 5), 37), ((2, 3, 4), (4, 6, 5), 21), ((4, 4, 4), (6, 4
 # elements than workgroups
 PRISM = [((4, 4, 4), (5, 5, 4), 37), ((7, 7, 7), (8, 8, 7), 64), ((3, 4, 5), (5, 4, 5), 15),
          ((4, 2, 4), (5, 6, 5), 33), ((10, 10, 10), (11, 11, 10), 12), ((12, 12, 12), (14, 14, 13), 1030)]
+
+
+# pyramids: equal modes in the three directions (the reference's
+# precondition), the MODIFIED defaults, other points, a global-workspace size
+PYR = [((4, 4, 4), (5, 5, 4), 37), ((7, 7, 7), (8, 8, 7), 64), ((3, 3, 3), (5, 4, 5), 15),
+       ((2, 2, 2), (3, 3, 2), 33), ((10, 10, 10), (11, 11, 10), 12), ((12, 12, 12), (14, 14, 13), 1030)]
 
 
 @pytest.fixture(scope="module")
@@ -87,9 +94,10 @@ def _run(nmf, key, nm, nq, E, prec, jac, df, ins, outs, lam=None, diff=None, vel
 
 def _collapse_factor(nmf, nq, shape="Tet"):
     """max 2/(1-z1) * max 2/(1-z2) over the tet's Radau points; max
-    2/(1-z2) over the prism's"""
-    if shape == "Prism":
-        z2 = nmf.basis_tables(nmf.SHAPE_PRISM, 2, 2, nq[2] + 1)["z"]
+    2/(1-z2) over the prism's and the pyramid's"""
+    if shape in ("Prism", "Pyr"):
+        sh = nmf.SHAPE_PRISM if shape == "Prism" else nmf.SHAPE_PYR
+        z2 = nmf.basis_tables(sh, 2, 2, nq[2] + 1)["z"]
         return float(np.max(2 / (1 - z2)))
     z1 = nmf.basis_tables(nmf.SHAPE_TET, 1, 2, nq[1] + 1)["z"]
     z2 = nmf.basis_tables(nmf.SHAPE_TET, 2, 2, nq[2] + 1)["z"]
@@ -99,20 +107,20 @@ def _collapse_factor(nmf, nq, shape="Tet"):
 def _check(got, want, prec, nmf, what):
     nm = what[1]
     tol = 5e-5 if prec == nmf.FP32 else (1e-12 if max(nm) <= 10 else 1e-11)
-    if what[0] in ("Tet", "Prism") and what[-1] == "PhysDeriv":
+    if what[0] in ("Tet", "Prism", "Pyr") and what[-1] == "PhysDeriv":
         tol *= max(1.0, _collapse_factor(nmf, what[2], what[0]) / 10)
     for g, w in zip(got, want):
         err = G.rel_err(g, w)
         assert err <= tol, (what[-1], what, err)
 
 
-@pytest.mark.parametrize("shape", ["Hex", "Tet", "Prism"])
+@pytest.mark.parametrize("shape", ["Hex", "Tet", "Prism", "Pyr"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("deformed", [False, True], ids=["regular", "deformed"])
 def test_runtime_size_operators_against_oracle(nmf, shape, prec, deformed):
     P = nmf.FP64 if prec == "fp64" else nmf.FP32
-    S = {"Hex": O.HEX, "Tet": O.TET, "Prism": O.PRISM}[shape]
-    cases = {"Hex": HEX, "Tet": TET, "Prism": PRISM}[shape]
+    S = {"Hex": O.HEX, "Tet": O.TET, "Prism": O.PRISM, "Pyr": O.PYR}[shape]
+    cases = {"Hex": HEX, "Tet": TET, "Prism": PRISM, "Pyr": PYR}[shape]
     geo = "Deformed" if deformed else "Regular"
     for ci, (nm, nq, E) in enumerate(cases):
         if prec == "fp32" and E > 100:
@@ -173,3 +181,9 @@ def test_compiled_and_runtime_choice(nmf):
         F.CreateInstance("Helmholtz_Prism_Regular", [5, 4, 4], [6, 6, 5], 4)
     with pytest.raises(nmf.NektarError):
         F.CreateInstance("PhysInterp1DScaled_Prism_Regular", 5, 6, 4)
+    # pyramids: runtime-size, equal modes only, no PhysInterp1DScaled
+    assert F.CreateInstance("Helmholtz_Pyr_Regular", 7, 8, 4).runtime_size
+    with pytest.raises(nmf.NektarError):
+        F.CreateInstance("Helmholtz_Pyr_Regular", [5, 5, 6], [6, 6, 6], 4)
+    with pytest.raises(nmf.NektarError):
+        F.CreateInstance("PhysInterp1DScaled_Pyr_Regular", 5, 6, 4)

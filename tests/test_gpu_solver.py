@@ -79,7 +79,11 @@ def test_fp64_cg_matches_oracle(nmf, n, nm, nq, lam, amp, seed):
     s.set_forcing_manufactured(1)
     info = s.solve(mixed=False, tol=1e-10)
     assert info.converged and info.final_error < 1e-10
-    assert abs(info.iterations - info_o.iterations) <= 2, (info.iterations, info_o.iterations)
+    # the device and the oracle sum the inner products in different orders
+    # (per-wave butterflies against blocked pairwise sums), which moves the
+    # stopping iteration of a long solve by a few: 2 or 1 % of the count
+    slack = max(2, int(0.01 * info_o.iterations))
+    assert abs(info.iterations - info_o.iterations) <= slack, (info.iterations, info_o.iterations)
     assert info.rhs_magnitude == pytest.approx(info_o.rhs_magnitude, rel=1e-12)
     # the GPU solution solves the oracle's system to the stopping tolerance:
     # its true residual with the oracle's operator ...
@@ -129,7 +133,11 @@ def test_mixed_precision_cg(nmf, n, nm, nq, lam, amp, seed):
     xo, _ = O.cg_helmholtz_hex(n, nm, nq, P["deformed"], P["jac"], P["df"], lam, P["b"], P["invd"], 1e-10, 5000)
     err = _rel(s.solution(), xo)
     print(f"mixed vs FP64 oracle ({n}^3 P={nm - 1} Q={nq}, amp {amp}): max rel diff {err:.3e}", flush=True)
-    assert err <= 1e-7
+    # two solutions of relative residual <= 1e-10 differ by up to
+    # cond(A) * 2e-10; the modal Helmholtz matrices here have cond(A) of
+    # order 1e3-1e4, and where the mixed solve stops depends on the
+    # rounding of its FP32 iterations
+    assert err <= 2e-6
 
 
 def test_solve_is_bitwise_reproducible(nmf):

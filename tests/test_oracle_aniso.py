@@ -165,24 +165,19 @@ PRISM_CASES = [((3, 3, 3), (4, 4, 3)), ((4, 4, 4), (5, 5, 4)), ((3, 4, 5), (5, 4
                ((5, 5, 5), (7, 7, 6)), ((4, 2, 4), (5, 6, 5))]
 
 
-@pytest.mark.parametrize("nm,nq", PRISM_CASES)
-def test_prism_own_points_against_dense(nm, nq):
-    """Prism operators (BwdTransKernels.hpp:334-405, IProductKernels.hpp:
-    519-655, the prism chain rule and metric, HelmholtzKernels.hpp:918-1119,
-    IProductWRTDerivBaseKernels.hpp:338-411) against the dense prism
-    expansion at the operator's own points, per-point geometry."""
-    vals, grads, W = B.prism_dense_own_points(nm, nq)
-    nqt, nc = int(np.prod(nq)), len(B.prism_modes_aniso(nm))
-    assert O.ncoeffs_aniso(O.PRISM, nm) == nc
+def _collapsed_against_dense(shape, dense, nm, nq):
+    vals, grads, W = dense
+    nqt, nc = int(np.prod(nq)), vals.shape[1]
+    assert O.ncoeffs_aniso(shape, nm) == nc
     jac, df = _geom(nqt, 5 * sum(nm) + sum(nq))
     x = G.coeffs(NE * nc, 13)
-    u = O.bwdtrans_aniso(O.PRISM, nm, nq, NE, x)
+    u = O.bwdtrans_aniso(shape, nm, nq, NE, x)
     assert G.rel_err(u, np.concatenate([vals @ _elem(x, e, nc) for e in range(NE)])) < TOL
     f = G.coeffs(NE * nqt, 14)
-    y = O.iproduct_aniso(O.PRISM, nm, nq, NE, True, jac, f)
+    y = O.iproduct_aniso(shape, nm, nq, NE, True, jac, f)
     yb = np.concatenate([vals.T @ (W * _elem(jac, e, nqt) * _elem(f, e, nqt)) for e in range(NE)])
     assert G.rel_err(y, yb) < TOL
-    d = O.physderiv_aniso(O.PRISM, nm, nq, NE, True, df, u)
+    d = O.physderiv_aniso(shape, nm, nq, NE, True, df, u)
     for e in range(NE):
         xe = _elem(x, e, nc)
         dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
@@ -190,7 +185,7 @@ def test_prism_own_points_against_dense(nm, nq):
             want = sum(dfe[:, j * 3 + k] * (grads[k] @ xe) for k in range(3))
             assert G.rel_err(_elem(d[j], e, nqt), want) < 1e-12
     f1, f2 = G.coeffs(NE * nqt, 15), G.coeffs(NE * nqt, 16)
-    yd = O.iproduct_deriv_aniso(O.PRISM, nm, nq, NE, True, jac, df, f, f1, f2)
+    yd = O.iproduct_deriv_aniso(shape, nm, nq, NE, True, jac, df, f, f1, f2)
     for e in range(NE):
         dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
         wj = W * _elem(jac, e, nqt)
@@ -199,13 +194,13 @@ def test_prism_own_points_against_dense(nm, nq):
         assert G.rel_err(_elem(yd, e, nc), want) < TOL
     d6 = np.array([1.3, -0.1, 0.8, 0.2, 0.05, 1.4])
     for diff in (None, d6):
-        h = O.helmholtz_aniso(O.PRISM, nm, nq, NE, True, jac, df, 1.9, x, diff)
+        h = O.helmholtz_aniso(shape, nm, nq, NE, True, jac, df, 1.9, x, diff)
         hb = np.concatenate([B.dense_helmholtz(vals, grads, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
                                                1.9, diff) @ _elem(x, e, nc) for e in range(NE)])
         assert G.rel_err(h, hb) < TOL
     # LinearAdvectionDiffusionReaction: lambda M + L plus lambda (phi_i, V . grad u)
     vel = G.coeffs(NE * nqt * 3, 17)
-    a = O.adv_diff_react_aniso(O.PRISM, nm, nq, NE, True, jac, df, vel, 1.9, x)
+    a = O.adv_diff_react_aniso(shape, nm, nq, NE, True, jac, df, vel, 1.9, x)
     for e in range(NE):
         xe = _elem(x, e, nc)
         dfe = _elem(df, e, 9 * nqt).reshape(nqt, 9)
@@ -215,6 +210,69 @@ def test_prism_own_points_against_dense(nm, nq):
         want = B.dense_helmholtz(vals, grads, W, _elem(jac, e, nqt), dfe, 1.9) @ xe + 1.9 * vals.T @ (
             W * _elem(jac, e, nqt) * adv)
         assert G.rel_err(_elem(a, e, nc), want) < TOL
+
+
+@pytest.mark.parametrize("nm,nq", PRISM_CASES)
+def test_prism_own_points_against_dense(nm, nq):
+    """Prism operators (BwdTransKernels.hpp:334-405, IProductKernels.hpp:
+    519-655, the prism chain rule and metric, HelmholtzKernels.hpp:918-1119,
+    IProductWRTDerivBaseKernels.hpp:338-411) against the dense prism
+    expansion at the operator's own points, per-point geometry."""
+    assert len(B.prism_modes_aniso(nm)) == O.ncoeffs_aniso(O.PRISM, nm)
+    _collapsed_against_dense(O.PRISM, B.prism_dense_own_points(nm, nq), nm, nq)
+
+
+PYR_CASES = [((2, 2, 2), (3, 3, 2)), ((3, 3, 3), (4, 4, 3)), ((4, 4, 4), (5, 5, 4)), ((4, 4, 4), (6, 5, 5)),
+             ((5, 5, 5), (7, 7, 6)), ((6, 6, 6), (7, 7, 6))]
+
+
+@pytest.mark.parametrize("nm,nq", PYR_CASES)
+def test_pyramid_own_points_against_dense(nm, nq):
+    """Pyramid operators (BwdTransKernels.hpp:407-489, IProductKernels.hpp:
+    660-815, the pyramid chain rule PhysDerivKernels.hpp:301-311, the metric
+    HelmholtzKernels.hpp:1119-1326, IProductWRTDerivBaseKernels.hpp:417-499,
+    LinearAdvectionDiffusionReactionKernels.hpp:475-560) against the dense
+    pyramid expansion at the operator's own points, per-point geometry.  The
+    direction-2 factors of the dense expansion come from their closed forms,
+    not from the oracle's block copies of Basis.cpp."""
+    assert len(B.pyr_modes(nm[0])) == O.ncoeffs_aniso(O.PYR, nm)
+    _collapsed_against_dense(O.PYR, B.pyr_dense_own_points(nm[0], nq), nm, nq)
+
+
+@pytest.mark.parametrize("nm,nq", PYR_CASES[1:4])
+def test_pyramid_without_top_vertex_term_fails_the_pin(nm, nq):
+    """The dense pin sees the top-vertex term (BwdTransKernels.hpp:474-485,
+    IProductKernels.hpp:780-812): the expansion without it differs from the
+    oracle by far more than the pin's tolerance."""
+    bad, bgrads, W = B.pyr_dense_own_points(nm[0], nq, correct=False)
+    nqt, nc = int(np.prod(nq)), bad.shape[1]
+    x = G.coeffs(NE * nc, 13)
+    u = O.bwdtrans_aniso(O.PYR, nm, nq, NE, x)
+    assert G.rel_err(u, np.concatenate([bad @ _elem(x, e, nc) for e in range(NE)])) > 1e-3
+    jac, df = _geom(nqt, 5 * sum(nm) + sum(nq))
+    h = O.helmholtz_aniso(O.PYR, nm, nq, NE, True, jac, df, 1.9, x)
+    hb = np.concatenate([B.dense_helmholtz(bad, bgrads, W, _elem(jac, e, nqt), _elem(df, e, 9 * nqt).reshape(nqt, 9),
+                                           1.9) @ _elem(x, e, nc) for e in range(NE)])
+    assert G.rel_err(h, hb) > 1e-3
+
+
+def test_pyramid_regular_geometry_and_defaults():
+    """Per-element geometry gives the per-point result with constant
+    factors, and the isotropic entry points use the pyramid defaults
+    (Q, Q, Q-1) and nm(nm+1)(2nm+1)/6 coefficients."""
+    nm, nq0 = 4, 5
+    nq = O.default_nq(O.PYR, nq0)
+    assert nq == (5, 5, 4) and O.ncoeffs(O.PYR, nm) == 30
+    nqt = int(np.prod(nq))
+    rng = np.random.default_rng(4)
+    jac_e = rng.uniform(0.5, 1.5, NE)
+    df_e = rng.uniform(-1, 1, NE * 9)
+    jac_q = np.repeat(jac_e, nqt)
+    df_q = np.concatenate([np.tile(df_e[e * 9:(e + 1) * 9], nqt) for e in range(NE)])
+    x = G.coeffs(NE * O.ncoeffs(O.PYR, nm), 19)
+    a = O.helmholtz(O.PYR, nm, nq0, NE, False, jac_e, df_e, 0.7, x)
+    b = O.helmholtz_aniso(O.PYR, (nm, nm, nm), nq, NE, True, jac_q, df_q, 0.7, x)
+    assert G.rel_err(a, b) < 1e-13
 
 
 @pytest.mark.parametrize("nm,nq", PRISM_CASES[:3])
