@@ -202,6 +202,32 @@ def bench_mfma(N, torch, steps=10, warmup=2):
     return out
 
 
+def bench_serve_sessions(repeats=3):
+    """HelmSolve requests given as the reference's session files
+    (Helmholtz3D_CG_Hex, _Tet, _Hex_AllBCs and a trilinear-hex session, each
+    with the full and the element-local CG), all arriving at once: served
+    with 4 solves in flight against one at a time.  Reading the sessions and
+    building the problems on the host is part of every request's latency."""
+    from nektar_amd import contfield as cf
+    import test_contfield_cpu as T
+
+    g = os.path.join(ROOT, "tests", "golden", "helmholtz3d")
+    sessions = [cf.read_session(os.path.join(g, f)) for f in
+                ("Helmholtz3D_Hex_P6.xml", "Helmholtz3D_Tet.xml", "Helmholtz3D_Hex_AllBCs_P6.xml")]
+    sessions.append(T.perturbed_hex_session())
+    reqs = [dict(session=s, solver=sv, tol=1e-14, arrival_s=0.0) for s in sessions
+            for sv in ("ConjugateGradient", "ConjugateGradientLoc")] * repeats
+    out = {"requests": len(reqs)}
+    cf.serve_sessions(reqs[:4], max_active=4)  # warm (kernels, tables)
+    for name, act in (("served", 4), ("one_at_a_time", 1)):
+        res, st = cf.serve_sessions(reqs, max_active=act)
+        out[name] = {"makespan_s": round(st["makespan_s"], 3), "p50_ms": round(st["p50_ms"], 1),
+                     "p99_ms": round(st["p99_ms"], 1), "peak_active": st["peak_active"],
+                     "all_converged": all(r["converged"] for r in res)}
+    out["served_over_one_at_a_time"] = round(out["one_at_a_time"]["makespan_s"] / out["served"]["makespan_s"], 3)
+    return out
+
+
 def rotated_box_map(n, nm, seed=9):
     """The 32^3-type structured box with every element in a random local
     frame (axis permutation and reflections): map and sign of every local
@@ -540,6 +566,8 @@ def main():
                     help="skip the per-operator timings (hex and tet, 32,768 elements, P=6)")
     ap.add_argument("--no-mfma", action="store_true",
                     help="skip the vector against matrix-core (MFMA) Helmholtz comparison at P = 7..9")
+    ap.add_argument("--no-serve-sessions", action="store_true",
+                    help="skip serving the reference's session files")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 512 MiB ten times (PMC byte-counter calibration)")
@@ -649,6 +677,8 @@ def main():
         extra["batch"] = bench_batch(N, torch, steps=10, warmup=2)
     if not args.no_serve:
         extra["serve"] = bench_serve(N, args.serve_requests, budget_gib=64, streams=4, max_active=16)
+    if not args.no_serve_sessions:
+        extra["serve_sessions"] = bench_serve_sessions()
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "GDOF/s", "n_gpus": 1,

@@ -397,20 +397,31 @@ def dirichlet_problem(session: Session, var: str = "u"):
 
 
 def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit: int = 20000,
-                    solver: str = "ConjugateGradient"):
+                    solver: str = "ConjugateGradient", stream=None, problem=None):
     """ContField::HelmSolve of the session's Forcing with its Dirichlet
     conditions and parameter Lambda (the Helmholtz3D demo; session_path may
     also be a Session already read): returns
     (L2 error, Linf error, SolveInfo, element-local solution) against the
     session's ExactSolution at the quadrature points.  Meshes of
     (trilinear) hexahedra or of tetrahedra, NUMMODES per the expansion,
-    the MODIFIED default points; GPU operators and CG."""
+    the MODIFIED default points; GPU operators and CG.  stream: a
+    torch.cuda.Stream for every device operation of the solve (default:
+    the current stream); problem: dirichlet_problem(session, var) if
+    already built."""
+    import torch
+
+    st = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(st):
+        return _helmholtz_solve(session_path, var, tol, maxit, solver, st, problem)
+
+
+def _helmholtz_solve(session_path, var, tol, maxit, solver, st, problem):
     import torch
 
     from . import AssemblyMap, FP64, GetOperatorFactory, LinearSystem
 
     s = session_path if isinstance(session_path, Session) else read_session(session_path)
-    P = dirichlet_problem(s, var)
+    P = problem if problem is not None else dirichlet_problem(s, var)
     tag, nm, nq0, jac, df, X = P["tag"], P["nm"], P["nq0"], P["jac"], P["df"], P["X"]
     E = len(P["eids"])
     lam = s.params.get("Lambda", 0.0)
@@ -423,7 +434,8 @@ def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit
     helm.SetJac(jac)
     helm.SetDF(df)
     helm.SetLambda(lam)
-    ls = LinearSystem(AssemblyMap(P["map"], P["sign"], P["nglobal"], P["ndir"]), [helm], solver=solver)
+    ls = LinearSystem(AssemblyMap(P["map"], P["sign"], P["nglobal"], P["ndir"]), [helm], solver=solver,
+                      stream=st.cuda_stream)
 
     names = dict(s.params, x=X[..., 0], y=X[..., 1], z=X[..., 2], t=0.0)
     f = np.broadcast_to(evaluate(s.functions["Forcing"][var], names), X.shape[:2]).astype(float).reshape(-1)
@@ -446,7 +458,7 @@ def helmholtz_solve(session_path: str, var: str = "u", tol: float = 1e-14, maxit
     ls.solution_local(xl)
     uq = torch.zeros(bwd.output_size(), dtype=torch.float64, device=dev)
     bwd(xl, uq)
-    torch.cuda.synchronize()
+    st.synchronize()
     u = uq.cpu().numpy().reshape(E, -1)
     if tag == "Hex":
         l2, linf = errors(u, exact, jac, *P["weights"])
@@ -686,3 +698,129 @@ def tet_errors(u, exact, jac, w0, w1, w2):
     wq = np.einsum("k,j,i->kji", 0.25 * w2, 0.5 * w1, w0).reshape(-1)
     err = np.asarray(u).reshape(len(jac), -1) - np.asarray(exact).reshape(len(jac), -1)
     return float(np.sqrt(np.sum(wq[None, :] * np.asarray(jac)[:, None] * err * err))), float(np.max(np.abs(err)))
+
+
+# ---------------------------------------------------------------------
+# Serving HelmSolve requests given as session files
+
+
+def estimate_bytes(P) -> int:
+    """Upper bound of the device bytes one helmholtz_solve of problem P
+    holds: per-point geometry of the three operators (df and J, 10 values,
+    or per element), element-local coefficient and point vectors, the
+    signed map, the global CG vectors and the Robin rows, in FP64."""
+    E = len(P["eids"])
+    nqt = int(P["X"].shape[1])
+    nq = nqt if P["deformed"] else 1
+    nc = P["map"].size // max(E, 1)
+    ng = int(P["nglobal"])
+    geo = 3 * 11 * E * max(nq, 1)
+    local = E * (8 * nc + 4 * nqt)
+    glob = 10 * ng
+    robin = 0 if P["robin"] is None else 2 * int(P["robin"][3].size)
+    return int(8 * (geo + local + glob + robin) + 4 * 2 * P["map"].size + (1 << 20))
+
+
+def serve_sessions(requests, hbm_budget_bytes: int = 8 << 30, max_active: int = 4, var: str = "u"):
+    """A serving loop for HelmSolve requests given as session files (or
+    Sessions already read): request i = dict(session=..., solver=...,
+    tol=..., maxit=..., arrival_s=...).  Requests are replayed at their
+    arrival times; on arrival the session is read and its problem built
+    (mesh, geometry, C0 map, boundary data: nektar_amd.io and this module),
+    then it waits until fewer than max_active solves run and its device
+    bytes (estimate_bytes) fit in what the active solves leave of the
+    budget, taking the first request that fits (no request is overtaken
+    more than 8 times).  Each admitted solve runs on a worker thread with a
+    stream of its own, so solves overlap on the device; its device memory
+    is freed when it finishes.  Returns (results, stats): results[i] has
+    l2, linf, iterations, converged, final_error, arrival_s, admit_s,
+    finish_s, latency_ms, queue_ms and bytes; stats has makespan_s,
+    peak_active, peak_bytes and the p50/p99 latency."""
+    import threading
+    import time
+
+    import torch
+
+    n = len(requests)
+    order = sorted(range(n), key=lambda i: requests[i].get("arrival_s", 0.0))
+    results = [None] * n
+    lock = threading.Condition()
+    state = dict(active=0, bytes=0, peak_active=0, peak_bytes=0, error=None)
+    streams = [torch.cuda.Stream() for _ in range(max_active)]
+    free_streams = list(range(max_active))
+    t0 = time.perf_counter()
+
+    def now():
+        return time.perf_counter() - t0
+
+    def work(i, P, sess, nbytes, sidx, t_arr, t_adm):
+        r = requests[i]
+        try:
+            l2, linf, info, _ = helmholtz_solve(sess, var, r.get("tol", 1e-14), r.get("maxit", 20000),
+                                                r.get("solver", "ConjugateGradient"), streams[sidx], problem=P)
+            t_fin = now()
+            results[i] = dict(l2=l2, linf=linf, iterations=info.iterations, converged=bool(info.converged),
+                              final_error=info.final_error, arrival_s=t_arr, admit_s=t_adm, finish_s=t_fin,
+                              latency_ms=(t_fin - t_arr) * 1e3, queue_ms=(t_adm - t_arr) * 1e3, bytes=nbytes)
+        except Exception as exc:  # reported after the loop
+            state["error"] = exc
+        finally:
+            streams[sidx].synchronize()
+            with lock:
+                state["active"] -= 1
+                state["bytes"] -= nbytes
+                free_streams.append(sidx)
+                lock.notify_all()
+
+    waiting, threads, overtaken = [], [], {}
+    k = 0
+    while k < n or waiting:
+        # arrivals: read the session and build the problem (host side)
+        while k < n and requests[order[k]].get("arrival_s", 0.0) <= now():
+            i = order[k]
+            k += 1
+            sess = requests[i]["session"]
+            sess = sess if isinstance(sess, Session) else read_session(sess)
+            P = dirichlet_problem(sess, var)
+            nb = estimate_bytes(P)
+            if nb > hbm_budget_bytes:
+                raise ValueError(f"request {i} needs ~{nb} device bytes, more than the budget {hbm_budget_bytes}")
+            waiting.append((i, P, sess, nb, requests[i].get("arrival_s", 0.0)))
+            overtaken[i] = 0
+        with lock:
+            if state["error"] is not None:
+                break
+            chosen = None
+            for w in waiting:
+                if state["active"] < max_active and state["bytes"] + w[3] <= hbm_budget_bytes:
+                    chosen = w
+                    break
+                if overtaken[w[0]] >= 8:  # the oldest waiting request blocks the rest
+                    break
+            if chosen is not None:
+                for w in waiting:
+                    if w is chosen:
+                        break
+                    overtaken[w[0]] += 1
+                waiting.remove(chosen)
+                state["active"] += 1
+                state["bytes"] += chosen[3]
+                state["peak_active"] = max(state["peak_active"], state["active"])
+                state["peak_bytes"] = max(state["peak_bytes"], state["bytes"])
+                sidx = free_streams.pop()
+            elif waiting or k < n:
+                nxt = requests[order[k]].get("arrival_s", 0.0) - now() if k < n else 0.05
+                lock.wait(timeout=max(1e-3, min(0.05, nxt)))
+        if chosen is not None:
+            i, P, sess, nb, t_arr = chosen
+            th = threading.Thread(target=work, args=(i, P, sess, nb, sidx, t_arr, now()))
+            th.start()
+            threads.append(th)
+    for th in threads:
+        th.join()
+    if state["error"] is not None:
+        raise state["error"]
+    lat = np.array([r["latency_ms"] for r in results])
+    stats = dict(makespan_s=now(), peak_active=state["peak_active"], peak_bytes=state["peak_bytes"],
+                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
+    return results, stats
