@@ -172,18 +172,36 @@ struct Tab {
 // of units (LP values), so that the row reads of neighbouring work-items
 // start in different banks; rows that fill their units exactly, and FP32
 // rows, keep the plain pitch (both measured faster).
-template <typename T, int NM, int NQ>
+// With G the Modified_C rows (NC + NM of them, the larger table) are read
+// from the padded rows of the table buffer in global memory (pitch RP,
+// cached) and only the B1 rows are staged, which leaves LDS for more
+// resident workgroups.
+template <typename T, int NM, int NQ, bool G = false>
 struct LTab {
     using Dm = Dims<NM, NQ>;
     static constexpr int VW = 16 / sizeof(T);
     static constexpr int UNITS = (Dm::NQ2 + VW - 1) / VW;
     static constexpr bool PAD = sizeof(T) == 8 && Dm::NQ2 % VW != 0 && UNITS % 2 == 0;
     static constexpr int LP = (PAD ? UNITS + 1 : UNITS) * VW;
-    static constexpr int SIZE = (Dm::NB1R + Dm::NB2R) * LP;
-    const T *l;
+    static constexpr int NROWS = Dm::NB1R + (G ? 0 : Dm::NB2R);
+    static constexpr int SIZE = NROWS * LP;
+    const T *l;  // staged rows in LDS
+    const T *g;  // the table buffer
     __device__ __forceinline__ const T *B1(int m) const { return l + m * LP; }
-    __device__ __forceinline__ const T *B2(int m) const { return l + (Dm::NB1R + m) * LP; }
+    __device__ __forceinline__ const T *B2(int m) const
+    {
+        if constexpr (G) return g + Dm::oB1P + (Dm::NB1R + m) * Dm::RP;
+        return l + (Dm::NB1R + m) * LP;
+    }
 };
+
+// The Helmholtz and inner-product kernels read the Modified_C rows from
+// global memory from Q = 11 on: at P = 9 that takes FP64 Helmholtz from
+// 59 to 41 KB of LDS per workgroup (2 -> 3 waves per SIMD) and its time
+// down by an eighth; at lower orders, and in BwdTrans, staging measured
+// faster.
+template <int NQ>
+constexpr bool kB2Global = NQ >= 11;
 
 // Work split of the (p,q) pair stages over TPE work-items.  With room for
 // two work-items per pair, the forward r-contraction gives each a half of
@@ -201,6 +219,9 @@ struct PairSplit {
     static constexpr int KW = KG == 2 ? KW2 : Dm::NQ2;
     static constexpr int RG = TWO ? 2 : 1;
     static constexpr int RW = (NM + RG - 1) / RG;
+    // reads of a B2 row stay inside its pitch in the table buffer too
+    static_assert((KG == 2 ? 2 * KW : (Dm::NQ2 + L::VW - 1) / L::VW * L::VW) <= Dm::RP,
+                  "B2 row reads must stay inside the table row");
 };
 
 // N consecutive values of a staged row, read as 16-byte vectors (rows are
@@ -221,18 +242,18 @@ __device__ __forceinline__ void load_row(const T *row, T (&b)[N])
 
 // Copy the padded B1/B2 rows of the table buffer (pitch RP) into LDS
 // (pitch LP), whole block.  Units past RP are the zero padding.
-template <typename T, int NM, int NQ>
+template <typename T, int NM, int NQ, bool G = false>
 __device__ __forceinline__ void stage_tables(const T *__restrict__ tabp, T *lt)
 {
     using Dm = Dims<NM, NQ>;
-    using L = LTab<T, NM, NQ>;
+    using L = LTab<T, NM, NQ, G>;
     constexpr int VW = L::VW;
     using V = T __attribute__((ext_vector_type(VW)));
     static_assert(Dm::RP % VW == 0 && Dm::oB1P % VW == 0, "table rows must be whole vectors");
     constexpr int UG = Dm::RP / VW, UL = L::LP / VW;
     const V *src = reinterpret_cast<const V *>(tabp + Dm::oB1P);
     V *dst = reinterpret_cast<V *>(lt);
-    for (int x = threadIdx.x; x < (Dm::NB1R + Dm::NB2R) * UL; x += blockDim.x) {
+    for (int x = threadIdx.x; x < L::NROWS * UL; x += blockDim.x) {
         const int r = x / UL, c = x % UL;
         V v = V(T(0));
         if (c < UG) v = src[r * UG + c];
@@ -248,8 +269,8 @@ __device__ __forceinline__ void stage_tables(const T *__restrict__ tabp, T *lt)
 // The triangular loops run over all NM rows: the coefficients (or partial
 // sums) beyond a pair's length are zero, and the rows they meet are real
 // modes of the next pair or the zero padding, so no lane needs a branch.
-template <typename T, int NM, int NQ, int TPE>
-__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
+template <typename T, int NM, int NQ, int TPE, bool G>
+__device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ, G> &lt,
                                               const T *__restrict__ xin, bool valid, int t, T *S1, T *S2, T *S3,
                                               T (&u)[NQ - 1])
 {
@@ -361,8 +382,8 @@ __device__ __forceinline__ void bwd_to_column(const Tab<T, NM, NQ> &tab, const L
 // corrections  : TOP(k), BOT(k)          -> S3[k], S3[NQ2 + k]
 // Stage I2 (p,k): sum_j B1 s            -> S1[pair*NQ2 + k]
 // Stage I3 (pairs): sum_k B2            -> out
-template <typename T, int NM, int NQ, int TPE>
-__device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
+template <typename T, int NM, int NQ, int TPE, bool G>
+__device__ __forceinline__ void iprod_from_column(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ, G> &lt,
                                                   const T (&y)[NQ - 1], bool valid, int t, T *S1, T *S2, T *S3,
                                                   T *__restrict__ xout)
 {
@@ -472,7 +493,7 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void bwdtrans_kernel(in
     __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
     stage_tables<T, NM, NQ>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ> lt{ltab};
+    const LTab<T, NM, NQ> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt;
@@ -495,10 +516,10 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void iproduct_kernel(in
     using Dm = Dims<NM, NQ>;
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
-    stage_tables<T, NM, NQ>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ> lt{ltab};
+    const LTab<T, NM, NQ, kB2Global<NQ>> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt;
@@ -626,8 +647,8 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void physderiv_kernel(
 // (each line belongs to one work-item, which reads it whole before writing
 // it), so the element needs two slabs.  The caller's writes of X and Y
 // must not be followed by a barrier yet.
-template <typename T, int NM, int NQ, int TPE>
-__device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ> &lt,
+template <typename T, int NM, int NQ, int TPE, bool G>
+__device__ __forceinline__ void grad_transpose_to_coeffs(const Tab<T, NM, NQ> &tab, const LTab<T, NM, NQ, G> &lt,
                                                          bool valid, int t, int e, T (&y)[NQ - 1],
                                                          const T (&v2)[NQ - 1], T *X, T *Y, T *__restrict__ out)
 {
@@ -678,7 +699,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
     using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ> lt{ltab};
+    const LTab<T, NM, NQ, kB2Global<NQ>> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blk * Ti::EPB + le;
     const bool valid = e < nelmt, col = t < Ti::NCOL;
@@ -858,8 +879,8 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void helmholtz_kernel(i
 {
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
-    stage_tables<T, NM, NQ>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
     helmholtz_block<T, NM, NQ, DEF, TPE>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab);
 }
 
@@ -877,8 +898,8 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void adv_diff_react_ker
 {
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
-    stage_tables<T, NM, NQ>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
     helmholtz_block<T, NM, NQ, DEF, TPE, true>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab, adv);
 }
 
@@ -890,8 +911,8 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void helmholtz_grouped_
 {
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
-    stage_tables<T, NM, NQ>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
     const int b = blockIdx.x;
     int it = 0;
     while (it + 1 < count && b >= items[it + 1].block0) ++it;
@@ -913,10 +934,10 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void iproduct_deriv_ker
     using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ>::SIZE];
-    stage_tables<T, NM, NQ>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ> lt{ltab};
+    const LTab<T, NM, NQ, kB2Global<NQ>> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt, col = t < Ti::NCOL;
