@@ -80,7 +80,7 @@ size_t HexSolver::estimate_bytes(int n, int nm, int nq, bool deformed, bool mixe
     const size_t nloc = nel * nm * nm * nm;
     const size_t geo = nel * 7 * (deformed ? (size_t)nq * nq * nq : 1);
     size_t b = 8 * (NG * (5 + 6) + nloc + geo);
-    if (mixed) b += 4 * (NG * 6 + nloc + geo);
+    if (mixed) b += 4 * (NG * 7 + nloc + geo);
     b += 8 * nel * (size_t)nq * nq * nq;  // forcing / quadrature scratch
     // the small buffers, so that the estimate bounds device_bytes(): mesh
     // vertices, points and weights, FP64 + FP32 tables, the diagonal's
@@ -171,7 +171,7 @@ size_t HexSolver::device_bytes() const
 {
     size_t s = 0;
     for (const DeviceBuffer *b : {&X_, &z_, &w_, &tab64_, &tab32_, &dtab_, &geo64_, &geo32_, &b_, &x_, &invd_,
-                                  &sloc64_, &tmp64_, &r64_, &sloc32_, &state_, &pa_, &pmu_, &scalar_})
+                                  &invd32_, &sloc64_, &tmp64_, &r64_, &sloc32_, &state_, &pa_, &pmu_, &scalar_})
         s += b->bytes();
     for (const auto &v : cg64_) s += v.bytes();
     for (const auto &v : cg32_) s += v.bytes();
@@ -333,6 +333,7 @@ void HexSolver::ensure_fp32()
     if (!cg32_[0].get()) {
         for (auto &v : cg32_) v.allocate(NG_ * sizeof(float));
         sloc32_.allocate((size_t)nelmt_ * nc_ * sizeof(float));
+        invd32_.allocate(NG_ * sizeof(float));
     }
 }
 
@@ -350,10 +351,10 @@ void HexSolver::capture_pcg32()
         NMF_HIP(solver::launch_assemble<float>(stream_, n_, nm_, N_, sloc32_.as<float>(), q, p, pmu_.as<double>(),
                                                done));
         NMF_HIP(solver::launch_pcg_alpha(stream_, st, pmu_.as<double>(), nb_));
-        NMF_HIP(solver::launch_pcg_update_xr<float>(stream_, N_, st, x, r, p, q, invd_.as<double>(),
+        NMF_HIP(solver::launch_pcg_update_xr<float>(stream_, N_, st, x, r, p, q, invd32_.as<float>(),
                                                     pa_.as<double>()));
         NMF_HIP(solver::launch_pcg_beta(stream_, st, pa_.as<double>(), nb_));
-        NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd_.as<double>()));
+        NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd32_.as<float>()));
     }
     NMF_HIP(hipStreamEndCapture(stream_, &g));
     hipError_t e = hipGraphInstantiate(&graph_[2], g, nullptr, nullptr, 0);
@@ -423,7 +424,11 @@ void HexSolver::begin_solve_enqueue(bool mixed, double tol, int maxit)
                                         pmu_.as<double>()));
         NMF_HIP(solver::launch_sum_partials(stream_, pmu_.as<double>(), nb_, dsc + 1));
         NMF_HIP(solver::launch_mixed_state_init(stream_, st, dsc, tol * tol, maxit, kReliableDelta * kReliableDelta));
-        NMF_HIP(solver::launch_pcg_init<float>(stream_, N_, r64_.as<double>(), invd_.as<double>(), y, r, p,
+        // the FP32 iterations precondition with an FP32 copy of the diagonal
+        // (4 bytes less per row in each of their two vector updates), taken
+        // here so that it follows every rebuild of the FP64 one
+        NMF_HIP(solver::launch_cast(stream_, NG_, invd_.as<double>(), invd32_.as<float>()));
+        NMF_HIP(solver::launch_pcg_init<float>(stream_, N_, r64_.as<double>(), invd32_.as<float>(), y, r, p,
                                                pa_.as<double>()));
         NMF_HIP(solver::launch_pcg_scalar_init(stream_, st, pa_.as<double>(), nb_));
         capture_pcg32();
@@ -443,9 +448,9 @@ void HexSolver::enqueue_reliable_update()
     apply_global_fp64(x_.as<double>(), tmp64_.as<double>());
     NMF_HIP(solver::launch_residual(stream_, N_, b_.as<double>(), tmp64_.as<double>(), r64_.as<double>(),
                                     pmu_.as<double>()));
-    NMF_HIP(solver::launch_ru_resume(stream_, N_, r64_.as<double>(), r, invd_.as<double>(), pa_.as<double>()));
+    NMF_HIP(solver::launch_ru_resume(stream_, N_, r64_.as<double>(), r, invd32_.as<float>(), pa_.as<double>()));
     NMF_HIP(solver::launch_ru_scalar(stream_, st, pa_.as<double>(), pmu_.as<double>(), nb_));
-    NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd_.as<double>()));
+    NMF_HIP(solver::launch_pcg_update_p<float>(stream_, N_, st, r, p, invd32_.as<float>()));
 }
 
 bool HexSolver::advance(bool wait)

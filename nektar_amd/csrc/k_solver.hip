@@ -584,7 +584,7 @@ __global__ void cg_scalar_kernel(CGState *st, const double *__restrict__ pa, con
 // ---- Hestenes-Stiefel PCG ------------------------------------------
 // r = b (non-Dirichlet), x = 0, p = z = M^{-1} r; partials {r.z, r.r, b.b}
 template <typename T>
-__global__ void pcg_init_kernel(int N, const double *__restrict__ b, const double *__restrict__ inv_diag,
+__global__ void pcg_init_kernel(int N, const double *__restrict__ b, const T *__restrict__ inv_diag,
                                 T *__restrict__ x, T *__restrict__ r, T *__restrict__ p, double *__restrict__ partials)
 {
     const size_t NG = (size_t)N * N * N;
@@ -594,7 +594,7 @@ __global__ void pcg_init_kernel(int N, const double *__restrict__ b, const doubl
         const double bv = b[G];
         acc[2] += bv * bv;
         const T rv = on_boundary(gx, gy, gz, N) ? T(0) : T(bv);
-        const double zv = inv_diag[G] * (double)rv;
+        const double zv = (double)inv_diag[G] * (double)rv;
         x[G] = T(0);
         r[G] = rv;
         p[G] = T(zv);
@@ -643,7 +643,7 @@ __global__ void pcg_alpha_kernel(CGState *st, const double *__restrict__ pmu, in
 template <typename T>
 __global__ void pcg_update_xr_kernel(size_t NG, const CGState *__restrict__ st, T *__restrict__ x, T *__restrict__ r,
                                      const T *__restrict__ p, const T *__restrict__ q,
-                                     const double *__restrict__ inv_diag, double *__restrict__ partials)
+                                     const T *__restrict__ inv_diag, double *__restrict__ partials)
 {
     if (st->done || st->pause) return;
     const T alpha = T(st->alpha);
@@ -652,7 +652,7 @@ __global__ void pcg_update_xr_kernel(size_t NG, const CGState *__restrict__ st, 
         x[G] = alpha * p[G] + x[G];
         const T rv = -alpha * q[G] + r[G];
         r[G] = rv;
-        acc[0] += (double)rv * (inv_diag[G] * (double)rv);
+        acc[0] += (double)rv * ((double)inv_diag[G] * (double)rv);
         acc[1] += (double)rv * (double)rv;
     }
     block_sum<2>(acc, partials + 2 * blockIdx.x);
@@ -695,24 +695,24 @@ __global__ void pcg_beta_kernel(CGState *st, const double *__restrict__ pa, int 
 // p = M^{-1} r + beta p
 template <typename T>
 __global__ void pcg_update_p_kernel(size_t NG, const CGState *__restrict__ st, const T *__restrict__ r,
-                                    T *__restrict__ p, const double *__restrict__ inv_diag)
+                                    T *__restrict__ p, const T *__restrict__ inv_diag)
 {
     if (st->done || st->pause) return;
     const double beta = st->beta;
     for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x)
-        p[G] = T(inv_diag[G] * (double)r[G] + beta * (double)p[G]);
+        p[G] = T((double)inv_diag[G] * (double)r[G] + beta * (double)p[G]);
 }
 
 // reliable update, part 1: r32 = r64 on non-Dirichlet rows; partial r.z
 __global__ void ru_resume_kernel(int N, const double *__restrict__ r64, float *__restrict__ r32,
-                                 const double *__restrict__ inv_diag, double *__restrict__ partials)
+                                 const float *__restrict__ inv_diag, double *__restrict__ partials)
 {
     const size_t NG = (size_t)N * N * N;
     double acc[1] = {0.0};
     for (size_t G = blockIdx.x * (size_t)blockDim.x + threadIdx.x; G < NG; G += (size_t)gridDim.x * blockDim.x) {
         const float rv = (float)r64[G];  // r64 is 0 on Dirichlet rows
         r32[G] = rv;
-        acc[0] += (double)rv * (inv_diag[G] * (double)rv);
+        acc[0] += (double)rv * ((double)inv_diag[G] * (double)rv);
     }
     block_sum<1>(acc, partials + blockIdx.x);
 }
@@ -944,7 +944,7 @@ hipError_t launch_cg_scalar(hipStream_t stream, CGState *st, const double *pa, c
 }
 
 template <typename T>
-hipError_t launch_pcg_init(hipStream_t stream, int N, const double *b, const double *inv_diag, T *x, T *r, T *p,
+hipError_t launch_pcg_init(hipStream_t stream, int N, const double *b, const T *inv_diag, T *x, T *r, T *p,
                            double *partials)
 {
     NMF_LAUNCH((pcg_init_kernel<T>), grid_for((size_t)N * N * N), N, b, inv_diag, x, r, p, partials);
@@ -965,7 +965,7 @@ hipError_t launch_pcg_alpha(hipStream_t stream, CGState *st, const double *pmu, 
 
 template <typename T>
 hipError_t launch_pcg_update_xr(hipStream_t stream, int N, const CGState *st, T *x, T *r, const T *p, const T *q,
-                                const double *inv_diag, double *partials)
+                                const T *inv_diag, double *partials)
 {
     const size_t NG = (size_t)N * N * N;
     NMF_LAUNCH((pcg_update_xr_kernel<T>), grid_for(NG), NG, st, x, r, p, q, inv_diag, partials);
@@ -979,27 +979,27 @@ hipError_t launch_pcg_beta(hipStream_t stream, CGState *st, const double *pa, in
 }
 
 template <typename T>
-hipError_t launch_pcg_update_p(hipStream_t stream, int N, const CGState *st, const T *r, T *p, const double *inv_diag)
+hipError_t launch_pcg_update_p(hipStream_t stream, int N, const CGState *st, const T *r, T *p, const T *inv_diag)
 {
     const size_t NG = (size_t)N * N * N;
     NMF_LAUNCH((pcg_update_p_kernel<T>), grid_for(NG), NG, st, r, p, inv_diag);
     return hipGetLastError();
 }
 
-template hipError_t launch_pcg_init<float>(hipStream_t, int, const double *, const double *, float *, float *, float *,
+template hipError_t launch_pcg_init<float>(hipStream_t, int, const double *, const float *, float *, float *, float *,
                                            double *);
 template hipError_t launch_pcg_init<double>(hipStream_t, int, const double *, const double *, double *, double *,
                                             double *, double *);
 template hipError_t launch_pcg_update_xr<float>(hipStream_t, int, const CGState *, float *, float *, const float *,
-                                                const float *, const double *, double *);
+                                                const float *, const float *, double *);
 template hipError_t launch_pcg_update_xr<double>(hipStream_t, int, const CGState *, double *, double *, const double *,
                                                  const double *, const double *, double *);
 template hipError_t launch_pcg_update_p<float>(hipStream_t, int, const CGState *, const float *, float *,
-                                               const double *);
+                                               const float *);
 template hipError_t launch_pcg_update_p<double>(hipStream_t, int, const CGState *, const double *, double *,
                                                 const double *);
 
-hipError_t launch_ru_resume(hipStream_t stream, int N, const double *r64, float *r32, const double *inv_diag,
+hipError_t launch_ru_resume(hipStream_t stream, int N, const double *r64, float *r32, const float *inv_diag,
                             double *partials)
 {
     NMF_LAUNCH(ru_resume_kernel, grid_for((size_t)N * N * N), N, r64, r32, inv_diag, partials);

@@ -59,19 +59,19 @@ hipError_t launch_cg_scalar(hipStream_t, CGState *st, const double *pa, const do
 hipError_t launch_residual(hipStream_t, int N, const double *b, const double *s, double *r, double *partials);
 // Hestenes-Stiefel PCG (used for the FP32 inner solve of the mixed path)
 template <typename T>
-hipError_t launch_pcg_init(hipStream_t, int N, const double *b, const double *inv_diag, T *x, T *r, T *p,
+hipError_t launch_pcg_init(hipStream_t, int N, const double *b, const T *inv_diag, T *x, T *r, T *p,
                            double *partials);
 hipError_t launch_pcg_scalar_init(hipStream_t, CGState *st, const double *pa, int nb);
 hipError_t launch_pcg_alpha(hipStream_t, CGState *st, const double *pmu, int nb);
 template <typename T>
 hipError_t launch_pcg_update_xr(hipStream_t, int N, const CGState *st, T *x, T *r, const T *p, const T *q,
-                                const double *inv_diag, double *partials);
+                                const T *inv_diag, double *partials);
 hipError_t launch_pcg_beta(hipStream_t, CGState *st, const double *pa, int nb);
 template <typename T>
-hipError_t launch_pcg_update_p(hipStream_t, int N, const CGState *st, const T *r, T *p, const double *inv_diag);
+hipError_t launch_pcg_update_p(hipStream_t, int N, const CGState *st, const T *r, T *p, const T *inv_diag);
 // reliable update: r32 = r64 (non-Dirichlet), partial r.z; then the scalar
 // step that tests the true residual and restarts beta from it
-hipError_t launch_ru_resume(hipStream_t, int N, const double *r64, float *r32, const double *inv_diag,
+hipError_t launch_ru_resume(hipStream_t, int N, const double *r64, float *r32, const float *inv_diag,
                             double *partials);
 hipError_t launch_ru_scalar(hipStream_t, CGState *st, const double *pa, const double *prr, int nb);
 template <typename T>
@@ -231,6 +231,7 @@ private:
 
     DeviceBuffer X_, z_, w_, tab64_, tab32_, dtab_, geo64_, geo32_;
     DeviceBuffer b_, x_, invd_, sloc64_, tmp64_, r64_;
+    DeviceBuffer invd32_;  // FP32 copy of invd_ for the FP32 iterations of the mixed solve
     // CG work vectors per precision
     DeviceBuffer cg64_[6], cg32_[6], sloc32_;  // x r p q w s
     DeviceBuffer state_, pa_, pmu_, scalar_;
