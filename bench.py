@@ -165,7 +165,9 @@ def bench_mfma(N, torch, steps=10, warmup=2):
     (24^3), FP64 and FP32, regular geometry, and the deformed (per-point
     geometry) kernel at P = 7.  Every tile is timed by the autotuner; the
     fastest tile of each family is reported with its algorithmic TFLOP/s
-    and fraction of the precision's peak."""
+    and fraction of the precision's peak, and for the deformed kernel (the
+    apply of the curved P = 7 mesh, HBM-bound) its algorithmic GB/s and
+    fraction of the 8 TB/s HBM peak."""
     import numpy as np
 
     E = 24 ** 3
@@ -196,6 +198,13 @@ def bench_mfma(N, torch, steps=10, warmup=2):
                 t = min(cand, key=cand.get)
                 r[fam] = {"tile": t, "us": round(cand[t], 2), "tflops": round(F / cand[t] / 1e6, 2),
                           "fp_frac": round(F / cand[t] / 1e6 / peak, 3)}
+                if deformed:
+                    # per-point geometry makes this kernel HBM-bound: its
+                    # algorithmic bytes (coefficients in and out, 7 geometry
+                    # values per point) against the 8 TB/s peak
+                    B = hex_bytes_per_elem(nm, nq, True, 8 if prec == N.FP64 else 4) * E
+                    r[fam]["alg_gbs"] = round(B / cand[t] / 1e3, 1)
+                    r[fam]["hbm_frac"] = round(B / cand[t] / 1e3 / PEAK_HBM_GBS, 3)
             r["mfma_over_valu"] = round(r["valu"]["us"] / r["mfma"]["us"], 3)
             out[f"P{nm - 1}_Q{nq}_{'deformed' if deformed else 'regular'}_{tag}"] = r
             del op
