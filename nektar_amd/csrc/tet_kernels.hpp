@@ -183,7 +183,9 @@ struct LTab {
     static constexpr int UNITS = (Dm::NQ2 + VW - 1) / VW;
     static constexpr bool PAD = sizeof(T) == 8 && Dm::NQ2 % VW != 0 && UNITS % 2 == 0;
     static constexpr int LP = (PAD ? UNITS + 1 : UNITS) * VW;
-    static constexpr int NROWS = Dm::NB1R + (G ? 0 : Dm::NB2R);
+    // with G only B1 is staged, and without its last zero row: the pair
+    // stages read rows pb + q <= modeb(NM-1, 0) + NM-1 = NB1R - 2
+    static constexpr int NROWS = G ? Dm::NB1R - 1 : Dm::NB1R + Dm::NB2R;
     static constexpr int SIZE = NROWS * LP;
     const T *l;  // staged rows in LDS
     const T *g;  // the table buffer
@@ -202,6 +204,10 @@ struct LTab {
 // faster.
 template <int NQ>
 constexpr bool kB2Global = NQ >= 11;
+// Those FP64 kernels then fit 4 workgroups per CU in LDS, and ask for the
+// registers of 4 waves per SIMD (<= 128 VGPRs) to match.
+template <typename T, int NQ>
+constexpr int kTetWaves = (sizeof(T) == 8 && kB2Global<NQ>) ? 4 : 1;
 
 // Work split of the (p,q) pair stages over TPE work-items.  With room for
 // two work-items per pair, the forward r-contraction gives each a half of
@@ -508,7 +514,7 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void bwdtrans_kernel(in
 }
 
 template <typename T, int NM, int NQ, bool DEF, int TPE>
-__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void iproduct_kernel(int nelmt, const T *__restrict__ tabp,
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) void iproduct_kernel(int nelmt, const T *__restrict__ tabp,
                                                                            const T *__restrict__ jac,
                                                                            const T *__restrict__ in,
                                                                            T *__restrict__ out)
@@ -872,7 +878,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
 }
 
 template <typename T, int NM, int NQ, bool DEF, int TPE>
-__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void helmholtz_kernel(int nelmt, const T *__restrict__ tabp,
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) void helmholtz_kernel(int nelmt, const T *__restrict__ tabp,
                                                                             const T *__restrict__ geo, T lambda,
                                                                             const T *__restrict__ in,
                                                                             T *__restrict__ out)
@@ -906,7 +912,7 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void adv_diff_react_ker
 // Grouped form: several operators of the same kind in one launch; block b
 // belongs to the item whose [block0, next block0) range holds it.
 template <typename T, int NM, int NQ, bool DEF, int TPE>
-__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void helmholtz_grouped_kernel(
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) void helmholtz_grouped_kernel(
     int count, const GroupItem<T> *__restrict__ items, const T *__restrict__ tabp)
 {
     using Ti = Tile<NM, NQ, TPE>;
@@ -926,7 +932,7 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void helmholtz_grouped_
 // t1 <- f2 (t1 + f3 t2) with f0 = h2 h3, f1 = h0, f2 = h3, f3 = h1, then
 // weighted by W J.  Geometry packed as for the Helmholtz kernel (df, J).
 template <typename T, int NM, int NQ, bool DEF, int TPE>
-__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void iproduct_deriv_kernel(
+__global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) void iproduct_deriv_kernel(
     int nelmt, const T *__restrict__ tabp, const T *__restrict__ geo, const T *__restrict__ in0,
     const T *__restrict__ in1, const T *__restrict__ in2, T *__restrict__ out)
 {
