@@ -198,16 +198,16 @@ struct LTab {
 };
 
 // The Helmholtz and inner-product kernels read the Modified_C rows from
-// global memory from Q = 11 on: at P = 9 that takes FP64 Helmholtz from
-// 59 to 41 KB of LDS per workgroup (2 -> 3 waves per SIMD) and its time
-// down by an eighth; at lower orders, and in BwdTrans, staging measured
-// faster.
-template <int NQ>
-constexpr bool kB2Global = NQ >= 11;
+// global memory from Q = 10 on in FP64 and Q = 11 in FP32: at P = 9 that
+// takes FP64 Helmholtz from 59 to 41 KB of LDS per workgroup (2 -> 3
+// waves per SIMD) and its time down by an eighth; at lower orders, and in
+// BwdTrans, staging measured faster.
+template <typename T, int NQ>
+constexpr bool kB2Global = NQ >= (sizeof(T) == 8 ? 10 : 11);
 // Those FP64 kernels then fit 4 workgroups per CU in LDS, and ask for the
 // registers of 4 waves per SIMD (<= 128 VGPRs) to match.
 template <typename T, int NQ>
-constexpr int kTetWaves = (sizeof(T) == 8 && kB2Global<NQ>) ? 4 : 1;
+constexpr int kTetWaves = (sizeof(T) == 8 && kB2Global<T, NQ>) ? 4 : 1;
 
 // Work split of the (p,q) pair stages over TPE work-items.  With room for
 // two work-items per pair, the forward r-contraction gives each a half of
@@ -522,10 +522,10 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) voi
     using Dm = Dims<NM, NQ>;
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
-    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<T, NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<T, NQ>>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ, kB2Global<NQ>> lt{ltab, tabp};
+    const LTab<T, NM, NQ, kB2Global<T, NQ>> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt;
@@ -705,7 +705,7 @@ __device__ __forceinline__ void helmholtz_block(int blk, int nelmt, const T *__r
     using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ, kB2Global<NQ>> lt{ltab, tabp};
+    const LTab<T, NM, NQ, kB2Global<T, NQ>> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blk * Ti::EPB + le;
     const bool valid = e < nelmt, col = t < Ti::NCOL;
@@ -885,8 +885,8 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) voi
 {
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
-    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<T, NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<T, NQ>>(tabp, ltab);
     helmholtz_block<T, NM, NQ, DEF, TPE>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab);
 }
 
@@ -904,8 +904,8 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK)) void adv_diff_react_ker
 {
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
-    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<T, NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<T, NQ>>(tabp, ltab);
     helmholtz_block<T, NM, NQ, DEF, TPE, true>(blockIdx.x, nelmt, tabp, geo, lambda, in, out, lds, ltab, adv);
 }
 
@@ -917,8 +917,8 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) voi
 {
     using Ti = Tile<NM, NQ, TPE>;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
-    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<T, NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<T, NQ>>(tabp, ltab);
     const int b = blockIdx.x;
     int it = 0;
     while (it + 1 < count && b >= items[it + 1].block0) ++it;
@@ -940,10 +940,10 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) voi
     using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<NQ>>::SIZE];
-    stage_tables<T, NM, NQ, kB2Global<NQ>>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<T, NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2Global<T, NQ>>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ, kB2Global<NQ>> lt{ltab, tabp};
+    const LTab<T, NM, NQ, kB2Global<T, NQ>> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt, col = t < Ti::NCOL;
