@@ -73,7 +73,8 @@ def _oracle(opname, nm, nq0, nel, deformed, jac, df, vel, xs):
 
 
 @pytest.mark.parametrize("opname", OPS)
-@pytest.mark.parametrize("nm,nq0", [(2, 3), (4, 5), (5, 7), (7, 8), (7, 9), (8, 9), (10, 11), (10, 12)])
+@pytest.mark.parametrize("nm,nq0", [(2, 3), (4, 5), (5, 7), (7, 8), (7, 9), (8, 9), (8, 10), (9, 10), (10, 11),
+                                     (10, 12)])
 @pytest.mark.parametrize("deformed", [False, True], ids=["regular", "deformed"])
 @pytest.mark.parametrize("fp32", [False, True], ids=["fp64", "fp32"])
 def test_tiles_bit_identical_and_oracle(nmf, opname, nm, nq0, deformed, fp32):
