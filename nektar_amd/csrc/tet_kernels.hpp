@@ -206,6 +206,11 @@ template <typename T, int NQ>
 constexpr bool kB2Global = NQ >= (sizeof(T) == 8 ? 10 : 11);
 // Those FP64 kernels then fit 4 workgroups per CU in LDS, and ask for the
 // registers of 4 waves per SIMD (<= 128 VGPRs) to match.
+// IProductWRTDerivBase gains from the global rows one order earlier in
+// FP32 (Q = 10: 69.5 -> 60 us at P = 8, 16,384 tets), where the Helmholtz
+// kernel loses.
+template <typename T, int NQ>
+constexpr bool kB2GlobalIPD = kB2Global<T, NQ> || (sizeof(T) == 4 && NQ == 10);
 template <typename T, int NQ>
 constexpr int kTetWaves = (sizeof(T) == 8 && kB2Global<T, NQ>) ? 4 : 1;
 
@@ -940,10 +945,10 @@ __global__ __launch_bounds__((Tile<NM, NQ, TPE>::BLOCK), (kTetWaves<T, NQ>)) voi
     using Ti = Tile<NM, NQ, TPE>;
     constexpr int NQ0 = Dm::NQ0, NQ1 = Dm::NQ1, NQ2 = Dm::NQ2, NQT = Dm::NQT;
     __shared__ T lds[Ti::EPB][2][Slab<T, NM, NQ>::SIZE];
-    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2Global<T, NQ>>::SIZE];
-    stage_tables<T, NM, NQ, kB2Global<T, NQ>>(tabp, ltab);
+    __shared__ __align__(16) T ltab[LTab<T, NM, NQ, kB2GlobalIPD<T, NQ>>::SIZE];
+    stage_tables<T, NM, NQ, kB2GlobalIPD<T, NQ>>(tabp, ltab);
     const Tab<T, NM, NQ> tab{tabp};
-    const LTab<T, NM, NQ, kB2Global<T, NQ>> lt{ltab, tabp};
+    const LTab<T, NM, NQ, kB2GlobalIPD<T, NQ>> lt{ltab, tabp};
     const int le = threadIdx.x / TPE, t = threadIdx.x % TPE;
     const int e = blockIdx.x * Ti::EPB + le;
     const bool valid = e < nelmt, col = t < Ti::NCOL;
