@@ -206,11 +206,11 @@ template <typename T, int NQ>
 constexpr bool kB2Global = NQ >= (sizeof(T) == 8 ? 10 : 11);
 // Those FP64 kernels then fit 4 workgroups per CU in LDS, and ask for the
 // registers of 4 waves per SIMD (<= 128 VGPRs) to match.
-// IProductWRTDerivBase gains from the global rows one order earlier in
-// FP32 (Q = 10: 69.5 -> 60 us at P = 8, 16,384 tets), where the Helmholtz
-// kernel loses.
+// IProductWRTDerivBase gains from the global rows where the Helmholtz
+// kernel loses: in FP32 from Q = 10 (69.5 -> 60 us at P = 8, 16,384
+// tets), in FP64 from Q = 8 (P = 6: 131 -> 125 us, P = 7: 85 -> 81 us).
 template <typename T, int NQ>
-constexpr bool kB2GlobalIPD = kB2Global<T, NQ> || (sizeof(T) == 4 && NQ == 10);
+constexpr bool kB2GlobalIPD = kB2Global<T, NQ> || (sizeof(T) == 4 && NQ == 10) || (sizeof(T) == 8 && NQ >= 8);
 template <typename T, int NQ>
 constexpr int kTetWaves = (sizeof(T) == 8 && kB2Global<T, NQ>) ? 4 : 1;
 
